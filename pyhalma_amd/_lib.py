@@ -1,0 +1,85 @@
+"""ctypes binding of libhalma.so (the C ABI declared in include/halma.h).
+
+torch is imported first so that the HIP runtime already loaded by torch
+(libamdhip64.so.7) is the one libhalma binds to: device pointers and streams then
+come straight from torch tensors and ``torch.cuda.current_stream()``.
+
+There is no fallback: if the library is missing or a GPU call fails, an exception is
+raised.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime libhalma binds to)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhalma.so")
+
+_c = ctypes
+_p = _c.c_void_p
+_i64 = _c.c_int64
+_i32 = _c.c_int32
+_dbl = _c.c_double
+_size = _c.c_size_t
+_i64p = _c.POINTER(_c.c_int64)
+
+# name: (restype, [argtypes])
+SIGNATURES = {
+    "halma_last_error": (_c.c_char_p, []),
+    "halma_version": (_c.c_int, []),
+    "halma_device_cu_count": (_c.c_int, []),
+    "halma_pot_sum_workspace_size": (_size, [_i64, _i64]),
+    "halma_pot_sum_f32": (_c.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p,
+                                     _size, _p]),
+    "halma_pot_error_bound": (_dbl, []),
+    "halma_pot_sum_batched_workspace_size": (_size, [_i64, _i64p, _i64p]),
+    "halma_pot_sum_batched_f32": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _i64p, _p, _p, _p,
+                                             _p, _p, _size, _p]),
+    "halma_pot_sum_f64_ordered": (_c.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p]),
+    "halma_pot_sum_f64_plain": (_c.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p]),
+    "halma_tune_set": (_c.c_int, [_c.c_int, _c.c_int]),
+    "halma_tune_get": (_c.c_int, [_c.c_int]),
+}
+
+TUNE_TESTS_PER_LANE = 1
+TUNE_SUBTILE = 2
+TUNE_MIN_CHUNK = 3
+TUNE_ITEMS_PER_WAVE = 4
+TUNE_WAVES_PER_CU = 5
+
+F32 = 0
+F64 = 1
+
+_LIB = None
+
+
+class HalmaError(RuntimeError):
+    """A libhalma entry point returned a non-zero code."""
+
+
+def load():
+    """Load libhalma.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libhalma.so not found at {LIB_PATH}; build it with "
+                "`python -m pyhalma_amd.build` (hipcc, gfx950)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = load().halma_last_error().decode(errors="replace")
+        raise HalmaError(f"{what}: code {rc}: {msg}")
+
+
+def i64_array(values):
+    arr = (ctypes.c_int64 * len(values))(*[int(v) for v in values])
+    return arr

@@ -1,0 +1,80 @@
+// Internal interface of the pair-potential kernels (shared by potential.hip and the
+// batched unbinding driver).
+#pragma once
+#include <vector>
+
+#include "halma_common.h"
+
+namespace halma {
+
+// Per-halo descriptor of one batched pair-sum launch.  Sources of halo h are
+// src4[src_off, src_off + src_len), tests tst4[tst_off, tst_off + tst_len); chunk c of
+// the sources covers [c*chunk, (c+1)*chunk); partial sums of test i for chunk c live at
+// part[part_off + c*stride + i].  With a dynamic length array (self-potential of a
+// shrinking bound set) src_len = tst_len = dyn_len[h] at run time, never above the
+// planned lengths.
+struct __attribute__((aligned(8))) HaloDesc {
+    int64_t src_off, tst_off, part_off;
+    int32_t src_len, tst_len, chunk, stride;
+};
+
+struct PotPlan {
+    std::vector<HaloDesc> halos;
+    std::vector<int4> items;   // {halo, test tile, source chunk, 0}, largest first
+    int64_t n_src = 0, n_tst = 0, n_part = 0;
+    int tests_per_lane = 2, subtile = 8;
+    int grid = 0;              // persistent workgroups (256 threads)
+};
+
+// Plans a batched launch from HOST offset arrays (H+1 entries each).
+int plan_pot(int64_t n_halo, const int64_t* src_off, const int64_t* tst_off, PotPlan& plan);
+
+// Launches the persistent FP32 kernel for a plan whose tables are already on the
+// device.  dyn_len may be null.  head must be a device int (reset by this launcher).
+int launch_pot_f32(const PotPlan& plan, const float4* src4, const float4* tst4,
+                   const HaloDesc* d_halos, const int* dyn_len, const int4* d_items,
+                   float2* part, int* head, hipStream_t stream);
+
+// Packs SoA FP32 (m, x, y, z) into AoS float4 (x, y, z, m); m may be null (-> 0).
+int launch_pack4(const float* m, const float* x, const float* y, const float* z,
+                 int64_t n, float4* out, hipStream_t stream);
+
+// Source accessors for the ordered FP64 sum.
+struct SrcAoS {
+    const float4* __restrict__ p;
+    __device__ __forceinline__ float4 operator()(int64_t j) const { return p[j]; }
+};
+struct SrcSoA {
+    const float* __restrict__ m;
+    const float* __restrict__ x;
+    const float* __restrict__ y;
+    const float* __restrict__ z;
+    __device__ __forceinline__ float4 operator()(int64_t j) const {
+        return make_float4(x[j], y[j], z[j], m[j]);
+    }
+};
+
+// FP64 sum in the oracle's fixed order (64 strided partials of ascending j, then the
+// pairwise tree) for one test position against ns sources; used by the band re-test.
+// Must be called by a whole wavefront; every lane returns the same value.
+template <class Src>
+__device__ __forceinline__ double wave_pot_f64_ordered(Src src, int64_t ns, float tx,
+                                                        float ty, float tz) {
+    const int lane = __lane_id();
+    double acc = 0.0;
+    const double xd = tx, yd = ty, zd = tz;
+    for (int64_t j = lane; j < ns; j += kWave) {
+        const float4 q = src(j);
+        if (q.x != tx && q.y != ty && q.z != tz) {
+            const double dx = static_cast<double>(q.x) - xd;
+            const double dy = static_cast<double>(q.y) - yd;
+            const double dz = static_cast<double>(q.z) - zd;
+            const double d2 = (dx * dx + dy * dy) + dz * dz;
+            acc = acc + static_cast<double>(q.w) / __builtin_sqrt(d2);
+        }
+    }
+    for (int s = 32; s >= 1; s >>= 1) acc = acc + __shfl_down(acc, s, kWave);
+    return __shfl(acc, 0, kWave);
+}
+
+}  // namespace halma

@@ -1,0 +1,446 @@
+// Pair-potential kernels: sum_j m_j / |x_j - x_i| over the sources whose three
+// coordinates all differ from test i's (particle_subroutines.f90:466-514).
+//
+// FP32 path (the hot one): one persistent launch; every wavefront is an independent
+// worker that pops queue items {halo, 64*R-test tile, source chunk} (largest first)
+// with one vector atomic.  Each lane holds R test positions in registers; sources are
+// wave-uniform and arrive through scalar loads (AoS float4 x,y,z,m), so the inner loop
+// is pure VALU: 3 sub, mul + 2 fma, v_rsq_f32, mul, the skip test, add.  SUB terms are
+// summed plainly, then Kahan-added into the running (sum, compensation) pair; partials
+// of split source ranges are merged in FP64 in chunk order (deterministic).
+#include <algorithm>
+#include <numeric>
+
+#include "halma_common.h"
+#include "potential.h"
+#include "staging.h"
+#include "tuning.h"
+
+namespace halma {
+
+__device__ __forceinline__ float pair_term(const float4 s, float xi, float yi, float zi) {
+    const float dx = s.x - xi;
+    const float dy = s.y - yi;
+    const float dz = s.z - zi;
+    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float t = s.w * __builtin_amdgcn_rsqf(d2);
+    // x_j /= x_i .and. y_j /= y_i .and. z_j /= z_i: a difference is zero exactly when
+    // the coordinates are equal (FP32 subtraction keeps denormals), so the pair counts
+    // iff the smallest |difference| is positive (one v_min3_f32 with |.| modifiers).
+    const float mn = fminf(fminf(fabsf(dx), fabsf(dy)), fabsf(dz));
+    return mn > 0.0f ? t : 0.0f;
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+template <int R, int SUB>
+__global__ __launch_bounds__(256) void k_pot_f32(const float4* __restrict__ src,
+                                                 const float4* __restrict__ tst,
+                                                 const HaloDesc* __restrict__ halos,
+                                                 const int* __restrict__ dyn_len,
+                                                 const int4* __restrict__ items, int n_items,
+                                                 float2* __restrict__ part,
+                                                 int* __restrict__ head) {
+    const int lane = __lane_id();
+    for (;;) {
+        int k = 0;
+        if (lane == 0) k = atomicAdd(head, 1);
+        k = __builtin_amdgcn_readfirstlane(k);
+        if (k >= n_items) break;
+        const int4 it = uniform_load(items + k);
+        const HaloDesc d = uniform_load(halos + it.x);
+        int ns = d.src_len, nt = d.tst_len;
+        if (dyn_len) {
+            ns = __builtin_amdgcn_readfirstlane(uniform_load(dyn_len + it.x));
+            nt = ns;
+        }
+        const int i0 = it.y * (R * kWave);
+        const int j0 = it.z * d.chunk;
+        const int j1 = min(ns, j0 + d.chunk);
+        if (i0 >= nt || j0 >= j1) continue;
+
+        float xi[R], yi[R], zi[R], as[R], ac[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = min(i0 + r * kWave + lane, nt - 1);
+            const float4 p = tst[d.tst_off + i];
+            xi[r] = p.x;
+            yi[r] = p.y;
+            zi[r] = p.z;
+            as[r] = 0.0f;
+            ac[r] = 0.0f;
+        }
+        const float4* sp = src + d.src_off;
+        int j = j0;
+        for (; j + SUB <= j1; j += SUB) {
+            float ps[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) ps[r] = 0.0f;
+#pragma unroll
+            for (int s = 0; s < SUB; ++s) {
+                const float4 q = uniform_load(sp + j + s);
+#pragma unroll
+                for (int r = 0; r < R; ++r) ps[r] += pair_term(q, xi[r], yi[r], zi[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) kahan_add(as[r], ac[r], ps[r]);
+        }
+        if (j < j1) {
+            float ps[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) ps[r] = 0.0f;
+            for (; j < j1; ++j) {
+                const float4 q = uniform_load(sp + j);
+#pragma unroll
+                for (int r = 0; r < R; ++r) ps[r] += pair_term(q, xi[r], yi[r], zi[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) kahan_add(as[r], ac[r], ps[r]);
+        }
+        float2* out = part + d.part_off + static_cast<int64_t>(it.z) * d.stride;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = i0 + r * kWave + lane;
+            if (i < nt) out[i] = make_float2(as[r], ac[r]);
+        }
+    }
+}
+
+__global__ void k_fill_i32(int* p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_pack4(const float* __restrict__ m, const float* __restrict__ x,
+                        const float* __restrict__ y, const float* __restrict__ z, int64_t n,
+                        float4* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float4(x[i], y[i], z[i], m ? m[i] : 0.0f);
+}
+
+// halo id of every test (tests of halo h are [tst_off, tst_off + tst_len))
+__global__ void k_tst_halo(const HaloDesc* __restrict__ halos, int n_halo,
+                           int* __restrict__ tst_halo) {
+    const int h = blockIdx.x;
+    if (h >= n_halo) return;
+    const HaloDesc d = halos[h];
+    for (int i = threadIdx.x; i < d.tst_len; i += blockDim.x) tst_halo[d.tst_off + i] = h;
+}
+
+// Merge per-chunk partials (chunk order) into FP64 sums.
+__global__ void k_pot_merge(const HaloDesc* __restrict__ halos, const int* __restrict__ tst_halo,
+                            int64_t n_tst, const float2* __restrict__ part,
+                            double* __restrict__ out_f64, float* __restrict__ out_f32) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n_tst) return;
+    const HaloDesc d = halos[tst_halo[g]];
+    const int64_t i = g - d.tst_off;
+    const int nch = d.src_len > 0 ? (d.src_len + d.chunk - 1) / d.chunk : 0;
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) {
+        const float2 p = part[d.part_off + static_cast<int64_t>(c) * d.stride + i];
+        s += static_cast<double>(p.x) + static_cast<double>(p.y);
+    }
+    if (out_f64) out_f64[g] = s;
+    if (out_f32) out_f32[g] = static_cast<float>(s);
+}
+
+__global__ __launch_bounds__(256) void k_pot_f64_ordered(SrcSoA src, int64_t ns,
+                                                         const float* __restrict__ tx,
+                                                         const float* __restrict__ ty,
+                                                         const float* __restrict__ tz,
+                                                         int64_t nt, double* __restrict__ out) {
+    const int64_t w = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    if (w >= nt) return;
+    const double s = wave_pot_f64_ordered(src, ns, tx[w], ty[w], tz[w]);
+    if (__lane_id() == 0) out[w] = s;
+}
+
+__global__ __launch_bounds__(256) void k_pot_f64_plain(
+    const double* __restrict__ m, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, int64_t ns, const double* __restrict__ tx,
+    const double* __restrict__ ty, const double* __restrict__ tz, int64_t nt,
+    double* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const double xi = tx[i], yi = ty[i], zi = tz[i];
+    double acc = 0.0;
+    for (int64_t j = 0; j < ns; ++j) {
+        if (x[j] != xi && y[j] != yi && z[j] != zi) {
+            const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
+            acc += m[j] / __builtin_sqrt((dx * dx + dy * dy) + dz * dz);
+        }
+    }
+    out[i] = acc;
+}
+
+// ------------------------------------------------------------------ host side ----
+
+int plan_pot(int64_t H, const int64_t* src_off, const int64_t* tst_off, PotPlan& plan) {
+    const Tuning t = g_tuning;
+    plan = PotPlan();
+    plan.tests_per_lane = t.tests_per_lane;
+    plan.subtile = t.subtile;
+    const int tile = t.tests_per_lane * kWave;
+    plan.halos.resize(H);
+    int cus = device_cu_count();
+    if (cus <= 0) cus = 256;  // planning without a device (workspace queries)
+    const int64_t waves = static_cast<int64_t>(cus) * t.waves_per_cu;
+    plan.grid = static_cast<int>(waves / 4);
+    const int64_t target = waves * t.items_per_wave;
+    int64_t base_items = 0;
+    double work = 0.0;
+    for (int64_t h = 0; h < H; ++h) {
+        const int64_t ns = src_off[h + 1] - src_off[h];
+        const int64_t nt = tst_off[h + 1] - tst_off[h];
+        if (ns < 0 || nt < 0 || ns > INT32_MAX || nt > INT32_MAX)
+            return set_error(HALMA_E_ARG, "halo %lld: bad offsets", (long long)h);
+        base_items += (nt + tile - 1) / tile;
+        work += static_cast<double>(ns) * static_cast<double>(nt);
+    }
+    int64_t part = 0;
+    std::vector<double> key(H);
+    for (int64_t h = 0; h < H; ++h) {
+        const int64_t ns = src_off[h + 1] - src_off[h];
+        const int64_t nt = tst_off[h + 1] - tst_off[h];
+        const int64_t tiles = (nt + tile - 1) / tile;
+        int64_t nch = 1;
+        if (base_items < target && tiles > 0 && ns > 0 && work > 0.0) {
+            const double want = static_cast<double>(target) *
+                                (static_cast<double>(ns) * static_cast<double>(nt) / work) /
+                                static_cast<double>(tiles);
+            const int64_t max_ch = std::max<int64_t>(1, ns / t.min_chunk);
+            nch = std::min<int64_t>(max_ch, std::max<int64_t>(1, static_cast<int64_t>(want)));
+        }
+        int64_t chunk = ns > 0 ? (ns + nch - 1) / nch : 1;
+        chunk = (chunk + t.subtile - 1) / t.subtile * t.subtile;
+        nch = ns > 0 ? (ns + chunk - 1) / chunk : 0;
+        HaloDesc& d = plan.halos[h];
+        d.src_off = src_off[h];
+        d.tst_off = tst_off[h];
+        d.part_off = part;
+        d.src_len = static_cast<int32_t>(ns);
+        d.tst_len = static_cast<int32_t>(nt);
+        d.chunk = static_cast<int32_t>(chunk);
+        d.stride = static_cast<int32_t>(nt);
+        part += nch * nt;
+        key[h] = static_cast<double>(std::min<int64_t>(nt, tile)) * static_cast<double>(chunk);
+    }
+    std::vector<int64_t> order(H);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return key[a] > key[b]; });
+    for (int64_t h : order) {
+        const HaloDesc& d = plan.halos[h];
+        const int tiles = (d.tst_len + tile - 1) / tile;
+        const int nch = d.src_len > 0 ? (d.src_len + d.chunk - 1) / d.chunk : 0;
+        for (int c = 0; c < nch; ++c)
+            for (int ti = 0; ti < tiles; ++ti)
+                plan.items.push_back(make_int4(static_cast<int>(h), ti, c, 0));
+    }
+    if (plan.items.size() > static_cast<size_t>(INT32_MAX))
+        return set_error(HALMA_E_LIMIT, "too many work items");
+    plan.n_src = src_off[H] - src_off[0];
+    plan.n_tst = tst_off[H] - tst_off[0];
+    plan.n_part = part;
+    return 0;
+}
+
+int launch_pack4(const float* m, const float* x, const float* y, const float* z, int64_t n,
+                 float4* out, hipStream_t stream) {
+    if (n <= 0) return 0;
+    const int64_t blocks = (n + 255) / 256;
+    k_pack4<<<dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream>>>(m, x, y, z, n, out);
+    HALMA_LAUNCH_CHECK();
+    return 0;
+}
+
+template <int R, int SUB>
+static int launch_pot_t(const PotPlan& plan, const float4* src4, const float4* tst4,
+                        const HaloDesc* d_halos, const int* dyn_len, const int4* d_items,
+                        float2* part, int* head, hipStream_t stream) {
+    k_fill_i32<<<1, 64, 0, stream>>>(head, 1, 0);
+    HALMA_LAUNCH_CHECK();
+    const int n_items = static_cast<int>(plan.items.size());
+    if (n_items == 0) return 0;
+    const int grid = std::max(1, std::min(plan.grid, (n_items + 3) / 4));
+    k_pot_f32<R, SUB><<<grid, 256, 0, stream>>>(src4, tst4, d_halos, dyn_len, d_items, n_items,
+                                                part, head);
+    HALMA_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_pot_f32(const PotPlan& plan, const float4* src4, const float4* tst4,
+                   const HaloDesc* d_halos, const int* dyn_len, const int4* d_items,
+                   float2* part, int* head, hipStream_t stream) {
+#define HALMA_POT_CASE(R_, S_)                                                            \
+    if (plan.tests_per_lane == R_ && plan.subtile == S_)                                  \
+        return launch_pot_t<R_, S_>(plan, src4, tst4, d_halos, dyn_len, d_items, part, head, \
+                                    stream);
+    HALMA_POT_CASE(1, 4)
+    HALMA_POT_CASE(1, 8)
+    HALMA_POT_CASE(2, 4)
+    HALMA_POT_CASE(2, 8)
+    HALMA_POT_CASE(4, 4)
+    HALMA_POT_CASE(4, 8)
+#undef HALMA_POT_CASE
+    return set_error(HALMA_E_STATE, "no kernel for R=%d SUB=%d", plan.tests_per_lane,
+                     plan.subtile);
+}
+
+// Workspace layout of a batched pair-sum call.
+struct PotWs {
+    float4* src4;
+    float4* tst4;
+    HaloDesc* halos;
+    int4* items;
+    float2* part;
+    int* tst_halo;
+    int* head;
+    size_t bytes;
+};
+
+static PotWs carve_pot(void* base, const PotPlan& p) {
+    Carver c(base);
+    PotWs w;
+    w.src4 = c.take<float4>(std::max<int64_t>(p.n_src, 1));
+    w.tst4 = c.take<float4>(std::max<int64_t>(p.n_tst, 1));
+    w.halos = c.take<HaloDesc>(std::max<size_t>(p.halos.size(), 1));
+    w.items = c.take<int4>(std::max<size_t>(p.items.size(), 1));
+    w.part = c.take<float2>(std::max<int64_t>(p.n_part, 1));
+    w.tst_halo = c.take<int>(std::max<int64_t>(p.n_tst, 1));
+    w.head = c.take<int>(64);
+    w.bytes = c.used;
+    return w;
+}
+
+static int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const float* sx,
+                       const float* sy, const float* sz, const int64_t* tst_off,
+                       const float* tx, const float* ty, const float* tz, double* out_f64,
+                       float* out_f32, void* ws, size_t ws_bytes, hipStream_t stream) {
+    PotPlan plan;
+    int rc = plan_pot(H, src_off, tst_off, plan);
+    if (rc) return rc;
+    PotWs w = carve_pot(ws, plan);
+    if (ws_bytes < w.bytes)
+        return set_error(HALMA_E_WORKSPACE, "workspace %zu < required %zu", ws_bytes, w.bytes);
+    if (plan.n_tst == 0) return 0;
+    const int64_t s0 = src_off[0], t0 = tst_off[0];
+    // the descriptors index the packed copies from zero
+    for (auto& d : plan.halos) {
+        d.src_off -= s0;
+        d.tst_off -= t0;
+    }
+    rc = launch_pack4(sm + s0, sx + s0, sy + s0, sz + s0, plan.n_src, w.src4, stream);
+    if (rc) return rc;
+    rc = launch_pack4(nullptr, tx + t0, ty + t0, tz + t0, plan.n_tst, w.tst4, stream);
+    if (rc) return rc;
+    rc = upload_async(w.halos, plan.halos.data(), plan.halos.size() * sizeof(HaloDesc), stream);
+    if (rc) return rc;
+    rc = upload_async(w.items, plan.items.data(), plan.items.size() * sizeof(int4), stream);
+    if (rc) return rc;
+    k_tst_halo<<<static_cast<unsigned>(H), 256, 0, stream>>>(w.halos, static_cast<int>(H),
+                                                             w.tst_halo);
+    HALMA_LAUNCH_CHECK();
+    rc = launch_pot_f32(plan, w.src4, w.tst4, w.halos, nullptr, w.items, w.part, w.head, stream);
+    if (rc) return rc;
+    const int64_t blocks = (plan.n_tst + 255) / 256;
+    k_pot_merge<<<static_cast<unsigned>(blocks), 256, 0, stream>>>(
+        w.halos, w.tst_halo, plan.n_tst, w.part, out_f64 ? out_f64 + t0 : nullptr,
+        out_f32 ? out_f32 + t0 : nullptr);
+    HALMA_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace halma
+
+using namespace halma;
+
+extern "C" {
+
+double halma_pot_error_bound(void) { return pot_error_bound(g_tuning.subtile); }
+
+size_t halma_pot_sum_batched_workspace_size(int64_t H, const int64_t* src_off,
+                                            const int64_t* tst_off) {
+    if (H < 0 || !src_off || !tst_off) return 0;
+    PotPlan plan;
+    if (plan_pot(H, src_off, tst_off, plan)) return 0;
+    return carve_pot(nullptr, plan).bytes;
+}
+
+size_t halma_pot_sum_workspace_size(int64_t n_src, int64_t n_test) {
+    const int64_t so[2] = {0, n_src}, to[2] = {0, n_test};
+    return halma_pot_sum_batched_workspace_size(1, so, to);
+}
+
+int halma_pot_sum_batched_f32(int64_t H, const int64_t* src_off, const float* sm,
+                              const float* sx, const float* sy, const float* sz,
+                              const int64_t* tst_off, const float* tx, const float* ty,
+                              const float* tz, double* out_f64, void* ws, size_t ws_bytes,
+                              void* stream) {
+    if (H < 0 || !src_off || !tst_off || !out_f64)
+        return set_error(HALMA_E_ARG, "halma_pot_sum_batched_f32: null argument");
+    if (H == 0) return 0;
+    for (int64_t h = 0; h < H; ++h)
+        if (src_off[h + 1] < src_off[h] || tst_off[h + 1] < tst_off[h])
+            return set_error(HALMA_E_ARG, "offsets must be non-decreasing");
+    if ((src_off[H] > src_off[0] && (!sm || !sx || !sy || !sz)) ||
+        (tst_off[H] > tst_off[0] && (!tx || !ty || !tz)))
+        return set_error(HALMA_E_ARG, "halma_pot_sum_batched_f32: null data pointer");
+    return pot_batched(H, src_off, sm, sx, sy, sz, tst_off, tx, ty, tz, out_f64, nullptr, ws,
+                       ws_bytes, static_cast<hipStream_t>(stream));
+}
+
+int halma_pot_sum_f32(const float* m, const float* x, const float* y, const float* z,
+                      int64_t n_src, const float* tx, const float* ty, const float* tz,
+                      int64_t n_test, float* out, double* out_f64, void* ws, size_t ws_bytes,
+                      void* stream) {
+    if (n_src < 0 || n_test < 0) return set_error(HALMA_E_ARG, "negative size");
+    if (n_test > 0 && !out && !out_f64)
+        return set_error(HALMA_E_ARG, "halma_pot_sum_f32: no output");
+    if ((n_src > 0 && (!m || !x || !y || !z)) || (n_test > 0 && (!tx || !ty || !tz)))
+        return set_error(HALMA_E_ARG, "halma_pot_sum_f32: null data pointer");
+    if (n_src > INT32_MAX || n_test > INT32_MAX)
+        return set_error(HALMA_E_LIMIT, "at most 2^31-1 sources and tests per halo");
+    const int64_t so[2] = {0, n_src}, to[2] = {0, n_test};
+    return pot_batched(1, so, m, x, y, z, to, tx, ty, tz, out_f64, out, ws, ws_bytes,
+                       static_cast<hipStream_t>(stream));
+}
+
+int halma_pot_sum_f64_ordered(const float* m, const float* x, const float* y, const float* z,
+                              int64_t n_src, const float* tx, const float* ty,
+                              const float* tz, int64_t n_test, double* out, void* stream) {
+    if (n_src < 0 || n_test < 0) return set_error(HALMA_E_ARG, "negative size");
+    if (n_test == 0) return 0;
+    if (!out || !tx || !ty || !tz || (n_src > 0 && (!m || !x || !y || !z)))
+        return set_error(HALMA_E_ARG, "halma_pot_sum_f64_ordered: null pointer");
+    const int64_t blocks = (n_test + 3) / 4;  // four wavefronts per block
+    if (blocks > INT32_MAX) return set_error(HALMA_E_LIMIT, "too many tests");
+    SrcSoA src{m, x, y, z};
+    k_pot_f64_ordered<<<static_cast<unsigned>(blocks), 256, 0,
+                        static_cast<hipStream_t>(stream)>>>(src, n_src, tx, ty, tz, n_test,
+                                                             out);
+    HALMA_LAUNCH_CHECK();
+    return 0;
+}
+
+int halma_pot_sum_f64_plain(const double* m, const double* x, const double* y, const double* z,
+                            int64_t n_src, const double* tx, const double* ty,
+                            const double* tz, int64_t n_test, double* out, void* stream) {
+    if (n_src < 0 || n_test < 0) return set_error(HALMA_E_ARG, "negative size");
+    if (n_test == 0) return 0;
+    if (!out || !tx || !ty || !tz || (n_src > 0 && (!m || !x || !y || !z)))
+        return set_error(HALMA_E_ARG, "halma_pot_sum_f64_plain: null pointer");
+    const int64_t blocks = (n_test + 255) / 256;
+    k_pot_f64_plain<<<static_cast<unsigned>(blocks), 256, 0, static_cast<hipStream_t>(stream)>>>(
+        m, x, y, z, n_src, tx, ty, tz, n_test, out);
+    HALMA_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // extern "C"

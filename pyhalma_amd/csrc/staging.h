@@ -1,0 +1,17 @@
+// Stream-ordered host->device upload of small host-built tables (work plans, offsets).
+// The bytes are first copied into a pinned staging slot; the slot is reused only after
+// the event recorded behind its previous copy has completed, so the caller's host
+// buffer may be freed as soon as upload_async returns.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace halma {
+
+int upload_async(void* dst_device, const void* src_host, size_t bytes, hipStream_t stream);
+
+// A pinned host word array for small device->host results (counters) read after a
+// stream synchronisation.  One per host thread and device; grows on demand.
+long long* pinned_words(int count);
+
+}  // namespace halma

@@ -1,0 +1,25 @@
+// Process-wide tuning of the FP32 pair kernels (written by halma_tune_set).
+#pragma once
+
+namespace halma {
+
+struct Tuning {
+    int tests_per_lane = 2;   // R: test particles held in registers per lane
+    int subtile = 8;          // sources summed in plain FP32 before each Kahan add
+    int min_chunk = 512;      // smallest source chunk when a source range is split
+    int items_per_wave = 4;   // queue items targeted per resident wave
+    int waves_per_cu = 16;    // persistent waves per compute unit
+};
+
+extern Tuning g_tuning;
+
+// Rigorous relative error bound of the compensated FP32 sum for subtile SUB:
+// per term 5.5u (FP32 differences and squares 2.5u after the square root, v_rsq_f32
+// <= 1 ulp = 2u, the mass product u), plain sub-tile sums (SUB-1)u, Kahan 2u, plus
+// FP64 merge and higher-order terms (covered by the extra 0.5u and 1e-11).
+inline double pot_error_bound(int sub) {
+    const double u = 5.9604644775390625e-08;  // 2^-24
+    return (sub + 7.0) * u + 1e-11;
+}
+
+}  // namespace halma
