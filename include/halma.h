@@ -96,6 +96,22 @@ int halma_pot_sum_f64_plain(const double *m, const double *x, const double *y,
                             const double *ty, const double *tz, int64_t n_test,
                             double *out, void *stream);
 
+/* ------------------------------------------------------------------------------------
+ * Friends-of-friends.  Replaces fof.friends_of_friends (python_scripts/fof.py:18-79, the
+ * scipy-KDTree breadth-first search) and, for the same groups, the third-party pyfof
+ * call of fof.py:184-187.
+ *
+ * labels[i] = smallest particle index of i's connected component in the graph with an
+ * edge wherever ((dx*dx + dy*dy) + dz*dz) <= ll*ll in FP64 (FP32 coordinates are
+ * widened exactly), non-periodic.  Singletons label themselves.  Coordinates may span
+ * at most 2^21 sub-cells of side ll/sqrt(3) per axis (HALMA_E_LIMIT otherwise).
+ * Synchronises the stream (it checks the kernels' error flag before returning).
+ * ---------------------------------------------------------------------------------- */
+size_t halma_fof_workspace_size(int64_t n, int dtype);
+int halma_fof_labels(const void *x, const void *y, const void *z, int dtype, int64_t n,
+                     double ll, int32_t *labels, void *workspace, size_t workspace_bytes,
+                     void *stream);
+
 /* Tuning knobs of the FP32 pair kernels (set by the autotuner; read back with get).
  *   HALMA_TUNE_TESTS_PER_LANE : test particles per lane (1, 2, 4)
  *   HALMA_TUNE_SUBTILE        : sources per plain FP32 sub-tile before the Kahan add

@@ -37,6 +37,8 @@ SIGNATURES = {
                                              _p, _p, _size, _p]),
     "halma_pot_sum_f64_ordered": (_c.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p]),
     "halma_pot_sum_f64_plain": (_c.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p]),
+    "halma_fof_workspace_size": (_size, [_i64, _c.c_int]),
+    "halma_fof_labels": (_c.c_int, [_p, _p, _p, _c.c_int, _i64, _dbl, _p, _p, _size, _p]),
     "halma_tune_set": (_c.c_int, [_c.c_int, _c.c_int]),
     "halma_tune_get": (_c.c_int, [_c.c_int]),
 }

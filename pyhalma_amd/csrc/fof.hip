@@ -1,0 +1,555 @@
+// Friends-of-friends labels on the GPU: connected components of the graph with an edge
+// wherever ((dx*dx + dy*dy) + dz*dz) <= ll*ll in FP64 (fof.py:18-79 through scipy's
+// KDTree; inclusive, non-periodic), canonical label = minimum particle index.
+//
+// 1. bounding box (two-pass reduction);
+// 2. sub-cells of side s = ll/sqrt(3) * (1 - 1e-6): two particles in one sub-cell are
+//    within ll under the FP64 test (|dx|,|dy|,|dz| <= s(1+1e-9) after rounding of the
+//    cell index), so a sub-cell is one vertex of the cell graph and needs no pair test;
+//    every link spans at most 2 sub-cells per axis (ll/s < 1.733);
+// 3. occupied sub-cells go into an open-addressing hash table (64-bit keys, vector
+//    atomics); counts and per-particle ranks come from the same pass;
+// 4. an exclusive scan of the table numbers the cells and gives their particle ranges;
+//    particles are scattered into cell order with their coordinates;
+// 5. one wavefront per cell looks up its 62 half-stencil neighbours (one lane each) and,
+//    for every neighbour not already in the same component, searches for one linking
+//    pair with 64 lanes in parallel (early exit) and hooks the two roots (lock-free
+//    union-find over cells: the larger root index is hooked under the smaller, so parents
+//    only decrease and no cycle can form; finds halve paths);
+// 6. roots are read back with atomic reads (see k_roots), each component's minimum
+//    particle index is taken over its cells, and every particle gets it as label.
+#include <math.h>
+
+#include "halma_common.h"
+#include "scan.h"
+#include "staging.h"
+
+namespace halma {
+
+namespace {
+
+constexpr uint64_t kEmpty = ~0ull;
+constexpr int kCellBits = 21;
+constexpr double kCellLimit = 2097152.0;  // 2^21 cells per axis
+constexpr int kBoxBlocks = 1024;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t k) {
+    k ^= k >> 30;
+    k *= 0xbf58476d1ce4e5b9ull;
+    k ^= k >> 27;
+    k *= 0x94d049bb133111ebull;
+    k ^= k >> 31;
+    return k;
+}
+
+template <typename T>
+__device__ __forceinline__ int ld_relaxed(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int uf_find(int* P, int x) {
+    int p = __hip_atomic_load(P + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (p != x) {
+        const int g = __hip_atomic_load(P + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (g == p) return p;
+        __hip_atomic_store(P + x, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = g;
+        p = __hip_atomic_load(P + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return x;
+}
+
+__device__ void uf_union(int* P, int a, int b) {
+    for (;;) {
+        a = uf_find(P, a);
+        b = uf_find(P, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        // hook root a (> b) under b
+        int expected = a;
+        if (__hip_atomic_compare_exchange_strong(P + a, &expected, b, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return;
+        a = expected;  // a was hooked meanwhile: continue from its new parent
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bbox_partial(const T* __restrict__ x,
+                                                      const T* __restrict__ y,
+                                                      const T* __restrict__ z, int64_t n,
+                                                      double* __restrict__ part) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double v[3] = {static_cast<double>(x[i]), static_cast<double>(y[i]),
+                             static_cast<double>(z[i])};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], v[d]);
+            hi[d] = fmax(hi[d], v[d]);
+        }
+    }
+    __shared__ double sh[4][6];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        for (int s = 32; s >= 1; s >>= 1) {
+            lo[d] = fmin(lo[d], __shfl_down(lo[d], s, kWave));
+            hi[d] = fmax(hi[d], __shfl_down(hi[d], s, kWave));
+        }
+    }
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0)
+        for (int d = 0; d < 3; ++d) {
+            sh[w][d] = lo[d];
+            sh[w][3 + d] = hi[d];
+        }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double v = sh[0][threadIdx.x];
+        for (int k = 1; k < 4; ++k)
+            v = threadIdx.x < 3 ? fmin(v, sh[k][threadIdx.x]) : fmax(v, sh[k][threadIdx.x]);
+        part[blockIdx.x * 6 + threadIdx.x] = v;
+    }
+}
+
+__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ box) {
+    if (threadIdx.x < 6) {
+        double v = part[threadIdx.x];
+        for (int b = 1; b < nb; ++b)
+            v = threadIdx.x < 3 ? fmin(v, part[b * 6 + threadIdx.x])
+                                : fmax(v, part[b * 6 + threadIdx.x]);
+        box[threadIdx.x] = v;
+    }
+}
+
+__global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_i32n(int* p, int64_t n, int v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fof_bin(const T* __restrict__ x, const T* __restrict__ y,
+                                                 const T* __restrict__ z, int64_t n,
+                                                 const double* __restrict__ box, double inv_s,
+                                                 uint64_t* __restrict__ tkey,
+                                                 int* __restrict__ tcnt, uint64_t mask,
+                                                 int* __restrict__ pslot, int* __restrict__ prank,
+                                                 int* __restrict__ err) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double q[3] = {(static_cast<double>(x[i]) - box[0]) * inv_s,
+                         (static_cast<double>(y[i]) - box[1]) * inv_s,
+                         (static_cast<double>(z[i]) - box[2]) * inv_s};
+    uint64_t key = 0;
+    bool bad = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        bad |= !(q[d] >= 0.0 && q[d] < kCellLimit);  // also catches NaN
+        const uint64_t c = bad ? 0 : static_cast<uint64_t>(floor(q[d]));
+        key = (key << kCellBits) | c;
+    }
+    if (bad) {
+        atomicOr(err, 1);
+        key = 0;
+    }
+    uint64_t h = mix64(key) & mask;
+    for (uint64_t probe = 0;; ++probe) {
+        const uint64_t k = __hip_atomic_load(tkey + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == key) break;
+        if (k == kEmpty) {
+            uint64_t expected = kEmpty;
+            if (__hip_atomic_compare_exchange_strong(tkey + h, &expected, key, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT) ||
+                expected == key)
+                break;
+        }
+        if (probe > mask) {  // table full (cannot happen with capacity >= 2n)
+            atomicOr(err, 2);
+            pslot[i] = 0;
+            prank[i] = 0;
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+    pslot[i] = static_cast<int>(h);
+    prank[i] = atomicAdd(tcnt + h, 1);
+}
+
+// packed (count << 32 | occupied) per table slot for one scan that yields both the
+// particle offset (high word) and the cell number (low word) of every occupied slot
+__global__ void k_slot_pack(const int* __restrict__ tcnt, int64_t cap, int64_t* __restrict__ v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cap) return;
+    const int c = tcnt[i];
+    v[i] = (static_cast<int64_t>(c) << 32) | (c > 0 ? 1 : 0);
+}
+
+__global__ void k_cells(const int* __restrict__ tcnt, const uint64_t* __restrict__ tkey,
+                        const int64_t* __restrict__ scan, int64_t cap,
+                        int* __restrict__ cell_start, int* __restrict__ cell_cnt,
+                        uint64_t* __restrict__ cell_key, int* __restrict__ parent,
+                        int* __restrict__ comp_min) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cap) return;
+    const int c = tcnt[i];
+    if (c <= 0) return;
+    const int64_t v = scan[i];
+    const int cell = static_cast<int>(v & 0xffffffff);
+    cell_start[cell] = static_cast<int>(v >> 32);
+    cell_cnt[cell] = c;
+    cell_key[cell] = tkey[i];
+    parent[cell] = cell;
+    comp_min[cell] = INT32_MAX;
+}
+
+template <typename T>
+__global__ void k_scatter(const T* __restrict__ x, const T* __restrict__ y,
+                          const T* __restrict__ z, int64_t n, const int* __restrict__ pslot,
+                          const int* __restrict__ prank, const int64_t* __restrict__ scan,
+                          int* __restrict__ sidx, T* __restrict__ sx, T* __restrict__ sy,
+                          T* __restrict__ sz, int* __restrict__ pcell) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t v = scan[pslot[i]];
+    const int64_t pos = (v >> 32) + prank[i];
+    sidx[pos] = static_cast<int>(i);
+    sx[pos] = x[i];
+    sy[pos] = y[i];
+    sz[pos] = z[i];
+    pcell[i] = static_cast<int>(v & 0xffffffff);
+}
+
+__device__ __forceinline__ int table_find(const uint64_t* __restrict__ tkey, uint64_t mask,
+                                          uint64_t key) {
+    uint64_t h = mix64(key) & mask;
+    for (uint64_t probe = 0; probe <= mask; ++probe) {
+        const uint64_t k = tkey[h];
+        if (k == key) return static_cast<int>(h);
+        if (k == kEmpty) return -1;
+        h = (h + 1) & mask;
+    }
+    return -1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fof_link(
+    const uint64_t* __restrict__ tkey, uint64_t mask, const int64_t* __restrict__ scan,
+    const int* __restrict__ cell_start, const int* __restrict__ cell_cnt,
+    const uint64_t* __restrict__ cell_key, const T* __restrict__ sx, const T* __restrict__ sy,
+    const T* __restrict__ sz, const int64_t* __restrict__ totals, double ll2,
+    int* __restrict__ parent) {
+    const int lane = __lane_id();
+    const int64_t n_cells = totals[0] & 0xffffffff;
+    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
+    const uint64_t cmask = (1ull << kCellBits) - 1;
+    for (int64_t c64 = wave0; c64 < n_cells; c64 += nwaves) {
+        const int c = static_cast<int>(c64);
+        const uint64_t key = cell_key[c];
+        const int64_t cx = static_cast<int64_t>((key >> (2 * kCellBits)) & cmask);
+        const int64_t cy = static_cast<int64_t>((key >> kCellBits) & cmask);
+        const int64_t cz = static_cast<int64_t>(key & cmask);
+        // lane k < 62 -> the k-th offset after the centre in (dz, dy, dx) order
+        int dcell = -1;
+        if (lane < 62) {
+            const int idx = 63 + lane;
+            const int64_t nz = cz + idx / 25 - 2;
+            const int64_t ny = cy + (idx / 5) % 5 - 2;
+            const int64_t nx = cx + idx % 5 - 2;
+            if (nx >= 0 && ny >= 0 && nz >= 0 && nx <= static_cast<int64_t>(cmask) &&
+                ny <= static_cast<int64_t>(cmask) && nz <= static_cast<int64_t>(cmask)) {
+                const uint64_t nkey = (static_cast<uint64_t>(nx) << (2 * kCellBits)) |
+                                      (static_cast<uint64_t>(ny) << kCellBits) |
+                                      static_cast<uint64_t>(nz);
+                const int slot = table_find(tkey, mask, nkey);
+                if (slot >= 0) dcell = static_cast<int>(scan[slot] & 0xffffffff);
+            }
+        }
+        uint64_t found = __ballot(dcell >= 0);
+        const int sc = cell_start[c];
+        const int nc = cell_cnt[c];
+        while (found) {
+            const int l = __builtin_ctzll(found);
+            found &= found - 1;
+            const int d = __shfl(dcell, l, kWave);
+            if (uf_find(parent, c) == uf_find(parent, d)) continue;
+            const int sd = cell_start[d];
+            const int nd = cell_cnt[d];
+            // lanes over the larger cell, loop over the smaller one
+            const bool c_small = nc <= nd;
+            const int sA = c_small ? sc : sd, nA = c_small ? nc : nd;
+            const int sB = c_small ? sd : sc, nB = c_small ? nd : nc;
+            bool linked = false;
+            for (int b0 = 0; b0 < nB && !linked; b0 += kWave) {
+                const int b = b0 + lane;
+                const bool valid = b < nB;
+                const int bb = valid ? sB + b : sB;
+                const double xb = static_cast<double>(sx[bb]);
+                const double yb = static_cast<double>(sy[bb]);
+                const double zb = static_cast<double>(sz[bb]);
+                for (int a = 0; a < nA; ++a) {
+                    const double dx = static_cast<double>(sx[sA + a]) - xb;
+                    const double dy = static_cast<double>(sy[sA + a]) - yb;
+                    const double dz = static_cast<double>(sz[sA + a]) - zb;
+                    const double d2 = (dx * dx + dy * dy) + dz * dz;
+                    if (__any(valid && d2 <= ll2)) {
+                        linked = true;
+                        break;
+                    }
+                }
+            }
+            if (linked && lane == 0) uf_union(parent, c, d);
+        }
+    }
+}
+
+// Roots after the link phase.  Parents are read with read-modify-write atomics (a
+// fetch-or of 0): on a multi-XCD device a relaxed load may return a value cached before
+// another XCD's hook, and a stale "I am a root" answer would split a component, while
+// an atomic read sees the value at the coherence point.  Results go to root[] (parent is
+// not modified here), and every root is re-checked the same way: a non-root target is
+// reported through err (bit 4) instead of producing a wrong partition.
+__device__ __forceinline__ int coherent_read(int* p) {
+    return __hip_atomic_fetch_or(p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_roots(const int64_t* __restrict__ totals, int* __restrict__ parent,
+                        int* __restrict__ root) {
+    const int64_t n_cells = totals[0] & 0xffffffff;
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n_cells;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int x = static_cast<int>(c);
+        for (int p = coherent_read(parent + x); p != x; p = coherent_read(parent + x)) x = p;
+        root[c] = x;
+    }
+}
+
+__global__ void k_check_roots(const int64_t* __restrict__ totals, int* __restrict__ parent,
+                              const int* __restrict__ root, int* __restrict__ err) {
+    const int64_t n_cells = totals[0] & 0xffffffff;
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n_cells;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int r = root[c];
+        if (coherent_read(parent + r) != r) atomicOr(err, 4);
+    }
+}
+
+// wave per cell: minimum particle index of the cell, folded into its root
+__global__ __launch_bounds__(256) void k_cell_min(const int64_t* __restrict__ totals,
+                                                  const int* __restrict__ cell_start,
+                                                  const int* __restrict__ cell_cnt,
+                                                  const int* __restrict__ sidx,
+                                                  const int* __restrict__ root,
+                                                  int* __restrict__ comp_min) {
+    const int64_t n_cells = totals[0] & 0xffffffff;
+    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
+    for (int64_t c = wave0; c < n_cells; c += nwaves) {
+        const int s = cell_start[c], n = cell_cnt[c];
+        int m = INT32_MAX;
+        for (int k = __lane_id(); k < n; k += kWave) m = min(m, sidx[s + k]);
+        for (int o = 32; o >= 1; o >>= 1) m = min(m, __shfl_down(m, o, kWave));
+        if (__lane_id() == 0) atomicMin(comp_min + root[c], m);
+    }
+}
+
+__global__ void k_labels(int64_t n, const int* __restrict__ pcell, const int* __restrict__ root,
+                         const int* __restrict__ comp_min, int* __restrict__ labels) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) labels[i] = comp_min[root[pcell[i]]];
+}
+
+struct FofWs {
+    double* box_part;
+    double* box;
+    uint64_t* tkey;
+    int* tcnt;
+    int64_t* scan;
+    int64_t* totals;
+    void* scan_ws;
+    int* pslot;
+    int* prank;
+    int* pcell;
+    int* cell_start;
+    int* cell_cnt;
+    uint64_t* cell_key;
+    int* parent;
+    int* root;
+    int* comp_min;
+    int* sidx;
+    void* sx;
+    void* sy;
+    void* sz;
+    int* err;
+    size_t bytes;
+    int64_t cap;
+};
+
+int64_t table_capacity(int64_t n) {
+    int64_t cap = 1024;
+    while (cap < 2 * n) cap <<= 1;
+    return cap;
+}
+
+FofWs carve_fof(void* base, int64_t n, size_t elt) {
+    Carver c(base);
+    FofWs w;
+    const int64_t nn = n > 0 ? n : 1;
+    w.cap = table_capacity(n);
+    w.box_part = c.take<double>(kBoxBlocks * 6);
+    w.box = c.take<double>(8);
+    w.tkey = c.take<uint64_t>(w.cap);
+    w.tcnt = c.take<int>(w.cap);
+    w.scan = c.take<int64_t>(w.cap);
+    w.totals = c.take<int64_t>(4);
+    w.scan_ws = c.take<char>(scan_workspace_bytes(w.cap));
+    w.pslot = c.take<int>(nn);
+    w.prank = c.take<int>(nn);
+    w.pcell = c.take<int>(nn);
+    w.cell_start = c.take<int>(nn);
+    w.cell_cnt = c.take<int>(nn);
+    w.cell_key = c.take<uint64_t>(nn);
+    w.parent = c.take<int>(nn);
+    w.root = c.take<int>(nn);
+    w.comp_min = c.take<int>(nn);
+    w.sidx = c.take<int>(nn);
+    w.sx = c.take<char>(nn * elt);
+    w.sy = c.take<char>(nn * elt);
+    w.sz = c.take<char>(nn * elt);
+    w.err = c.take<int>(64);
+    w.bytes = c.used;
+    return w;
+}
+
+inline unsigned blocks_for(int64_t n, int t = 256) {
+    return static_cast<unsigned>((n + t - 1) / t);
+}
+
+template <typename T>
+int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* labels,
+            const FofWs& w, hipStream_t st) {
+    const int grid_cells = device_cu_count() > 0 ? device_cu_count() * 8 : 2048;
+    k_fill_i32n<<<1, 64, 0, st>>>(w.err, 1, 0);
+    HALMA_LAUNCH_CHECK();
+    k_bbox_partial<T><<<kBoxBlocks, 256, 0, st>>>(x, y, z, n, w.box_part);
+    HALMA_LAUNCH_CHECK();
+    k_bbox_final<<<1, 64, 0, st>>>(w.box_part, kBoxBlocks, w.box);
+    HALMA_LAUNCH_CHECK();
+    k_fill_u64<<<blocks_for(w.cap), 256, 0, st>>>(w.tkey, w.cap, kEmpty);
+    HALMA_LAUNCH_CHECK();
+    k_fill_i32n<<<blocks_for(w.cap), 256, 0, st>>>(w.tcnt, w.cap, 0);
+    HALMA_LAUNCH_CHECK();
+    // sub-cell side: ll/sqrt(3) with a 1e-6 relative margin (see the file header)
+    const double side = ll / sqrt(3.0) * (1.0 - 1e-6);
+    const double inv_s = 1.0 / side;
+    k_fof_bin<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.box, inv_s, w.tkey, w.tcnt,
+                                                static_cast<uint64_t>(w.cap - 1), w.pslot,
+                                                w.prank, w.err);
+    HALMA_LAUNCH_CHECK();
+    k_slot_pack<<<blocks_for(w.cap), 256, 0, st>>>(w.tcnt, w.cap, w.scan);
+    HALMA_LAUNCH_CHECK();
+    int rc = exclusive_scan_i64(w.scan, w.scan, w.cap, w.totals, w.scan_ws, st);
+    if (rc) return rc;
+    k_cells<<<blocks_for(w.cap), 256, 0, st>>>(w.tcnt, w.tkey, w.scan, w.cap, w.cell_start,
+                                               w.cell_cnt, w.cell_key, w.parent, w.comp_min);
+    HALMA_LAUNCH_CHECK();
+    k_scatter<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.pslot, w.prank, w.scan, w.sidx,
+                                                static_cast<T*>(w.sx), static_cast<T*>(w.sy),
+                                                static_cast<T*>(w.sz), w.pcell);
+    HALMA_LAUNCH_CHECK();
+    k_fof_link<T><<<grid_cells, 256, 0, st>>>(
+        w.tkey, static_cast<uint64_t>(w.cap - 1), w.scan, w.cell_start, w.cell_cnt, w.cell_key,
+        static_cast<const T*>(w.sx), static_cast<const T*>(w.sy), static_cast<const T*>(w.sz),
+        w.totals, ll * ll, w.parent);
+    HALMA_LAUNCH_CHECK();
+    k_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.parent, w.root);
+    HALMA_LAUNCH_CHECK();
+    k_check_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.parent, w.root, w.err);
+    HALMA_LAUNCH_CHECK();
+    k_cell_min<<<grid_cells, 256, 0, st>>>(w.totals, w.cell_start, w.cell_cnt, w.sidx, w.root,
+                                           w.comp_min);
+    HALMA_LAUNCH_CHECK();
+    k_labels<<<blocks_for(n), 256, 0, st>>>(n, w.pcell, w.root, w.comp_min, labels);
+    HALMA_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace
+
+// Shared with the group builder: runs FoF into `labels` and leaves the cell structures in
+// the workspace.  Asynchronous; *err (device) is non-zero on a range or table failure.
+int fof_labels_async(const void* x, const void* y, const void* z, int dtype, int64_t n,
+                     double ll, int* labels, void* ws, size_t ws_bytes, int** err_out,
+                     int64_t** ncells_out, hipStream_t st) {
+    const size_t elt = dtype == HALMA_F64 ? sizeof(double) : sizeof(float);
+    FofWs w = carve_fof(ws, n, elt);
+    if (ws_bytes < w.bytes)
+        return set_error(HALMA_E_WORKSPACE, "workspace %zu < required %zu", ws_bytes, w.bytes);
+    if (err_out) *err_out = w.err;
+    if (ncells_out) *ncells_out = w.totals;
+    if (dtype == HALMA_F64)
+        return fof_run<double>(static_cast<const double*>(x), static_cast<const double*>(y),
+                               static_cast<const double*>(z), n, ll, labels, w, st);
+    return fof_run<float>(static_cast<const float*>(x), static_cast<const float*>(y),
+                          static_cast<const float*>(z), n, ll, labels, w, st);
+}
+
+size_t fof_workspace_bytes(int64_t n, int dtype) {
+    return carve_fof(nullptr, n, dtype == HALMA_F64 ? sizeof(double) : sizeof(float)).bytes;
+}
+
+int fof_check_error(const int* d_err, hipStream_t st) {
+    long long* h = pinned_words(2);
+    if (!h) return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
+    h[0] = 0;
+    HALMA_HIP(hipMemcpyAsync(h, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HALMA_HIP(hipStreamSynchronize(st));
+    const int e = static_cast<int>(h[0] & 0xffffffff);
+    if (e & 1)
+        return set_error(HALMA_E_LIMIT,
+                         "friends-of-friends: coordinates span more than 2^21 sub-cells of "
+                         "side ll/sqrt(3) on an axis, or are not finite");
+    if (e & 2) return set_error(HALMA_E_STATE, "friends-of-friends: cell table overflow");
+    if (e & 4) return set_error(HALMA_E_STATE, "friends-of-friends: inconsistent union-find roots");
+    return 0;
+}
+
+}  // namespace halma
+
+using namespace halma;
+
+extern "C" {
+
+size_t halma_fof_workspace_size(int64_t n, int dtype) {
+    if (n < 0 || (dtype != HALMA_F32 && dtype != HALMA_F64)) return 0;
+    return fof_workspace_bytes(n, dtype);
+}
+
+int halma_fof_labels(const void* x, const void* y, const void* z, int dtype, int64_t n,
+                     double ll, int32_t* labels, void* ws, size_t ws_bytes, void* stream) {
+    if (n < 0) return set_error(HALMA_E_ARG, "negative particle count");
+    if (n == 0) return 0;
+    if (n > INT32_MAX - 1) return set_error(HALMA_E_LIMIT, "at most 2^31-2 particles");
+    if (!x || !y || !z || !labels) return set_error(HALMA_E_ARG, "null pointer");
+    if (dtype != HALMA_F32 && dtype != HALMA_F64) return set_error(HALMA_E_ARG, "bad dtype");
+    if (!(ll > 0.0) || !isfinite(ll))
+        return set_error(HALMA_E_ARG, "linking length must be positive and finite");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int* err = nullptr;
+    int rc = fof_labels_async(x, y, z, dtype, n, ll, labels, ws, ws_bytes, &err, nullptr, st);
+    if (rc) return rc;
+    return fof_check_error(err, st);
+}
+
+}  // extern "C"

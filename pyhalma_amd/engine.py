@@ -90,6 +90,35 @@ def pot_sum_f64_plain(m, x, y, z, tx, ty, tz):
     return out
 
 
+def _coords(x, y, z):
+    """Device coordinate tensors sharing one float dtype (float32 kept, else float64)."""
+    from .device import float_tensor
+    x, y, z = float_tensor(x), float_tensor(y), float_tensor(z)
+    if not (x.dtype == y.dtype == z.dtype):
+        x, y, z = x.double(), y.double(), z.double()
+    if not (x.numel() == y.numel() == z.numel()):
+        raise ValueError("coordinate arrays must have equal lengths")
+    return x, y, z
+
+
+def fof_labels(x, y, z, linking_length):
+    """Friends-of-friends labels (int32 device tensor): the minimum particle index of
+    each particle's component (fof.py:18-79 semantics; singletons label themselves)."""
+    from .device import dtype_flag
+    L = _lib_()
+    x, y, z = _coords(x, y, z)
+    n = int(x.numel())
+    labels = torch.empty(n, dtype=torch.int32, device=x.device)
+    if n == 0:
+        return labels
+    flag = dtype_flag(x)
+    ws = workspace(L.halma_fof_workspace_size(n, flag))
+    rc = L.halma_fof_labels(ptr(x), ptr(y), ptr(z), flag, n, float(linking_length),
+                            ptr(labels), ptr(ws), ws.numel(), stream_ptr())
+    _lib.check(rc, "halma_fof_labels")
+    return labels
+
+
 def tune_set(key, value):
     _lib.check(_lib_().halma_tune_set(int(key), int(value)), "halma_tune_set")
 
