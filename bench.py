@@ -1,0 +1,219 @@
+"""Benchmark: FoF + per-group reductions + batched iterative unbinding on one GPU.
+
+Workload: 1000 Plummer clumps in a 40 Mpc box, clump sizes p(N) ~ N^-1.385 on
+[100, 20000] (~1.9M clump stars) plus 10% uniform field stars (~2.1M particles in all),
+FP32 positions, velocities and log-uniform masses 1e4-1e6 Msun, linking length 0.2x the
+mean interparticle spacing, groups of >= 50 members, unbinding in physical mode
+(E = K + U, factor 1) iterated to convergence.  Synthetic data generated from a seed;
+inputs are resident in HBM before the timed region.
+
+One step = friends-of-friends labels -> groups -> group reductions (mass, centre of mass,
+CM velocity, furthest particle) -> density peaks -> batched unbinding.  The plain FP64
+comparator (same method: one launch per halo, FP64 sums, no batching, no graphs) runs
+once after the timed steps; the CPU baseline runs the oracle's FoF port and the
+reference's own FP32 pair routine (compiled from its Fortran source, OpenMP on all host
+cores) on a bounded sub-box of the same workload.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "particles/s FoF+unbinding at 2M stars, batched FP32 vs plain FP64 on same GPU"
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (AMD published, packed FMA)
+FLOP_PER_PAIR = 11         # particle_subroutines.f90:502-505 as written
+MINP = 50
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=20261204)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparator")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-frac", type=float, default=0.25,
+                    help="fraction of the box (along x) used for the CPU baseline sample")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the halo-finding path would shard haloes across GPUs; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from pyhalma_amd import engine, synth
+
+    torch.cuda.init()
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    w = synth.clumps(seed=args.seed)
+    n = len(w["x"])
+    ll = w["ll"]
+    log(f"workload: {n} particles, {len(w['sizes'])} clumps, ll={ll:.5f} Mpc "
+        f"(generated in {time.time() - t0:.1f}s)")
+    D = {k: torch.from_numpy(w[k]).to(dev) for k in ("x", "y", "z", "vx", "vy", "vz", "mass")}
+    torch.cuda.synchronize()
+
+    def step(time_pot=False):
+        lab = engine.fof_labels(D["x"], D["y"], D["z"], ll)
+        groups = engine.groups_from_labels(lab, MINP)
+        red = engine.group_reduce(groups, D["x"], D["y"], D["z"], D["mass"], D["vx"], D["vy"],
+                                  D["vz"])
+        peak = engine.density_peak(groups, D["x"], D["y"], D["z"], D["mass"], ll, reduced=red)
+        ub = engine.unbind(groups, D["x"], D["y"], D["z"], D["vx"], D["vy"], D["vz"],
+                           D["mass"], factor=1.0, time_pot=time_pot)
+        return lab, groups, red, peak, ub
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    pot_ns = pot_launches = pairs = 0
+    for i in range(args.steps):
+        lab, groups, red, peak, ub = step(time_pot=True)
+        pot_ns += ub.stats["pot_ns"]
+        pot_launches += ub.stats["pot_launches"]
+        pairs += ub.stats["pairs"]
+    torch.cuda.synchronize()
+    t_end = time.perf_counter()
+    step_s = (t_end - t_start) / args.steps
+    log(f"timed {args.steps} steps: {step_s * 1e3:.2f} ms/step; groups={groups.n_groups} "
+        f"members={groups.n_members} passes={ub.stats['passes']} "
+        f"retested={ub.stats['retested']}")
+
+    # roofline of the dominant kernel (the FP32 pair kernel), from its own events
+    pot_s_per_launch = pot_ns * 1e-9 / max(pot_launches, 1)
+    pairs_per_launch = pairs / max(pot_launches, 1)
+    achieved = FLOP_PER_PAIR * pairs_per_launch / pot_s_per_launch / 1e12
+    roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3),
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "kernel": "k_pot_f32 (persistent FP32 pair kernel)",
+                "pairs_per_launch": int(pairs_per_launch),
+                "launch_ms": round(pot_s_per_launch * 1e3, 4)}
+
+    out = {
+        "metric": METRIC, "value": n / step_s, "unit": "particles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_s * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Plummer clumps + uniform field)",
+        "config": {"workload": "1000 Plummer clumps in a 40 Mpc box, p(N)~N^-1.385 on "
+                               "[100, 20000], +10% field, FP32, ll=0.2 mean spacing, "
+                               "groups>=50, physical-mode iterative unbinding",
+                   "particles": n, "groups": groups.n_groups,
+                   "members": groups.n_members, "passes": ub.stats["passes"]},
+        "roofline": roofline,
+    }
+
+    if not args.no_plain:
+        # the plain FP64 comparator on the same groups (FoF and CSR are shared: FoF has
+        # one implementation; the comparator replaces reductions + unbinding)
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        lab2 = engine.fof_labels(D["x"], D["y"], D["z"], ll)
+        g2 = engine.groups_from_labels(lab2, MINP)
+        red2 = engine.group_reduce(g2, D["x"], D["y"], D["z"], D["mass"], D["vx"], D["vy"],
+                                   D["vz"])
+        pb, piters = engine.unbind_plain_f64(g2, D["x"], D["y"], D["z"], D["vx"], D["vy"],
+                                             D["vz"], D["mass"], factor=1.0)
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - tp0
+        same_groups = torch.equal(g2.members[: g2.n_members], groups.members[: groups.n_members])
+        same_masks = torch.equal(pb, ub.bound)
+        same_iters = bool(np.array_equal(piters, ub.iters.cpu().numpy()))
+        del red2
+        out["plain_f64"] = {"ms": plain_s * 1e3, "value": n / plain_s,
+                            "speedup": plain_s / step_s, "groups_identical": bool(same_groups),
+                            "masks_identical": bool(same_masks),
+                            "passes_identical": same_iters}
+        log(f"plain FP64: {plain_s * 1e3:.1f} ms -> speedup {plain_s / step_s:.1f}x; "
+            f"masks identical: {same_masks}")
+
+    # per-halo potential error against the FP64 ordered sum (first pass, sample of haloes)
+    sizes = groups.sizes()
+    order = np.argsort(sizes)
+    sample = np.unique(np.concatenate([order[-3:], order[:: max(1, len(order) // 12)]]))
+    mem = groups.members[: groups.n_members]
+    errs = []
+    for h in sample:
+        a, b = int(groups.offsets_host[h]), int(groups.offsets_host[h + 1])
+        idx = mem[a:b].long()
+        args4 = [D[k][idx] for k in ("mass", "x", "y", "z")]
+        s32 = engine.pot_sum(*args4, args4[1], args4[2], args4[3], out_dtype=np.float64)
+        s64 = engine.pot_sum_f64_ordered(*args4, args4[1], args4[2], args4[3])
+        errs.append(float(((s32 - s64).abs() / s64).max()))
+    out["pot_rel_err_max"] = max(errs)
+    out["pot_error_bound"] = engine.pot_error_bound()
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_frac)
+    print(json.dumps(out))
+
+
+def cpu_baseline(w, frac):
+    """CPU reference path on a sub-box (x < -L/2 + frac*L) of the same workload: the
+    oracle's cell-list FoF port, groups >= 50, numpy reductions, and the reference's own
+    FP32 pair routine (particle_subroutines.f90:466-514 compiled from its Fortran source
+    into oracle/_ref, OpenMP on every host core; the oracle's C port of the same loop
+    when that build is absent) inside the same iterated energy test."""
+    import numpy as np
+
+    import oracle
+
+    ncores = os.cpu_count() or 1
+    L = w["box"]
+    sel = w["x"] < (-L / 2 + frac * L)
+    x, y, z = w["x"][sel], w["y"][sel], w["z"][sel]
+    vx, vy, vz, m = w["vx"][sel], w["vy"][sel], w["vz"][sel], w["mass"][sel]
+    nsub = int(sel.sum())
+    use_ref = oracle.have_ref()
+    pot = (lambda *a: oracle.ref_pot_sum_f32(*a, nthreads=ncores)) if use_ref else \
+        (lambda *a: oracle.pot_sum_f32(*a, nthreads=ncores))
+    t0 = time.perf_counter()
+    lab = oracle.fof_labels_c(x, y, z, w["ll"])
+    groups = [np.asarray(g) for g in oracle.labels_to_groups(lab, 50)]
+    cm = [oracle.center_of_mass(g, x, y, z, m) for g in groups]
+    G = oracle.G_CONST
+    for g, (cx, cy, cz, M) in zip(groups, cm):
+        b = np.ones(len(g), bool)
+        for _ in range(100):
+            s = g[b]
+            mm = m[s].astype(np.float64)
+            vc = [float((mm * v[s]).sum() / mm.sum()) for v in (vx, vy, vz)]
+            S = pot(m[s], x[s], y[s], z[s], x[s], y[s], z[s]).astype(np.float64)
+            E = 0.5 * ((vx[s] - vc[0]) ** 2 + (vy[s] - vc[1]) ** 2 + (vz[s] - vc[2]) ** 2) - G * S
+            if (E <= 0).all():
+                break
+            pos = np.nonzero(b)[0]
+            b[pos[E > 0]] = False
+    dt = time.perf_counter() - t0
+    return {"value": nsub / dt, "unit": "particles/s", "cores": ncores,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"sub-box x < -L/2 + {frac}L of the same workload: {nsub} particles, "
+                      f"{len(groups)} groups >= 50; FoF = C cell-list port, pair sums = "
+                      f"{'reference Fortran routine' if use_ref else 'C port of it'} "
+                      f"(FP32, OpenMP {ncores} threads), iterated energy test; {dt:.1f} s",
+            "seconds": dt}
+
+
+if __name__ == "__main__":
+    main()

@@ -112,6 +112,86 @@ int halma_fof_labels(const void *x, const void *y, const void *z, int dtype, int
                      double ll, int32_t *labels, void *workspace, size_t workspace_bytes,
                      void *stream);
 
+/* ------------------------------------------------------------------------------------
+ * Groups.  From FoF labels to the groups of at least min_size members (the size filter
+ * of good_groups, pyHALMA.py:148-149,592), in CSR form: groups ordered by label (their
+ * minimum member index, the order fof.py:34-77 emits them), members ascending (fof.py:76).
+ * members needs n entries, offsets n+1.  Synchronises the stream (returns the counts).
+ * ---------------------------------------------------------------------------------- */
+size_t halma_groups_workspace_size(int64_t n);
+int halma_groups_from_labels(const int32_t *labels, int64_t n, int32_t min_size,
+                             int32_t *members, int64_t *offsets, int64_t *n_groups,
+                             int64_t *n_members, void *workspace, size_t workspace_bytes,
+                             void *stream);
+
+/* Per-group reductions (halo_properties.py:16-24 total_mass, :26-43 center_of_mass,
+ * :45-60 CM_velocity, :98-111 furthest_particle), FP64 with double-double accumulation.
+ * out has HALMA_GROUP_FIELDS doubles per group:
+ *   [0] M  [1..3] centre of mass (0 if M <= 0)  [4..6] mass-weighted velocity
+ *   (0 if M <= 0 or no velocities)  [7] distance of the furthest member from the centre
+ *   (the given centres when non-NULL, 3 per group, else the centre of mass)
+ *   [8..10] min x,y,z  [11..13] max x,y,z  [14] sum |m| (upper bound)
+ *   [15] 1 if every partial sum of the group's masses is exact in FP64, else 0.
+ * Velocity pointers may be NULL.  Stream-ordered. */
+#define HALMA_GROUP_FIELDS 16
+int halma_group_reduce(int64_t n_groups, const int64_t *offsets, const int32_t *members,
+                       const void *x, const void *y, const void *z, int pos_dtype,
+                       const void *vx, const void *vy, const void *vz, int vel_dtype,
+                       const void *m, int mass_dtype, const double *centres, double *out,
+                       void *stream);
+
+/* density_peak (halo_properties.py:518-556) for every group: node coordinates (3 doubles
+ * per group) of the heaviest cell of the 50^3 nearest-node grid spanning
+ * [min - ll, max + ll] per axis.  `reduced` is halma_group_reduce's output for the same
+ * groups.  Synchronises the stream. */
+size_t halma_density_peak_workspace_size(int64_t n_groups);
+int halma_density_peak(int64_t n_groups, const int64_t *offsets, const int32_t *members,
+                       const void *x, const void *y, const void *z, int pos_dtype,
+                       const void *m, int mass_dtype, double ll, const double *reduced,
+                       double *out, void *workspace, size_t workspace_bytes, void *stream);
+
+/* ------------------------------------------------------------------------------------
+ * Batched iterative unbinding.  The energy test of halo_properties.py:342-359 (stars,
+ * factor = FACTOR_V^2) and halo_gas.py:456-476 (gas, factor = 2):
+ *   E_i = 1/2 |v_i - v_cm|^2 - factor * G * S_i,   G = 4.3e-9 (km/s)^2 Mpc / Msun,
+ * S_i = sum over the halo's bound members j of m_j / r_ij (FP32-cast positions and
+ * masses, particle_subroutines.f90:466-514), bound iff E_i <= 0; v_cm is the mass-weighted
+ * velocity of the bound members.  Every halo is iterated until a pass removes nobody (at
+ * most max_iter passes); max_iter = 1 is the reference's single test with v_cm over all
+ * members.  Haloes are the CSR ranges offsets[h]..offsets[h+1] of `members` (particle
+ * indices into the global arrays); offsets is a HOST array of n_halo+1 entries.
+ * Decisions are exact with respect to the FP64 oracle: members whose FP32-compensated E
+ * lies inside the rigorous error band are re-tested with an FP64 sum in the oracle's
+ * fixed order.  Outputs (device): bound[k] per member position (1 bound, 0 not); iters
+ * (n_halo int32, may be NULL) passes per halo; vcm (3 per halo, may be NULL); pot (per
+ * member, the last S used, may be NULL).  stats (HOST, 8 int64, may be NULL): [0] passes,
+ * [1] members re-tested in FP64, [2] queue items per pass, [3] tests per lane,
+ * [4] sub-tile, [5] pair-kernel time in ns and [6] timed launches (with
+ * HALMA_UNBIND_TIME_POT: hipEvents recorded around the pair kernel on its stream),
+ * [7] pairs evaluated.  flags: HALMA_UNBIND_NO_GRAPH replays passes as plain launches
+ * instead of a captured hipGraph.  Synchronises the stream once per pass.
+ * ---------------------------------------------------------------------------------- */
+#define HALMA_UNBIND_NO_GRAPH 1
+#define HALMA_UNBIND_TIME_POT 2
+size_t halma_unbind_workspace_size(int64_t n_halo, const int64_t *offsets);
+int halma_unbind_batched(int64_t n_halo, const int64_t *offsets, const int32_t *members,
+                         const void *x, const void *y, const void *z, int pos_dtype,
+                         const void *vx, const void *vy, const void *vz, int vel_dtype,
+                         const void *m, int mass_dtype, double factor, int max_iter, int flags,
+                         uint8_t *bound, int32_t *iters, double *vcm, double *pot,
+                         int64_t *stats, void *workspace, size_t workspace_bytes, void *stream);
+
+/* The plain FP64 comparator of the same method: one halo at a time, one launch per step,
+ * FP64 storage, sequential per-thread sums with FP64 sqrt and divide, no batching and no
+ * graphs.  iters_host: HOST int32 per halo (may be NULL).  Synchronises often. */
+size_t halma_unbind_plain_workspace_size(int64_t n_halo, const int64_t *offsets);
+int halma_unbind_plain_f64(int64_t n_halo, const int64_t *offsets, const int32_t *members,
+                           const void *x, const void *y, const void *z, int pos_dtype,
+                           const void *vx, const void *vy, const void *vz, int vel_dtype,
+                           const void *m, int mass_dtype, double factor, int max_iter,
+                           uint8_t *bound, int32_t *iters_host, void *workspace,
+                           size_t workspace_bytes, void *stream);
+
 /* Tuning knobs of the FP32 pair kernels (set by the autotuner; read back with get).
  *   HALMA_TUNE_TESTS_PER_LANE : test particles per lane (1, 2, 4)
  *   HALMA_TUNE_SUBTILE        : sources per plain FP32 sub-tile before the Kahan add

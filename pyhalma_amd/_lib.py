@@ -39,6 +39,22 @@ SIGNATURES = {
     "halma_pot_sum_f64_plain": (_c.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p]),
     "halma_fof_workspace_size": (_size, [_i64, _c.c_int]),
     "halma_fof_labels": (_c.c_int, [_p, _p, _p, _c.c_int, _i64, _dbl, _p, _p, _size, _p]),
+    "halma_groups_workspace_size": (_size, [_i64]),
+    "halma_groups_from_labels": (_c.c_int, [_p, _i64, _i32, _p, _p, _i64p, _i64p, _p, _size,
+                                            _p]),
+    "halma_group_reduce": (_c.c_int, [_i64, _p, _p, _p, _p, _p, _c.c_int, _p, _p, _p, _c.c_int,
+                                      _p, _c.c_int, _p, _p, _p]),
+    "halma_density_peak_workspace_size": (_size, [_i64]),
+    "halma_density_peak": (_c.c_int, [_i64, _p, _p, _p, _p, _p, _c.c_int, _p, _c.c_int, _dbl,
+                                      _p, _p, _p, _size, _p]),
+    "halma_unbind_workspace_size": (_size, [_i64, _i64p]),
+    "halma_unbind_batched": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _c.c_int, _p, _p, _p,
+                                        _c.c_int, _p, _c.c_int, _dbl, _c.c_int, _c.c_int, _p,
+                                        _p, _p, _p, _i64p, _p, _size, _p]),
+    "halma_unbind_plain_workspace_size": (_size, [_i64, _i64p]),
+    "halma_unbind_plain_f64": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _c.c_int, _p, _p, _p,
+                                          _c.c_int, _p, _c.c_int, _dbl, _c.c_int, _p,
+                                          _c.POINTER(_c.c_int32), _p, _size, _p]),
     "halma_tune_set": (_c.c_int, [_c.c_int, _c.c_int]),
     "halma_tune_get": (_c.c_int, [_c.c_int]),
 }
@@ -51,6 +67,9 @@ TUNE_WAVES_PER_CU = 5
 
 F32 = 0
 F64 = 1
+GROUP_FIELDS = 16
+UNBIND_NO_GRAPH = 1
+UNBIND_TIME_POT = 2
 
 _LIB = None
 

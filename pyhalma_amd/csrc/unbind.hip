@@ -1,0 +1,795 @@
+// Batched iterative unbinding.
+//
+// The energy test of halo_properties.py:342-359 / halo_gas.py:456-476,
+//   E_i = 1/2 |v_i - v_cm|^2 - factor * G * S_i,  bound iff E_i <= 0,
+// with S_i the pair sum of particle_subroutines.f90:466-514 over the halo's bound
+// members (positions and masses cast to FP32 as halo_gas.py:172-178 does), iterated per
+// halo until a pass removes nobody: each pass recomputes v_cm (mass-weighted over the
+// bound set, halo_properties.py:45-60) and S over the bound set and removes members with
+// E > 0.  (The reference evaluates the test once; iterating it is this path's design.)
+//
+// One pass on the GPU, for all haloes at once:
+//   1. k_compact     one workgroup per active halo: stable compaction of its bound
+//                    members into the front of its range (AoS float4) and the
+//                    double-double v_cm over them;
+//   2. k_pot_f32     the persistent FP32 pair kernel over a static queue planned for the
+//                    initial sizes (largest first); haloes' live lengths come from a
+//                    device array, converged haloes have length 0 and cost one fetch;
+//   3. k_energy      merges the chunk partials (FP64), evaluates E in FP64 and decides
+//                    every member whose |E| exceeds the rigorous band
+//                    factor*G*(B*S) + 8 ulp(kin + factor*G*S), B = FP32 error bound;
+//   4. k_retest      one wavefront per member inside the band: S in FP64 with the
+//                    oracle's fixed summation order, so E and the decision are the
+//                    oracle's bit for bit;
+//   5. k_finish      per-halo pass counts, convergence flags, the removed total.
+// Passes 1-5 are captured once into a hipGraph and replayed; the host reads one counter
+// per pass to stop.
+#include <math.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "halma_common.h"
+#include "potential.h"
+#include "staging.h"
+#include "tuning.h"
+
+namespace halma {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned blocks_for(int64_t n, int t = kThreads) {
+    return static_cast<unsigned>((n + t - 1) / t);
+}
+
+__device__ __forceinline__ double ldf(const void* p, int dtype, int64_t i) {
+    return dtype == HALMA_F64 ? static_cast<const double*>(p)[i]
+                              : static_cast<double>(static_cast<const float*>(p)[i]);
+}
+
+struct Inputs {
+    const void *x, *y, *z;
+    int pos_t;
+    const void *vx, *vy, *vz;
+    int vel_t;
+    const void* m;
+    int mass_t;
+};
+
+// member k -> FP32 position/mass (the potential's inputs), FP64 velocity and mass
+__global__ void k_gather(Inputs in, const int32_t* __restrict__ mem, int64_t nm,
+                         float4* __restrict__ p4, double* __restrict__ vel,
+                         double* __restrict__ m64, uint8_t* __restrict__ bound) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    const int64_t i = mem[k];
+    p4[k] = make_float4(static_cast<float>(ldf(in.x, in.pos_t, i)),
+                        static_cast<float>(ldf(in.y, in.pos_t, i)),
+                        static_cast<float>(ldf(in.z, in.pos_t, i)),
+                        static_cast<float>(ldf(in.m, in.mass_t, i)));
+    vel[3 * k + 0] = ldf(in.vx, in.vel_t, i);
+    vel[3 * k + 1] = ldf(in.vy, in.vel_t, i);
+    vel[3 * k + 2] = ldf(in.vz, in.vel_t, i);
+    m64[k] = ldf(in.m, in.mass_t, i);
+    bound[k] = 1;
+}
+
+__global__ void k_member_halo(const int64_t* __restrict__ off, int H, int32_t* __restrict__ mh) {
+    const int h = blockIdx.x;
+    if (h >= H) return;
+    for (int64_t k = off[h] + threadIdx.x; k < off[h + 1]; k += blockDim.x) mh[k] = h;
+}
+
+__global__ void k_init_halo(const int64_t* __restrict__ off, int H, int32_t* __restrict__ active,
+                            int32_t* __restrict__ iters, int32_t* __restrict__ removed) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    active[h] = off[h + 1] > off[h] ? 1 : 0;
+    iters[h] = 0;
+    removed[h] = 0;
+}
+
+__global__ void k_zero_counters(unsigned long long* __restrict__ c, int n) {
+    if (static_cast<int>(threadIdx.x) < n) c[threadIdx.x] = 0;
+}
+
+// block-wide exclusive scan of one int per thread; returns the block total
+__device__ int block_excl_scan(int v, int& excl, int* sh) {
+    const int lane = __lane_id();
+    const int w = threadIdx.x / kWave;
+    int x = v;
+    for (int s = 1; s < kWave; s <<= 1) {
+        const int y = __shfl_up(x, s, kWave);
+        if (lane >= s) x += y;
+    }
+    if (lane == kWave - 1) sh[w] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int k = 0; k < kThreads / kWave; ++k) {
+        if (k < w) base += sh[k];
+        tot += sh[k];
+    }
+    excl = base + x - v;
+    __syncthreads();
+    return tot;
+}
+
+__global__ __launch_bounds__(kThreads) void k_compact(
+    const int64_t* __restrict__ off, int H, const int32_t* __restrict__ active,
+    const uint8_t* __restrict__ bound, const float4* __restrict__ p4,
+    const double* __restrict__ vel, const double* __restrict__ m64, float4* __restrict__ src4,
+    int32_t* __restrict__ cidx, int32_t* __restrict__ dyn_len, double* __restrict__ vcm) {
+    __shared__ int shs[kThreads / kWave];
+    __shared__ double shd[2 * (kThreads / kWave)];
+    for (int h = blockIdx.x; h < H; h += gridDim.x) {
+        if (!active[h]) {
+            if (threadIdx.x == 0) dyn_len[h] = 0;
+            continue;
+        }
+        const int64_t b = off[h], e = off[h + 1];
+        int run = 0;
+        double acc[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        for (int64_t k0 = b; k0 < e; k0 += kThreads) {
+            const int64_t k = k0 + threadIdx.x;
+            const int keep = (k < e && bound[k]) ? 1 : 0;
+            int excl;
+            const int tot = block_excl_scan(keep, excl, shs);
+            if (keep) {
+                const int64_t pos = b + run + excl;
+                src4[pos] = p4[k];
+                cidx[pos] = static_cast<int32_t>(k);
+                const double m = m64[k];
+                dd_add(acc[0][0], acc[0][1], m);
+                dd_add(acc[1][0], acc[1][1], m * vel[3 * k + 0]);
+                dd_add(acc[2][0], acc[2][1], m * vel[3 * k + 1]);
+                dd_add(acc[3][0], acc[3][1], m * vel[3 * k + 2]);
+            }
+            run += tot;
+        }
+        // block double-double reduction of the four sums (thread 0 holds the result)
+        for (int q = 0; q < 4; ++q) {
+            double hi = acc[q][0], lo = acc[q][1];
+            for (int s = 32; s >= 1; s >>= 1) {
+                const double h2 = __shfl_down(hi, s, kWave);
+                const double l2 = __shfl_down(lo, s, kWave);
+                dd_add_dd(hi, lo, h2, l2);
+            }
+            const int w = threadIdx.x / kWave;
+            if (__lane_id() == 0) {
+                shd[2 * w] = hi;
+                shd[2 * w + 1] = lo;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                hi = shd[0];
+                lo = shd[1];
+                for (int k = 1; k < kThreads / kWave; ++k) dd_add_dd(hi, lo, shd[2 * k], shd[2 * k + 1]);
+                acc[q][0] = hi;
+                acc[q][1] = lo;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const double M = acc[0][0] + acc[0][1];
+            for (int d = 0; d < 3; ++d)
+                vcm[3 * h + d] = M > 0.0 ? (acc[1 + d][0] + acc[1 + d][1]) / M : 0.0;
+            dyn_len[h] = run;
+        }
+        __syncthreads();
+    }
+}
+
+struct EnergyArgs {
+    const HaloDesc* halos;
+    const int32_t* mhalo;
+    const int32_t* dyn_len;
+    const int32_t* cidx;
+    const float2* part;
+    const double* vel;
+    const double* vcm;
+    double cg;      // factor * G
+    double bound_rel;  // FP32 relative error bound B
+    uint8_t* bound;
+    int32_t* removed;
+    int32_t* band;
+    unsigned long long* counters;  // [0] removed total, [1] band count, [2] band total
+    double* s_out;        // optional: S per member position (FP64)
+    int64_t nm;
+};
+
+__device__ __forceinline__ double kinetic(const double* vel, int64_t k, const double* vc) {
+    const double dvx = vel[3 * k + 0] - vc[0];
+    const double dvy = vel[3 * k + 1] - vc[1];
+    const double dvz = vel[3 * k + 2] - vc[2];
+    return 0.5 * ((dvx * dvx + dvy * dvy) + dvz * dvz);
+}
+
+__global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= a.nm) return;
+    const int h = a.mhalo[g];
+    const HaloDesc d = a.halos[h];
+    const int64_t p = g - d.src_off;
+    const int n = a.dyn_len[h];
+    if (p >= n) return;
+    const int nch = (n + d.chunk - 1) / d.chunk;
+    double S = 0.0;
+    for (int c = 0; c < nch; ++c) {
+        const float2 q = a.part[d.part_off + static_cast<int64_t>(c) * d.stride + p];
+        S += static_cast<double>(q.x) + static_cast<double>(q.y);
+    }
+    const int64_t k = a.cidx[g];
+    const double kin = kinetic(a.vel, k, a.vcm + 3 * h);
+    const double pot = a.cg * S;
+    const double E = kin - pot;
+    const double band = pot * a.bound_rel * 1.01 + 8.0 * 0x1p-53 * (kin + pot);
+    if (a.s_out) a.s_out[k] = S;
+    if (fabs(E) <= band) {
+        const unsigned long long at = atomicAdd(a.counters + 1, 1ull);
+        a.band[at] = static_cast<int32_t>(g);
+        atomicAdd(a.counters + 2, 1ull);
+    } else if (E > 0.0) {
+        a.bound[k] = 0;
+        atomicAdd(a.removed + h, 1);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_retest(EnergyArgs a, const float4* __restrict__ src4) {
+    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
+    const int64_t nb = static_cast<int64_t>(a.counters[1]);
+    for (int64_t w = wave0; w < nb; w += nwaves) {
+        const int64_t g = a.band[w];
+        const int h = a.mhalo[g];
+        const HaloDesc d = a.halos[h];
+        const int n = a.dyn_len[h];
+        const float4 t = src4[g];
+        SrcAoS src{src4 + d.src_off};
+        const double S = wave_pot_f64_ordered(src, n, t.x, t.y, t.z);
+        if (__lane_id() == 0) {
+            const int64_t k = a.cidx[g];
+            const double E = kinetic(a.vel, k, a.vcm + 3 * h) - a.cg * S;
+            if (a.s_out) a.s_out[k] = S;
+            if (E > 0.0) {
+                a.bound[k] = 0;
+                atomicAdd(a.removed + h, 1);
+            }
+        }
+    }
+}
+
+__global__ void k_finish(int H, const int32_t* __restrict__ dyn_len, int32_t* __restrict__ active,
+                         int32_t* __restrict__ iters, int32_t* __restrict__ removed,
+                         unsigned long long* __restrict__ counters) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h == 0) counters[1] = 0;  // the band list is rebuilt by the next pass
+    if (h >= H) return;
+    if (!active[h]) return;
+    if (dyn_len[h] > 0) {
+        iters[h] += 1;
+        const unsigned long long n = static_cast<unsigned long long>(dyn_len[h]);
+        atomicAdd(counters + 3, n * n);  // pairs evaluated by the pair kernel this pass
+    }
+    const int r = removed[h];
+    if (r == 0 || dyn_len[h] == 0) active[h] = 0;
+    if (r) atomicAdd(counters + 0, static_cast<unsigned long long>(r));
+    removed[h] = 0;
+}
+
+// lengths used by the pair kernel: 0 for haloes that are no longer active
+__global__ void k_work_len(int H, const int32_t* __restrict__ active,
+                           const int32_t* __restrict__ dyn_len, int32_t* __restrict__ wlen) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < H) wlen[h] = active[h] ? dyn_len[h] : 0;
+}
+
+struct UnbWs {
+    float4* p4;
+    double* vel;
+    double* m64;
+    float4* src4;
+    int32_t* cidx;
+    int32_t* mhalo;
+    int32_t* dyn_len;
+    int32_t* wlen;
+    int32_t* active;
+    int32_t* iters;
+    int32_t* removed;
+    double* vcm;
+    int32_t* band;
+    unsigned long long* counters;
+    HaloDesc* halos;
+    int4* items;
+    float2* part;
+    int* head;
+    int64_t* off;
+    size_t bytes;
+};
+
+UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan) {
+    Carver c(base);
+    const int64_t n1 = nm > 0 ? nm : 1;
+    const int64_t h1 = H > 0 ? H : 1;
+    UnbWs w;
+    w.p4 = c.take<float4>(n1);
+    w.vel = c.take<double>(3 * n1);
+    w.m64 = c.take<double>(n1);
+    w.src4 = c.take<float4>(n1);
+    w.cidx = c.take<int32_t>(n1);
+    w.mhalo = c.take<int32_t>(n1);
+    w.dyn_len = c.take<int32_t>(h1);
+    w.wlen = c.take<int32_t>(h1);
+    w.active = c.take<int32_t>(h1);
+    w.iters = c.take<int32_t>(h1);
+    w.removed = c.take<int32_t>(h1);
+    w.vcm = c.take<double>(3 * h1);
+    w.band = c.take<int32_t>(n1);
+    w.counters = c.take<unsigned long long>(8);
+    w.halos = c.take<HaloDesc>(std::max<size_t>(plan.halos.size(), 1));
+    w.items = c.take<int4>(std::max<size_t>(plan.items.size(), 1));
+    w.part = c.take<float2>(std::max<int64_t>(plan.n_part, 1));
+    w.head = c.take<int>(64);
+    w.off = c.take<int64_t>(h1 + 1);
+    w.bytes = c.used;
+    return w;
+}
+
+// library-owned capture stream per device (default priority)
+hipStream_t capture_stream() {
+    static std::mutex mu;
+    static std::vector<std::pair<int, hipStream_t>> streams;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& p : streams)
+        if (p.first == dev) return p.second;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    streams.emplace_back(dev, s);
+    return s;
+}
+
+struct PassLaunch {
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // optional timing of the pair kernel
+    int64_t H, nm;
+    int grid_halo, grid_retest;
+    const PotPlan* plan;
+    UnbWs w;
+    uint8_t* bound;
+    EnergyArgs ea;
+};
+
+int launch_pass(const PassLaunch& L, hipStream_t st) {
+    const UnbWs& w = L.w;
+    k_compact<<<L.grid_halo, kThreads, 0, st>>>(w.off, static_cast<int>(L.H), w.active, L.bound,
+                                                w.p4, w.vel, w.m64, w.src4, w.cidx, w.dyn_len,
+                                                w.vcm);
+    HALMA_LAUNCH_CHECK();
+    k_work_len<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.active, w.dyn_len,
+                                                     w.wlen);
+    HALMA_LAUNCH_CHECK();
+    if (L.ev0) HALMA_HIP(hipEventRecord(L.ev0, st));
+    int rc = launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, w.wlen, w.items, w.part, w.head, st);
+    if (rc) return rc;
+    if (L.ev1) HALMA_HIP(hipEventRecord(L.ev1, st));
+    if (L.nm > 0) {
+        k_energy<<<blocks_for(L.nm), kThreads, 0, st>>>(L.ea);
+        HALMA_LAUNCH_CHECK();
+        k_retest<<<L.grid_retest, kThreads, 0, st>>>(L.ea, w.src4);
+        HALMA_LAUNCH_CHECK();
+    }
+    k_finish<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.dyn_len, w.active,
+                                                   w.iters, w.removed, w.counters);
+    HALMA_LAUNCH_CHECK();
+    return 0;
+}
+
+int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs in,
+               double factor, int max_iter, int flags, uint8_t* bound_out, int32_t* iters_out,
+               double* vcm_out, double* s_out, int64_t* stats, void* ws, size_t ws_bytes,
+               hipStream_t st) {
+    const int64_t nm = h_off[H] - h_off[0];
+    // every halo's tests are its own (live) members
+    std::vector<int64_t> rel(H + 1);
+    for (int64_t h = 0; h <= H; ++h) rel[h] = h_off[h] - h_off[0];
+    PotPlan plan;
+    int rc = plan_pot(H, rel.data(), rel.data(), plan);
+    if (rc) return rc;
+    UnbWs w = carve_unb(ws, H, nm, plan);
+    if (ws_bytes < w.bytes)
+        return set_error(HALMA_E_WORKSPACE, "workspace %zu < required %zu", ws_bytes, w.bytes);
+    rc = upload_async(w.off, rel.data(), (H + 1) * sizeof(int64_t), st);
+    if (rc) return rc;
+    rc = upload_async(w.halos, plan.halos.data(), plan.halos.size() * sizeof(HaloDesc), st);
+    if (rc) return rc;
+    rc = upload_async(w.items, plan.items.data(), plan.items.size() * sizeof(int4), st);
+    if (rc) return rc;
+    const int32_t* mem = members + h_off[0];
+    if (nm > 0) {
+        k_gather<<<blocks_for(nm), kThreads, 0, st>>>(in, mem, nm, w.p4, w.vel, w.m64, bound_out);
+        HALMA_LAUNCH_CHECK();
+        k_member_halo<<<static_cast<unsigned>(H), kThreads, 0, st>>>(w.off, static_cast<int>(H),
+                                                                     w.mhalo);
+        HALMA_LAUNCH_CHECK();
+    }
+    k_init_halo<<<blocks_for(H), kThreads, 0, st>>>(w.off, static_cast<int>(H), w.active, w.iters,
+                                                    w.removed);
+    HALMA_LAUNCH_CHECK();
+    k_zero_counters<<<1, 64, 0, st>>>(w.counters, 8);
+    HALMA_LAUNCH_CHECK();
+
+    double G = 4.3 * 1e-3;
+    G *= 1e-6;
+    PassLaunch L;
+    L.H = H;
+    L.nm = nm;
+    const int cus = std::max(device_cu_count(), 1);
+    L.grid_halo = static_cast<int>(std::min<int64_t>(H, 65535));
+    L.grid_retest = cus * 4;
+    L.plan = &plan;
+    L.w = w;
+    L.bound = bound_out;
+    L.ea.halos = w.halos;
+    L.ea.mhalo = w.mhalo;
+    L.ea.dyn_len = w.dyn_len;
+    L.ea.cidx = w.cidx;
+    L.ea.part = w.part;
+    L.ea.vel = w.vel;
+    L.ea.vcm = w.vcm;
+    L.ea.cg = factor * G;
+    L.ea.bound_rel = pot_error_bound(plan.subtile);
+    L.ea.bound = bound_out;
+    L.ea.removed = w.removed;
+    L.ea.band = w.band;
+    L.ea.counters = w.counters;
+    L.ea.s_out = s_out;
+    L.ea.nm = nm;
+
+    const bool use_graph = (flags & HALMA_UNBIND_NO_GRAPH) == 0;
+    const bool timing = (flags & HALMA_UNBIND_TIME_POT) != 0;
+    if (timing) {
+        HALMA_HIP(hipEventCreate(&L.ev0));
+        HALMA_HIP(hipEventCreate(&L.ev1));
+    }
+    double pot_ms = 0.0;
+    long long pot_launches = 0;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (use_graph) {
+        hipStream_t cs = capture_stream();
+        if (!cs) return set_error(HALMA_E_STATE, "cannot create the capture stream");
+        HALMA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+        rc = launch_pass(L, cs);
+        hipGraph_t g2 = nullptr;
+        const hipError_t ec = hipStreamEndCapture(cs, &g2);
+        if (rc) {
+            if (g2) (void)hipGraphDestroy(g2);
+            return rc;
+        }
+        HALMA_HIP(ec);
+        graph = g2;
+        HALMA_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    long long* host = pinned_words(8);
+    if (!host) return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
+    int passes = 0;
+    long long band_total = 0;
+    int status = 0;
+    for (; passes < max_iter; ++passes) {
+        if (use_graph) {
+            const hipError_t e = hipGraphLaunch(exec, st);
+            if (e != hipSuccess) {
+                status = hip_fail(e, "hipGraphLaunch", __FILE__, __LINE__);
+                break;
+            }
+        } else {
+            status = launch_pass(L, st);
+            if (status) break;
+        }
+        const hipError_t e1 = hipMemcpyAsync(host, w.counters, 4 * sizeof(long long),
+                                             hipMemcpyDeviceToHost, st);
+        const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(st) : e1;
+        if (e2 != hipSuccess) {
+            status = hip_fail(e2, "counter read-back", __FILE__, __LINE__);
+            break;
+        }
+        band_total = host[2];
+        if (timing) {
+            float ms = 0.0f;
+            const hipError_t e4 = hipEventElapsedTime(&ms, L.ev0, L.ev1);
+            if (e4 != hipSuccess) {
+                status = hip_fail(e4, "pair-kernel timing", __FILE__, __LINE__);
+                break;
+            }
+            pot_ms += ms;
+            ++pot_launches;
+        }
+        if (host[0] == 0) {
+            ++passes;
+            break;
+        }
+        // reset the removed total for the next pass (stream-ordered, outside the graph)
+        k_zero_counters<<<1, 64, 0, st>>>(w.counters, 1);
+        const hipError_t e3 = hipGetLastError();
+        if (e3 != hipSuccess) {
+            status = hip_fail(e3, "counter reset", __FILE__, __LINE__);
+            break;
+        }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (L.ev0) (void)hipEventDestroy(L.ev0);
+    if (L.ev1) (void)hipEventDestroy(L.ev1);
+    if (status) return status;
+    if (iters_out) HALMA_HIP(hipMemcpyAsync(iters_out, w.iters, H * sizeof(int32_t),
+                                            hipMemcpyDeviceToDevice, st));
+    if (vcm_out) HALMA_HIP(hipMemcpyAsync(vcm_out, w.vcm, 3 * H * sizeof(double),
+                                          hipMemcpyDeviceToDevice, st));
+    if (stats) {
+        stats[0] = passes;
+        stats[1] = band_total;
+        stats[2] = plan.items.size();
+        stats[3] = plan.tests_per_lane;
+        stats[4] = plan.subtile;
+        stats[5] = static_cast<int64_t>(pot_ms * 1e6);  // ns
+        stats[6] = pot_launches;
+        stats[7] = host[3];
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------ plain FP64 path ----
+// One halo at a time, one launch per step, FP64 storage and arithmetic, sequential
+// per-thread sums: the comparator the batched path is measured against.
+__global__ void k_plain_gather(Inputs in, const int32_t* __restrict__ mem, int64_t n,
+                               double* __restrict__ px, double* __restrict__ py,
+                               double* __restrict__ pz, double* __restrict__ pm,
+                               double* __restrict__ vel, double* __restrict__ m64,
+                               uint8_t* __restrict__ bound) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = mem[k];
+    // positions and masses as the FP32 values the pair sum uses, held in FP64
+    px[k] = static_cast<float>(ldf(in.x, in.pos_t, i));
+    py[k] = static_cast<float>(ldf(in.y, in.pos_t, i));
+    pz[k] = static_cast<float>(ldf(in.z, in.pos_t, i));
+    pm[k] = static_cast<float>(ldf(in.m, in.mass_t, i));
+    vel[3 * k + 0] = ldf(in.vx, in.vel_t, i);
+    vel[3 * k + 1] = ldf(in.vy, in.vel_t, i);
+    vel[3 * k + 2] = ldf(in.vz, in.vel_t, i);
+    m64[k] = ldf(in.m, in.mass_t, i);
+    bound[k] = 1;
+}
+
+// single block: compaction of the bound set of one halo and its v_cm (plain FP64 sums)
+__global__ __launch_bounds__(kThreads) void k_plain_compact(
+    int64_t n, const uint8_t* __restrict__ bound, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ pm,
+    const double* __restrict__ vel, const double* __restrict__ m64, double* __restrict__ cx,
+    double* __restrict__ cy, double* __restrict__ cz, double* __restrict__ cm,
+    int32_t* __restrict__ cidx, int32_t* __restrict__ cnt, double* __restrict__ vcm) {
+    __shared__ int shs[kThreads / kWave];
+    __shared__ double red[4][kThreads];
+    int run = 0;
+    double s[4] = {0, 0, 0, 0};
+    for (int64_t k0 = 0; k0 < n; k0 += kThreads) {
+        const int64_t k = k0 + threadIdx.x;
+        const int keep = (k < n && bound[k]) ? 1 : 0;
+        int excl;
+        const int tot = block_excl_scan(keep, excl, shs);
+        if (keep) {
+            const int64_t pos = run + excl;
+            cx[pos] = px[k];
+            cy[pos] = py[k];
+            cz[pos] = pz[k];
+            cm[pos] = pm[k];
+            cidx[pos] = static_cast<int32_t>(k);
+            s[0] += m64[k];
+            for (int d = 0; d < 3; ++d) s[1 + d] += m64[k] * vel[3 * k + d];
+        }
+        run += tot;
+    }
+    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = s[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[4] = {0, 0, 0, 0};
+        for (int k = 0; k < kThreads; ++k)
+            for (int q = 0; q < 4; ++q) t[q] += red[q][k];
+        for (int d = 0; d < 3; ++d) vcm[d] = t[0] > 0.0 ? t[1 + d] / t[0] : 0.0;
+        *cnt = run;
+    }
+}
+
+__global__ void k_plain_energy(const int32_t* __restrict__ cnt, const int32_t* __restrict__ cidx,
+                               const double* __restrict__ S, const double* __restrict__ vel,
+                               const double* __restrict__ vcm, double cg,
+                               uint8_t* __restrict__ bound, int32_t* __restrict__ removed) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= *cnt) return;
+    const int k = cidx[p];
+    const double E = kinetic(vel, k, vcm) - cg * S[p];
+    if (E > 0.0) {
+        bound[k] = 0;
+        atomicAdd(removed, 1);
+    }
+}
+
+}  // namespace
+
+size_t unbind_workspace_bytes(int64_t H, const int64_t* h_off) {
+    std::vector<int64_t> rel(H + 1);
+    for (int64_t h = 0; h <= H; ++h) rel[h] = h_off[h] - h_off[0];
+    PotPlan plan;
+    if (plan_pot(H, rel.data(), rel.data(), plan)) return 0;
+    return carve_unb(nullptr, H, h_off[H] - h_off[0], plan).bytes;
+}
+
+int unbind_plain_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs in,
+                     double factor, int max_iter, uint8_t* bound_out, int32_t* iters_host,
+                     void* ws, size_t ws_bytes, hipStream_t st) {
+    int64_t nmax = 0;
+    for (int64_t h = 0; h < H; ++h) nmax = std::max(nmax, h_off[h + 1] - h_off[h]);
+    Carver c(ws);
+    const int64_t n1 = nmax > 0 ? nmax : 1;
+    double* px = c.take<double>(n1);
+    double* py = c.take<double>(n1);
+    double* pz = c.take<double>(n1);
+    double* pm = c.take<double>(n1);
+    double* vel = c.take<double>(3 * n1);
+    double* m64 = c.take<double>(n1);
+    double* cx = c.take<double>(n1);
+    double* cy = c.take<double>(n1);
+    double* cz = c.take<double>(n1);
+    double* cm = c.take<double>(n1);
+    int32_t* cidx = c.take<int32_t>(n1);
+    double* S = c.take<double>(n1);
+    double* vcm = c.take<double>(4);
+    int32_t* cnt = c.take<int32_t>(64);
+    int32_t* removed = c.take<int32_t>(64);
+    if (ws_bytes < c.used) return set_error(HALMA_E_WORKSPACE, "workspace too small");
+    double G = 4.3 * 1e-3;
+    G *= 1e-6;
+    const double cg = factor * G;
+    long long* host = pinned_words(2);
+    if (!host) return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
+    int* hcnt = reinterpret_cast<int*>(host);
+    for (int64_t h = 0; h < H; ++h) {
+        const int64_t n = h_off[h + 1] - h_off[h];
+        if (iters_host) iters_host[h] = 0;
+        if (n == 0) continue;
+        uint8_t* bh = bound_out + (h_off[h] - h_off[0]);
+        k_plain_gather<<<blocks_for(n), kThreads, 0, st>>>(in, members + h_off[h], n, px, py, pz,
+                                                           pm, vel, m64, bh);
+        HALMA_LAUNCH_CHECK();
+        for (int it = 0; it < max_iter; ++it) {
+            k_plain_compact<<<1, kThreads, 0, st>>>(n, bh, px, py, pz, pm, vel, m64, cx, cy, cz,
+                                                    cm, cidx, cnt, vcm);
+            HALMA_LAUNCH_CHECK();
+            HALMA_HIP(hipMemcpyAsync(hcnt, cnt, sizeof(int), hipMemcpyDeviceToHost, st));
+            HALMA_HIP(hipStreamSynchronize(st));
+            const int live = hcnt[0];
+            if (live == 0) break;
+            if (iters_host) iters_host[h] = it + 1;
+            int rc = halma_pot_sum_f64_plain(cm, cx, cy, cz, live, cx, cy, cz, live, S, st);
+            if (rc) return rc;
+            k_zero_counters<<<1, 64, 0, st>>>(reinterpret_cast<unsigned long long*>(removed), 1);
+            HALMA_LAUNCH_CHECK();
+            k_plain_energy<<<blocks_for(live), kThreads, 0, st>>>(cnt, cidx, S, vel, vcm, cg, bh,
+                                                                  removed);
+            HALMA_LAUNCH_CHECK();
+            HALMA_HIP(hipMemcpyAsync(hcnt, removed, sizeof(int), hipMemcpyDeviceToHost, st));
+            HALMA_HIP(hipStreamSynchronize(st));
+            if (hcnt[0] == 0) break;
+        }
+    }
+    return 0;
+}
+
+size_t unbind_plain_workspace_bytes(int64_t H, const int64_t* h_off) {
+    int64_t nmax = 0;
+    for (int64_t h = 0; h < H; ++h) nmax = std::max(nmax, h_off[h + 1] - h_off[h]);
+    const int64_t n1 = nmax > 0 ? nmax : 1;
+    Carver c(nullptr);
+    for (int k = 0; k < 4; ++k) c.take<double>(n1);
+    c.take<double>(3 * n1);
+    c.take<double>(n1);
+    for (int k = 0; k < 4; ++k) c.take<double>(n1);
+    c.take<int32_t>(n1);
+    c.take<double>(n1);
+    c.take<double>(4);
+    c.take<int32_t>(64);
+    c.take<int32_t>(64);
+    return c.used;
+}
+
+}  // namespace halma
+
+using namespace halma;
+
+namespace {
+int check_unbind_args(int64_t H, const int64_t* h_off, const int32_t* members, const void* x,
+                      const void* y, const void* z, int pos_t, const void* vx, const void* vy,
+                      const void* vz, int vel_t, const void* m, int mass_t, double factor,
+                      int max_iter, uint8_t* bound) {
+    if (H < 0 || !h_off) return set_error(HALMA_E_ARG, "bad halo offsets");
+    for (int64_t h = 0; h < H; ++h)
+        if (h_off[h + 1] < h_off[h]) return set_error(HALMA_E_ARG, "offsets must not decrease");
+    if (H > INT32_MAX) return set_error(HALMA_E_LIMIT, "too many haloes");
+    if (h_off[H] - h_off[0] > INT32_MAX) return set_error(HALMA_E_LIMIT, "too many members");
+    for (int64_t h = 0; h < H; ++h)
+        if (h_off[h + 1] - h_off[h] > INT32_MAX) return set_error(HALMA_E_LIMIT, "halo too large");
+    if (h_off[H] > h_off[0] &&
+        (!members || !x || !y || !z || !vx || !vy || !vz || !m || !bound))
+        return set_error(HALMA_E_ARG, "null pointer");
+    auto bad = [](int t) { return t != HALMA_F32 && t != HALMA_F64; };
+    if (bad(pos_t) || bad(vel_t) || bad(mass_t)) return set_error(HALMA_E_ARG, "bad dtype");
+    if (!(factor >= 0.0) || !isfinite(factor))
+        return set_error(HALMA_E_ARG, "factor must be finite and non-negative");
+    if (max_iter < 1) return set_error(HALMA_E_ARG, "max_iter must be >= 1");
+    return 0;
+}
+
+Inputs make_inputs(const void* x, const void* y, const void* z, int pos_t, const void* vx,
+                   const void* vy, const void* vz, int vel_t, const void* m, int mass_t) {
+    Inputs in;
+    in.x = x;
+    in.y = y;
+    in.z = z;
+    in.pos_t = pos_t;
+    in.vx = vx;
+    in.vy = vy;
+    in.vz = vz;
+    in.vel_t = vel_t;
+    in.m = m;
+    in.mass_t = mass_t;
+    return in;
+}
+}  // namespace
+
+extern "C" {
+
+size_t halma_unbind_workspace_size(int64_t n_halo, const int64_t* offsets) {
+    if (n_halo < 0 || !offsets) return 0;
+    return unbind_workspace_bytes(n_halo, offsets);
+}
+
+int halma_unbind_batched(int64_t n_halo, const int64_t* offsets, const int32_t* members,
+                         const void* x, const void* y, const void* z, int pos_dtype,
+                         const void* vx, const void* vy, const void* vz, int vel_dtype,
+                         const void* m, int mass_dtype, double factor, int max_iter, int flags,
+                         uint8_t* bound, int32_t* iters, double* vcm, double* pot,
+                         int64_t* stats, void* ws, size_t ws_bytes, void* stream) {
+    int rc = check_unbind_args(n_halo, offsets, members, x, y, z, pos_dtype, vx, vy, vz,
+                               vel_dtype, m, mass_dtype, factor, max_iter, bound);
+    if (rc) return rc;
+    if (n_halo == 0) return 0;
+    for (int64_t h = 0; h < n_halo; ++h)
+        if (offsets[h + 1] - offsets[h] < 0) return set_error(HALMA_E_ARG, "bad offsets");
+    Inputs in = make_inputs(x, y, z, pos_dtype, vx, vy, vz, vel_dtype, m, mass_dtype);
+    return unbind_run(n_halo, offsets, members, in, factor, max_iter, flags, bound, iters, vcm,
+                      pot, stats, ws, ws_bytes, static_cast<hipStream_t>(stream));
+}
+
+size_t halma_unbind_plain_workspace_size(int64_t n_halo, const int64_t* offsets) {
+    if (n_halo < 0 || !offsets) return 0;
+    return unbind_plain_workspace_bytes(n_halo, offsets);
+}
+
+int halma_unbind_plain_f64(int64_t n_halo, const int64_t* offsets, const int32_t* members,
+                           const void* x, const void* y, const void* z, int pos_dtype,
+                           const void* vx, const void* vy, const void* vz, int vel_dtype,
+                           const void* m, int mass_dtype, double factor, int max_iter,
+                           uint8_t* bound, int32_t* iters_host, void* ws, size_t ws_bytes,
+                           void* stream) {
+    int rc = check_unbind_args(n_halo, offsets, members, x, y, z, pos_dtype, vx, vy, vz,
+                               vel_dtype, m, mass_dtype, factor, max_iter, bound);
+    if (rc) return rc;
+    Inputs in = make_inputs(x, y, z, pos_dtype, vx, vy, vz, vel_dtype, m, mass_dtype);
+    return unbind_plain_run(n_halo, offsets, members, in, factor, max_iter, bound, iters_host,
+                            ws, ws_bytes, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

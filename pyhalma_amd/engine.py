@@ -119,6 +119,186 @@ def fof_labels(x, y, z, linking_length):
     return labels
 
 
+class Groups:
+    """Groups in CSR form on the device: members[offsets[g]:offsets[g+1]] are the particle
+    indices of group g (ascending), groups ordered by minimum member index."""
+
+    def __init__(self, members, offsets, offsets_host):
+        self.members = members            # int32 device tensor
+        self.offsets = offsets            # int64 device tensor (G+1)
+        self.offsets_host = offsets_host  # numpy int64 (G+1)
+
+    @property
+    def n_groups(self):
+        return len(self.offsets_host) - 1
+
+    @property
+    def n_members(self):
+        return int(self.offsets_host[-1]) if len(self.offsets_host) else 0
+
+    def sizes(self):
+        return np.diff(self.offsets_host)
+
+    def to_lists(self):
+        mem = self.members[: self.n_members].cpu().numpy()
+        return [mem[a:b].tolist() for a, b in zip(self.offsets_host[:-1], self.offsets_host[1:])]
+
+    def to_arrays(self):
+        mem = self.members[: self.n_members].cpu().numpy().astype(np.int64)
+        return [mem[a:b] for a, b in zip(self.offsets_host[:-1], self.offsets_host[1:])]
+
+
+def groups_from_labels(labels, min_size=2):
+    """Groups with at least min_size members from FoF labels (good_groups filter,
+    pyHALMA.py:148-149)."""
+    L = _lib_()
+    labels = to_dev(labels, np.int32)
+    n = int(labels.numel())
+    members = torch.empty(max(n, 1), dtype=torch.int32, device=labels.device)
+    offsets = torch.zeros(n + 1, dtype=torch.int64, device=labels.device)
+    ng = ctypes_i64()
+    nm = ctypes_i64()
+    if n > 0:
+        ws = workspace(L.halma_groups_workspace_size(n))
+        rc = L.halma_groups_from_labels(ptr(labels), n, int(min_size), ptr(members),
+                                        ptr(offsets), ng, nm, ptr(ws), ws.numel(), stream_ptr())
+        _lib.check(rc, "halma_groups_from_labels")
+    G = int(ng.value)
+    offsets = offsets[: G + 1]
+    if G == 0:
+        offsets = torch.zeros(1, dtype=torch.int64, device=labels.device)
+    return Groups(members, offsets, offsets.cpu().numpy())
+
+
+def fof_groups(x, y, z, linking_length, min_size=2):
+    """friends-of-friends straight to groups (labels, Groups)."""
+    lab = fof_labels(x, y, z, linking_length)
+    return lab, groups_from_labels(lab, min_size)
+
+
+def ctypes_i64(v=0):
+    import ctypes
+    return ctypes.c_int64(v)
+
+
+def _vec(a):
+    from .device import float_tensor
+    return float_tensor(a)
+
+
+def group_reduce(groups, x, y, z, m, vx=None, vy=None, vz=None, centres=None):
+    """Per-group reductions, float64 (G, 16): [M, cx, cy, cz, vx, vy, vz, rmax,
+    xmin, ymin, zmin, xmax, ymax, zmax, sum|m|, exact-sum flag]."""
+    from .device import dtype_flag
+    L = _lib_()
+    x, y, z = _coords(x, y, z)
+    m = _vec(m)
+    G = groups.n_groups
+    out = torch.zeros((max(G, 1), _lib.GROUP_FIELDS), dtype=torch.float64, device=x.device)
+    if G == 0:
+        return out[:0]
+    if vx is not None:
+        vx, vy, vz = _coords(vx, vy, vz)
+        vflag = dtype_flag(vx)
+    else:
+        vflag = _lib.F64
+    cen = None
+    if centres is not None:
+        cen = to_dev(np.asarray(centres, np.float64).reshape(G, 3), np.float64)
+    rc = L.halma_group_reduce(G, ptr(groups.offsets), ptr(groups.members), ptr(x), ptr(y),
+                              ptr(z), dtype_flag(x), ptr(vx), ptr(vy), ptr(vz), vflag, ptr(m),
+                              dtype_flag(m), ptr(cen), ptr(out), stream_ptr())
+    _lib.check(rc, "halma_group_reduce")
+    return out
+
+
+def density_peak(groups, x, y, z, m, ll, reduced=None):
+    """density_peak (halo_properties.py:518-556) of every group: (G, 3) float64."""
+    from .device import dtype_flag
+    L = _lib_()
+    x, y, z = _coords(x, y, z)
+    m = _vec(m)
+    G = groups.n_groups
+    if G == 0:
+        return torch.zeros((0, 3), dtype=torch.float64, device=x.device)
+    if reduced is None:
+        reduced = group_reduce(groups, x, y, z, m)
+    out = torch.zeros((G, 3), dtype=torch.float64, device=x.device)
+    ws = workspace(L.halma_density_peak_workspace_size(G))
+    rc = L.halma_density_peak(G, ptr(groups.offsets), ptr(groups.members), ptr(x), ptr(y),
+                              ptr(z), dtype_flag(x), ptr(m), dtype_flag(m), float(ll),
+                              ptr(reduced), ptr(out), ptr(ws), ws.numel(), stream_ptr())
+    _lib.check(rc, "halma_density_peak")
+    return out
+
+
+class UnbindResult:
+    def __init__(self, bound, iters, vcm, pot, stats):
+        self.bound = bound      # uint8 device tensor, member order
+        self.iters = iters      # int32 device tensor per halo
+        self.vcm = vcm          # float64 device tensor (H, 3)
+        self.pot = pot          # float64 device tensor per member (last S) or None
+        self.stats = stats      # dict
+
+
+def unbind(groups, x, y, z, vx, vy, vz, m, factor=1.0, max_iter=100, use_graph=True,
+           with_pot=False, time_pot=False):
+    """Batched iterative unbinding of every group (see halma_unbind_batched)."""
+    from .device import dtype_flag
+    L = _lib_()
+    x, y, z = _coords(x, y, z)
+    vx, vy, vz = _coords(vx, vy, vz)
+    m = _vec(m)
+    H = groups.n_groups
+    nm = groups.n_members
+    dev = x.device
+    bound = torch.zeros(max(nm, 1), dtype=torch.uint8, device=dev)
+    iters = torch.zeros(max(H, 1), dtype=torch.int32, device=dev)
+    vcm = torch.zeros((max(H, 1), 3), dtype=torch.float64, device=dev)
+    pot = (torch.full((max(nm, 1),), float("nan"), dtype=torch.float64, device=dev)
+           if with_pot else None)
+    import ctypes
+    st = (ctypes.c_int64 * 8)()
+    if H > 0:
+        off = _lib.i64_array(groups.offsets_host)
+        ws = workspace(L.halma_unbind_workspace_size(H, off))
+        rc = L.halma_unbind_batched(H, off, ptr(groups.members), ptr(x), ptr(y), ptr(z),
+                                    dtype_flag(x), ptr(vx), ptr(vy), ptr(vz), dtype_flag(vx),
+                                    ptr(m), dtype_flag(m), float(factor), int(max_iter),
+                                    (0 if use_graph else _lib.UNBIND_NO_GRAPH)
+                                    | (_lib.UNBIND_TIME_POT if time_pot else 0), ptr(bound),
+                                    ptr(iters), ptr(vcm), ptr(pot), st, ptr(ws), ws.numel(),
+                                    stream_ptr())
+        _lib.check(rc, "halma_unbind_batched")
+    s = dict(passes=int(st[0]), retested=int(st[1]), items=int(st[2]),
+             tests_per_lane=int(st[3]), subtile=int(st[4]), pot_ns=int(st[5]),
+             pot_launches=int(st[6]), pairs=int(st[7]))
+    return UnbindResult(bound[:nm], iters[:H], vcm[:H], None if pot is None else pot[:nm], s)
+
+
+def unbind_plain_f64(groups, x, y, z, vx, vy, vz, m, factor=1.0, max_iter=100):
+    """The plain FP64 comparator of the same method (per-halo launches)."""
+    import ctypes
+    from .device import dtype_flag
+    L = _lib_()
+    x, y, z = _coords(x, y, z)
+    vx, vy, vz = _coords(vx, vy, vz)
+    m = _vec(m)
+    H = groups.n_groups
+    nm = groups.n_members
+    bound = torch.zeros(max(nm, 1), dtype=torch.uint8, device=x.device)
+    iters = (ctypes.c_int32 * max(H, 1))()
+    if H > 0:
+        off = _lib.i64_array(groups.offsets_host)
+        ws = workspace(L.halma_unbind_plain_workspace_size(H, off))
+        rc = L.halma_unbind_plain_f64(H, off, ptr(groups.members), ptr(x), ptr(y), ptr(z),
+                                      dtype_flag(x), ptr(vx), ptr(vy), ptr(vz), dtype_flag(vx),
+                                      ptr(m), dtype_flag(m), float(factor), int(max_iter),
+                                      ptr(bound), iters, ptr(ws), ws.numel(), stream_ptr())
+        _lib.check(rc, "halma_unbind_plain_f64")
+    return bound[:nm], np.array(iters[:H], dtype=np.int32)
+
+
 def tune_set(key, value):
     _lib.check(_lib_().halma_tune_set(int(key), int(value)), "halma_tune_set")
 

@@ -1,0 +1,136 @@
+"""Batched unbinding on the GPU against the FP64 oracle.
+
+Bars: bound masks and per-halo pass counts identical to oracle.unbind_iterative; v_cm
+within 1e-14; the FP32 potentials within the rigorous bound; a knife-edge set where
+almost every particle sits inside the FP32 error band (so the FP64 re-test decides) still
+matches the oracle bit for bit; graph replay and plain launches agree; the plain FP64
+comparator agrees with the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _groups(w, min_size=50):
+    from pyhalma_amd import engine
+
+    lab = oracle.fof_labels_c(w["x"], w["y"], w["z"], w["ll"])
+    return engine.groups_from_labels(lab.astype(np.int32), min_size)
+
+
+def _oracle(g, w, factor, max_iter=100):
+    return oracle.unbind_iterative(g.offsets_host, g.members[: g.n_members].cpu().numpy(),
+                                   w["x"], w["y"], w["z"], w["vx"], w["vy"], w["vz"],
+                                   w["mass"], factor, max_iter)
+
+
+@pytest.mark.parametrize("factor", [1.0, 9.0])
+def test_unbind_matches_oracle(gpu, factor):
+    from pyhalma_amd import engine
+
+    w = __import__("pyhalma_amd.synth", fromlist=["x"]).clumps(seed=12, scale=0.01)
+    g = _groups(w)
+    print(f"factor={factor} haloes={g.n_groups} members={g.n_members}", flush=True)
+    r = engine.unbind(g, w["x"], w["y"], w["z"], w["vx"], w["vy"], w["vz"], w["mass"],
+                      factor=factor, with_pot=True)
+    bound, iters, S = _oracle(g, w, factor)
+    got = r.bound.cpu().numpy().astype(bool)
+    assert np.array_equal(got, bound), int((got != bound).sum())
+    assert np.array_equal(r.iters.cpu().numpy(), iters)
+    pot = r.pot.cpu().numpy()
+    ok = ~np.isnan(S)
+    rel = np.abs(pot[ok] - S[ok]) / S[ok]
+    assert rel.max() <= engine.pot_error_bound()
+    print(f"passes={r.stats['passes']} retested={r.stats['retested']} "
+          f"unbound={int((~bound).sum())} max_rel_pot_err={rel.max():.2e}", flush=True)
+
+
+def test_unbind_plummer_background(gpu):
+    from pyhalma_amd import engine, synth
+
+    w = synth.plummer_with_background(n_sphere=4000, n_bg=800)
+    g = _groups(w, min_size=50)
+    r = engine.unbind(g, w["x"], w["y"], w["z"], w["vx"], w["vy"], w["vz"], w["mass"], 1.0)
+    bound, iters, _ = _oracle(g, w, 1.0)
+    assert np.array_equal(r.bound.cpu().numpy().astype(bool), bound)
+    assert np.array_equal(r.iters.cpu().numpy(), iters)
+
+
+def test_unbind_knife_edge_band(gpu):
+    """Velocities set so that E = kin - G*S sits within ~1e-8 relative of zero for every
+    particle: the FP32 band contains them all and the FP64 re-test must decide exactly as
+    the oracle does.  Mirror-symmetric positions and velocities keep v_cm = 0."""
+    from pyhalma_amd import engine, synth
+
+    rng = np.random.default_rng(8)
+    half = synth.plummer_positions(rng, 1500, 0.005)
+    pos = np.concatenate([half, -half]).astype(np.float32).astype(np.float64)
+    m = np.full(len(pos), 1e5)
+    S = oracle.pot_sum_f64(m, pos[:, 0], pos[:, 1], pos[:, 2], pos[:, 0], pos[:, 1], pos[:, 2])
+    target = oracle.G_CONST * S * (1.0 + rng.uniform(-3e-8, 3e-8, len(pos)))
+    speed = np.sqrt(2.0 * target[:1500])
+    u = rng.normal(size=(1500, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    v = np.concatenate([u * speed[:, None], -u * speed[:, None]])
+    # the mirrored half has its own S; set its speed from its own target instead
+    sp2 = np.sqrt(2.0 * target[1500:])
+    v[1500:] = -u * sp2[:, None]
+    # re-balance momentum exactly: equal masses, v_cm must vanish -> scale pairs alike
+    vv = v.copy()
+    vv[1500:] = -vv[:1500]
+    w = dict(x=pos[:, 0], y=pos[:, 1], z=pos[:, 2], vx=vv[:, 0], vy=vv[:, 1], vz=vv[:, 2],
+             mass=m, ll=1.0)
+    members = np.arange(len(pos), dtype=np.int32)
+    import torch
+    g = engine.Groups(torch.from_numpy(members).cuda(),
+                      torch.tensor([0, len(pos)], dtype=torch.int64).cuda(),
+                      np.array([0, len(pos)], np.int64))
+    r = engine.unbind(g, w["x"], w["y"], w["z"], w["vx"], w["vy"], w["vz"], m, 1.0, max_iter=1)
+    bound, _, _ = oracle.unbind_iterative(g.offsets_host, members, w["x"], w["y"], w["z"],
+                                          w["vx"], w["vy"], w["vz"], m, 1.0, max_iter=1)
+    got = r.bound.cpu().numpy().astype(bool)
+    print(f"retested={r.stats['retested']} of {len(pos)}; bound={int(bound.sum())}", flush=True)
+    assert r.stats["retested"] > len(pos) // 2
+    assert 0 < bound.sum() < len(pos)
+    assert np.array_equal(got, bound)
+
+
+def test_unbind_graph_vs_launches_and_plain(gpu):
+    from pyhalma_amd import engine, synth
+
+    w = synth.clumps(seed=21, scale=0.005)
+    g = _groups(w)
+    args = (g, w["x"], w["y"], w["z"], w["vx"], w["vy"], w["vz"], w["mass"], 1.0)
+    a = engine.unbind(*args, use_graph=True)
+    b = engine.unbind(*args, use_graph=False)
+    assert np.array_equal(a.bound.cpu().numpy(), b.bound.cpu().numpy())
+    assert np.array_equal(a.iters.cpu().numpy(), b.iters.cpu().numpy())
+    pb, piters = engine.unbind_plain_f64(*args)
+    bound, iters, _ = _oracle(g, w, 1.0)
+    diff = int((pb.cpu().numpy().astype(bool) != bound).sum())
+    assert diff == 0, diff
+    assert np.array_equal(piters, iters)
+
+
+def test_unbind_empty_and_tiny(gpu):
+    from pyhalma_amd import engine
+    import torch
+
+    e = engine.Groups(torch.zeros(1, dtype=torch.int32).cuda(),
+                      torch.zeros(1, dtype=torch.int64).cuda(), np.zeros(1, np.int64))
+    z = np.zeros(3)
+    r = engine.unbind(e, z, z, z, z, z, z, z)
+    assert r.bound.numel() == 0
+    # two bodies at rest: bound; two bodies flying apart: unbound
+    x = np.array([0.0, 0.001, 5.0, 5.001])
+    y = np.array([0.0, 0.0005, 5.0, 5.0005])
+    zz = np.array([0.0, 0.0002, 5.0, 5.0002])
+    vx = np.array([0.0, 0.0, -1e4, 1e4])
+    m = np.full(4, 1e8)
+    members = torch.arange(4, dtype=torch.int32).cuda()
+    g = engine.Groups(members, torch.tensor([0, 2, 4], dtype=torch.int64).cuda(),
+                      np.array([0, 2, 4], np.int64))
+    r = engine.unbind(g, x, y, zz, vx, np.zeros(4), np.zeros(4), m, 1.0)
+    assert r.bound.cpu().numpy().tolist() == [1, 1, 0, 0]
