@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparator")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-frac", type=float, default=0.25,
+    ap.add_argument("--cpu-frac", type=float, default=0.5,
                     help="fraction of the box (along x) used for the CPU baseline sample")
     return ap.parse_args()
 
@@ -197,6 +197,8 @@ def cpu_baseline(w, frac):
         b = np.ones(len(g), bool)
         for _ in range(100):
             s = g[b]
+            if s.size == 0:
+                break
             mm = m[s].astype(np.float64)
             vc = [float((mm * v[s]).sum() / mm.sum()) for v in (vx, vy, vz)]
             S = pot(m[s], x[s], y[s], z[s], x[s], y[s], z[s]).astype(np.float64)

@@ -78,12 +78,15 @@ __global__ void k_seg_keep(const int64_t* __restrict__ seg_begin, const int64_t*
     ksize[s] = k ? size : 0;
 }
 
-__global__ void k_emit(const int32_t* __restrict__ sv, const int64_t* __restrict__ segid,
-                       const int64_t* __restrict__ seg_begin, const int32_t* __restrict__ keep,
-                       const int64_t* __restrict__ moff, int64_t n, int32_t* __restrict__ members) {
+// segid is the exclusive scan of the head flags, so the segment holding position p is
+// segid[p] + head[p] - 1 (the inclusive count of heads up to p, minus one)
+__global__ void k_emit(const int32_t* __restrict__ sv, const int32_t* __restrict__ head,
+                       const int64_t* __restrict__ segid, const int64_t* __restrict__ seg_begin,
+                       const int32_t* __restrict__ keep, const int64_t* __restrict__ moff,
+                       int64_t n, int32_t* __restrict__ members) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const int64_t s = segid[p];
+    const int64_t s = segid[p] + head[p] - 1;
     if (keep[s]) members[moff[s] + (p - seg_begin[s])] = sv[p];
 }
 
@@ -577,8 +580,8 @@ int groups_csr_run(const int32_t* labels, int64_t n, int min_size, int32_t* memb
     if (rc) return rc;
     rc = exclusive_scan_i64(w.ksize, w.moff, n, w.totals + 2, w.scan_ws, st);
     if (rc) return rc;
-    k_emit<<<blocks_for(n), kThreads, 0, st>>>(w.sv, w.segid, w.seg_begin, w.keep, w.moff, n,
-                                               members);
+    k_emit<<<blocks_for(n), kThreads, 0, st>>>(w.sv, w.head, w.segid, w.seg_begin, w.keep,
+                                               w.moff, n, members);
     HALMA_LAUNCH_CHECK();
     k_offsets<<<blocks_for(n), kThreads, 0, st>>>(w.keep, w.gidx, w.moff, w.totals, n, offsets);
     HALMA_LAUNCH_CHECK();

@@ -22,8 +22,8 @@
 //                    oracle's fixed summation order, so E and the decision are the
 //                    oracle's bit for bit;
 //   5. k_finish      per-halo pass counts, convergence flags, the removed total.
-// Passes 1-5 are captured once into a hipGraph and replayed; the host reads one counter
-// per pass to stop.
+// Each pass is captured once into hipGraphs (prepare / pair kernel / decisions) and
+// replayed; the host reads one counter per pass to stop.
 #include <math.h>
 
 #include <algorithm>
@@ -353,7 +353,6 @@ hipStream_t capture_stream() {
 }
 
 struct PassLaunch {
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // optional timing of the pair kernel
     int64_t H, nm;
     int grid_halo, grid_retest;
     const PotPlan* plan;
@@ -362,7 +361,10 @@ struct PassLaunch {
     EnergyArgs ea;
 };
 
-int launch_pass(const PassLaunch& L, hipStream_t st) {
+// A pass in three parts (each captured into its own graph): prepare (compaction, live
+// lengths), the pair kernel, and the energy decisions.  Timing events, when requested,
+// are recorded on the stream between the parts, around the pair kernel alone.
+int launch_pre(const PassLaunch& L, hipStream_t st) {
     const UnbWs& w = L.w;
     k_compact<<<L.grid_halo, kThreads, 0, st>>>(w.off, static_cast<int>(L.H), w.active, L.bound,
                                                 w.p4, w.vel, w.m64, w.src4, w.cidx, w.dyn_len,
@@ -371,10 +373,16 @@ int launch_pass(const PassLaunch& L, hipStream_t st) {
     k_work_len<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.active, w.dyn_len,
                                                      w.wlen);
     HALMA_LAUNCH_CHECK();
-    if (L.ev0) HALMA_HIP(hipEventRecord(L.ev0, st));
-    int rc = launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, w.wlen, w.items, w.part, w.head, st);
-    if (rc) return rc;
-    if (L.ev1) HALMA_HIP(hipEventRecord(L.ev1, st));
+    return 0;
+}
+
+int launch_pot(const PassLaunch& L, hipStream_t st) {
+    const UnbWs& w = L.w;
+    return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, w.wlen, w.items, w.part, w.head, st);
+}
+
+int launch_post(const PassLaunch& L, hipStream_t st) {
+    const UnbWs& w = L.w;
     if (L.nm > 0) {
         k_energy<<<blocks_for(L.nm), kThreads, 0, st>>>(L.ea);
         HALMA_LAUNCH_CHECK();
@@ -384,6 +392,24 @@ int launch_pass(const PassLaunch& L, hipStream_t st) {
     k_finish<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.dyn_len, w.active,
                                                    w.iters, w.removed, w.counters);
     HALMA_LAUNCH_CHECK();
+    return 0;
+}
+
+typedef int (*PartFn)(const PassLaunch&, hipStream_t);
+
+int capture_part(PartFn fn, const PassLaunch& L, hipStream_t cs, hipGraph_t* g,
+                 hipGraphExec_t* e) {
+    HALMA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    const int rc = fn(L, cs);
+    hipGraph_t g2 = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &g2);
+    if (rc) {
+        if (g2) (void)hipGraphDestroy(g2);
+        return rc;
+    }
+    HALMA_HIP(ec);
+    *g = g2;
+    HALMA_HIP(hipGraphInstantiate(e, g2, nullptr, nullptr, 0));
     return 0;
 }
 
@@ -450,45 +476,62 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
 
     const bool use_graph = (flags & HALMA_UNBIND_NO_GRAPH) == 0;
     const bool timing = (flags & HALMA_UNBIND_TIME_POT) != 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (timing) {
-        HALMA_HIP(hipEventCreate(&L.ev0));
-        HALMA_HIP(hipEventCreate(&L.ev1));
+        HALMA_HIP(hipEventCreate(&ev0));
+        HALMA_HIP(hipEventCreate(&ev1));
     }
     double pot_ms = 0.0;
     long long pot_launches = 0;
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
+    const PartFn parts[3] = {launch_pre, launch_pot, launch_post};
+    hipGraph_t graphs[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t execs[3] = {nullptr, nullptr, nullptr};
+    auto cleanup = [&]() {
+        for (int q = 0; q < 3; ++q) {
+            if (execs[q]) (void)hipGraphExecDestroy(execs[q]);
+            if (graphs[q]) (void)hipGraphDestroy(graphs[q]);
+        }
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    };
     if (use_graph) {
         hipStream_t cs = capture_stream();
-        if (!cs) return set_error(HALMA_E_STATE, "cannot create the capture stream");
-        HALMA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
-        rc = launch_pass(L, cs);
-        hipGraph_t g2 = nullptr;
-        const hipError_t ec = hipStreamEndCapture(cs, &g2);
-        if (rc) {
-            if (g2) (void)hipGraphDestroy(g2);
-            return rc;
+        if (!cs) {
+            cleanup();
+            return set_error(HALMA_E_STATE, "cannot create the capture stream");
         }
-        HALMA_HIP(ec);
-        graph = g2;
-        HALMA_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (int q = 0; q < 3; ++q) {
+            rc = capture_part(parts[q], L, cs, &graphs[q], &execs[q]);
+            if (rc) {
+                cleanup();
+                return rc;
+            }
+        }
     }
+    auto run_part = [&](int q) -> int {
+        if (use_graph) {
+            const hipError_t e = hipGraphLaunch(execs[q], st);
+            return e == hipSuccess ? 0 : hip_fail(e, "hipGraphLaunch", __FILE__, __LINE__);
+        }
+        return parts[q](L, st);
+    };
     long long* host = pinned_words(8);
-    if (!host) return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
+    if (!host) {
+        cleanup();
+        return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
+    }
     int passes = 0;
     long long band_total = 0;
     int status = 0;
     for (; passes < max_iter; ++passes) {
-        if (use_graph) {
-            const hipError_t e = hipGraphLaunch(exec, st);
-            if (e != hipSuccess) {
-                status = hip_fail(e, "hipGraphLaunch", __FILE__, __LINE__);
-                break;
-            }
-        } else {
-            status = launch_pass(L, st);
-            if (status) break;
-        }
+        status = run_part(0);
+        if (!status && timing) status = hipEventRecord(ev0, st) == hipSuccess ? 0 :
+            set_error(HALMA_E_STATE, "event record failed");
+        if (!status) status = run_part(1);
+        if (!status && timing) status = hipEventRecord(ev1, st) == hipSuccess ? 0 :
+            set_error(HALMA_E_STATE, "event record failed");
+        if (!status) status = run_part(2);
+        if (status) break;
         const hipError_t e1 = hipMemcpyAsync(host, w.counters, 4 * sizeof(long long),
                                              hipMemcpyDeviceToHost, st);
         const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(st) : e1;
@@ -499,7 +542,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         band_total = host[2];
         if (timing) {
             float ms = 0.0f;
-            const hipError_t e4 = hipEventElapsedTime(&ms, L.ev0, L.ev1);
+            const hipError_t e4 = hipEventElapsedTime(&ms, ev0, ev1);
             if (e4 != hipSuccess) {
                 status = hip_fail(e4, "pair-kernel timing", __FILE__, __LINE__);
                 break;
@@ -519,10 +562,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
             break;
         }
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (L.ev0) (void)hipEventDestroy(L.ev0);
-    if (L.ev1) (void)hipEventDestroy(L.ev1);
+    cleanup();
     if (status) return status;
     if (iters_out) HALMA_HIP(hipMemcpyAsync(iters_out, w.iters, H * sizeof(int32_t),
                                             hipMemcpyDeviceToDevice, st));

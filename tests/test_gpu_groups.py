@@ -33,10 +33,13 @@ def test_groups_from_labels_matches_oracle(gpu):
 def test_groups_edge_cases(gpu):
     from pyhalma_amd import engine
 
+    print("case=empty", flush=True)
     assert engine.groups_from_labels(np.zeros(0, np.int32)).n_groups == 0
     lab = np.arange(10, dtype=np.int32)  # all singletons
+    print("case=singletons", flush=True)
     assert engine.groups_from_labels(lab, 2).n_groups == 0
     assert engine.groups_from_labels(lab, 1).to_lists() == [[i] for i in range(10)]
+    print("case=one-group", flush=True)
     lab = np.zeros(300000, np.int32)  # one big group
     g = engine.groups_from_labels(lab, 2)
     assert g.n_groups == 1 and g.to_lists()[0] == list(range(300000))
