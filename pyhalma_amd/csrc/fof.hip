@@ -117,13 +117,27 @@ __global__ __launch_bounds__(256) void k_bbox_partial(const T* __restrict__ x,
     }
 }
 
-__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ box) {
+__global__ __launch_bounds__(256) void k_bbox_final(const double* __restrict__ part, int nb,
+                                                    double* __restrict__ box) {
+    __shared__ double sh[4][6];
+    double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int d = 0; d < 6; ++d)
+            v[d] = d < 3 ? fmin(v[d], part[b * 6 + d]) : fmax(v[d], part[b * 6 + d]);
+    for (int d = 0; d < 6; ++d)
+        for (int s = 32; s >= 1; s >>= 1) {
+            const double o = __shfl_down(v[d], s, kWave);
+            v[d] = d < 3 ? fmin(v[d], o) : fmax(v[d], o);
+        }
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0)
+        for (int d = 0; d < 6; ++d) sh[w][d] = v[d];
+    __syncthreads();
     if (threadIdx.x < 6) {
-        double v = part[threadIdx.x];
-        for (int b = 1; b < nb; ++b)
-            v = threadIdx.x < 3 ? fmin(v, part[b * 6 + threadIdx.x])
-                                : fmax(v, part[b * 6 + threadIdx.x]);
-        box[threadIdx.x] = v;
+        double r = sh[0][threadIdx.x];
+        for (int k = 1; k < 4; ++k)
+            r = threadIdx.x < 3 ? fmin(r, sh[k][threadIdx.x]) : fmax(r, sh[k][threadIdx.x]);
+        box[threadIdx.x] = r;
     }
 }
 
@@ -444,7 +458,7 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     HALMA_LAUNCH_CHECK();
     k_bbox_partial<T><<<kBoxBlocks, 256, 0, st>>>(x, y, z, n, w.box_part);
     HALMA_LAUNCH_CHECK();
-    k_bbox_final<<<1, 64, 0, st>>>(w.box_part, kBoxBlocks, w.box);
+    k_bbox_final<<<1, 256, 0, st>>>(w.box_part, kBoxBlocks, w.box);
     HALMA_LAUNCH_CHECK();
     k_fill_u64<<<blocks_for(w.cap), 256, 0, st>>>(w.tkey, w.cap, kEmpty);
     HALMA_LAUNCH_CHECK();

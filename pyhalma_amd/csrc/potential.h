@@ -29,11 +29,22 @@ struct PotPlan {
 // Plans a batched launch from HOST offset arrays (H+1 entries each).
 int plan_pot(int64_t n_halo, const int64_t* src_off, const int64_t* tst_off, PotPlan& plan);
 
+// Live set sizes for a plan launched repeatedly over shrinking sets (all pointers may
+// be null for a static launch).  When set, halo h has tst_len[h] tests and src_len[h]
+// sources (never above the planned lengths), and its sources are read from src_alt
+// instead of the main source array when src_sel[h] != 0 (same offsets).
+struct DynSets {
+    const int* tst_len = nullptr;
+    const int* src_len = nullptr;
+    const int* src_sel = nullptr;
+    const float4* src_alt = nullptr;
+};
+
 // Launches the persistent FP32 kernel for a plan whose tables are already on the
-// device.  dyn_len may be null.  head must be a device int (reset by this launcher).
+// device.  head must be a device int (reset by this launcher).
 int launch_pot_f32(const PotPlan& plan, const float4* src4, const float4* tst4,
-                   const HaloDesc* d_halos, const int* dyn_len, const int4* d_items,
-                   float2* part, int* head, hipStream_t stream);
+                   const HaloDesc* d_halos, DynSets dyn, const int4* d_items, float2* part,
+                   int* head, hipStream_t stream);
 
 // Packs SoA FP32 (m, x, y, z) into AoS float4 (x, y, z, m); m may be null (-> 0).
 int launch_pack4(const float* m, const float* x, const float* y, const float* z,

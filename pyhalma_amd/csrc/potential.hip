@@ -42,7 +42,7 @@ template <int R, int SUB>
 __global__ __launch_bounds__(256) void k_pot_f32(const float4* __restrict__ src,
                                                  const float4* __restrict__ tst,
                                                  const HaloDesc* __restrict__ halos,
-                                                 const int* __restrict__ dyn_len,
+                                                 DynSets dyn,
                                                  const int4* __restrict__ items, int n_items,
                                                  float2* __restrict__ part,
                                                  int* __restrict__ head) {
@@ -55,9 +55,11 @@ __global__ __launch_bounds__(256) void k_pot_f32(const float4* __restrict__ src,
         const int4 it = uniform_load(items + k);
         const HaloDesc d = uniform_load(halos + it.x);
         int ns = d.src_len, nt = d.tst_len;
-        if (dyn_len) {
-            ns = __builtin_amdgcn_readfirstlane(uniform_load(dyn_len + it.x));
-            nt = ns;
+        const float4* sbase = src;
+        if (dyn.tst_len) {
+            nt = __builtin_amdgcn_readfirstlane(uniform_load(dyn.tst_len + it.x));
+            ns = __builtin_amdgcn_readfirstlane(uniform_load(dyn.src_len + it.x));
+            if (__builtin_amdgcn_readfirstlane(uniform_load(dyn.src_sel + it.x))) sbase = dyn.src_alt;
         }
         const int i0 = it.y * (R * kWave);
         const int j0 = it.z * d.chunk;
@@ -75,7 +77,7 @@ __global__ __launch_bounds__(256) void k_pot_f32(const float4* __restrict__ src,
             as[r] = 0.0f;
             ac[r] = 0.0f;
         }
-        const float4* sp = src + d.src_off;
+        const float4* sp = sbase + d.src_off;
         int j = j0;
         for (; j + SUB <= j1; j += SUB) {
             float ps[R];
@@ -262,25 +264,25 @@ int launch_pack4(const float* m, const float* x, const float* y, const float* z,
 
 template <int R, int SUB>
 static int launch_pot_t(const PotPlan& plan, const float4* src4, const float4* tst4,
-                        const HaloDesc* d_halos, const int* dyn_len, const int4* d_items,
+                        const HaloDesc* d_halos, DynSets dyn, const int4* d_items,
                         float2* part, int* head, hipStream_t stream) {
     k_fill_i32<<<1, 64, 0, stream>>>(head, 1, 0);
     HALMA_LAUNCH_CHECK();
     const int n_items = static_cast<int>(plan.items.size());
     if (n_items == 0) return 0;
     const int grid = std::max(1, std::min(plan.grid, (n_items + 3) / 4));
-    k_pot_f32<R, SUB><<<grid, 256, 0, stream>>>(src4, tst4, d_halos, dyn_len, d_items, n_items,
+    k_pot_f32<R, SUB><<<grid, 256, 0, stream>>>(src4, tst4, d_halos, dyn, d_items, n_items,
                                                 part, head);
     HALMA_LAUNCH_CHECK();
     return 0;
 }
 
 int launch_pot_f32(const PotPlan& plan, const float4* src4, const float4* tst4,
-                   const HaloDesc* d_halos, const int* dyn_len, const int4* d_items,
-                   float2* part, int* head, hipStream_t stream) {
+                   const HaloDesc* d_halos, DynSets dyn, const int4* d_items, float2* part,
+                   int* head, hipStream_t stream) {
 #define HALMA_POT_CASE(R_, S_)                                                            \
     if (plan.tests_per_lane == R_ && plan.subtile == S_)                                  \
-        return launch_pot_t<R_, S_>(plan, src4, tst4, d_halos, dyn_len, d_items, part, head, \
+        return launch_pot_t<R_, S_>(plan, src4, tst4, d_halos, dyn, d_items, part, head,  \
                                     stream);
     HALMA_POT_CASE(1, 4)
     HALMA_POT_CASE(1, 8)
@@ -347,7 +349,7 @@ static int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const
     k_tst_halo<<<static_cast<unsigned>(H), 256, 0, stream>>>(w.halos, static_cast<int>(H),
                                                              w.tst_halo);
     HALMA_LAUNCH_CHECK();
-    rc = launch_pot_f32(plan, w.src4, w.tst4, w.halos, nullptr, w.items, w.part, w.head, stream);
+    rc = launch_pot_f32(plan, w.src4, w.tst4, w.halos, DynSets(), w.items, w.part, w.head, stream);
     if (rc) return rc;
     const int64_t blocks = (plan.n_tst + 255) / 256;
     k_pot_merge<<<static_cast<unsigned>(blocks), 256, 0, stream>>>(

@@ -10,17 +10,26 @@
 //
 // One pass on the GPU, for all haloes at once:
 //   1. k_compact     one workgroup per active halo: stable compaction of its bound
-//                    members into the front of its range (AoS float4) and the
-//                    double-double v_cm over them;
+//                    members into the front of its range (AoS float4), of the members
+//                    removed by the previous pass into a second list, the double-double
+//                    v_cm over the bound set, and the pass mode: FULL (first pass, or the
+//                    previous pass removed more than a quarter of the bound set) sums over
+//                    the bound set afresh; INCREMENTAL subtracts the removed members'
+//                    contributions from each bound member's stored sum,
+//                    S_new = S_old - sum_{j removed} m_j / r_ij, which costs
+//                    n_bound x n_removed pairs instead of n_bound^2;
 //   2. k_pot_f32     the persistent FP32 pair kernel over a static queue planned for the
-//                    initial sizes (largest first); haloes' live lengths come from a
-//                    device array, converged haloes have length 0 and cost one fetch;
-//   3. k_energy      merges the chunk partials (FP64), evaluates E in FP64 and decides
-//                    every member whose |E| exceeds the rigorous band
-//                    factor*G*(B*S) + 8 ulp(kin + factor*G*S), B = FP32 error bound;
+//                    initial sizes (largest first); live test/source counts and the
+//                    source list come from device arrays, so empty chunks and converged
+//                    haloes cost one queue fetch;
+//   3. k_energy      merges the chunk partials (FP64), updates S and its absolute error
+//                    bound (the bounds of every FP32 sum that went into S add up),
+//                    evaluates E in FP64 and decides every member whose |E| exceeds
+//                    factor*G*bound + 8 ulp(kin + factor*G*S);
 //   4. k_retest      one wavefront per member inside the band: S in FP64 with the
-//                    oracle's fixed summation order, so E and the decision are the
-//                    oracle's bit for bit;
+//                    oracle's fixed summation order over the bound set, so E and the
+//                    decision are the oracle's bit for bit (S and its bound are reset to
+//                    that value);
 //   5. k_finish      per-halo pass counts, convergence flags, the removed total.
 // Each pass is captured once into hipGraphs (prepare / pair kernel / decisions) and
 // replayed; the host reads one counter per pass to stop.
@@ -40,6 +49,11 @@ namespace halma {
 namespace {
 
 constexpr int kThreads = 256;
+
+// member states
+constexpr uint8_t kGone = 0;      // removed by an earlier pass
+constexpr uint8_t kBound = 1;     // currently bound
+constexpr uint8_t kJustGone = 2;  // removed by the last pass (its contribution is pending)
 
 inline unsigned blocks_for(int64_t n, int t = kThreads) {
     return static_cast<unsigned>((n + t - 1) / t);
@@ -62,7 +76,7 @@ struct Inputs {
 // member k -> FP32 position/mass (the potential's inputs), FP64 velocity and mass
 __global__ void k_gather(Inputs in, const int32_t* __restrict__ mem, int64_t nm,
                          float4* __restrict__ p4, double* __restrict__ vel,
-                         double* __restrict__ m64, uint8_t* __restrict__ bound) {
+                         double* __restrict__ m64, uint8_t* __restrict__ state) {
     const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= nm) return;
     const int64_t i = mem[k];
@@ -74,7 +88,7 @@ __global__ void k_gather(Inputs in, const int32_t* __restrict__ mem, int64_t nm,
     vel[3 * k + 1] = ldf(in.vy, in.vel_t, i);
     vel[3 * k + 2] = ldf(in.vz, in.vel_t, i);
     m64[k] = ldf(in.m, in.mass_t, i);
-    bound[k] = 1;
+    state[k] = kBound;
 }
 
 __global__ void k_member_halo(const int64_t* __restrict__ off, int H, int32_t* __restrict__ mh) {
@@ -119,24 +133,33 @@ __device__ int block_excl_scan(int v, int& excl, int* sh) {
 
 __global__ __launch_bounds__(kThreads) void k_compact(
     const int64_t* __restrict__ off, int H, const int32_t* __restrict__ active,
-    const uint8_t* __restrict__ bound, const float4* __restrict__ p4,
-    const double* __restrict__ vel, const double* __restrict__ m64, float4* __restrict__ src4,
-    int32_t* __restrict__ cidx, int32_t* __restrict__ dyn_len, double* __restrict__ vcm) {
+    const int32_t* __restrict__ pass_no, uint8_t* __restrict__ state,
+    const float4* __restrict__ p4, const double* __restrict__ vel,
+    const double* __restrict__ m64, float4* __restrict__ src4, int32_t* __restrict__ cidx,
+    float4* __restrict__ rem4, int32_t* __restrict__ n_bound, int32_t* __restrict__ n_rem,
+    int32_t* __restrict__ mode, double* __restrict__ vcm) {
     __shared__ int shs[kThreads / kWave];
     __shared__ double shd[2 * (kThreads / kWave)];
+    const bool first = *pass_no == 0;
     for (int h = blockIdx.x; h < H; h += gridDim.x) {
         if (!active[h]) {
-            if (threadIdx.x == 0) dyn_len[h] = 0;
+            if (threadIdx.x == 0) {
+                n_bound[h] = 0;
+                n_rem[h] = 0;
+            }
             continue;
         }
         const int64_t b = off[h], e = off[h + 1];
-        int run = 0;
+        int run = 0, rrun = 0;
         double acc[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
         for (int64_t k0 = b; k0 < e; k0 += kThreads) {
             const int64_t k = k0 + threadIdx.x;
-            const int keep = (k < e && bound[k]) ? 1 : 0;
-            int excl;
+            const uint8_t st = k < e ? state[k] : kGone;
+            const int keep = st == kBound ? 1 : 0;
+            const int gone = st == kJustGone ? 1 : 0;
+            int excl, rexcl;
             const int tot = block_excl_scan(keep, excl, shs);
+            const int rtot = block_excl_scan(gone, rexcl, shs);
             if (keep) {
                 const int64_t pos = b + run + excl;
                 src4[pos] = p4[k];
@@ -147,7 +170,12 @@ __global__ __launch_bounds__(kThreads) void k_compact(
                 dd_add(acc[2][0], acc[2][1], m * vel[3 * k + 1]);
                 dd_add(acc[3][0], acc[3][1], m * vel[3 * k + 2]);
             }
+            if (gone) {
+                rem4[b + rrun + rexcl] = p4[k];
+                state[k] = kGone;
+            }
             run += tot;
+            rrun += rtot;
         }
         // block double-double reduction of the four sums (thread 0 holds the result)
         for (int q = 0; q < 4; ++q) {
@@ -176,7 +204,10 @@ __global__ __launch_bounds__(kThreads) void k_compact(
             const double M = acc[0][0] + acc[0][1];
             for (int d = 0; d < 3; ++d)
                 vcm[3 * h + d] = M > 0.0 ? (acc[1 + d][0] + acc[1 + d][1]) / M : 0.0;
-            dyn_len[h] = run;
+            n_bound[h] = run;
+            n_rem[h] = rrun;
+            // a full recompute when more than a quarter of the set left last pass
+            mode[h] = (first || 4 * static_cast<int64_t>(rrun) > run) ? 1 : 0;
         }
         __syncthreads();
     }
@@ -185,18 +216,23 @@ __global__ __launch_bounds__(kThreads) void k_compact(
 struct EnergyArgs {
     const HaloDesc* halos;
     const int32_t* mhalo;
-    const int32_t* dyn_len;
+    const int32_t* n_bound;
+    const int32_t* n_src;
+    const int32_t* mode;
     const int32_t* cidx;
     const float2* part;
     const double* vel;
     const double* vcm;
-    double cg;      // factor * G
-    double bound_rel;  // FP32 relative error bound B
-    uint8_t* bound;
+    double cg;         // factor * G
+    double bound_rel;  // FP32 relative error bound B of one pair sum
+    uint8_t* state;
+    double* sacc;      // running S per member position
+    double* eacc;      // absolute error bound of sacc
     int32_t* removed;
     int32_t* band;
-    unsigned long long* counters;  // [0] removed total, [1] band count, [2] band total
-    double* s_out;        // optional: S per member position (FP64)
+    unsigned long long* counters;  // [0] removed total, [1] band count, [2] band total,
+                                   // [3] pairs evaluated
+    double* s_out;     // optional: S per member position (FP64)
     int64_t nm;
 };
 
@@ -212,27 +248,41 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     if (g >= a.nm) return;
     const int h = a.mhalo[g];
     const HaloDesc d = a.halos[h];
-    const int64_t p = g - d.src_off;
-    const int n = a.dyn_len[h];
+    const int64_t p = g - d.tst_off;
+    const int n = a.n_bound[h];
     if (p >= n) return;
-    const int nch = (n + d.chunk - 1) / d.chunk;
-    double S = 0.0;
+    const int ns = a.n_src[h];
+    const int nch = (ns + d.chunk - 1) / d.chunk;
+    double sum = 0.0;
     for (int c = 0; c < nch; ++c) {
         const float2 q = a.part[d.part_off + static_cast<int64_t>(c) * d.stride + p];
-        S += static_cast<double>(q.x) + static_cast<double>(q.y);
+        sum += static_cast<double>(q.x) + static_cast<double>(q.y);
     }
     const int64_t k = a.cidx[g];
+    double S, err;
+    if (a.mode[h]) {
+        S = sum;
+        err = a.bound_rel * sum;
+    } else {
+        S = a.sacc[k] - sum;
+        err = a.eacc[k] + a.bound_rel * sum;
+    }
+    a.sacc[k] = S;
+    a.eacc[k] = err;
     const double kin = kinetic(a.vel, k, a.vcm + 3 * h);
     const double pot = a.cg * S;
     const double E = kin - pot;
-    const double band = pot * a.bound_rel * 1.01 + 8.0 * 0x1p-53 * (kin + pot);
+    // FP32 bound, FP64 rounding of the updates and of E, and the oracle's own FP64
+    // rounding (far below 1e-10 relative for any halo that fits in memory)
+    const double band = a.cg * err * 1.01 + 8.0 * 0x1p-53 * (kin + fabs(pot)) +
+                        1e-10 * fabs(pot);
     if (a.s_out) a.s_out[k] = S;
     if (fabs(E) <= band) {
         const unsigned long long at = atomicAdd(a.counters + 1, 1ull);
         a.band[at] = static_cast<int32_t>(g);
         atomicAdd(a.counters + 2, 1ull);
     } else if (E > 0.0) {
-        a.bound[k] = 0;
+        a.state[k] = kJustGone;
         atomicAdd(a.removed + h, 1);
     }
 }
@@ -245,59 +295,88 @@ __global__ __launch_bounds__(kThreads) void k_retest(EnergyArgs a, const float4*
         const int64_t g = a.band[w];
         const int h = a.mhalo[g];
         const HaloDesc d = a.halos[h];
-        const int n = a.dyn_len[h];
+        const int n = a.n_bound[h];
         const float4 t = src4[g];
         SrcAoS src{src4 + d.src_off};
         const double S = wave_pot_f64_ordered(src, n, t.x, t.y, t.z);
         if (__lane_id() == 0) {
             const int64_t k = a.cidx[g];
             const double E = kinetic(a.vel, k, a.vcm + 3 * h) - a.cg * S;
+            a.sacc[k] = S;
+            a.eacc[k] = 1e-12 * S;
             if (a.s_out) a.s_out[k] = S;
             if (E > 0.0) {
-                a.bound[k] = 0;
+                a.state[k] = kJustGone;
                 atomicAdd(a.removed + h, 1);
             }
         }
     }
 }
 
-__global__ void k_finish(int H, const int32_t* __restrict__ dyn_len, int32_t* __restrict__ active,
+__global__ void k_finish(int H, const int32_t* __restrict__ n_bound,
+                         const int32_t* __restrict__ n_src, int32_t* __restrict__ active,
                          int32_t* __restrict__ iters, int32_t* __restrict__ removed,
-                         unsigned long long* __restrict__ counters) {
+                         int32_t* __restrict__ pass_no, unsigned long long* __restrict__ counters) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h == 0) counters[1] = 0;  // the band list is rebuilt by the next pass
+    if (h == 0) {
+        counters[1] = 0;  // the band list is rebuilt by the next pass
+        *pass_no += 1;
+    }
     if (h >= H) return;
     if (!active[h]) return;
-    if (dyn_len[h] > 0) {
+    if (n_bound[h] > 0) {
         iters[h] += 1;
-        const unsigned long long n = static_cast<unsigned long long>(dyn_len[h]);
-        atomicAdd(counters + 3, n * n);  // pairs evaluated by the pair kernel this pass
+        // pairs evaluated by the pair kernel this pass
+        atomicAdd(counters + 3, static_cast<unsigned long long>(n_bound[h]) *
+                                    static_cast<unsigned long long>(n_src[h]));
     }
     const int r = removed[h];
-    if (r == 0 || dyn_len[h] == 0) active[h] = 0;
+    if (r == 0 || n_bound[h] == 0) active[h] = 0;
     if (r) atomicAdd(counters + 0, static_cast<unsigned long long>(r));
     removed[h] = 0;
 }
 
-// lengths used by the pair kernel: 0 for haloes that are no longer active
-__global__ void k_work_len(int H, const int32_t* __restrict__ active,
-                           const int32_t* __restrict__ dyn_len, int32_t* __restrict__ wlen) {
+// live counts for the pair kernel: tests = bound set; sources = bound set (full pass) or
+// the members removed by the previous pass (incremental pass); 0 when inactive
+__global__ void k_live_sets(int H, const int32_t* __restrict__ active,
+                            const int32_t* __restrict__ n_bound, const int32_t* __restrict__ n_rem,
+                            const int32_t* __restrict__ mode, int32_t* __restrict__ tst_len,
+                            int32_t* __restrict__ src_len, int32_t* __restrict__ src_sel) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h < H) wlen[h] = active[h] ? dyn_len[h] : 0;
+    if (h >= H) return;
+    const bool on = active[h] != 0;
+    tst_len[h] = on ? n_bound[h] : 0;
+    src_len[h] = on ? (mode[h] ? n_bound[h] : n_rem[h]) : 0;
+    src_sel[h] = mode[h] ? 0 : 1;
+}
+
+__global__ void k_final_mask(int64_t nm, const uint8_t* __restrict__ state,
+                             uint8_t* __restrict__ bound) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < nm) bound[k] = state[k] == kBound ? 1 : 0;
 }
 
 struct UnbWs {
     float4* p4;
     double* vel;
     double* m64;
+    uint8_t* state;
     float4* src4;
+    float4* rem4;
     int32_t* cidx;
     int32_t* mhalo;
-    int32_t* dyn_len;
-    int32_t* wlen;
+    double* sacc;
+    double* eacc;
+    int32_t* n_bound;
+    int32_t* n_rem;
+    int32_t* mode;
+    int32_t* tst_len;
+    int32_t* src_len;
+    int32_t* src_sel;
     int32_t* active;
     int32_t* iters;
     int32_t* removed;
+    int32_t* pass_no;
     double* vcm;
     int32_t* band;
     unsigned long long* counters;
@@ -317,14 +396,23 @@ UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan) {
     w.p4 = c.take<float4>(n1);
     w.vel = c.take<double>(3 * n1);
     w.m64 = c.take<double>(n1);
+    w.state = c.take<uint8_t>(n1);
     w.src4 = c.take<float4>(n1);
+    w.rem4 = c.take<float4>(n1);
     w.cidx = c.take<int32_t>(n1);
     w.mhalo = c.take<int32_t>(n1);
-    w.dyn_len = c.take<int32_t>(h1);
-    w.wlen = c.take<int32_t>(h1);
+    w.sacc = c.take<double>(n1);
+    w.eacc = c.take<double>(n1);
+    w.n_bound = c.take<int32_t>(h1);
+    w.n_rem = c.take<int32_t>(h1);
+    w.mode = c.take<int32_t>(h1);
+    w.tst_len = c.take<int32_t>(h1);
+    w.src_len = c.take<int32_t>(h1);
+    w.src_sel = c.take<int32_t>(h1);
     w.active = c.take<int32_t>(h1);
     w.iters = c.take<int32_t>(h1);
     w.removed = c.take<int32_t>(h1);
+    w.pass_no = c.take<int32_t>(64);
     w.vcm = c.take<double>(3 * h1);
     w.band = c.take<int32_t>(n1);
     w.counters = c.take<unsigned long long>(8);
@@ -366,19 +454,25 @@ struct PassLaunch {
 // are recorded on the stream between the parts, around the pair kernel alone.
 int launch_pre(const PassLaunch& L, hipStream_t st) {
     const UnbWs& w = L.w;
-    k_compact<<<L.grid_halo, kThreads, 0, st>>>(w.off, static_cast<int>(L.H), w.active, L.bound,
-                                                w.p4, w.vel, w.m64, w.src4, w.cidx, w.dyn_len,
-                                                w.vcm);
+    k_compact<<<L.grid_halo, kThreads, 0, st>>>(w.off, static_cast<int>(L.H), w.active, w.pass_no,
+                                                w.state, w.p4, w.vel, w.m64, w.src4, w.cidx,
+                                                w.rem4, w.n_bound, w.n_rem, w.mode, w.vcm);
     HALMA_LAUNCH_CHECK();
-    k_work_len<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.active, w.dyn_len,
-                                                     w.wlen);
+    k_live_sets<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.active, w.n_bound,
+                                                      w.n_rem, w.mode, w.tst_len, w.src_len,
+                                                      w.src_sel);
     HALMA_LAUNCH_CHECK();
     return 0;
 }
 
 int launch_pot(const PassLaunch& L, hipStream_t st) {
     const UnbWs& w = L.w;
-    return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, w.wlen, w.items, w.part, w.head, st);
+    DynSets dyn;
+    dyn.tst_len = w.tst_len;
+    dyn.src_len = w.src_len;
+    dyn.src_sel = w.src_sel;
+    dyn.src_alt = w.rem4;
+    return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, dyn, w.items, w.part, w.head, st);
 }
 
 int launch_post(const PassLaunch& L, hipStream_t st) {
@@ -389,8 +483,9 @@ int launch_post(const PassLaunch& L, hipStream_t st) {
         k_retest<<<L.grid_retest, kThreads, 0, st>>>(L.ea, w.src4);
         HALMA_LAUNCH_CHECK();
     }
-    k_finish<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.dyn_len, w.active,
-                                                   w.iters, w.removed, w.counters);
+    k_finish<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.n_bound, w.src_len,
+                                                   w.active, w.iters, w.removed, w.pass_no,
+                                                   w.counters);
     HALMA_LAUNCH_CHECK();
     return 0;
 }
@@ -435,7 +530,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     if (rc) return rc;
     const int32_t* mem = members + h_off[0];
     if (nm > 0) {
-        k_gather<<<blocks_for(nm), kThreads, 0, st>>>(in, mem, nm, w.p4, w.vel, w.m64, bound_out);
+        k_gather<<<blocks_for(nm), kThreads, 0, st>>>(in, mem, nm, w.p4, w.vel, w.m64, w.state);
         HALMA_LAUNCH_CHECK();
         k_member_halo<<<static_cast<unsigned>(H), kThreads, 0, st>>>(w.off, static_cast<int>(H),
                                                                      w.mhalo);
@@ -445,6 +540,8 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
                                                     w.removed);
     HALMA_LAUNCH_CHECK();
     k_zero_counters<<<1, 64, 0, st>>>(w.counters, 8);
+    HALMA_LAUNCH_CHECK();
+    k_zero_counters<<<1, 64, 0, st>>>(reinterpret_cast<unsigned long long*>(w.pass_no), 1);
     HALMA_LAUNCH_CHECK();
 
     double G = 4.3 * 1e-3;
@@ -460,14 +557,18 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.bound = bound_out;
     L.ea.halos = w.halos;
     L.ea.mhalo = w.mhalo;
-    L.ea.dyn_len = w.dyn_len;
+    L.ea.n_bound = w.n_bound;
+    L.ea.n_src = w.src_len;
+    L.ea.mode = w.mode;
     L.ea.cidx = w.cidx;
     L.ea.part = w.part;
     L.ea.vel = w.vel;
     L.ea.vcm = w.vcm;
     L.ea.cg = factor * G;
     L.ea.bound_rel = pot_error_bound(plan.subtile);
-    L.ea.bound = bound_out;
+    L.ea.state = w.state;
+    L.ea.sacc = w.sacc;
+    L.ea.eacc = w.eacc;
     L.ea.removed = w.removed;
     L.ea.band = w.band;
     L.ea.counters = w.counters;
@@ -564,6 +665,10 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     }
     cleanup();
     if (status) return status;
+    if (nm > 0) {
+        k_final_mask<<<blocks_for(nm), kThreads, 0, st>>>(nm, w.state, bound_out);
+        HALMA_LAUNCH_CHECK();
+    }
     if (iters_out) HALMA_HIP(hipMemcpyAsync(iters_out, w.iters, H * sizeof(int32_t),
                                             hipMemcpyDeviceToDevice, st));
     if (vcm_out) HALMA_HIP(hipMemcpyAsync(vcm_out, w.vcm, 3 * H * sizeof(double),
