@@ -7,8 +7,9 @@ mean interparticle spacing, groups of >= 50 members, unbinding in physical mode
 (E = K + U, factor 1) iterated to convergence.  Synthetic data generated from a seed;
 inputs are resident in HBM before the timed region.
 
-One step = friends-of-friends labels -> groups -> group reductions (mass, centre of mass,
-CM velocity, furthest particle) -> density peaks -> batched unbinding.  The plain FP64
+One step = friends-of-friends labels -> groups -> one gather of the members' data into
+group order -> group reductions (mass, centre of mass, CM velocity, furthest particle)
+-> density peaks -> batched unbinding.  The plain FP64
 comparator (same method: one launch per halo, FP64 sums, no batching, no graphs) runs
 once after the timed steps; the CPU baseline runs the oracle's FoF port and the
 reference's own FP32 pair routine (compiled from its Fortran source, OpenMP on all host
@@ -40,6 +41,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparator")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 256 MiB device copy first (scales profiler byte counters)")
     ap.add_argument("--cpu-frac", type=float, default=0.5,
                     help="fraction of the box (along x) used for the CPU baseline sample")
     return ap.parse_args()
@@ -71,15 +74,23 @@ def main():
         f"(generated in {time.time() - t0:.1f}s)")
     D = {k: torch.from_numpy(w[k]).to(dev) for k in ("x", "y", "z", "vx", "vy", "vz", "mass")}
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def step(time_pot=False):
         lab = engine.fof_labels(D["x"], D["y"], D["z"], ll)
         groups = engine.groups_from_labels(lab, MINP)
-        red = engine.group_reduce(groups, D["x"], D["y"], D["z"], D["mass"], D["vx"], D["vy"],
-                                  D["vz"])
-        peak = engine.density_peak(groups, D["x"], D["y"], D["z"], D["mass"], ll, reduced=red)
-        ub = engine.unbind(groups, D["x"], D["y"], D["z"], D["vx"], D["vy"], D["vz"],
-                           D["mass"], factor=1.0, time_pot=time_pot)
+        md = engine.gather_members(groups, D["x"], D["y"], D["z"], D["mass"], D["vx"], D["vy"],
+                                   D["vz"])
+        g = md.groups
+        red = engine.group_reduce(g, md.x, md.y, md.z, md.m, md.vx, md.vy, md.vz)
+        peak = engine.density_peak(g, md.x, md.y, md.z, md.m, ll, reduced=red)
+        ub = engine.unbind(g, md.x, md.y, md.z, md.vx, md.vy, md.vz, md.m, factor=1.0,
+                           time_pot=time_pot)
         return lab, groups, red, peak, ub
 
     for i in range(args.warmup):

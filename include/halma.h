@@ -140,6 +140,21 @@ int halma_group_reduce(int64_t n_groups, const int64_t *offsets, const int32_t *
                        const void *m, int mass_dtype, const double *centres, double *out,
                        void *stream);
 
+/* Member-ordered copies of the particle arrays: out_*[k] = in_*[members[k]] for the
+ * n_members members (x, y, z, m, and vx, vy, vz when given), in out_dtype.  Particles
+ * are first packed into 32/64-byte records so the gather costs one memory line per
+ * member instead of one per value.  Reductions, density peaks and unbinding can then run
+ * on the copies with members = 0..n_members-1.  The workspace holds n_particles records.
+ * Stream-ordered. */
+size_t halma_gather_workspace_size(int64_t n_particles, int dtype);
+int halma_gather_members(const int32_t *members, int64_t n_members, int64_t n_particles,
+                         const void *x, const void *y, const void *z, int pos_dtype,
+                         const void *vx, const void *vy, const void *vz, int vel_dtype,
+                         const void *m, int mass_dtype, int out_dtype, void *out_x,
+                         void *out_y, void *out_z, void *out_m, void *out_vx, void *out_vy,
+                         void *out_vz, void *workspace, size_t workspace_bytes,
+                         void *stream);
+
 /* density_peak (halo_properties.py:518-556) for every group: node coordinates (3 doubles
  * per group) of the heaviest cell of the 50^3 nearest-node grid spanning
  * [min - ll, max + ll] per axis.  `reduced` is halma_group_reduce's output for the same

@@ -44,6 +44,10 @@ SIGNATURES = {
                                             _p]),
     "halma_group_reduce": (_c.c_int, [_i64, _p, _p, _p, _p, _p, _c.c_int, _p, _p, _p, _c.c_int,
                                       _p, _c.c_int, _p, _p, _p]),
+    "halma_gather_workspace_size": (_size, [_i64, _c.c_int]),
+    "halma_gather_members": (_c.c_int, [_p, _i64, _i64, _p, _p, _p, _c.c_int, _p, _p, _p,
+                                        _c.c_int, _p, _c.c_int, _c.c_int, _p, _p, _p, _p, _p,
+                                        _p, _p, _p, _size, _p]),
     "halma_density_peak_workspace_size": (_size, [_i64]),
     "halma_density_peak": (_c.c_int, [_i64, _p, _p, _p, _p, _p, _c.c_int, _p, _c.c_int, _dbl,
                                       _p, _p, _p, _size, _p]),

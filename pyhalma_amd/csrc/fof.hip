@@ -33,6 +33,21 @@ constexpr int kCellBits = 21;
 constexpr double kCellLimit = 2097152.0;  // 2^21 cells per axis
 constexpr int kBoxBlocks = 1024;
 
+// Table slot of a cell key: the (x, y) column is hashed and z is added, so the cells of
+// one column sit in consecutive slots and the 5 z-neighbours of a stencil column are one
+// or two memory lines apart.
+__device__ __forceinline__ uint64_t mix64(uint64_t k);
+__device__ __forceinline__ uint64_t slot_of(uint64_t key, uint64_t mask) {
+    return (mix64(key >> kCellBits) + (key & ((1ull << kCellBits) - 1))) & mask;
+}
+
+// 16-byte table entry: cell key (kEmpty when free), particle count, cell number
+struct __attribute__((aligned(16))) Entry {
+    uint64_t key;
+    int32_t cnt;
+    int32_t cell;
+};
+
 __device__ __forceinline__ uint64_t mix64(uint64_t k) {
     k ^= k >> 30;
     k *= 0xbf58476d1ce4e5b9ull;
@@ -141,9 +156,13 @@ __global__ __launch_bounds__(256) void k_bbox_final(const double* __restrict__ p
     }
 }
 
-__global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t v) {
+__global__ void k_clear_table(Entry* t, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    if (i < n) {
+        t[i].key = kEmpty;
+        t[i].cnt = 0;
+        t[i].cell = -1;
+    }
 }
 
 __global__ void k_fill_i32n(int* p, int64_t n, int v) {
@@ -155,8 +174,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_fof_bin(const T* __restrict__ x, const T* __restrict__ y,
                                                  const T* __restrict__ z, int64_t n,
                                                  const double* __restrict__ box, double inv_s,
-                                                 uint64_t* __restrict__ tkey,
-                                                 int* __restrict__ tcnt, uint64_t mask,
+                                                 Entry* __restrict__ tab, uint64_t mask,
                                                  int* __restrict__ pslot, int* __restrict__ prank,
                                                  int* __restrict__ err) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -176,13 +194,13 @@ __global__ __launch_bounds__(256) void k_fof_bin(const T* __restrict__ x, const 
         atomicOr(err, 1);
         key = 0;
     }
-    uint64_t h = mix64(key) & mask;
+    uint64_t h = slot_of(key, mask);
     for (uint64_t probe = 0;; ++probe) {
-        const uint64_t k = __hip_atomic_load(tkey + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t k = __hip_atomic_load(&tab[h].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (k == key) break;
         if (k == kEmpty) {
             uint64_t expected = kEmpty;
-            if (__hip_atomic_compare_exchange_strong(tkey + h, &expected, key, __ATOMIC_RELAXED,
+            if (__hip_atomic_compare_exchange_strong(&tab[h].key, &expected, key, __ATOMIC_RELAXED,
                                                      __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_AGENT) ||
                 expected == key)
@@ -197,32 +215,32 @@ __global__ __launch_bounds__(256) void k_fof_bin(const T* __restrict__ x, const 
         h = (h + 1) & mask;
     }
     pslot[i] = static_cast<int>(h);
-    prank[i] = atomicAdd(tcnt + h, 1);
+    prank[i] = atomicAdd(&tab[h].cnt, 1);
 }
 
 // packed (count << 32 | occupied) per table slot for one scan that yields both the
 // particle offset (high word) and the cell number (low word) of every occupied slot
-__global__ void k_slot_pack(const int* __restrict__ tcnt, int64_t cap, int64_t* __restrict__ v) {
+__global__ void k_slot_pack(const Entry* __restrict__ tab, int64_t cap, int64_t* __restrict__ v) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= cap) return;
-    const int c = tcnt[i];
+    const int c = tab[i].cnt;
     v[i] = (static_cast<int64_t>(c) << 32) | (c > 0 ? 1 : 0);
 }
 
-__global__ void k_cells(const int* __restrict__ tcnt, const uint64_t* __restrict__ tkey,
-                        const int64_t* __restrict__ scan, int64_t cap,
+__global__ void k_cells(Entry* __restrict__ tab, const int64_t* __restrict__ scan, int64_t cap,
                         int* __restrict__ cell_start, int* __restrict__ cell_cnt,
                         uint64_t* __restrict__ cell_key, int* __restrict__ parent,
                         int* __restrict__ comp_min) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= cap) return;
-    const int c = tcnt[i];
+    const int c = tab[i].cnt;
     if (c <= 0) return;
     const int64_t v = scan[i];
     const int cell = static_cast<int>(v & 0xffffffff);
+    tab[i].cell = cell;
     cell_start[cell] = static_cast<int>(v >> 32);
     cell_cnt[cell] = c;
-    cell_key[cell] = tkey[i];
+    cell_key[cell] = tab[i].key;
     parent[cell] = cell;
     comp_min[cell] = INT32_MAX;
 }
@@ -244,12 +262,13 @@ __global__ void k_scatter(const T* __restrict__ x, const T* __restrict__ y,
     pcell[i] = static_cast<int>(v & 0xffffffff);
 }
 
-__device__ __forceinline__ int table_find(const uint64_t* __restrict__ tkey, uint64_t mask,
+// cell number of a key, or -1
+__device__ __forceinline__ int table_find(const Entry* __restrict__ tab, uint64_t mask,
                                           uint64_t key) {
-    uint64_t h = mix64(key) & mask;
+    uint64_t h = slot_of(key, mask);
     for (uint64_t probe = 0; probe <= mask; ++probe) {
-        const uint64_t k = tkey[h];
-        if (k == key) return static_cast<int>(h);
+        const uint64_t k = tab[h].key;
+        if (k == key) return tab[h].cell;
         if (k == kEmpty) return -1;
         h = (h + 1) & mask;
     }
@@ -258,7 +277,7 @@ __device__ __forceinline__ int table_find(const uint64_t* __restrict__ tkey, uin
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_fof_link(
-    const uint64_t* __restrict__ tkey, uint64_t mask, const int64_t* __restrict__ scan,
+    const Entry* __restrict__ tab, uint64_t mask,
     const int* __restrict__ cell_start, const int* __restrict__ cell_cnt,
     const uint64_t* __restrict__ cell_key, const T* __restrict__ sx, const T* __restrict__ sy,
     const T* __restrict__ sz, const int64_t* __restrict__ totals, double ll2,
@@ -286,8 +305,7 @@ __global__ __launch_bounds__(256) void k_fof_link(
                 const uint64_t nkey = (static_cast<uint64_t>(nx) << (2 * kCellBits)) |
                                       (static_cast<uint64_t>(ny) << kCellBits) |
                                       static_cast<uint64_t>(nz);
-                const int slot = table_find(tkey, mask, nkey);
-                if (slot >= 0) dcell = static_cast<int>(scan[slot] & 0xffffffff);
+                dcell = table_find(tab, mask, nkey);
             }
         }
         uint64_t found = __ballot(dcell >= 0);
@@ -387,8 +405,7 @@ __global__ void k_labels(int64_t n, const int* __restrict__ pcell, const int* __
 struct FofWs {
     double* box_part;
     double* box;
-    uint64_t* tkey;
-    int* tcnt;
+    Entry* tab;
     int64_t* scan;
     int64_t* totals;
     void* scan_ws;
@@ -423,8 +440,7 @@ FofWs carve_fof(void* base, int64_t n, size_t elt) {
     w.cap = table_capacity(n);
     w.box_part = c.take<double>(kBoxBlocks * 6);
     w.box = c.take<double>(8);
-    w.tkey = c.take<uint64_t>(w.cap);
-    w.tcnt = c.take<int>(w.cap);
+    w.tab = c.take<Entry>(w.cap);
     w.scan = c.take<int64_t>(w.cap);
     w.totals = c.take<int64_t>(4);
     w.scan_ws = c.take<char>(scan_workspace_bytes(w.cap));
@@ -460,30 +476,28 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     HALMA_LAUNCH_CHECK();
     k_bbox_final<<<1, 256, 0, st>>>(w.box_part, kBoxBlocks, w.box);
     HALMA_LAUNCH_CHECK();
-    k_fill_u64<<<blocks_for(w.cap), 256, 0, st>>>(w.tkey, w.cap, kEmpty);
-    HALMA_LAUNCH_CHECK();
-    k_fill_i32n<<<blocks_for(w.cap), 256, 0, st>>>(w.tcnt, w.cap, 0);
+    k_clear_table<<<blocks_for(w.cap), 256, 0, st>>>(w.tab, w.cap);
     HALMA_LAUNCH_CHECK();
     // sub-cell side: ll/sqrt(3) with a 1e-6 relative margin (see the file header)
     const double side = ll / sqrt(3.0) * (1.0 - 1e-6);
     const double inv_s = 1.0 / side;
-    k_fof_bin<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.box, inv_s, w.tkey, w.tcnt,
+    k_fof_bin<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.box, inv_s, w.tab,
                                                 static_cast<uint64_t>(w.cap - 1), w.pslot,
                                                 w.prank, w.err);
     HALMA_LAUNCH_CHECK();
-    k_slot_pack<<<blocks_for(w.cap), 256, 0, st>>>(w.tcnt, w.cap, w.scan);
+    k_slot_pack<<<blocks_for(w.cap), 256, 0, st>>>(w.tab, w.cap, w.scan);
     HALMA_LAUNCH_CHECK();
     int rc = exclusive_scan_i64(w.scan, w.scan, w.cap, w.totals, w.scan_ws, st);
     if (rc) return rc;
-    k_cells<<<blocks_for(w.cap), 256, 0, st>>>(w.tcnt, w.tkey, w.scan, w.cap, w.cell_start,
-                                               w.cell_cnt, w.cell_key, w.parent, w.comp_min);
+    k_cells<<<blocks_for(w.cap), 256, 0, st>>>(w.tab, w.scan, w.cap, w.cell_start, w.cell_cnt,
+                                               w.cell_key, w.parent, w.comp_min);
     HALMA_LAUNCH_CHECK();
     k_scatter<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.pslot, w.prank, w.scan, w.sidx,
                                                 static_cast<T*>(w.sx), static_cast<T*>(w.sy),
                                                 static_cast<T*>(w.sz), w.pcell);
     HALMA_LAUNCH_CHECK();
     k_fof_link<T><<<grid_cells, 256, 0, st>>>(
-        w.tkey, static_cast<uint64_t>(w.cap - 1), w.scan, w.cell_start, w.cell_cnt, w.cell_key,
+        w.tab, static_cast<uint64_t>(w.cap - 1), w.cell_start, w.cell_cnt, w.cell_key,
         static_cast<const T*>(w.sx), static_cast<const T*>(w.sy), static_cast<const T*>(w.sz),
         w.totals, ll * ll, w.parent);
     HALMA_LAUNCH_CHECK();

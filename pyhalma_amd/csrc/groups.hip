@@ -554,6 +554,48 @@ __global__ __launch_bounds__(kThreads) void k_dense_pick(PeakArgs a, int first) 
     if (threadIdx.x == 0) write_peak(a, gr, g, node);
 }
 
+// ------------------------------------------------------------- member gather ----
+// Particle data arrive as separate arrays in particle order, and group members are
+// particle indices in no spatial order, so gathering seven arrays member by member costs
+// one memory line per value.  Packing each particle into one 32-byte (FP32) or 64-byte
+// (FP64) record first makes the gather one line per member; the member-ordered arrays
+// it writes are then read contiguously by every per-group kernel.
+template <typename T>
+__global__ void k_pack_records(const void* x, const void* y, const void* z, int pos_t,
+                               const void* vx, const void* vy, const void* vz, int vel_t,
+                               const void* m, int mass_t, int64_t n, T* __restrict__ rec) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T* r = rec + 8 * i;
+    r[0] = static_cast<T>(ldf(x, pos_t, i));
+    r[1] = static_cast<T>(ldf(y, pos_t, i));
+    r[2] = static_cast<T>(ldf(z, pos_t, i));
+    r[3] = static_cast<T>(ldf(m, mass_t, i));
+    r[4] = vx ? static_cast<T>(ldf(vx, vel_t, i)) : T(0);
+    r[5] = vx ? static_cast<T>(ldf(vy, vel_t, i)) : T(0);
+    r[6] = vx ? static_cast<T>(ldf(vz, vel_t, i)) : T(0);
+    r[7] = T(0);
+}
+
+template <typename T>
+__global__ void k_gather_records(const T* __restrict__ rec, const int32_t* __restrict__ mem,
+                                 int64_t nm, T* __restrict__ ox, T* __restrict__ oy,
+                                 T* __restrict__ oz, T* __restrict__ om, T* __restrict__ ovx,
+                                 T* __restrict__ ovy, T* __restrict__ ovz) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    const T* r = rec + 8 * static_cast<int64_t>(mem[k]);
+    ox[k] = r[0];
+    oy[k] = r[1];
+    oz[k] = r[2];
+    om[k] = r[3];
+    if (ovx) {
+        ovx[k] = r[4];
+        ovy[k] = r[5];
+        ovz[k] = r[6];
+    }
+}
+
 }  // namespace
 
 size_t groups_csr_workspace_bytes(int64_t n) { return carve_csr(nullptr, n).bytes; }
@@ -735,6 +777,53 @@ int halma_density_peak(int64_t n_groups, const int64_t* offsets, const int32_t* 
     if (h[0] & 8)
         return set_error(HALMA_E_LIMIT, "density peak: more than %d near-tied nodes",
                          kMaxCandidates);
+    return 0;
+}
+
+size_t halma_gather_workspace_size(int64_t n_particles, int dtype) {
+    if (n_particles < 0) return 0;
+    const size_t elt = dtype == HALMA_F64 ? sizeof(double) : sizeof(float);
+    return align256(static_cast<size_t>(n_particles > 0 ? n_particles : 1) * 8 * elt);
+}
+
+int halma_gather_members(const int32_t* members, int64_t n_members, int64_t n_particles,
+                         const void* x, const void* y, const void* z, int pos_dtype,
+                         const void* vx, const void* vy, const void* vz, int vel_dtype,
+                         const void* m, int mass_dtype, int out_dtype, void* ox, void* oy,
+                         void* oz, void* om, void* ovx, void* ovy, void* ovz, void* ws,
+                         size_t ws_bytes, void* stream) {
+    if (n_members < 0 || n_particles < 0) return set_error(HALMA_E_ARG, "negative size");
+    if (n_members == 0) return 0;
+    if (!members || !x || !y || !z || !m || !ox || !oy || !oz || !om)
+        return set_error(HALMA_E_ARG, "null pointer");
+    if (bad_dtype(pos_dtype) || bad_dtype(mass_dtype) || bad_dtype(out_dtype) ||
+        (vx && bad_dtype(vel_dtype)))
+        return set_error(HALMA_E_ARG, "bad dtype");
+    if (vx && (!vy || !vz || !ovx || !ovy || !ovz))
+        return set_error(HALMA_E_ARG, "velocities need all three input and output arrays");
+    if (ws_bytes < halma_gather_workspace_size(n_particles, out_dtype))
+        return set_error(HALMA_E_WORKSPACE, "workspace too small");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (out_dtype == HALMA_F64) {
+        double* rec = static_cast<double*>(ws);
+        k_pack_records<double><<<blocks_for(n_particles), kThreads, 0, st>>>(
+            x, y, z, pos_dtype, vx, vy, vz, vel_dtype, m, mass_dtype, n_particles, rec);
+        HALMA_LAUNCH_CHECK();
+        k_gather_records<double><<<blocks_for(n_members), kThreads, 0, st>>>(
+            rec, members, n_members, static_cast<double*>(ox), static_cast<double*>(oy),
+            static_cast<double*>(oz), static_cast<double*>(om), static_cast<double*>(vx ? ovx : nullptr),
+            static_cast<double*>(ovy), static_cast<double*>(ovz));
+    } else {
+        float* rec = static_cast<float*>(ws);
+        k_pack_records<float><<<blocks_for(n_particles), kThreads, 0, st>>>(
+            x, y, z, pos_dtype, vx, vy, vz, vel_dtype, m, mass_dtype, n_particles, rec);
+        HALMA_LAUNCH_CHECK();
+        k_gather_records<float><<<blocks_for(n_members), kThreads, 0, st>>>(
+            rec, members, n_members, static_cast<float*>(ox), static_cast<float*>(oy),
+            static_cast<float*>(oz), static_cast<float*>(om), static_cast<float*>(vx ? ovx : nullptr),
+            static_cast<float*>(ovy), static_cast<float*>(ovz));
+    }
+    HALMA_LAUNCH_CHECK();
     return 0;
 }
 

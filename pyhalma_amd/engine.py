@@ -186,6 +186,50 @@ def _vec(a):
     return float_tensor(a)
 
 
+class MemberData:
+    """Member-ordered copies of the particle arrays of a Groups object, with `groups`
+    rewritten to index them (members = 0..n_members-1, same offsets)."""
+
+    def __init__(self, groups, x, y, z, m, vx, vy, vz):
+        self.groups = groups
+        self.x, self.y, self.z, self.m = x, y, z, m
+        self.vx, self.vy, self.vz = vx, vy, vz
+
+
+def gather_members(groups, x, y, z, m, vx=None, vy=None, vz=None):
+    """One packed-record gather of the members' data (halma_gather_members)."""
+    from .device import dtype_flag
+    L = _lib_()
+    x, y, z = _coords(x, y, z)
+    m = _vec(m)
+    if vx is not None:
+        vx, vy, vz = _coords(vx, vy, vz)
+    flags = [x.dtype, m.dtype] + ([vx.dtype] if vx is not None else [])
+    odt = torch.float32 if all(f == torch.float32 for f in flags) else torch.float64
+    nm = groups.n_members
+    n = int(x.numel())
+    dev = x.device
+    outs = [torch.empty(max(nm, 1), dtype=odt, device=dev) for _ in range(7 if vx is not None else 4)]
+    oflag = _lib.F32 if odt == torch.float32 else _lib.F64
+    if nm > 0:
+        ws = workspace(L.halma_gather_workspace_size(n, oflag))
+        v = (ptr(vx), ptr(vy), ptr(vz)) if vx is not None else (None, None, None)
+        ov = (ptr(outs[4]), ptr(outs[5]), ptr(outs[6])) if vx is not None else (None, None, None)
+        rc = L.halma_gather_members(ptr(groups.members), nm, n, ptr(x), ptr(y), ptr(z),
+                                    dtype_flag(x), *v,
+                                    dtype_flag(vx) if vx is not None else _lib.F64,
+                                    ptr(m), dtype_flag(m), oflag, ptr(outs[0]), ptr(outs[1]),
+                                    ptr(outs[2]), ptr(outs[3]), *ov, ptr(ws), ws.numel(),
+                                    stream_ptr())
+        _lib.check(rc, "halma_gather_members")
+    local = Groups(torch.arange(max(nm, 1), dtype=torch.int32, device=dev), groups.offsets,
+                   groups.offsets_host)
+    outs = [o[:nm] for o in outs]
+    if vx is None:
+        outs += [None, None, None]
+    return MemberData(local, outs[0], outs[1], outs[2], outs[3], outs[4], outs[5], outs[6])
+
+
 def group_reduce(groups, x, y, z, m, vx=None, vy=None, vz=None, centres=None):
     """Per-group reductions, float64 (G, 16): [M, cx, cy, cz, vx, vy, vz, rmax,
     xmin, ymin, zmin, xmax, ymax, zmax, sum|m|, exact-sum flag]."""

@@ -128,3 +128,32 @@ def test_dropin_reductions(gpu):
     assert hp.center_of_mass([], x, y, z, m) == (0.0, 0.0, 0.0, 0.0)
     with pytest.raises(ValueError):
         hp.density_peak([], x, y, z, m, 0.1)
+
+
+def test_gather_members_then_reduce(gpu):
+    """Member-ordered copies give the same reductions, peaks and unbinding as the
+    particle-ordered arrays."""
+    from pyhalma_amd import engine
+
+    w = _clumps(0.01, seed=5)
+    lab = oracle.fof_labels_c(w["x"], w["y"], w["z"], w["ll"])
+    g = engine.groups_from_labels(lab.astype(np.int32), 20)
+    md = engine.gather_members(g, w["x"], w["y"], w["z"], w["mass"], w["vx"], w["vy"], w["vz"])
+    mem = g.members[: g.n_members].cpu().numpy()
+    assert np.array_equal(md.x.cpu().numpy(), w["x"][mem])
+    assert np.array_equal(md.vz.cpu().numpy(), w["vz"][mem])
+    a = engine.group_reduce(g, w["x"], w["y"], w["z"], w["mass"], w["vx"], w["vy"], w["vz"])
+    b = engine.group_reduce(md.groups, md.x, md.y, md.z, md.m, md.vx, md.vy, md.vz)
+    assert torch_equal(a, b)
+    pa = engine.density_peak(g, w["x"], w["y"], w["z"], w["mass"], w["ll"])
+    pb = engine.density_peak(md.groups, md.x, md.y, md.z, md.m, w["ll"])
+    assert torch_equal(pa, pb)
+    ua = engine.unbind(g, w["x"], w["y"], w["z"], w["vx"], w["vy"], w["vz"], w["mass"])
+    ub = engine.unbind(md.groups, md.x, md.y, md.z, md.vx, md.vy, md.vz, md.m)
+    assert torch_equal(ua.bound, ub.bound)
+
+
+def torch_equal(a, b):
+    import torch
+
+    return bool(torch.equal(a, b))
