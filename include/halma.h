@@ -104,7 +104,8 @@ int halma_pot_sum_f64_plain(const double *m, const double *x, const double *y,
  * labels[i] = smallest particle index of i's connected component in the graph with an
  * edge wherever ((dx*dx + dy*dy) + dz*dz) <= ll*ll in FP64 (FP32 coordinates are
  * widened exactly), non-periodic.  Singletons label themselves.  Coordinates may span
- * at most 2^21 sub-cells of side ll/sqrt(3) per axis (HALMA_E_LIMIT otherwise).
+ * at most 2^21 sub-cells of side ll/sqrt(3) per axis and n <= 2^30 (HALMA_E_LIMIT
+ * otherwise).
  * Synchronises the stream (it checks the kernels' error flag before returning).
  * ---------------------------------------------------------------------------------- */
 size_t halma_fof_workspace_size(int64_t n, int dtype);

@@ -10,6 +10,7 @@ raised.
 import ctypes
 import os
 
+import numpy as np
 import torch  # noqa: F401  (loads the HIP runtime libhalma binds to)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -106,5 +107,7 @@ def check(rc, what=""):
 
 
 def i64_array(values):
-    arr = (ctypes.c_int64 * len(values))(*[int(v) for v in values])
-    return arr
+    """A host int64 array for an ``int64_t*`` argument (no copy when ``values`` is
+    already a contiguous int64 numpy array; the pointer keeps the array alive)."""
+    arr = np.ascontiguousarray(values, dtype=np.int64)
+    return arr.ctypes.data_as(_i64p)

@@ -8,7 +8,8 @@
 //    cell index), so a sub-cell is one vertex of the cell graph and needs no pair test;
 //    every link spans at most 2 sub-cells per axis (ll/s < 1.733);
 // 3. occupied sub-cells go into an open-addressing hash table (64-bit keys, vector
-//    atomics); counts and per-particle ranks come from the same pass;
+//    atomics); counts, per-particle ranks and each cell's minimum particle index come
+//    from the same pass, aggregated per block in LDS;
 // 4. an exclusive scan of the table numbers the cells and gives their particle ranges;
 //    particles are scattered into cell order with their coordinates;
 // 5. one wavefront per cell looks up its 62 half-stencil neighbours (one lane each) and,
@@ -161,7 +162,7 @@ __global__ void k_clear_table(Entry* t, int64_t n) {
     if (i < n) {
         t[i].key = kEmpty;
         t[i].cnt = 0;
-        t[i].cell = -1;
+        t[i].cell = INT32_MAX;
     }
 }
 
@@ -170,53 +171,106 @@ __global__ void k_fill_i32n(int* p, int64_t n, int v) {
     if (i < n) p[i] = v;
 }
 
+// Binning, aggregated per block: a block takes kBinChunk particles, finds (or inserts)
+// each particle's cell in the global table, and counts its particles per cell in an LDS
+// table first; one global atomic per distinct cell and block then reserves the block's
+// ranks (and folds the block's minimum particle index into the entry's `cell` field,
+// which holds the cell minimum until k_cells numbers the cells).  Dense haloes put 10^4
+// particles into one sub-cell, and one global atomic per particle on that entry would
+// serialise them all.
+constexpr int kBinThreads = 256;
+constexpr int kBinPerThread = 8;
+constexpr int kBinChunk = kBinThreads * kBinPerThread;
+constexpr int kBinLds = 2 * kBinChunk;  // LDS slots, load factor <= 1/2
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_fof_bin(const T* __restrict__ x, const T* __restrict__ y,
-                                                 const T* __restrict__ z, int64_t n,
-                                                 const double* __restrict__ box, double inv_s,
-                                                 Entry* __restrict__ tab, uint64_t mask,
-                                                 int* __restrict__ pslot, int* __restrict__ prank,
-                                                 int* __restrict__ err) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double q[3] = {(static_cast<double>(x[i]) - box[0]) * inv_s,
-                         (static_cast<double>(y[i]) - box[1]) * inv_s,
-                         (static_cast<double>(z[i]) - box[2]) * inv_s};
-    uint64_t key = 0;
-    bool bad = false;
+__global__ __launch_bounds__(kBinThreads) void k_fof_bin(
+    const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, int64_t n,
+    const double* __restrict__ box, double inv_s, Entry* __restrict__ tab, uint64_t mask,
+    int* __restrict__ pslot, int* __restrict__ prank, int* __restrict__ err) {
+    __shared__ int l_slot[kBinLds];
+    __shared__ int l_cnt[kBinLds];
+    __shared__ int l_min[kBinLds];
+    for (int k = threadIdx.x; k < kBinLds; k += kBinThreads) {
+        l_slot[k] = -1;
+        l_cnt[k] = 0;
+        l_min[k] = INT32_MAX;
+    }
+    __syncthreads();
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kBinChunk + threadIdx.x;
+    int loc[kBinPerThread], rk[kBinPerThread];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        bad |= !(q[d] >= 0.0 && q[d] < kCellLimit);  // also catches NaN
-        const uint64_t c = bad ? 0 : static_cast<uint64_t>(floor(q[d]));
-        key = (key << kCellBits) | c;
-    }
-    if (bad) {
-        atomicOr(err, 1);
-        key = 0;
-    }
-    uint64_t h = slot_of(key, mask);
-    for (uint64_t probe = 0;; ++probe) {
-        const uint64_t k = __hip_atomic_load(&tab[h].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (k == key) break;
-        if (k == kEmpty) {
-            uint64_t expected = kEmpty;
-            if (__hip_atomic_compare_exchange_strong(&tab[h].key, &expected, key, __ATOMIC_RELAXED,
-                                                     __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT) ||
-                expected == key)
-                break;
+    for (int j = 0; j < kBinPerThread; ++j) {
+        loc[j] = -1;
+        rk[j] = 0;
+        const int64_t i = base + j * kBinThreads;
+        if (i >= n) continue;
+        const double q[3] = {(static_cast<double>(x[i]) - box[0]) * inv_s,
+                             (static_cast<double>(y[i]) - box[1]) * inv_s,
+                             (static_cast<double>(z[i]) - box[2]) * inv_s};
+        uint64_t key = 0;
+        bool bad = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            bad |= !(q[d] >= 0.0 && q[d] < kCellLimit);  // also catches NaN
+            const uint64_t c = bad ? 0 : static_cast<uint64_t>(floor(q[d]));
+            key = (key << kCellBits) | c;
         }
-        if (probe > mask) {  // table full (cannot happen with capacity >= 2n)
+        if (bad) {
+            atomicOr(err, 1);
+            key = 0;
+        }
+        uint64_t h = slot_of(key, mask);
+        bool full = false;
+        for (uint64_t probe = 0;; ++probe) {
+            const uint64_t k = __hip_atomic_load(&tab[h].key, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            if (k == key) break;
+            if (k == kEmpty) {
+                uint64_t expected = kEmpty;
+                if (__hip_atomic_compare_exchange_strong(&tab[h].key, &expected, key,
+                                                         __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT) ||
+                    expected == key)
+                    break;
+            }
+            if (probe > mask) {  // table full (cannot happen with capacity >= 2n)
+                full = true;
+                break;
+            }
+            h = (h + 1) & mask;
+        }
+        if (full) {
             atomicOr(err, 2);
             pslot[i] = 0;
             prank[i] = 0;
-            return;
+            continue;
         }
-        h = (h + 1) & mask;
+        pslot[i] = static_cast<int>(h);
+        unsigned l = (static_cast<unsigned>(h) * 0x9e3779b1u) >> 20;  // 12 bits
+        for (;;) {
+            const int old = atomicCAS(&l_slot[l], -1, static_cast<int>(h));
+            if (old == -1 || old == static_cast<int>(h)) break;
+            l = (l + 1) & (kBinLds - 1);
+        }
+        loc[j] = static_cast<int>(l);
+        rk[j] = atomicAdd(&l_cnt[l], 1);
+        atomicMin(&l_min[l], static_cast<int>(i));
     }
-    pslot[i] = static_cast<int>(h);
-    prank[i] = atomicAdd(&tab[h].cnt, 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kBinLds; k += kBinThreads) {
+        const int h = l_slot[k];
+        if (h >= 0) {
+            l_cnt[k] = atomicAdd(&tab[h].cnt, l_cnt[k]);
+            atomicMin(&tab[h].cell, l_min[k]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBinPerThread; ++j)
+        if (loc[j] >= 0) prank[base + j * kBinThreads] = l_cnt[loc[j]] + rk[j];
 }
+static_assert(kBinLds == 4096, "the LDS slot hash takes the top 12 bits");
 
 // packed (count << 32 | occupied) per table slot for one scan that yields both the
 // particle offset (high word) and the cell number (low word) of every occupied slot
@@ -229,14 +283,15 @@ __global__ void k_slot_pack(const Entry* __restrict__ tab, int64_t cap, int64_t*
 
 __global__ void k_cells(Entry* __restrict__ tab, const int64_t* __restrict__ scan, int64_t cap,
                         int* __restrict__ cell_start, int* __restrict__ cell_cnt,
-                        uint64_t* __restrict__ cell_key, int* __restrict__ parent,
-                        int* __restrict__ comp_min) {
+                        uint64_t* __restrict__ cell_key, int* __restrict__ cell_min,
+                        int* __restrict__ parent, int* __restrict__ comp_min) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= cap) return;
     const int c = tab[i].cnt;
     if (c <= 0) return;
     const int64_t v = scan[i];
     const int cell = static_cast<int>(v & 0xffffffff);
+    cell_min[cell] = tab[i].cell;  // the binning left the cell's minimum index here
     tab[i].cell = cell;
     cell_start[cell] = static_cast<int>(v >> 32);
     cell_cnt[cell] = c;
@@ -249,13 +304,12 @@ template <typename T>
 __global__ void k_scatter(const T* __restrict__ x, const T* __restrict__ y,
                           const T* __restrict__ z, int64_t n, const int* __restrict__ pslot,
                           const int* __restrict__ prank, const int64_t* __restrict__ scan,
-                          int* __restrict__ sidx, T* __restrict__ sx, T* __restrict__ sy,
+                          T* __restrict__ sx, T* __restrict__ sy,
                           T* __restrict__ sz, int* __restrict__ pcell) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t v = scan[pslot[i]];
     const int64_t pos = (v >> 32) + prank[i];
-    sidx[pos] = static_cast<int>(i);
     sx[pos] = x[i];
     sy[pos] = y[i];
     sz[pos] = z[i];
@@ -367,32 +421,16 @@ __global__ void k_roots(const int64_t* __restrict__ totals, int* __restrict__ pa
     }
 }
 
+// also folds every cell's minimum particle index into its component's minimum
 __global__ void k_check_roots(const int64_t* __restrict__ totals, int* __restrict__ parent,
-                              const int* __restrict__ root, int* __restrict__ err) {
+                              const int* __restrict__ root, const int* __restrict__ cell_min,
+                              int* __restrict__ comp_min, int* __restrict__ err) {
     const int64_t n_cells = totals[0] & 0xffffffff;
     for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n_cells;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int r = root[c];
         if (coherent_read(parent + r) != r) atomicOr(err, 4);
-    }
-}
-
-// wave per cell: minimum particle index of the cell, folded into its root
-__global__ __launch_bounds__(256) void k_cell_min(const int64_t* __restrict__ totals,
-                                                  const int* __restrict__ cell_start,
-                                                  const int* __restrict__ cell_cnt,
-                                                  const int* __restrict__ sidx,
-                                                  const int* __restrict__ root,
-                                                  int* __restrict__ comp_min) {
-    const int64_t n_cells = totals[0] & 0xffffffff;
-    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
-    for (int64_t c = wave0; c < n_cells; c += nwaves) {
-        const int s = cell_start[c], n = cell_cnt[c];
-        int m = INT32_MAX;
-        for (int k = __lane_id(); k < n; k += kWave) m = min(m, sidx[s + k]);
-        for (int o = 32; o >= 1; o >>= 1) m = min(m, __shfl_down(m, o, kWave));
-        if (__lane_id() == 0) atomicMin(comp_min + root[c], m);
+        atomicMin(comp_min + r, cell_min[c]);
     }
 }
 
@@ -418,7 +456,7 @@ struct FofWs {
     int* parent;
     int* root;
     int* comp_min;
-    int* sidx;
+    int* cell_min;
     void* sx;
     void* sy;
     void* sz;
@@ -453,7 +491,7 @@ FofWs carve_fof(void* base, int64_t n, size_t elt) {
     w.parent = c.take<int>(nn);
     w.root = c.take<int>(nn);
     w.comp_min = c.take<int>(nn);
-    w.sidx = c.take<int>(nn);
+    w.cell_min = c.take<int>(nn);
     w.sx = c.take<char>(nn * elt);
     w.sy = c.take<char>(nn * elt);
     w.sz = c.take<char>(nn * elt);
@@ -481,18 +519,18 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     // sub-cell side: ll/sqrt(3) with a 1e-6 relative margin (see the file header)
     const double side = ll / sqrt(3.0) * (1.0 - 1e-6);
     const double inv_s = 1.0 / side;
-    k_fof_bin<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.box, inv_s, w.tab,
-                                                static_cast<uint64_t>(w.cap - 1), w.pslot,
-                                                w.prank, w.err);
+    k_fof_bin<T><<<blocks_for(n, kBinChunk), kBinThreads, 0, st>>>(
+        x, y, z, n, w.box, inv_s, w.tab, static_cast<uint64_t>(w.cap - 1), w.pslot, w.prank,
+        w.err);
     HALMA_LAUNCH_CHECK();
     k_slot_pack<<<blocks_for(w.cap), 256, 0, st>>>(w.tab, w.cap, w.scan);
     HALMA_LAUNCH_CHECK();
     int rc = exclusive_scan_i64(w.scan, w.scan, w.cap, w.totals, w.scan_ws, st);
     if (rc) return rc;
     k_cells<<<blocks_for(w.cap), 256, 0, st>>>(w.tab, w.scan, w.cap, w.cell_start, w.cell_cnt,
-                                               w.cell_key, w.parent, w.comp_min);
+                                               w.cell_key, w.cell_min, w.parent, w.comp_min);
     HALMA_LAUNCH_CHECK();
-    k_scatter<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.pslot, w.prank, w.scan, w.sidx,
+    k_scatter<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.pslot, w.prank, w.scan,
                                                 static_cast<T*>(w.sx), static_cast<T*>(w.sy),
                                                 static_cast<T*>(w.sz), w.pcell);
     HALMA_LAUNCH_CHECK();
@@ -503,10 +541,8 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     HALMA_LAUNCH_CHECK();
     k_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.parent, w.root);
     HALMA_LAUNCH_CHECK();
-    k_check_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.parent, w.root, w.err);
-    HALMA_LAUNCH_CHECK();
-    k_cell_min<<<grid_cells, 256, 0, st>>>(w.totals, w.cell_start, w.cell_cnt, w.sidx, w.root,
-                                           w.comp_min);
+    k_check_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.parent, w.root, w.cell_min,
+                                              w.comp_min, w.err);
     HALMA_LAUNCH_CHECK();
     k_labels<<<blocks_for(n), 256, 0, st>>>(n, w.pcell, w.root, w.comp_min, labels);
     HALMA_LAUNCH_CHECK();
@@ -568,7 +604,7 @@ int halma_fof_labels(const void* x, const void* y, const void* z, int dtype, int
                      double ll, int32_t* labels, void* ws, size_t ws_bytes, void* stream) {
     if (n < 0) return set_error(HALMA_E_ARG, "negative particle count");
     if (n == 0) return 0;
-    if (n > INT32_MAX - 1) return set_error(HALMA_E_LIMIT, "at most 2^31-2 particles");
+    if (n > (int64_t{1} << 30)) return set_error(HALMA_E_LIMIT, "at most 2^30 particles");
     if (!x || !y || !z || !labels) return set_error(HALMA_E_ARG, "null pointer");
     if (dtype != HALMA_F32 && dtype != HALMA_F64) return set_error(HALMA_E_ARG, "bad dtype");
     if (!(ll > 0.0) || !isfinite(ll))

@@ -21,23 +21,30 @@ struct __attribute__((aligned(8))) HaloDesc {
 struct PotPlan {
     std::vector<HaloDesc> halos;
     std::vector<int4> items;   // {halo, test tile, source chunk, 0}, largest first
+    std::vector<int32_t> order;  // the haloes in item order (largest item first)
+    int64_t n_items = 0;         // item count (items is filled only when asked for)
     int64_t n_src = 0, n_tst = 0, n_part = 0;
     int tests_per_lane = 2, subtile = 8;
     int grid = 0;              // persistent workgroups (256 threads)
 };
 
-// Plans a batched launch from HOST offset arrays (H+1 entries each).
-int plan_pot(int64_t n_halo, const int64_t* src_off, const int64_t* tst_off, PotPlan& plan);
+// Plans a batched launch from HOST offset arrays (H+1 entries each).  The item list is
+// built only with with_items (callers that generate items on the device need the count).
+int plan_pot(int64_t n_halo, const int64_t* src_off, const int64_t* tst_off, PotPlan& plan,
+             bool with_items = true);
 
 // Live set sizes for a plan launched repeatedly over shrinking sets (all pointers may
 // be null for a static launch).  When set, halo h has tst_len[h] tests and src_len[h]
 // sources (never above the planned lengths), and its sources are read from src_alt
 // instead of the main source array when src_sel[h] != 0 (same offsets).
+// When n_items is set, the item list was written on the device and holds *n_items
+// entries (never more than the planned count, which bounds it).
 struct DynSets {
     const int* tst_len = nullptr;
     const int* src_len = nullptr;
     const int* src_sel = nullptr;
     const float4* src_alt = nullptr;
+    const int64_t* n_items = nullptr;
 };
 
 // Launches the persistent FP32 kernel for a plan whose tables are already on the

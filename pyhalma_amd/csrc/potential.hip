@@ -18,17 +18,20 @@
 
 namespace halma {
 
-__device__ __forceinline__ float pair_term(const float4 s, float xi, float yi, float zi) {
+// acc + m_j / r_ij for one source s = (x, y, z, m) and one test, with the reference's
+// rule that a pair counts only if x_j /= x_i .and. y_j /= y_i .and. z_j /= z_i: a
+// difference is zero exactly when the coordinates are equal (FP32 subtraction keeps
+// denormals), so the pair counts iff the smallest |difference| is positive (one
+// v_min3_f32 with |.| modifiers).  The mass multiply and the add are one FMA.
+__device__ __forceinline__ float pair_acc(const float4 s, float xi, float yi, float zi,
+                                          float acc) {
     const float dx = s.x - xi;
     const float dy = s.y - yi;
     const float dz = s.z - zi;
     const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-    const float t = s.w * __builtin_amdgcn_rsqf(d2);
-    // x_j /= x_i .and. y_j /= y_i .and. z_j /= z_i: a difference is zero exactly when
-    // the coordinates are equal (FP32 subtraction keeps denormals), so the pair counts
-    // iff the smallest |difference| is positive (one v_min3_f32 with |.| modifiers).
     const float mn = fminf(fminf(fabsf(dx), fabsf(dy)), fabsf(dz));
-    return mn > 0.0f ? t : 0.0f;
+    const float inv = mn > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+    return __builtin_fmaf(s.w, inv, acc);
 }
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
@@ -47,10 +50,21 @@ __global__ __launch_bounds__(256) void k_pot_f32(const float4* __restrict__ src,
                                                  float2* __restrict__ part,
                                                  int* __restrict__ head) {
     const int lane = __lane_id();
-    for (;;) {
-        int k = 0;
-        if (lane == 0) k = atomicAdd(head, 1);
-        k = __builtin_amdgcn_readfirstlane(k);
+    const int n_waves = static_cast<int>(gridDim.x * (blockDim.x / kWave));
+    if (dyn.n_items) {
+        const int64_t nd = uniform_load(dyn.n_items);
+        n_items = static_cast<int>(nd < n_items ? nd : n_items);
+    }
+    // the first item of every wave is its wave number; later ones come from the queue
+    // (numbered from n_waves), so a launch with few items costs few atomics on `head`
+    int k = __builtin_amdgcn_readfirstlane(
+        static_cast<int>(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
+    for (bool first = true;; first = false) {
+        if (!first) {
+            int t = 0;
+            if (lane == 0) t = atomicAdd(head, 1);
+            k = n_waves + __builtin_amdgcn_readfirstlane(t);
+        }
         if (k >= n_items) break;
         const int4 it = uniform_load(items + k);
         const HaloDesc d = uniform_load(halos + it.x);
@@ -87,7 +101,7 @@ __global__ __launch_bounds__(256) void k_pot_f32(const float4* __restrict__ src,
             for (int s = 0; s < SUB; ++s) {
                 const float4 q = uniform_load(sp + j + s);
 #pragma unroll
-                for (int r = 0; r < R; ++r) ps[r] += pair_term(q, xi[r], yi[r], zi[r]);
+                for (int r = 0; r < R; ++r) ps[r] = pair_acc(q, xi[r], yi[r], zi[r], ps[r]);
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) kahan_add(as[r], ac[r], ps[r]);
@@ -99,7 +113,7 @@ __global__ __launch_bounds__(256) void k_pot_f32(const float4* __restrict__ src,
             for (; j < j1; ++j) {
                 const float4 q = uniform_load(sp + j);
 #pragma unroll
-                for (int r = 0; r < R; ++r) ps[r] += pair_term(q, xi[r], yi[r], zi[r]);
+                for (int r = 0; r < R; ++r) ps[r] = pair_acc(q, xi[r], yi[r], zi[r], ps[r]);
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) kahan_add(as[r], ac[r], ps[r]);
@@ -183,7 +197,8 @@ __global__ __launch_bounds__(256) void k_pot_f64_plain(
 
 // ------------------------------------------------------------------ host side ----
 
-int plan_pot(int64_t H, const int64_t* src_off, const int64_t* tst_off, PotPlan& plan) {
+int plan_pot(int64_t H, const int64_t* src_off, const int64_t* tst_off, PotPlan& plan,
+             bool with_items) {
     const Tuning t = g_tuning;
     plan = PotPlan();
     plan.tests_per_lane = t.tests_per_lane;
@@ -237,16 +252,29 @@ int plan_pot(int64_t H, const int64_t* src_off, const int64_t* tst_off, PotPlan&
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](int64_t a, int64_t b) { return key[a] > key[b]; });
-    for (int64_t h : order) {
+    plan.order.resize(H);
+    for (int64_t q = 0; q < H; ++q) plan.order[q] = static_cast<int32_t>(order[q]);
+    int64_t n_items = 0;
+    for (int64_t h = 0; h < H; ++h) {
         const HaloDesc& d = plan.halos[h];
-        const int tiles = (d.tst_len + tile - 1) / tile;
-        const int nch = d.src_len > 0 ? (d.src_len + d.chunk - 1) / d.chunk : 0;
-        for (int c = 0; c < nch; ++c)
-            for (int ti = 0; ti < tiles; ++ti)
-                plan.items.push_back(make_int4(static_cast<int>(h), ti, c, 0));
+        const int64_t tiles = (d.tst_len + tile - 1) / tile;
+        const int64_t nch = d.src_len > 0 ? (d.src_len + d.chunk - 1) / d.chunk : 0;
+        n_items += tiles * nch;
     }
-    if (plan.items.size() > static_cast<size_t>(INT32_MAX))
+    if (n_items > static_cast<int64_t>(INT32_MAX))
         return set_error(HALMA_E_LIMIT, "too many work items");
+    plan.n_items = n_items;
+    if (with_items) {
+        plan.items.reserve(static_cast<size_t>(n_items));
+        for (int64_t h : order) {
+            const HaloDesc& d = plan.halos[h];
+            const int tiles = (d.tst_len + tile - 1) / tile;
+            const int nch = d.src_len > 0 ? (d.src_len + d.chunk - 1) / d.chunk : 0;
+            for (int c = 0; c < nch; ++c)
+                for (int ti = 0; ti < tiles; ++ti)
+                    plan.items.push_back(make_int4(static_cast<int>(h), ti, c, 0));
+        }
+    }
     plan.n_src = src_off[H] - src_off[0];
     plan.n_tst = tst_off[H] - tst_off[0];
     plan.n_part = part;
@@ -268,7 +296,7 @@ static int launch_pot_t(const PotPlan& plan, const float4* src4, const float4* t
                         float2* part, int* head, hipStream_t stream) {
     k_fill_i32<<<1, 64, 0, stream>>>(head, 1, 0);
     HALMA_LAUNCH_CHECK();
-    const int n_items = static_cast<int>(plan.items.size());
+    const int n_items = static_cast<int>(plan.n_items);
     if (n_items == 0) return 0;
     const int grid = std::max(1, std::min(plan.grid, (n_items + 3) / 4));
     k_pot_f32<R, SUB><<<grid, 256, 0, stream>>>(src4, tst4, d_halos, dyn, d_items, n_items,
@@ -313,7 +341,7 @@ static PotWs carve_pot(void* base, const PotPlan& p) {
     w.src4 = c.take<float4>(std::max<int64_t>(p.n_src, 1));
     w.tst4 = c.take<float4>(std::max<int64_t>(p.n_tst, 1));
     w.halos = c.take<HaloDesc>(std::max<size_t>(p.halos.size(), 1));
-    w.items = c.take<int4>(std::max<size_t>(p.items.size(), 1));
+    w.items = c.take<int4>(std::max<int64_t>(p.n_items, 1));
     w.part = c.take<float2>(std::max<int64_t>(p.n_part, 1));
     w.tst_halo = c.take<int>(std::max<int64_t>(p.n_tst, 1));
     w.head = c.take<int>(64);

@@ -27,7 +27,16 @@ int g_next = 0;
 }  // namespace
 
 int upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+    const UploadPiece piece{0, src, bytes};
+    return upload_pieces(dst, bytes, &piece, 1, stream);
+}
+
+int upload_pieces(void* dst, size_t bytes, const UploadPiece* pieces, int n_pieces,
+                  hipStream_t stream) {
     if (bytes == 0) return 0;
+    for (int q = 0; q < n_pieces; ++q)
+        if (pieces[q].offset + pieces[q].bytes > bytes)
+            return set_error(HALMA_E_ARG, "upload piece outside the region");
     int dev = 0;
     HALMA_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_mu);
@@ -52,7 +61,9 @@ int upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream) {
         HALMA_HIP(hipHostMalloc(&s.host, cap, hipHostMallocDefault));
         s.cap = cap;
     }
-    memcpy(s.host, src, bytes);
+    for (int q = 0; q < n_pieces; ++q)
+        if (pieces[q].bytes)
+            memcpy(static_cast<char*>(s.host) + pieces[q].offset, pieces[q].src, pieces[q].bytes);
     HALMA_HIP(hipMemcpyAsync(dst, s.host, bytes, hipMemcpyHostToDevice, stream));
     HALMA_HIP(hipEventRecord(s.done, stream));
     s.pending = true;

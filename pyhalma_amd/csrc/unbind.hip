@@ -41,6 +41,7 @@
 
 #include "halma_common.h"
 #include "potential.h"
+#include "scan.h"
 #include "staging.h"
 #include "tuning.h"
 
@@ -131,85 +132,149 @@ __device__ int block_excl_scan(int v, int& excl, int* sh) {
     return tot;
 }
 
-__global__ __launch_bounds__(kThreads) void k_compact(
-    const int64_t* __restrict__ off, int H, const int32_t* __restrict__ active,
-    const int32_t* __restrict__ pass_no, uint8_t* __restrict__ state,
-    const float4* __restrict__ p4, const double* __restrict__ vel,
-    const double* __restrict__ m64, float4* __restrict__ src4, int32_t* __restrict__ cidx,
-    float4* __restrict__ rem4, int32_t* __restrict__ n_bound, int32_t* __restrict__ n_rem,
-    int32_t* __restrict__ mode, double* __restrict__ vcm) {
+// Compaction of the bound sets, in fixed chunks of kChunk member positions (a halo of n
+// members has ceil(n / kChunk) chunks, listed halo by halo): k_chunk_sums counts each
+// chunk's kept and just-removed members and sums the kept members' mass and momentum in
+// double-double; k_halo_sets turns the counts into per-chunk output offsets and merges
+// the sums in chunk order into v_cm; k_chunk_write writes the compacted lists.  Large
+// haloes thus spread over many blocks instead of one.
+constexpr int kChunk = 1024;
+
+// double-double block sum of (hi, lo) pairs; the result is valid in thread 0
+__device__ void block_dd_sum(double& hi, double& lo, double* shd) {
+    for (int s = 32; s >= 1; s >>= 1) {
+        const double h2 = __shfl_down(hi, s, kWave);
+        const double l2 = __shfl_down(lo, s, kWave);
+        dd_add_dd(hi, lo, h2, l2);
+    }
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0) {
+        shd[2 * w] = hi;
+        shd[2 * w + 1] = lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        hi = shd[0];
+        lo = shd[1];
+        for (int k = 1; k < kThreads / kWave; ++k) dd_add_dd(hi, lo, shd[2 * k], shd[2 * k + 1]);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void chunk_range(const int64_t* off, int2 ch, int64_t& b,
+                                            int64_t& e) {
+    b = off[ch.x] + static_cast<int64_t>(ch.y) * kChunk;
+    e = off[ch.x + 1] < b + kChunk ? off[ch.x + 1] : b + kChunk;
+}
+
+__global__ __launch_bounds__(kThreads) void k_chunk_sums(
+    const int2* __restrict__ chunks, int n_chunks, const int64_t* __restrict__ off,
+    const int32_t* __restrict__ active, const uint8_t* __restrict__ state,
+    const double* __restrict__ vel, const double* __restrict__ m64,
+    int32_t* __restrict__ ccnt, double* __restrict__ csum) {
     __shared__ int shs[kThreads / kWave];
     __shared__ double shd[2 * (kThreads / kWave)];
-    const bool first = *pass_no == 0;
-    for (int h = blockIdx.x; h < H; h += gridDim.x) {
-        if (!active[h]) {
-            if (threadIdx.x == 0) {
-                n_bound[h] = 0;
-                n_rem[h] = 0;
-            }
-            continue;
-        }
-        const int64_t b = off[h], e = off[h + 1];
-        int run = 0, rrun = 0;
+    for (int ci = blockIdx.x; ci < n_chunks; ci += gridDim.x) {
+        const int2 ch = chunks[ci];
+        if (!active[ch.x]) continue;  // block-uniform
+        int64_t b, e;
+        chunk_range(off, ch, b, e);
+        int packed = 0;  // kept count | just-removed count << 16
         double acc[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-        for (int64_t k0 = b; k0 < e; k0 += kThreads) {
-            const int64_t k = k0 + threadIdx.x;
-            const uint8_t st = k < e ? state[k] : kGone;
-            const int keep = st == kBound ? 1 : 0;
-            const int gone = st == kJustGone ? 1 : 0;
-            int excl, rexcl;
-            const int tot = block_excl_scan(keep, excl, shs);
-            const int rtot = block_excl_scan(gone, rexcl, shs);
-            if (keep) {
-                const int64_t pos = b + run + excl;
-                src4[pos] = p4[k];
-                cidx[pos] = static_cast<int32_t>(k);
+        for (int64_t k = b + threadIdx.x; k < e; k += kThreads) {
+            const uint8_t st = state[k];
+            if (st == kBound) {
+                packed += 1;
                 const double m = m64[k];
                 dd_add(acc[0][0], acc[0][1], m);
                 dd_add(acc[1][0], acc[1][1], m * vel[3 * k + 0]);
                 dd_add(acc[2][0], acc[2][1], m * vel[3 * k + 1]);
                 dd_add(acc[3][0], acc[3][1], m * vel[3 * k + 2]);
+            } else if (st == kJustGone) {
+                packed += 1 << 16;
             }
-            if (gone) {
-                rem4[b + rrun + rexcl] = p4[k];
-                state[k] = kGone;
-            }
-            run += tot;
-            rrun += rtot;
         }
-        // block double-double reduction of the four sums (thread 0 holds the result)
+        int excl;
+        const int tot = block_excl_scan(packed, excl, shs);
         for (int q = 0; q < 4; ++q) {
-            double hi = acc[q][0], lo = acc[q][1];
-            for (int s = 32; s >= 1; s >>= 1) {
-                const double h2 = __shfl_down(hi, s, kWave);
-                const double l2 = __shfl_down(lo, s, kWave);
-                dd_add_dd(hi, lo, h2, l2);
-            }
-            const int w = threadIdx.x / kWave;
-            if (__lane_id() == 0) {
-                shd[2 * w] = hi;
-                shd[2 * w + 1] = lo;
-            }
-            __syncthreads();
+            block_dd_sum(acc[q][0], acc[q][1], shd);
             if (threadIdx.x == 0) {
-                hi = shd[0];
-                lo = shd[1];
-                for (int k = 1; k < kThreads / kWave; ++k) dd_add_dd(hi, lo, shd[2 * k], shd[2 * k + 1]);
-                acc[q][0] = hi;
-                acc[q][1] = lo;
+                csum[8 * static_cast<int64_t>(ci) + 2 * q] = acc[q][0];
+                csum[8 * static_cast<int64_t>(ci) + 2 * q + 1] = acc[q][1];
             }
-            __syncthreads();
         }
         if (threadIdx.x == 0) {
-            const double M = acc[0][0] + acc[0][1];
-            for (int d = 0; d < 3; ++d)
-                vcm[3 * h + d] = M > 0.0 ? (acc[1 + d][0] + acc[1 + d][1]) / M : 0.0;
-            n_bound[h] = run;
-            n_rem[h] = rrun;
-            // a full recompute when more than a quarter of the set left last pass
-            mode[h] = (first || 4 * static_cast<int64_t>(rrun) > run) ? 1 : 0;
+            ccnt[2 * ci] = tot & 0xffff;
+            ccnt[2 * ci + 1] = tot >> 16;
         }
-        __syncthreads();
+    }
+}
+
+// one thread per halo: chunk offsets, bound/removed counts, v_cm and the pass mode
+__global__ void k_halo_sets(int H, const int32_t* __restrict__ cfirst,
+                            const int32_t* __restrict__ active, const int32_t* __restrict__ pass_no,
+                            int32_t* __restrict__ ccnt, const double* __restrict__ csum,
+                            int32_t* __restrict__ n_bound, int32_t* __restrict__ n_rem,
+                            int32_t* __restrict__ mode, double* __restrict__ vcm) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    if (!active[h]) {
+        n_bound[h] = 0;
+        n_rem[h] = 0;
+        return;
+    }
+    int run = 0, rrun = 0;
+    double acc[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int c = cfirst[h]; c < cfirst[h + 1]; ++c) {
+        const int kc = ccnt[2 * c], gc = ccnt[2 * c + 1];
+        ccnt[2 * c] = run;  // counts become the chunk's output offsets
+        ccnt[2 * c + 1] = rrun;
+        run += kc;
+        rrun += gc;
+        for (int q = 0; q < 4; ++q)
+            dd_add_dd(acc[q][0], acc[q][1], csum[8 * static_cast<int64_t>(c) + 2 * q],
+                      csum[8 * static_cast<int64_t>(c) + 2 * q + 1]);
+    }
+    const double M = acc[0][0] + acc[0][1];
+    for (int d = 0; d < 3; ++d)
+        vcm[3 * h + d] = M > 0.0 ? (acc[1 + d][0] + acc[1 + d][1]) / M : 0.0;
+    n_bound[h] = run;
+    n_rem[h] = rrun;
+    // a full recompute when more than a quarter of the set left last pass
+    mode[h] = (*pass_no == 0 || 4 * static_cast<int64_t>(rrun) > run) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_chunk_write(
+    const int2* __restrict__ chunks, int n_chunks, const int64_t* __restrict__ off,
+    const int32_t* __restrict__ active, const int32_t* __restrict__ ccnt,
+    uint8_t* __restrict__ state, const float4* __restrict__ p4, float4* __restrict__ src4,
+    int32_t* __restrict__ cidx, float4* __restrict__ rem4) {
+    __shared__ int shs[kThreads / kWave];
+    for (int ci = blockIdx.x; ci < n_chunks; ci += gridDim.x) {
+        const int2 ch = chunks[ci];
+        if (!active[ch.x]) continue;  // block-uniform
+        int64_t b, e;
+        chunk_range(off, ch, b, e);
+        const int64_t kbase = off[ch.x] + ccnt[2 * ci];
+        const int64_t gbase = off[ch.x] + ccnt[2 * ci + 1];
+        int run = 0, rrun = 0;
+        for (int64_t k0 = b; k0 < e; k0 += kThreads) {
+            const int64_t k = k0 + threadIdx.x;
+            const uint8_t st = k < e ? state[k] : kGone;
+            const int v = st == kBound ? 1 : (st == kJustGone ? (1 << 16) : 0);
+            int excl;
+            const int tot = block_excl_scan(v, excl, shs);
+            if (st == kBound) {
+                const int64_t pos = kbase + run + (excl & 0xffff);
+                src4[pos] = p4[k];
+                cidx[pos] = static_cast<int32_t>(k);
+            } else if (st == kJustGone) {
+                rem4[gbase + rrun + (excl >> 16)] = p4[k];
+                state[k] = kGone;
+            }
+            run += tot & 0xffff;
+            rrun += tot >> 16;
+        }
     }
 }
 
@@ -253,9 +318,19 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     if (p >= n) return;
     const int ns = a.n_src[h];
     const int nch = (ns + d.chunk - 1) / d.chunk;
+    // chunk partials summed in chunk order; loads issued eight at a time
+    const float2* pp = a.part + d.part_off + p;
     double sum = 0.0;
-    for (int c = 0; c < nch; ++c) {
-        const float2 q = a.part[d.part_off + static_cast<int64_t>(c) * d.stride + p];
+    int c = 0;
+    for (; c + 8 <= nch; c += 8) {
+        float2 q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = pp[static_cast<int64_t>(c + u) * d.stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sum += static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
+    }
+    for (; c < nch; ++c) {
+        const float2 q = pp[static_cast<int64_t>(c) * d.stride];
         sum += static_cast<double>(q.x) + static_cast<double>(q.y);
     }
     const int64_t k = a.cidx[g];
@@ -336,18 +411,57 @@ __global__ void k_finish(int H, const int32_t* __restrict__ n_bound,
     removed[h] = 0;
 }
 
-// live counts for the pair kernel: tests = bound set; sources = bound set (full pass) or
-// the members removed by the previous pass (incremental pass); 0 when inactive
-__global__ void k_live_sets(int H, const int32_t* __restrict__ active,
+// Live test/source sets of every halo for this pass (tests = the bound set; sources =
+// the bound set in a full pass, the members removed by the previous pass in an
+// incremental one; none when inactive) and its number of pair-kernel items (test tiles
+// x source chunks), listed in the plan's halo order (order[pos]).
+__global__ void k_live_sets(int H, const int32_t* __restrict__ order,
+                            const HaloDesc* __restrict__ halos, int tile,
+                            const int32_t* __restrict__ active,
                             const int32_t* __restrict__ n_bound, const int32_t* __restrict__ n_rem,
                             const int32_t* __restrict__ mode, int32_t* __restrict__ tst_len,
-                            int32_t* __restrict__ src_len, int32_t* __restrict__ src_sel) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= H) return;
+                            int32_t* __restrict__ src_len, int32_t* __restrict__ src_sel,
+                            int64_t* __restrict__ icnt) {
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= H) return;
+    const int h = order[pos];
     const bool on = active[h] != 0;
-    tst_len[h] = on ? n_bound[h] : 0;
-    src_len[h] = on ? (mode[h] ? n_bound[h] : n_rem[h]) : 0;
+    const int nt = on ? n_bound[h] : 0;
+    const int ns = on ? (mode[h] ? n_bound[h] : n_rem[h]) : 0;
+    tst_len[h] = nt;
+    src_len[h] = ns;
     src_sel[h] = mode[h] ? 0 : 1;
+    const int ch = halos[h].chunk;
+    icnt[pos] = (nt > 0 && ns > 0)
+                    ? static_cast<int64_t>((nt + tile - 1) / tile) * ((ns + ch - 1) / ch)
+                    : 0;
+}
+
+// The pass's item list: the items of the haloes with live work, in the plan's halo
+// order, test tiles fastest ({halo, tile, chunk, 0}, as in the plan).  ioff is the
+// exclusive scan of the per-position counts; the list never exceeds the planned one
+// (live sets only shrink, chunks are fixed), and is cut at cap all the same.
+__global__ void k_item_fill(int H, const int32_t* __restrict__ order,
+                            const int64_t* __restrict__ ioff, const int64_t* __restrict__ itotal,
+                            const int32_t* __restrict__ tst_len, int tile, int64_t cap,
+                            int4* __restrict__ items) {
+    const int64_t total = itotal[0] < cap ? itotal[0] : cap;
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int lo = 0, hi = H - 1;  // last position with ioff <= k (its count is > 0)
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ioff[mid] <= k)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int h = order[lo];
+        const int64_t local = k - ioff[lo];
+        const int tiles = (tst_len[h] + tile - 1) / tile;
+        items[k] = make_int4(h, static_cast<int>(local % tiles), static_cast<int>(local / tiles),
+                             0);
+    }
 }
 
 __global__ void k_final_mask(int64_t nm, const uint8_t* __restrict__ state,
@@ -382,17 +496,58 @@ struct UnbWs {
     unsigned long long* counters;
     HaloDesc* halos;
     int4* items;
+    int32_t* order;
+    int64_t* icnt;
+    int64_t* ioff;
+    int64_t* itotal;
+    void* iscan_ws;
     float2* part;
     int* head;
     int64_t* off;
+    int2* chunks;     // compaction chunks {halo, chunk number}, halo by halo
+    int32_t* cfirst;  // first chunk of each halo (H + 1)
+    int32_t* ccnt;    // per chunk: kept / removed counts, then their output offsets
+    double* csum;     // per chunk: double-double mass and momentum sums (8 doubles)
+    int64_t n_chunks;
+    // host tables (off, halos, order, chunks, cfirst) sit in one region, uploaded at once
+    char* tables;
+    size_t t_off, t_halos, t_order, t_chunks, t_cfirst, t_bytes;
     size_t bytes;
 };
 
-UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan) {
+int64_t count_chunks(int64_t H, const int64_t* h_off) {
+    int64_t n = 0;
+    for (int64_t h = 0; h < H; ++h) n += (h_off[h + 1] - h_off[h] + kChunk - 1) / kChunk;
+    return n;
+}
+
+UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, int64_t n_chunks) {
     Carver c(base);
     const int64_t n1 = nm > 0 ? nm : 1;
     const int64_t h1 = H > 0 ? H : 1;
+    const int64_t c1 = n_chunks > 0 ? n_chunks : 1;
     UnbWs w;
+    Carver t(nullptr);
+    w.t_off = t.used;
+    t.take<int64_t>(h1 + 1);
+    w.t_halos = t.used;
+    t.take<HaloDesc>(h1);
+    w.t_order = t.used;
+    t.take<int32_t>(h1);
+    w.t_chunks = t.used;
+    t.take<int2>(c1);
+    w.t_cfirst = t.used;
+    t.take<int32_t>(h1 + 1);
+    w.t_bytes = t.used;
+    w.tables = c.take<char>(w.t_bytes);
+    w.off = reinterpret_cast<int64_t*>(w.tables + w.t_off);
+    w.halos = reinterpret_cast<HaloDesc*>(w.tables + w.t_halos);
+    w.order = reinterpret_cast<int32_t*>(w.tables + w.t_order);
+    w.chunks = reinterpret_cast<int2*>(w.tables + w.t_chunks);
+    w.cfirst = reinterpret_cast<int32_t*>(w.tables + w.t_cfirst);
+    w.n_chunks = n_chunks;
+    w.ccnt = c.take<int32_t>(2 * c1);
+    w.csum = c.take<double>(8 * c1);
     w.p4 = c.take<float4>(n1);
     w.vel = c.take<double>(3 * n1);
     w.m64 = c.take<double>(n1);
@@ -416,11 +571,13 @@ UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan) {
     w.vcm = c.take<double>(3 * h1);
     w.band = c.take<int32_t>(n1);
     w.counters = c.take<unsigned long long>(8);
-    w.halos = c.take<HaloDesc>(std::max<size_t>(plan.halos.size(), 1));
-    w.items = c.take<int4>(std::max<size_t>(plan.items.size(), 1));
+    w.items = c.take<int4>(std::max<int64_t>(plan.n_items, 1));
+    w.icnt = c.take<int64_t>(h1);
+    w.ioff = c.take<int64_t>(h1);
+    w.itotal = c.take<int64_t>(4);
+    w.iscan_ws = c.take<char>(scan_workspace_bytes(h1));
     w.part = c.take<float2>(std::max<int64_t>(plan.n_part, 1));
     w.head = c.take<int>(64);
-    w.off = c.take<int64_t>(h1 + 1);
     w.bytes = c.used;
     return w;
 }
@@ -442,7 +599,7 @@ hipStream_t capture_stream() {
 
 struct PassLaunch {
     int64_t H, nm;
-    int grid_halo, grid_retest;
+    int grid_chunks, grid_retest;
     const PotPlan* plan;
     UnbWs w;
     uint8_t* bound;
@@ -454,13 +611,33 @@ struct PassLaunch {
 // are recorded on the stream between the parts, around the pair kernel alone.
 int launch_pre(const PassLaunch& L, hipStream_t st) {
     const UnbWs& w = L.w;
-    k_compact<<<L.grid_halo, kThreads, 0, st>>>(w.off, static_cast<int>(L.H), w.active, w.pass_no,
-                                                w.state, w.p4, w.vel, w.m64, w.src4, w.cidx,
-                                                w.rem4, w.n_bound, w.n_rem, w.mode, w.vcm);
+    const int nc = static_cast<int>(w.n_chunks);
+    if (nc > 0) {
+        k_chunk_sums<<<L.grid_chunks, kThreads, 0, st>>>(w.chunks, nc, w.off, w.active, w.state,
+                                                         w.vel, w.m64, w.ccnt, w.csum);
+        HALMA_LAUNCH_CHECK();
+    }
+    k_halo_sets<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.cfirst, w.active,
+                                                      w.pass_no, w.ccnt, w.csum, w.n_bound,
+                                                      w.n_rem, w.mode, w.vcm);
     HALMA_LAUNCH_CHECK();
-    k_live_sets<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.active, w.n_bound,
-                                                      w.n_rem, w.mode, w.tst_len, w.src_len,
-                                                      w.src_sel);
+    if (nc > 0) {
+        k_chunk_write<<<L.grid_chunks, kThreads, 0, st>>>(w.chunks, nc, w.off, w.active, w.ccnt,
+                                                          w.state, w.p4, w.src4, w.cidx, w.rem4);
+        HALMA_LAUNCH_CHECK();
+    }
+    const int tile = L.plan->tests_per_lane * kWave;
+    k_live_sets<<<blocks_for(L.H), kThreads, 0, st>>>(
+        static_cast<int>(L.H), w.order, w.halos, tile, w.active, w.n_bound, w.n_rem, w.mode,
+        w.tst_len, w.src_len, w.src_sel, w.icnt);
+    HALMA_LAUNCH_CHECK();
+    const int rc = exclusive_scan_i64(w.icnt, w.ioff, L.H, w.itotal, w.iscan_ws, st);
+    if (rc) return rc;
+    const int64_t cap = L.plan->n_items;
+    const unsigned fill_grid =
+        static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(1024, (cap + 255) / 256)));
+    k_item_fill<<<fill_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.ioff,
+                                                 w.itotal, w.tst_len, tile, cap, w.items);
     HALMA_LAUNCH_CHECK();
     return 0;
 }
@@ -472,6 +649,7 @@ int launch_pot(const PassLaunch& L, hipStream_t st) {
     dyn.src_len = w.src_len;
     dyn.src_sel = w.src_sel;
     dyn.src_alt = w.rem4;
+    dyn.n_items = w.itotal;
     return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, dyn, w.items, w.part, w.head, st);
 }
 
@@ -517,16 +695,32 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     std::vector<int64_t> rel(H + 1);
     for (int64_t h = 0; h <= H; ++h) rel[h] = h_off[h] - h_off[0];
     PotPlan plan;
-    int rc = plan_pot(H, rel.data(), rel.data(), plan);
+    int rc = plan_pot(H, rel.data(), rel.data(), plan, false);
     if (rc) return rc;
-    UnbWs w = carve_unb(ws, H, nm, plan);
+    const int64_t n_chunks = count_chunks(H, h_off);
+    if (n_chunks > INT32_MAX) return set_error(HALMA_E_LIMIT, "too many members");
+    UnbWs w = carve_unb(ws, H, nm, plan, n_chunks);
     if (ws_bytes < w.bytes)
         return set_error(HALMA_E_WORKSPACE, "workspace %zu < required %zu", ws_bytes, w.bytes);
-    rc = upload_async(w.off, rel.data(), (H + 1) * sizeof(int64_t), st);
-    if (rc) return rc;
-    rc = upload_async(w.halos, plan.halos.data(), plan.halos.size() * sizeof(HaloDesc), st);
-    if (rc) return rc;
-    rc = upload_async(w.items, plan.items.data(), plan.items.size() * sizeof(int4), st);
+    std::vector<int2> chunks(static_cast<size_t>(n_chunks));
+    std::vector<int32_t> cfirst(H + 1);
+    {
+        int32_t q = 0;
+        for (int64_t h = 0; h < H; ++h) {
+            cfirst[h] = q;
+            const int64_t nch = (rel[h + 1] - rel[h] + kChunk - 1) / kChunk;
+            for (int64_t c = 0; c < nch; ++c)
+                chunks[q++] = make_int2(static_cast<int>(h), static_cast<int>(c));
+        }
+        cfirst[H] = q;
+    }
+    const UploadPiece pieces[5] = {
+        {w.t_off, rel.data(), (H + 1) * sizeof(int64_t)},
+        {w.t_halos, plan.halos.data(), plan.halos.size() * sizeof(HaloDesc)},
+        {w.t_order, plan.order.data(), plan.order.size() * sizeof(int32_t)},
+        {w.t_chunks, chunks.data(), chunks.size() * sizeof(int2)},
+        {w.t_cfirst, cfirst.data(), cfirst.size() * sizeof(int32_t)}};
+    rc = upload_pieces(w.tables, w.t_bytes, pieces, 5, st);
     if (rc) return rc;
     const int32_t* mem = members + h_off[0];
     if (nm > 0) {
@@ -550,7 +744,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.H = H;
     L.nm = nm;
     const int cus = std::max(device_cu_count(), 1);
-    L.grid_halo = static_cast<int>(std::min<int64_t>(H, 65535));
+    L.grid_chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(n_chunks, cus * 8)));
     L.grid_retest = cus * 4;
     L.plan = &plan;
     L.w = w;
@@ -676,7 +870,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     if (stats) {
         stats[0] = passes;
         stats[1] = band_total;
-        stats[2] = plan.items.size();
+        stats[2] = plan.n_items;
         stats[3] = plan.tests_per_lane;
         stats[4] = plan.subtile;
         stats[5] = static_cast<int64_t>(pot_ms * 1e6);  // ns
@@ -768,8 +962,8 @@ size_t unbind_workspace_bytes(int64_t H, const int64_t* h_off) {
     std::vector<int64_t> rel(H + 1);
     for (int64_t h = 0; h <= H; ++h) rel[h] = h_off[h] - h_off[0];
     PotPlan plan;
-    if (plan_pot(H, rel.data(), rel.data(), plan)) return 0;
-    return carve_unb(nullptr, H, h_off[H] - h_off[0], plan).bytes;
+    if (plan_pot(H, rel.data(), rel.data(), plan, false)) return 0;
+    return carve_unb(nullptr, H, h_off[H] - h_off[0], plan, count_chunks(H, h_off)).bytes;
 }
 
 int unbind_plain_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs in,
