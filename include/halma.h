@@ -81,9 +81,9 @@ int halma_pot_sum_batched_f32(int64_t n_halo, const int64_t *src_off, const floa
                               const float *tz, double *out_f64, void *workspace,
                               size_t workspace_bytes, void *stream);
 
-/* The same sum in FP64 with the oracle's fixed summation order (64 strided partials
- * of ascending j, then a pairwise tree), one wavefront per test particle.  Bitwise
- * reproducible; used for the band re-test and exposed for verification. */
+/* The same sum in FP64 with the oracle's fixed summation order (256 strided partials
+ * of ascending j, then a pairwise tree), one 256-thread block per test particle.
+ * Bitwise reproducible; used for the band re-test and exposed for verification. */
 int halma_pot_sum_f64_ordered(const float *m, const float *x, const float *y,
                               const float *z, int64_t n_src, const float *tx,
                               const float *ty, const float *tz, int64_t n_test,
@@ -184,10 +184,12 @@ int halma_density_peak(int64_t n_groups, const int64_t *offsets, const int32_t *
  * [1] members re-tested in FP64, [2] queue items per pass, [3] tests per lane,
  * [4] sub-tile, [5] pair-kernel time in ns and [6] timed launches (with
  * HALMA_UNBIND_TIME_POT: hipEvents recorded around the pair kernel on its stream),
- * [7] pairs evaluated.  flags: HALMA_UNBIND_NO_GRAPH replays passes as plain launches
- * instead of a captured hipGraph.  Synchronises the stream once per pass.
+ * [7] pairs evaluated.  flags: HALMA_UNBIND_GRAPHS captures each part of a pass (prepare,
+ * pair kernel, decisions) into a hipGraph replayed every pass; by default the parts are
+ * plain launches (a few passes do not repay the capture).  Synchronises the stream once
+ * per pass.
  * ---------------------------------------------------------------------------------- */
-#define HALMA_UNBIND_NO_GRAPH 1
+#define HALMA_UNBIND_GRAPHS 1
 #define HALMA_UNBIND_TIME_POT 2
 size_t halma_unbind_workspace_size(int64_t n_halo, const int64_t *offsets);
 int halma_unbind_batched(int64_t n_halo, const int64_t *offsets, const int32_t *members,

@@ -11,7 +11,7 @@
  *                        sequential inner sum, the same three-coordinate skip rule.
  *   orc_pot_sum_f64      the FP64 direct-sum oracle of the same sum on FP32-cast
  *                        inputs (halo_gas.py:172-178 casts every input to float32).
- *                        Fixed summation order (64 strided partials + pairwise tree),
+ *                        Fixed summation order (256 strided partials + pairwise tree),
  *                        no contraction, IEEE sqrt and divide: its bits are the
  *                        contract the GPU FP64 band re-test reproduces.
  *   orc_pot_soft_f64 /   softened potential and acceleration for the centring
@@ -57,18 +57,20 @@ void orc_pot_sum_f32(int nthreads, int64_t ntotal, const float *m, const float *
 
 /* FP64 direct sum on FP32 inputs.  Same predicate as particle_subroutines.f90:499-501;
  * per included pair d2 = (dx*dx + dy*dy) + dz*dz, t = m / sqrt(d2).  Summation order
- * (fixed, so the GPU FP64 re-test can reproduce the bits with one wavefront per test
- * particle): 64 strided partial sums L[k] = sum over j = k, k+64, k+128, ... ascending,
- * each starting from 0.0, then the pairwise tree L[k] += L[k+s] for s = 32,16,...,1
- * (k < s); the result is L[0].  sqrt and divide are IEEE correctly rounded. */
+ * (fixed, so the GPU FP64 re-test can reproduce the bits with one 256-thread block per
+ * test particle): ORD_W = 256 strided partial sums L[k] = sum over j = k, k+256, ...
+ * ascending, each starting from 0.0, then the pairwise tree L[k] += L[k+s] for
+ * s = 128, 64, ..., 1 (k < s); the result is L[0].  sqrt and divide are IEEE correctly
+ * rounded. */
+#define ORD_W 256
 static inline double pot_one_f64(int64_t ntotal, const float *m, const float *x,
                                  const float *y, const float *z, float xi, float yi,
                                  float zi) {
-    double lane[64];
+    double lane[ORD_W];
     const double xd = xi, yd = yi, zd = zi;
-    for (int k = 0; k < 64; ++k) {
+    for (int k = 0; k < ORD_W; ++k) {
         double acc = 0.0;
-        for (int64_t j = k; j < ntotal; j += 64) {
+        for (int64_t j = k; j < ntotal; j += ORD_W) {
             if (x[j] != xi && y[j] != yi && z[j] != zi) {
                 const double dx = (double)x[j] - xd;
                 const double dy = (double)y[j] - yd;
@@ -79,7 +81,7 @@ static inline double pot_one_f64(int64_t ntotal, const float *m, const float *x,
         }
         lane[k] = acc;
     }
-    for (int s = 32; s >= 1; s >>= 1)
+    for (int s = ORD_W / 2; s >= 1; s >>= 1)
         for (int k = 0; k < s; ++k) lane[k] = lane[k] + lane[k + s];
     return lane[0];
 }

@@ -73,7 +73,7 @@ TUNE_WAVES_PER_CU = 5
 F32 = 0
 F64 = 1
 GROUP_FIELDS = 16
-UNBIND_NO_GRAPH = 1
+UNBIND_GRAPHS = 1
 UNBIND_TIME_POT = 2
 
 _LIB = None

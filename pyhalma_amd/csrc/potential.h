@@ -48,10 +48,11 @@ struct DynSets {
 };
 
 // Launches the persistent FP32 kernel for a plan whose tables are already on the
-// device.  head must be a device int (reset by this launcher).
+// device.  head must be a device int holding 0 at launch: this launcher clears it
+// first unless reset_head is false (the caller then clears it earlier in stream order).
 int launch_pot_f32(const PotPlan& plan, const float4* src4, const float4* tst4,
                    const HaloDesc* d_halos, DynSets dyn, const int4* d_items, float2* part,
-                   int* head, hipStream_t stream);
+                   int* head, hipStream_t stream, bool reset_head = true);
 
 // Packs SoA FP32 (m, x, y, z) into AoS float4 (x, y, z, m); m may be null (-> 0).
 int launch_pack4(const float* m, const float* x, const float* y, const float* z,
@@ -72,27 +73,57 @@ struct SrcSoA {
     }
 };
 
-// FP64 sum in the oracle's fixed order (64 strided partials of ascending j, then the
-// pairwise tree) for one test position against ns sources; used by the band re-test.
-// Must be called by a whole wavefront; every lane returns the same value.
-template <class Src>
-__device__ __forceinline__ double wave_pot_f64_ordered(Src src, int64_t ns, float tx,
-                                                        float ty, float tz) {
-    const int lane = __lane_id();
-    double acc = 0.0;
-    const double xd = tx, yd = ty, zd = tz;
-    for (int64_t j = lane; j < ns; j += kWave) {
-        const float4 q = src(j);
-        if (q.x != tx && q.y != ty && q.z != tz) {
-            const double dx = static_cast<double>(q.x) - xd;
-            const double dy = static_cast<double>(q.y) - yd;
-            const double dz = static_cast<double>(q.z) - zd;
-            const double d2 = (dx * dx + dy * dy) + dz * dz;
-            acc = acc + static_cast<double>(q.w) / __builtin_sqrt(d2);
-        }
+// FP64 sum in the oracle's fixed order for one test position against ns sources:
+// kOrdWidth strided partials L[k] = sum over j = k, k + kOrdWidth, ... ascending (each
+// from 0.0), then the pairwise tree L[k] = L[k] + L[k + s] for s = kOrdWidth/2, ..., 1
+// (k < s); the result is L[0].  Used by the band re-test and the FP64 checker.  Must be
+// called by a whole block of kOrdWidth threads (sh: kOrdWidth doubles of LDS); every
+// thread returns the same value.
+constexpr int kOrdWidth = 256;
+
+__device__ __forceinline__ double ord_term(double acc, float4 q, float tx, float ty, float tz,
+                                           double xd, double yd, double zd) {
+    if (q.x != tx && q.y != ty && q.z != tz) {
+        const double dx = static_cast<double>(q.x) - xd;
+        const double dy = static_cast<double>(q.y) - yd;
+        const double dz = static_cast<double>(q.z) - zd;
+        const double d2 = (dx * dx + dy * dy) + dz * dz;
+        acc = acc + static_cast<double>(q.w) / __builtin_sqrt(d2);
     }
-    for (int s = 32; s >= 1; s >>= 1) acc = acc + __shfl_down(acc, s, kWave);
-    return __shfl(acc, 0, kWave);
+    return acc;
+}
+
+template <class Src>
+__device__ double block_pot_f64_ordered(Src src, int64_t ns, float tx, float ty, float tz,
+                                        double* sh) {
+    const int t = threadIdx.x;
+    const double xd = tx, yd = ty, zd = tz;
+    double acc = 0.0;
+    int64_t j = t;
+    // four loads in flight per step; the additions stay in ascending j
+    for (; j + 3 * kOrdWidth < ns; j += 4 * kOrdWidth) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = src(j + u * kOrdWidth);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = ord_term(acc, q[u], tx, ty, tz, xd, yd, zd);
+    }
+    for (; j < ns; j += kOrdWidth) acc = ord_term(acc, src(j), tx, ty, tz, xd, yd, zd);
+    sh[t] = acc;
+    __syncthreads();
+    for (int s = kOrdWidth / 2; s >= kWave; s >>= 1) {
+        if (t < s) sh[t] = sh[t] + sh[t + s];
+        __syncthreads();
+    }
+    if (t < kWave) {
+        double r = sh[t];
+        for (int s = kWave / 2; s >= 1; s >>= 1) r = r + __shfl_down(r, s, kWave);
+        if (t == 0) sh[0] = r;
+    }
+    __syncthreads();
+    const double out = sh[0];
+    __syncthreads();
+    return out;
 }
 
 }  // namespace halma

@@ -166,15 +166,18 @@ __global__ void k_pot_merge(const HaloDesc* __restrict__ halos, const int* __res
     if (out_f32) out_f32[g] = static_cast<float>(s);
 }
 
-__global__ __launch_bounds__(256) void k_pot_f64_ordered(SrcSoA src, int64_t ns,
-                                                         const float* __restrict__ tx,
-                                                         const float* __restrict__ ty,
-                                                         const float* __restrict__ tz,
-                                                         int64_t nt, double* __restrict__ out) {
-    const int64_t w = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-    if (w >= nt) return;
-    const double s = wave_pot_f64_ordered(src, ns, tx[w], ty[w], tz[w]);
-    if (__lane_id() == 0) out[w] = s;
+// one block per test (grid-stride)
+__global__ __launch_bounds__(kOrdWidth) void k_pot_f64_ordered(SrcSoA src, int64_t ns,
+                                                               const float* __restrict__ tx,
+                                                               const float* __restrict__ ty,
+                                                               const float* __restrict__ tz,
+                                                               int64_t nt,
+                                                               double* __restrict__ out) {
+    __shared__ double sh[kOrdWidth];
+    for (int64_t i = blockIdx.x; i < nt; i += gridDim.x) {
+        const double s = block_pot_f64_ordered(src, ns, tx[i], ty[i], tz[i], sh);
+        if (threadIdx.x == 0) out[i] = s;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_pot_f64_plain(
@@ -293,9 +296,11 @@ int launch_pack4(const float* m, const float* x, const float* y, const float* z,
 template <int R, int SUB>
 static int launch_pot_t(const PotPlan& plan, const float4* src4, const float4* tst4,
                         const HaloDesc* d_halos, DynSets dyn, const int4* d_items,
-                        float2* part, int* head, hipStream_t stream) {
-    k_fill_i32<<<1, 64, 0, stream>>>(head, 1, 0);
-    HALMA_LAUNCH_CHECK();
+                        float2* part, int* head, bool reset_head, hipStream_t stream) {
+    if (reset_head) {
+        k_fill_i32<<<1, 64, 0, stream>>>(head, 1, 0);
+        HALMA_LAUNCH_CHECK();
+    }
     const int n_items = static_cast<int>(plan.n_items);
     if (n_items == 0) return 0;
     const int grid = std::max(1, std::min(plan.grid, (n_items + 3) / 4));
@@ -307,11 +312,11 @@ static int launch_pot_t(const PotPlan& plan, const float4* src4, const float4* t
 
 int launch_pot_f32(const PotPlan& plan, const float4* src4, const float4* tst4,
                    const HaloDesc* d_halos, DynSets dyn, const int4* d_items, float2* part,
-                   int* head, hipStream_t stream) {
+                   int* head, hipStream_t stream, bool reset_head) {
 #define HALMA_POT_CASE(R_, S_)                                                            \
     if (plan.tests_per_lane == R_ && plan.subtile == S_)                                  \
         return launch_pot_t<R_, S_>(plan, src4, tst4, d_halos, dyn, d_items, part, head,  \
-                                    stream);
+                                    reset_head, stream);
     HALMA_POT_CASE(1, 4)
     HALMA_POT_CASE(1, 8)
     HALMA_POT_CASE(2, 4)
@@ -449,10 +454,9 @@ int halma_pot_sum_f64_ordered(const float* m, const float* x, const float* y, co
     if (n_test == 0) return 0;
     if (!out || !tx || !ty || !tz || (n_src > 0 && (!m || !x || !y || !z)))
         return set_error(HALMA_E_ARG, "halma_pot_sum_f64_ordered: null pointer");
-    const int64_t blocks = (n_test + 3) / 4;  // four wavefronts per block
-    if (blocks > INT32_MAX) return set_error(HALMA_E_LIMIT, "too many tests");
+    const int64_t blocks = std::min<int64_t>(n_test, 65536);
     SrcSoA src{m, x, y, z};
-    k_pot_f64_ordered<<<static_cast<unsigned>(blocks), 256, 0,
+    k_pot_f64_ordered<<<static_cast<unsigned>(blocks), kOrdWidth, 0,
                         static_cast<hipStream_t>(stream)>>>(src, n_src, tx, ty, tz, n_test,
                                                              out);
     HALMA_LAUNCH_CHECK();

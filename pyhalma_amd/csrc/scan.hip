@@ -93,11 +93,48 @@ __global__ __launch_bounds__(kThreads) void k_scan_apply(const T* __restrict__ i
     }
 }
 
+// single block over the whole input, tile by tile with a carry (short inputs: one
+// launch instead of three)
+constexpr int64_t kSingleTiles = 8;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_scan_single(const T* __restrict__ in, int64_t n,
+                                                          int64_t* __restrict__ out,
+                                                          int64_t* __restrict__ total) {
+    __shared__ int64_t sh[kThreads / kWave + 1];
+    int64_t carry = 0;
+    for (int64_t t0 = 0; t0 < n; t0 += kTile) {
+        const int64_t base = t0 + threadIdx.x * kItems;
+        int64_t vals[kItems];
+        int64_t v = 0;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            vals[k] = ld(in, base + k, n);
+            v += vals[k];
+        }
+        int64_t tot;
+        const int64_t incl = block_incl_scan(v, sh, &tot);
+        int64_t run = carry + incl - v;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            if (base + k < n) out[base + k] = run;
+            run += vals[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
 template <typename T>
 int scan_impl(const T* in, int64_t* out, int64_t n, int64_t* total, void* ws,
               hipStream_t stream) {
     const int64_t nb = n > 0 ? (n + kTile - 1) / kTile : 0;
     int64_t* sums = static_cast<int64_t*>(ws);
+    if (nb > 0 && nb <= kSingleTiles) {
+        k_scan_single<T><<<1, kThreads, 0, stream>>>(in, n, out, total);
+        HALMA_LAUNCH_CHECK();
+        return 0;
+    }
     if (nb == 0) {
         if (total) {
             k_scan_sums<<<1, kThreads, 0, stream>>>(sums, 0, total);

@@ -9,30 +9,30 @@
 // E > 0.  (The reference evaluates the test once; iterating it is this path's design.)
 //
 // One pass on the GPU, for all haloes at once:
-//   1. k_compact     one workgroup per active halo: stable compaction of its bound
-//                    members into the front of its range (AoS float4), of the members
-//                    removed by the previous pass into a second list, the double-double
-//                    v_cm over the bound set, and the pass mode: FULL (first pass, or the
-//                    previous pass removed more than a quarter of the bound set) sums over
-//                    the bound set afresh; INCREMENTAL subtracts the removed members'
-//                    contributions from each bound member's stored sum,
-//                    S_new = S_old - sum_{j removed} m_j / r_ij, which costs
-//                    n_bound x n_removed pairs instead of n_bound^2;
-//   2. k_pot_f32     the persistent FP32 pair kernel over a static queue planned for the
-//                    initial sizes (largest first); live test/source counts and the
-//                    source list come from device arrays, so empty chunks and converged
-//                    haloes cost one queue fetch;
+//   1. compaction    k_chunk_sums / k_halo_sets / k_chunk_write over fixed chunks of
+//                    1024 member positions: stable compaction of each active halo's
+//                    bound members into the front of its range (AoS float4), of the
+//                    members removed by the previous pass into a second list, the
+//                    double-double v_cm over the bound set, and the pass mode: FULL
+//                    (first pass, or the previous pass removed more than a quarter of
+//                    the bound set) sums over the bound set afresh; INCREMENTAL subtracts
+//                    the removed members' contributions from each bound member's stored
+//                    sum, S_new = S_old - sum_{j removed} m_j / r_ij, which costs
+//                    n_bound x n_removed pairs instead of n_bound^2.  k_halo_sets also
+//                    sets the live test/source counts and each halo's number of
+//                    pair-kernel items; a scan and k_item_fill write the pass's queue
+//                    (haloes in the plan's largest-first order);
+//   2. k_pot_f32     the persistent FP32 pair kernel over that queue;
 //   3. k_energy      merges the chunk partials (FP64), updates S and its absolute error
 //                    bound (the bounds of every FP32 sum that went into S add up),
 //                    evaluates E in FP64 and decides every member whose |E| exceeds
 //                    factor*G*bound + 8 ulp(kin + factor*G*S);
-//   4. k_retest      one wavefront per member inside the band: S in FP64 with the
-//                    oracle's fixed summation order over the bound set, so E and the
-//                    decision are the oracle's bit for bit (S and its bound are reset to
-//                    that value);
+//   4. k_retest      one block per member inside the band: S in FP64 with the oracle's
+//                    fixed summation order over the bound set, so E and the decision are
+//                    the oracle's bit for bit (S and its bound are reset to that value);
 //   5. k_finish      per-halo pass counts, convergence flags, the removed total.
-// Each pass is captured once into hipGraphs (prepare / pair kernel / decisions) and
-// replayed; the host reads one counter per pass to stop.
+// The parts are plain launches (or, on request, hipGraphs captured once and replayed);
+// the host reads one counter per pass to stop.
 #include <math.h>
 
 #include <algorithm>
@@ -210,17 +210,34 @@ __global__ __launch_bounds__(kThreads) void k_chunk_sums(
     }
 }
 
-// one thread per halo: chunk offsets, bound/removed counts, v_cm and the pass mode
-__global__ void k_halo_sets(int H, const int32_t* __restrict__ cfirst,
+// One thread per halo, in the plan's halo order (h = order[pos]): chunk output offsets,
+// bound and removed counts, v_cm, the pass mode (full recompute, or incremental when at
+// most a quarter of the set left last pass), the live sets of the pair kernel (tests =
+// the bound set; sources = the bound set in a full pass, the members removed by the
+// previous pass in an incremental one; none when inactive) and the halo's number of
+// pair-kernel items (test tiles x source chunks).  Also clears the removed-members
+// counter for the pass.
+__global__ void k_halo_sets(int H, const int32_t* __restrict__ order,
+                            const HaloDesc* __restrict__ halos, int tile,
+                            const int32_t* __restrict__ cfirst,
                             const int32_t* __restrict__ active, const int32_t* __restrict__ pass_no,
                             int32_t* __restrict__ ccnt, const double* __restrict__ csum,
                             int32_t* __restrict__ n_bound, int32_t* __restrict__ n_rem,
-                            int32_t* __restrict__ mode, double* __restrict__ vcm) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= H) return;
+                            int32_t* __restrict__ mode, double* __restrict__ vcm,
+                            int32_t* __restrict__ tst_len, int32_t* __restrict__ src_len,
+                            int32_t* __restrict__ src_sel, int64_t* __restrict__ icnt,
+                            unsigned long long* __restrict__ counters) {
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos == 0) counters[0] = 0;
+    if (pos >= H) return;
+    const int h = order[pos];
     if (!active[h]) {
         n_bound[h] = 0;
         n_rem[h] = 0;
+        tst_len[h] = 0;
+        src_len[h] = 0;
+        src_sel[h] = 0;
+        icnt[pos] = 0;
         return;
     }
     int run = 0, rrun = 0;
@@ -238,10 +255,19 @@ __global__ void k_halo_sets(int H, const int32_t* __restrict__ cfirst,
     const double M = acc[0][0] + acc[0][1];
     for (int d = 0; d < 3; ++d)
         vcm[3 * h + d] = M > 0.0 ? (acc[1 + d][0] + acc[1 + d][1]) / M : 0.0;
+    const int full = (*pass_no == 0 || 4 * static_cast<int64_t>(rrun) > run) ? 1 : 0;
     n_bound[h] = run;
     n_rem[h] = rrun;
-    // a full recompute when more than a quarter of the set left last pass
-    mode[h] = (*pass_no == 0 || 4 * static_cast<int64_t>(rrun) > run) ? 1 : 0;
+    mode[h] = full;
+    const int nt = run;
+    const int ns = full ? run : rrun;
+    tst_len[h] = nt;
+    src_len[h] = ns;
+    src_sel[h] = full ? 0 : 1;
+    const int ch = halos[h].chunk;
+    icnt[pos] = (nt > 0 && ns > 0)
+                    ? static_cast<int64_t>((nt + tile - 1) / tile) * ((ns + ch - 1) / ch)
+                    : 0;
 }
 
 __global__ __launch_bounds__(kThreads) void k_chunk_write(
@@ -362,19 +388,20 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_retest(EnergyArgs a, const float4* __restrict__ src4) {
-    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
+// one block per band member (grid-stride): the FP64 sum in the oracle's order decides
+__global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
+                                                      const float4* __restrict__ src4) {
+    __shared__ double sh[kOrdWidth];
     const int64_t nb = static_cast<int64_t>(a.counters[1]);
-    for (int64_t w = wave0; w < nb; w += nwaves) {
+    for (int64_t w = blockIdx.x; w < nb; w += gridDim.x) {
         const int64_t g = a.band[w];
         const int h = a.mhalo[g];
         const HaloDesc d = a.halos[h];
         const int n = a.n_bound[h];
         const float4 t = src4[g];
         SrcAoS src{src4 + d.src_off};
-        const double S = wave_pot_f64_ordered(src, n, t.x, t.y, t.z);
-        if (__lane_id() == 0) {
+        const double S = block_pot_f64_ordered(src, n, t.x, t.y, t.z, sh);
+        if (threadIdx.x == 0) {
             const int64_t k = a.cidx[g];
             const double E = kinetic(a.vel, k, a.vcm + 3 * h) - a.cg * S;
             a.sacc[k] = S;
@@ -411,32 +438,6 @@ __global__ void k_finish(int H, const int32_t* __restrict__ n_bound,
     removed[h] = 0;
 }
 
-// Live test/source sets of every halo for this pass (tests = the bound set; sources =
-// the bound set in a full pass, the members removed by the previous pass in an
-// incremental one; none when inactive) and its number of pair-kernel items (test tiles
-// x source chunks), listed in the plan's halo order (order[pos]).
-__global__ void k_live_sets(int H, const int32_t* __restrict__ order,
-                            const HaloDesc* __restrict__ halos, int tile,
-                            const int32_t* __restrict__ active,
-                            const int32_t* __restrict__ n_bound, const int32_t* __restrict__ n_rem,
-                            const int32_t* __restrict__ mode, int32_t* __restrict__ tst_len,
-                            int32_t* __restrict__ src_len, int32_t* __restrict__ src_sel,
-                            int64_t* __restrict__ icnt) {
-    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos >= H) return;
-    const int h = order[pos];
-    const bool on = active[h] != 0;
-    const int nt = on ? n_bound[h] : 0;
-    const int ns = on ? (mode[h] ? n_bound[h] : n_rem[h]) : 0;
-    tst_len[h] = nt;
-    src_len[h] = ns;
-    src_sel[h] = mode[h] ? 0 : 1;
-    const int ch = halos[h].chunk;
-    icnt[pos] = (nt > 0 && ns > 0)
-                    ? static_cast<int64_t>((nt + tile - 1) / tile) * ((ns + ch - 1) / ch)
-                    : 0;
-}
-
 // The pass's item list: the items of the haloes with live work, in the plan's halo
 // order, test tiles fastest ({halo, tile, chunk, 0}, as in the plan).  ioff is the
 // exclusive scan of the per-position counts; the list never exceeds the planned one
@@ -444,7 +445,8 @@ __global__ void k_live_sets(int H, const int32_t* __restrict__ order,
 __global__ void k_item_fill(int H, const int32_t* __restrict__ order,
                             const int64_t* __restrict__ ioff, const int64_t* __restrict__ itotal,
                             const int32_t* __restrict__ tst_len, int tile, int64_t cap,
-                            int4* __restrict__ items) {
+                            int4* __restrict__ items, int* __restrict__ head) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *head = 0;  // the pair kernel's queue
     const int64_t total = itotal[0] < cap ? itotal[0] : cap;
     for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
          k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -612,24 +614,16 @@ struct PassLaunch {
 int launch_pre(const PassLaunch& L, hipStream_t st) {
     const UnbWs& w = L.w;
     const int nc = static_cast<int>(w.n_chunks);
+    const int tile = L.plan->tests_per_lane * kWave;
     if (nc > 0) {
         k_chunk_sums<<<L.grid_chunks, kThreads, 0, st>>>(w.chunks, nc, w.off, w.active, w.state,
                                                          w.vel, w.m64, w.ccnt, w.csum);
         HALMA_LAUNCH_CHECK();
     }
-    k_halo_sets<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.cfirst, w.active,
-                                                      w.pass_no, w.ccnt, w.csum, w.n_bound,
-                                                      w.n_rem, w.mode, w.vcm);
-    HALMA_LAUNCH_CHECK();
-    if (nc > 0) {
-        k_chunk_write<<<L.grid_chunks, kThreads, 0, st>>>(w.chunks, nc, w.off, w.active, w.ccnt,
-                                                          w.state, w.p4, w.src4, w.cidx, w.rem4);
-        HALMA_LAUNCH_CHECK();
-    }
-    const int tile = L.plan->tests_per_lane * kWave;
-    k_live_sets<<<blocks_for(L.H), kThreads, 0, st>>>(
-        static_cast<int>(L.H), w.order, w.halos, tile, w.active, w.n_bound, w.n_rem, w.mode,
-        w.tst_len, w.src_len, w.src_sel, w.icnt);
+    k_halo_sets<<<blocks_for(L.H), kThreads, 0, st>>>(
+        static_cast<int>(L.H), w.order, w.halos, tile, w.cfirst, w.active, w.pass_no, w.ccnt,
+        w.csum, w.n_bound, w.n_rem, w.mode, w.vcm, w.tst_len, w.src_len, w.src_sel, w.icnt,
+        w.counters);
     HALMA_LAUNCH_CHECK();
     const int rc = exclusive_scan_i64(w.icnt, w.ioff, L.H, w.itotal, w.iscan_ws, st);
     if (rc) return rc;
@@ -637,8 +631,14 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
     const unsigned fill_grid =
         static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(1024, (cap + 255) / 256)));
     k_item_fill<<<fill_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.ioff,
-                                                 w.itotal, w.tst_len, tile, cap, w.items);
+                                                 w.itotal, w.tst_len, tile, cap, w.items,
+                                                 w.head);
     HALMA_LAUNCH_CHECK();
+    if (nc > 0) {
+        k_chunk_write<<<L.grid_chunks, kThreads, 0, st>>>(w.chunks, nc, w.off, w.active, w.ccnt,
+                                                          w.state, w.p4, w.src4, w.cidx, w.rem4);
+        HALMA_LAUNCH_CHECK();
+    }
     return 0;
 }
 
@@ -650,7 +650,10 @@ int launch_pot(const PassLaunch& L, hipStream_t st) {
     dyn.src_sel = w.src_sel;
     dyn.src_alt = w.rem4;
     dyn.n_items = w.itotal;
-    return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, dyn, w.items, w.part, w.head, st);
+    // the queue head is cleared by k_item_fill in the prepare part, so that this part
+    // holds the pair kernel alone (its timing events bracket nothing else)
+    return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, dyn, w.items, w.part, w.head, st,
+                          false);
 }
 
 int launch_post(const PassLaunch& L, hipStream_t st) {
@@ -658,7 +661,7 @@ int launch_post(const PassLaunch& L, hipStream_t st) {
     if (L.nm > 0) {
         k_energy<<<blocks_for(L.nm), kThreads, 0, st>>>(L.ea);
         HALMA_LAUNCH_CHECK();
-        k_retest<<<L.grid_retest, kThreads, 0, st>>>(L.ea, w.src4);
+        k_retest<<<L.grid_retest, kOrdWidth, 0, st>>>(L.ea, w.src4);
         HALMA_LAUNCH_CHECK();
     }
     k_finish<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.n_bound, w.src_len,
@@ -769,7 +772,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.ea.s_out = s_out;
     L.ea.nm = nm;
 
-    const bool use_graph = (flags & HALMA_UNBIND_NO_GRAPH) == 0;
+    const bool use_graph = (flags & HALMA_UNBIND_GRAPHS) != 0;
     const bool timing = (flags & HALMA_UNBIND_TIME_POT) != 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (timing) {
@@ -849,13 +852,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
             ++passes;
             break;
         }
-        // reset the removed total for the next pass (stream-ordered, outside the graph)
-        k_zero_counters<<<1, 64, 0, st>>>(w.counters, 1);
-        const hipError_t e3 = hipGetLastError();
-        if (e3 != hipSuccess) {
-            status = hip_fail(e3, "counter reset", __FILE__, __LINE__);
-            break;
-        }
+        // the removed total is cleared by the next pass's k_halo_sets
     }
     cleanup();
     if (status) return status;

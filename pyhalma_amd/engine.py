@@ -285,7 +285,7 @@ class UnbindResult:
         self.stats = stats      # dict
 
 
-def unbind(groups, x, y, z, vx, vy, vz, m, factor=1.0, max_iter=100, use_graph=True,
+def unbind(groups, x, y, z, vx, vy, vz, m, factor=1.0, max_iter=100, use_graph=False,
            with_pot=False, time_pot=False):
     """Batched iterative unbinding of every group (see halma_unbind_batched)."""
     from .device import dtype_flag
@@ -309,7 +309,7 @@ def unbind(groups, x, y, z, vx, vy, vz, m, factor=1.0, max_iter=100, use_graph=T
         rc = L.halma_unbind_batched(H, off, ptr(groups.members), ptr(x), ptr(y), ptr(z),
                                     dtype_flag(x), ptr(vx), ptr(vy), ptr(vz), dtype_flag(vx),
                                     ptr(m), dtype_flag(m), float(factor), int(max_iter),
-                                    (0 if use_graph else _lib.UNBIND_NO_GRAPH)
+                                    (_lib.UNBIND_GRAPHS if use_graph else 0)
                                     | (_lib.UNBIND_TIME_POT if time_pot else 0), ptr(bound),
                                     ptr(iters), ptr(vcm), ptr(pot), st, ptr(ws), ws.numel(),
                                     stream_ptr())
