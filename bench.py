@@ -30,6 +30,11 @@ if ROOT not in sys.path:
 METRIC = "particles/s FoF+unbinding at 2M stars, batched FP32 vs plain FP64 on same GPU"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (AMD published, packed FMA)
 FLOP_PER_PAIR = 11         # particle_subroutines.f90:502-505 as written
+# HBM bytes per k_pot_f32 launch on this workload, averaged over the 8 unbinding
+# launches of a step: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from two rocprofv3 --pmc
+# passes, each scaled by what the counter reads for a 256 MiB device copy
+# (profiles/bench_step_2M_hbm_traffic.md)
+POT_TRAFFIC_BYTES = 63.8e6
 MINP = 50
 
 
@@ -72,8 +77,20 @@ def main():
     ll = w["ll"]
     log(f"workload: {n} particles, {len(w['sizes'])} clumps, ll={ll:.5f} Mpc "
         f"(generated in {time.time() - t0:.1f}s)")
+    torch.cuda.synchronize()
+    th0 = time.perf_counter()
     D = {k: torch.from_numpy(w[k]).to(dev) for k in ("x", "y", "z", "vx", "vy", "vz", "mass")}
     torch.cuda.synchronize()
+    h2d_s = time.perf_counter() - th0  # first upload: also pays allocator set-up
+    for k in ("x", "y", "z", "vx", "vy", "vz", "mass"):  # steady-state upload time
+        D[k].copy_(torch.from_numpy(w[k]))
+    torch.cuda.synchronize()
+    th0 = time.perf_counter()
+    for k in ("x", "y", "z", "vx", "vy", "vz", "mass"):
+        D[k].copy_(torch.from_numpy(w[k]))
+    torch.cuda.synchronize()
+    h2d_s = min(h2d_s, time.perf_counter() - th0)
+    h2d_bytes = sum(int(D[k].numel()) * D[k].element_size() for k in D)
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB
         dst = torch.empty_like(src)
@@ -117,7 +134,7 @@ def main():
     achieved = FLOP_PER_PAIR * pairs_per_launch / pot_s_per_launch / 1e12
     roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": POT_TRAFFIC_BYTES,
                 "kernel": "k_pot_f32 (persistent FP32 pair kernel)",
                 "pairs_per_launch": int(pairs_per_launch),
                 "launch_ms": round(pot_s_per_launch * 1e3, 4)}
@@ -133,6 +150,9 @@ def main():
                    "particles": n, "groups": groups.n_groups,
                    "members": groups.n_members, "passes": ub.stats["passes"]},
         "roofline": roofline,
+        # inputs handed over from (pageable) host memory instead of resident in HBM
+        "pcie_inclusive": {"h2d_ms": h2d_s * 1e3, "h2d_bytes": h2d_bytes,
+                           "value": n / (step_s + h2d_s)},
     }
 
     if not args.no_plain:
