@@ -210,6 +210,31 @@ int halma_unbind_plain_f64(int64_t n_halo, const int64_t *offsets, const int32_t
                            uint8_t *bound, int32_t *iters_host, void *workspace,
                            size_t workspace_bytes, void *stream);
 
+/* ------------------------------------------------------------------------------------
+ * Softened potential and its gradient, for the potential-minimum centring of haloes
+ * (the differentiable extension of the unbinding sum: no reference function computes
+ * it; the reference's most-bound position, halo_gas.py:498-634, is its discrete
+ * counterpart).  For each halo h (CSR offsets, HOST arrays of n_halo+1 entries) and each
+ * test i of h (FP64 positions tx, ty, tz):
+ *   pot[i]    = sum_j m_j (r_ij^2 + eps^2)^(-1/2)
+ *   acc[3i+k] = sum_j m_j (x_j - t_i)_k (r_ij^2 + eps^2)^(-3/2)     (= d pot[i] / d t_i)
+ *   wsum[i]   = sum_j m_j (r_ij^2 + eps^2)^(-3/2)
+ * over the sources j of h (FP32 sm, sx, sy, sz) whose three coordinates all differ
+ * from the test's (particle_subroutines.f90:499-501; drops the self pair).  Any of
+ * pot, acc, wsum may be NULL (the gradient pass runs only when acc or wsum is asked
+ * for).  FP32 compensated tiles merged in FP64: pot within halma_pot_error_bound()
+ * relative; acc and wsum within the same bound relative to sum_j |term_j|.  The pair
+ * terms are recomputed, never stored.  Stream-ordered.
+ * ---------------------------------------------------------------------------------- */
+size_t halma_soft_workspace_size(int64_t n_halo, const int64_t *src_off,
+                                 const int64_t *tst_off);
+int halma_pot_soft_batched(int64_t n_halo, const int64_t *src_off, const float *sm,
+                           const float *sx, const float *sy, const float *sz,
+                           const int64_t *tst_off, const double *tx, const double *ty,
+                           const double *tz, double eps, double *pot, double *acc,
+                           double *wsum, void *workspace, size_t workspace_bytes,
+                           void *stream);
+
 /* Tuning knobs of the FP32 pair kernels (set by the autotuner; read back with get).
  *   HALMA_TUNE_TESTS_PER_LANE : test particles per lane (1, 2, 4)
  *   HALMA_TUNE_SUBTILE        : sources per plain FP32 sub-tile before the Kahan add

@@ -1,0 +1,137 @@
+"""Differentiable halo potential and the potential-minimum centring.
+
+The softened pair sum of a halo,
+
+    S_i = sum_j m_j (|x_j - t_i|^2 + eps^2)^(-1/2)
+
+(the sum of particle_subroutines.f90:466-514 with a softening length and the same
+three-coordinate skip rule), is exposed as a torch.autograd.Function whose backward is
+the library's hand-written gradient kernel (the per-test acceleration
+sum_j m_j (x_j - t_i) (r^2 + eps^2)^(-3/2)); the pair terms are recomputed, never
+stored.  Gradients with respect to the sources (positions and masses) reuse the same
+kernels with tests and sources exchanged.
+
+``potential_energy`` is the halo's softened potential energy
+W_h = -G/2 sum_{i in h} m_i S_i as a differentiable function of the particle positions
+(dW/dx_k = -G m_k a_k).  ``refine_centres`` moves a seed centre per halo downhill on
+the potential Phi(c) = -G S(c): every step takes the autograd gradient of -S at the
+current centres and scales it per halo by 1 / w(c), w = sum_j m_j (r^2+eps^2)^(-3/2),
+which makes the step the fixed-point update c <- sum_j m_j w_j x_j / sum_j m_j w_j and
+never increases Phi (the kernel (r^2 + eps^2)^(-1/2) is convex and decreasing in r^2).
+The reference's discrete counterpart is the most-bound particle of
+halo_gas.py:498-634.
+"""
+import numpy as np
+import torch
+
+from . import engine
+
+G_CONST = 4.3 * 1e-3 * 1e-6  # (km/s)^2 Mpc / Msun, halo_gas.py:459-460
+
+
+def _cols(p):
+    return p[:, 0].contiguous(), p[:, 1].contiguous(), p[:, 2].contiguous()
+
+
+class SoftPotential(torch.autograd.Function):
+    """S (float64, one per test) for tests (T, 3) float64 against the sources of their
+    halo: src_pos (N, 3), src_m (N,), CSR host offsets src_off / tst_off (H+1 each).
+    ``aux`` (a dict or None) receives ``wsum`` (one per test) when the test gradient is
+    computed."""
+
+    @staticmethod
+    def forward(ctx, tests, src_pos, src_m, src_off, tst_off, eps, aux=None):
+        tx, ty, tz = _cols(tests.detach().double())
+        sx, sy, sz = _cols(src_pos.detach().float())
+        out = engine.pot_soft_batched(src_off, src_m.detach().float(), sx, sy, sz, tst_off,
+                                      tx, ty, tz, eps)["pot"]
+        ctx.save_for_backward(tests, src_pos, src_m)
+        ctx.src_off, ctx.tst_off, ctx.eps, ctx.aux = src_off, tst_off, float(eps), aux
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        tests, src_pos, src_m = ctx.saved_tensors
+        g = g.detach().double().contiguous()
+        gt = gp = gm = None
+        tx, ty, tz = _cols(tests.detach().double())
+        sx, sy, sz = _cols(src_pos.detach().float())
+        if ctx.needs_input_grad[0]:
+            r = engine.pot_soft_batched(ctx.src_off, src_m.detach().float(), sx, sy, sz,
+                                        ctx.tst_off, tx, ty, tz, ctx.eps, pot=False,
+                                        acc=True, wsum=ctx.aux is not None)
+            gt = (g[:, None] * r["acc"]).to(tests.dtype)
+            if ctx.aux is not None:
+                ctx.aux["wsum"] = r["wsum"]
+        if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
+            # dS_i/dx_j = m_j (t_i - x_j)(r^2+eps^2)^-3/2 and dS_i/dm_j = (r^2+eps^2)^-1/2:
+            # the same sums with the tests as sources (masses g_i) and the sources as tests
+            r = engine.pot_soft_batched(ctx.tst_off, g.float(), tx.float(), ty.float(),
+                                        tz.float(), ctx.src_off, sx.double(), sy.double(),
+                                        sz.double(), ctx.eps, pot=ctx.needs_input_grad[2],
+                                        acc=ctx.needs_input_grad[1])
+            if ctx.needs_input_grad[1]:
+                gp = (src_m.detach().double()[:, None] * r["acc"]).to(src_pos.dtype)
+            if ctx.needs_input_grad[2]:
+                gm = r["pot"].to(src_m.dtype)
+        return gt, gp, gm, None, None, None, None
+
+
+def soft_potential(tests, src_pos, src_m, src_off, tst_off, eps, aux=None):
+    return SoftPotential.apply(tests, src_pos, src_m, np.asarray(src_off, np.int64),
+                               np.asarray(tst_off, np.int64), float(eps), aux)
+
+
+def potential_energy(pos, m, offsets, eps, G=G_CONST):
+    """W_h = -G/2 sum_{i in h} m_i S_i (float64, one per halo) for particle positions
+    pos (N, 3) in halo order with CSR host offsets; differentiable in pos (and m)."""
+    offsets = np.asarray(offsets, np.int64)
+    S = soft_potential(pos.double(), pos.float(), m, offsets, offsets, eps)
+    e = m.double() * S
+    seg = torch.repeat_interleave(torch.arange(len(offsets) - 1, device=e.device),
+                                  torch.as_tensor(np.diff(offsets), device=e.device))
+    W = torch.zeros(len(offsets) - 1, dtype=torch.float64, device=e.device)
+    return -0.5 * G * W.index_add(0, seg, e)
+
+
+class CentringResult:
+    def __init__(self, centres, S, iters, steps):
+        self.centres = centres  # (H, 3) float64 device tensor
+        self.S = S              # softened sum at the centres (Phi = -G S)
+        self.iters = iters      # steps taken per halo
+        self.steps = steps      # last step length per halo (Mpc)
+
+
+def refine_centres(x, y, z, m, offsets, seeds, eps, max_iter=200, tol=1e-6):
+    """Potential-minimum centre per halo.  Particles x, y, z, m in halo order (CSR
+    host offsets, H+1 entries); seeds (H, 3) starting centres; eps the softening
+    length (scalar or one per halo is not supported: one value).  Iterates until
+    every halo's step is below tol * eps, or max_iter.  Each step is autograd's
+    gradient of -sum S at the centres through SoftPotential, scaled by 1 / w."""
+    offsets = np.asarray(offsets, np.int64)
+    H = len(offsets) - 1
+    dev = engine.to_dev(np.zeros(1), np.float64).device
+    pos = torch.stack([engine.to_dev(a, np.float32) for a in (x, y, z)], dim=1)
+    mt = engine.to_dev(m, np.float32)
+    c = torch.as_tensor(np.asarray(seeds, np.float64), device=dev).clone()
+    tst_off = np.arange(H + 1, dtype=np.int64)
+    iters = torch.zeros(H, dtype=torch.int32, device=dev)
+    active = torch.ones(H, dtype=torch.bool, device=dev)
+    step_len = torch.full((H,), float("inf"), dtype=torch.float64, device=dev)
+    S = None
+    for _ in range(max_iter):
+        cc = c.detach().requires_grad_(True)
+        aux = {}
+        S = soft_potential(cc, pos, mt, offsets, tst_off, eps, aux)
+        (grad,) = torch.autograd.grad(-S.sum(), cc)   # = -acc(c)
+        w = aux["wsum"]
+        step = torch.where(w[:, None] > 0, -grad / w[:, None], torch.zeros_like(grad))
+        step = torch.where(active[:, None], step, torch.zeros_like(step))
+        c = c + step
+        step_len = torch.where(active, step.norm(dim=1), step_len)
+        iters += active.to(torch.int32)
+        active = active & (step_len > tol * eps)
+        if not bool(active.any()):
+            break
+    S = soft_potential(c, pos, mt, offsets, tst_off, eps).detach()
+    return CentringResult(c.detach(), S, iters, step_len)

@@ -30,8 +30,9 @@ struct PotPlan {
 
 // Plans a batched launch from HOST offset arrays (H+1 entries each).  The item list is
 // built only with with_items (callers that generate items on the device need the count).
+// tests_per_lane > 0 overrides the tuned tile height (kernels with a fixed one).
 int plan_pot(int64_t n_halo, const int64_t* src_off, const int64_t* tst_off, PotPlan& plan,
-             bool with_items = true);
+             bool with_items = true, int tests_per_lane = 0);
 
 // Live set sizes for a plan launched repeatedly over shrinking sets (all pointers may
 // be null for a static launch).  When set, halo h has tst_len[h] tests and src_len[h]

@@ -201,8 +201,9 @@ __global__ __launch_bounds__(256) void k_pot_f64_plain(
 // ------------------------------------------------------------------ host side ----
 
 int plan_pot(int64_t H, const int64_t* src_off, const int64_t* tst_off, PotPlan& plan,
-             bool with_items) {
-    const Tuning t = g_tuning;
+             bool with_items, int tests_per_lane) {
+    Tuning t = g_tuning;
+    if (tests_per_lane > 0) t.tests_per_lane = tests_per_lane;
     plan = PotPlan();
     plan.tests_per_lane = t.tests_per_lane;
     plan.subtile = t.subtile;

@@ -90,6 +90,42 @@ def pot_sum_f64_plain(m, x, y, z, tx, ty, tz):
     return out
 
 
+def pot_soft_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, eps, pot=True, acc=False,
+                     wsum=False):
+    """Softened sums per halo (CSR host offsets; see halma_pot_soft_batched):
+    pot_i = sum_j m_j (r^2+eps^2)^-1/2, acc_i = d pot_i / d t_i (n_test x 3),
+    wsum_i = sum_j m_j (r^2+eps^2)^-3/2.  Sources FP32, tests FP64.  Returns a dict of
+    float64 device tensors for the requested outputs."""
+    L = _lib_()
+    src_off = np.asarray(src_off, np.int64)
+    tst_off = np.asarray(tst_off, np.int64)
+    H = len(src_off) - 1
+    if len(tst_off) != H + 1:
+        raise ValueError("offset arrays must have the same length")
+    m, x, y, z = (to_dev(a, np.float32) for a in (m, x, y, z))
+    tx, ty, tz = (to_dev(a, np.float64) for a in (tx, ty, tz))
+    nt = int(tx.numel())
+    dev = tx.device
+    out = {}
+    if pot:
+        out["pot"] = torch.zeros(nt, dtype=torch.float64, device=dev)
+    if acc:
+        out["acc"] = torch.zeros((nt, 3), dtype=torch.float64, device=dev)
+    if wsum:
+        out["wsum"] = torch.zeros(nt, dtype=torch.float64, device=dev)
+    if H == 0 or nt == 0:
+        return out
+    so = _lib.i64_array(src_off)
+    to = _lib.i64_array(tst_off)
+    ws = workspace(L.halma_soft_workspace_size(H, so, to))
+    rc = L.halma_pot_soft_batched(H, so, ptr(m), ptr(x), ptr(y), ptr(z), to, ptr(tx), ptr(ty),
+                                  ptr(tz), float(eps), ptr(out.get("pot")),
+                                  ptr(out.get("acc")), ptr(out.get("wsum")), ptr(ws),
+                                  ws.numel(), stream_ptr())
+    _lib.check(rc, "halma_pot_soft_batched")
+    return out
+
+
 def _coords(x, y, z):
     """Device coordinate tensors sharing one float dtype (float32 kept, else float64)."""
     from .device import float_tensor

@@ -1,0 +1,204 @@
+"""Softened potential, its hand-written gradient kernel behind torch.autograd, and the
+potential-minimum centring, against the FP64 oracle on the GPU.
+
+Bars: S within the FP32 bound of the FP64 oracle; the gradient and w within the bound
+relative to the sum of |terms|; autograd gradients (tests, source positions, source
+masses, and the halo potential energy w.r.t. positions) against central finite
+differences of the FP64 oracle and against the FP64 acceleration; refined centres are
+stationary in FP64, at least as deep as the most-bound particle (brute force over every
+particle), and the deepest point of an FP64 grid around them."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+EPS = 5e-4
+
+
+def _haloes(n_haloes=3, seed=5, nscale=0.01):
+    from pyhalma_amd import synth
+
+    return synth.hernquist_haloes(seed=seed, n_haloes=n_haloes, scale=1.0, nscale=nscale)
+
+
+def _terms_abs(m, x, y, z, t, eps):
+    """sum_j |m_j d_k (r^2+eps^2)^-3/2| per test and axis (FP64, numpy), and w."""
+    d = np.stack([x, y, z], 1).astype(np.float64)[None, :, :] - t[:, None, :]
+    keep = np.all(d != 0.0, axis=2)
+    r2 = (d * d).sum(2) + eps * eps
+    w = np.where(keep, m.astype(np.float64)[None, :] * r2 ** -1.5, 0.0)
+    return (np.abs(d) * w[:, :, None]).sum(1), w.sum(1)
+
+
+def test_soft_sums_vs_oracle(gpu):
+    from pyhalma_amd import engine
+
+    w = _haloes()
+    off = w["offsets"]
+    rng = np.random.default_rng(1)
+    H = len(off) - 1
+    tests, toff = [], [0]
+    for h in range(H):
+        c = w["centres"][h]
+        tests.append(c + rng.normal(scale=w["a"][h], size=(40, 3)))
+        toff.append(toff[-1] + 40)
+    t = np.concatenate(tests)
+    toff = np.array(toff)
+    r = engine.pot_soft_batched(off, w["mass"], w["x"], w["y"], w["z"], toff, t[:, 0], t[:, 1],
+                                t[:, 2], EPS, pot=True, acc=True, wsum=True)
+    pot, acc, ws = (r[k].cpu().numpy() for k in ("pot", "acc", "wsum"))
+    B = engine.pot_error_bound()
+    for h in range(H):
+        s, e = off[h], off[h + 1]
+        a, b = toff[h], toff[h + 1]
+        args = (w["mass"][s:e], w["x"][s:e], w["y"][s:e], w["z"][s:e])
+        So = oracle.pot_soft_f64(*args, t[a:b, 0], t[a:b, 1], t[a:b, 2], EPS)
+        Ao = oracle.acc_soft_f64(*args, t[a:b, 0], t[a:b, 1], t[a:b, 2], EPS)
+        mag, wo = _terms_abs(*args, t[a:b], EPS)
+        assert np.all(np.abs(pot[a:b] - So) <= B * So), h
+        assert np.all(np.abs(acc[a:b] - Ao) <= B * mag + 1e-300), h
+        assert np.all(np.abs(ws[a:b] - wo) <= B * wo), h
+    print(f"haloes={H} members={off[-1]} tests={len(t)}", flush=True)
+
+
+def test_autograd_test_gradient_vs_finite_differences(gpu):
+    from pyhalma_amd import centring, engine
+
+    w = _haloes(n_haloes=2, seed=9)
+    off = w["offsets"]
+    H = len(off) - 1
+    rng = np.random.default_rng(2)
+    t0 = np.concatenate([w["centres"][h] + rng.normal(scale=w["a"][h], size=(8, 3))
+                         for h in range(H)])
+    toff = np.arange(0, 8 * H + 1, 8)
+    dev = torch.device("cuda", 0)
+    pos = torch.stack([torch.from_numpy(w[k]).to(dev) for k in "xyz"], 1)
+    m = torch.from_numpy(w["mass"]).to(dev)
+    coef = torch.from_numpy(rng.uniform(0.5, 2.0, len(t0))).to(dev)
+    t = torch.from_numpy(t0).to(dev).requires_grad_(True)
+    S = centring.soft_potential(t, pos, m, off, toff, EPS)
+    (S * coef).sum().backward()
+    g = t.grad.cpu().numpy()
+    hstep = 1e-3 * EPS
+    fd = np.zeros_like(t0)
+    for h in range(H):
+        s, e = off[h], off[h + 1]
+        a, b = toff[h], toff[h + 1]
+        args = (w["mass"][s:e], w["x"][s:e], w["y"][s:e], w["z"][s:e])
+        for k in range(3):
+            tp, tm = t0[a:b].copy(), t0[a:b].copy()
+            tp[:, k] += hstep
+            tm[:, k] -= hstep
+            fd[a:b, k] = (oracle.pot_soft_f64(*args, tp[:, 0], tp[:, 1], tp[:, 2], EPS)
+                          - oracle.pot_soft_f64(*args, tm[:, 0], tm[:, 1], tm[:, 2], EPS)) / (2 * hstep)
+        Ao = oracle.acc_soft_f64(*args, t0[a:b, 0], t0[a:b, 1], t0[a:b, 2], EPS)
+        mag, _ = _terms_abs(*args, t0[a:b], EPS)
+        c = coef.cpu().numpy()[a:b, None]
+        assert np.all(np.abs(g[a:b] - c * Ao) <= c * engine.pot_error_bound() * mag)
+    np.testing.assert_allclose(g, coef.cpu().numpy()[:, None] * fd, rtol=1e-4,
+                               atol=1e-4 * np.abs(g).max())
+
+
+def test_autograd_source_gradients_vs_finite_differences(gpu):
+    from pyhalma_amd import centring
+
+    rng = np.random.default_rng(3)
+    n, nt = 60, 12
+    xs = rng.normal(scale=2e-3, size=(n, 3)).astype(np.float32)
+    ms = rng.uniform(1e4, 1e6, n).astype(np.float32)
+    t0 = rng.normal(scale=2e-3, size=(nt, 3))
+    coef = rng.uniform(0.5, 2.0, nt)
+    dev = torch.device("cuda", 0)
+    pos = torch.from_numpy(xs).to(dev).requires_grad_(True)
+    m = torch.from_numpy(ms).to(dev).requires_grad_(True)
+    t = torch.from_numpy(t0).to(dev)
+    S = centring.soft_potential(t, pos, m, [0, n], [0, nt], EPS)
+    (S * torch.from_numpy(coef).to(dev)).sum().backward()
+    gp = pos.grad.double().cpu().numpy()
+    gm = m.grad.double().cpu().numpy()
+
+    def L(xx, mm):
+        return float((coef * oracle.pot_soft_f64(mm, xx[:, 0], xx[:, 1], xx[:, 2], t0[:, 0],
+                                                 t0[:, 1], t0[:, 2], EPS)).sum())
+    # source positions are FP32 in the kernel: differentiate the oracle in FP64 at the
+    # same FP32 points by an exact analytic route (masses: S is linear in m_j)
+    for j in range(n):
+        e = np.zeros(n, np.float32)
+        e[j] = 1.0
+        d = np.stack([xs[j].astype(np.float64)] * nt) - t0
+        keep = np.all(d != 0.0, axis=1)
+        r2 = (d * d).sum(1) + EPS * EPS
+        want_m = float((coef * np.where(keep, r2 ** -0.5, 0.0)).sum())
+        assert abs(gm[j] - want_m) <= 2e-6 * abs(want_m)
+        want_p = -(coef[:, None] * np.where(keep[:, None], float(ms[j]) * d * r2[:, None] ** -1.5,
+                                            0.0)).sum(0)
+        mag = (coef[:, None] * np.abs(float(ms[j]) * d * r2[:, None] ** -1.5)).sum(0)
+        assert np.all(np.abs(gp[j] - want_p) <= 2e-6 * mag + 1e-30), j
+    # and the mass gradient is the derivative of the oracle's L
+    j = 7
+    h = 1.0
+    mp, mm_ = ms.astype(np.float64).copy(), ms.astype(np.float64).copy()
+    mp[j] += h
+    mm_[j] -= h
+    fd = (L(xs, mp.astype(np.float32)) - L(xs, mm_.astype(np.float32))) / (
+        float(np.float32(mp[j])) - float(np.float32(mm_[j])))
+    assert abs(gm[j] - fd) <= 1e-6 * abs(fd)
+
+
+def test_potential_energy_gradient_vs_fp64_acceleration(gpu):
+    from pyhalma_amd import centring
+
+    w = _haloes(n_haloes=2, seed=11, nscale=0.004)
+    off = w["offsets"]
+    dev = torch.device("cuda", 0)
+    pos = torch.stack([torch.from_numpy(w[k]).to(dev) for k in "xyz"], 1).double()
+    pos.requires_grad_(True)
+    m = torch.from_numpy(w["mass"]).to(dev)
+    W = centring.potential_energy(pos, m, off, EPS)
+    W.sum().backward()
+    g = pos.grad.cpu().numpy()
+    G = centring.G_CONST
+    for h in range(len(off) - 1):
+        s, e = off[h], off[h + 1]
+        args = (w["mass"][s:e], w["x"][s:e], w["y"][s:e], w["z"][s:e])
+        t = np.stack([w[k][s:e].astype(np.float64) for k in "xyz"], 1)
+        Ao = oracle.acc_soft_f64(*args, t[:, 0], t[:, 1], t[:, 2], EPS)
+        mag, _ = _terms_abs(*args, t, EPS)
+        want = -G * w["mass"][s:e, None].astype(np.float64) * Ao
+        tol = 4e-6 * G * w["mass"][s:e, None] * mag
+        assert np.all(np.abs(g[s:e] - want) <= tol), h
+    print(f"haloes={len(off) - 1} members={off[-1]}", flush=True)
+
+
+def test_refine_centres_reaches_fp64_minimum(gpu):
+    from pyhalma_amd import centring
+
+    w = _haloes(n_haloes=4, seed=13, nscale=0.02)
+    off = w["offsets"]
+    res = centring.refine_centres(w["x"], w["y"], w["z"], w["mass"], off, w["seeds"], EPS,
+                                  max_iter=300, tol=1e-5)
+    c = res.centres.cpu().numpy()
+    it = res.iters.cpu().numpy()
+    print(f"haloes={len(off) - 1} iters={it.tolist()}", flush=True)
+    for h in range(len(off) - 1):
+        s, e = off[h], off[h + 1]
+        args = (w["mass"][s:e], w["x"][s:e], w["y"][s:e], w["z"][s:e])
+        a = w["a"][h]
+        assert np.linalg.norm(c[h] - w["centres"][h]) < 0.1 * a
+        # stationary in FP64
+        Ao = oracle.acc_soft_f64(*args, c[h:h + 1, 0], c[h:h + 1, 1], c[h:h + 1, 2], EPS)[0]
+        _, wo = _terms_abs(*args, c[h:h + 1], EPS)
+        assert np.linalg.norm(Ao / wo[0]) < 1e-3 * EPS
+        Sc = oracle.pot_soft_f64(*args, c[h:h + 1, 0], c[h:h + 1, 1], c[h:h + 1, 2], EPS)[0]
+        # at least as deep as the most-bound particle (FP64 brute force over all of them)
+        Sp = oracle.pot_soft_f64(*args, *(w[k][s:e].astype(np.float64) for k in "xyz"), EPS)
+        assert Sc >= Sp.max()
+        # the deepest point of a 7^3 FP64 grid spanning +-EPS/4 around the centre
+        g1 = np.linspace(-EPS / 4, EPS / 4, 7)
+        gx, gy, gz = np.meshgrid(g1, g1, g1, indexing="ij")
+        pts = c[h] + np.stack([gx.ravel(), gy.ravel(), gz.ravel()], 1)
+        Sg = oracle.pot_soft_f64(*args, pts[:, 0], pts[:, 1], pts[:, 2], EPS)
+        assert Sc >= Sg.max() * (1 - 1e-12)
