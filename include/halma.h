@@ -211,6 +211,31 @@ int halma_unbind_plain_f64(int64_t n_halo, const int64_t *offsets, const int32_t
                            size_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------------------------------------------
+ * Batched single energy test with exact decisions.  Replaces the per-halo energy test
+ * of halo_gas.RPS (halo_gas.py:456-476: sources = gas + dark matter + stars, tests =
+ * gas cells, factor 2, v_h = the halo's velocity) and of
+ * escape_velocity_unbinding_fortran (halo_properties.py:333-359, factor FACTOR_V^2),
+ * for every halo in one call, without the reference's random subsampling of large
+ * source groups (every source is summed).
+ *   E_i = 1/2 |v_i - v_h|^2 - factor * G * S_i,  G = 4.3e-9,  bound[i] = (E_i <= 0)
+ * with S_i = sum_j m_j / r_ij over halo h's sources [src_off[h], src_off[h+1]) whose
+ * three coordinates all differ from test i's; tests [tst_off[h], tst_off[h+1]) (HOST
+ * offsets).  halo_v: 3 doubles per halo (device).  bound and s_out (may be NULL) are
+ * indexed like the tests.  Decisions equal those made with the FP64 sum in the oracle's
+ * fixed order (band re-test, see halma_unbind_batched).  stats (HOST, may be NULL):
+ * [0] tests re-tested in FP64 (the call then synchronises the stream).
+ * ---------------------------------------------------------------------------------- */
+size_t halma_energy_test_workspace_size(int64_t n_halo, const int64_t *src_off,
+                                        const int64_t *tst_off);
+int halma_energy_test_batched(int64_t n_halo, const int64_t *src_off, const float *sm,
+                              const float *sx, const float *sy, const float *sz,
+                              const int64_t *tst_off, const float *tx, const float *ty,
+                              const float *tz, const void *vx, const void *vy, const void *vz,
+                              int vel_dtype, const double *halo_v, double factor,
+                              uint8_t *bound, double *s_out, int64_t *stats, void *workspace,
+                              size_t workspace_bytes, void *stream);
+
+/* ------------------------------------------------------------------------------------
  * Softened potential and its gradient, for the potential-minimum centring of haloes
  * (the differentiable extension of the unbinding sum: no reference function computes
  * it; the reference's most-bound position, halo_gas.py:498-634, is its discrete

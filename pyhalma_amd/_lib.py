@@ -31,6 +31,10 @@ SIGNATURES = {
     "halma_device_cu_count": (_c.c_int, []),
     "halma_pot_sum_workspace_size": (_size, [_i64, _i64]),
     "halma_soft_workspace_size": (_size, [_i64, _i64p, _i64p]),
+    "halma_energy_test_workspace_size": (_size, [_i64, _i64p, _i64p]),
+    "halma_energy_test_batched": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _i64p, _p, _p, _p,
+                                             _p, _p, _p, _c.c_int, _p, _dbl, _p, _p, _i64p,
+                                             _p, _size, _p]),
     "halma_pot_soft_batched": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _i64p, _p, _p, _p,
                                           _dbl, _p, _p, _p, _p, _size, _p]),
     "halma_pot_sum_f32": (_c.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p,

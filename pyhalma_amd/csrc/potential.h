@@ -48,6 +48,14 @@ struct DynSets {
     const int64_t* n_items = nullptr;
 };
 
+// Batched FP32 compensated pair sums over H haloes (HOST CSR offsets), FP64 (and/or
+// FP32) result per test; scratch from pot_batched_workspace_bytes.  Stream-ordered.
+size_t pot_batched_workspace_bytes(int64_t H, const int64_t* src_off, const int64_t* tst_off);
+int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const float* sx,
+                const float* sy, const float* sz, const int64_t* tst_off, const float* tx,
+                const float* ty, const float* tz, double* out_f64, float* out_f32, void* ws,
+                size_t ws_bytes, hipStream_t stream);
+
 // Launches the persistent FP32 kernel for a plan whose tables are already on the
 // device.  head must be a device int holding 0 at launch: this launcher clears it
 // first unless reset_head is false (the caller then clears it earlier in stream order).

@@ -355,7 +355,13 @@ static PotWs carve_pot(void* base, const PotPlan& p) {
     return w;
 }
 
-static int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const float* sx,
+size_t pot_batched_workspace_bytes(int64_t H, const int64_t* src_off, const int64_t* tst_off) {
+    PotPlan plan;
+    if (plan_pot(H, src_off, tst_off, plan, false)) return 0;
+    return carve_pot(nullptr, plan).bytes;
+}
+
+int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const float* sx,
                        const float* sy, const float* sz, const int64_t* tst_off,
                        const float* tx, const float* ty, const float* tz, double* out_f64,
                        float* out_f32, void* ws, size_t ws_bytes, hipStream_t stream) {

@@ -126,6 +126,43 @@ def pot_soft_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, eps, pot=True, ac
     return out
 
 
+def energy_test_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, vx, vy, vz, halo_v,
+                        factor, with_pot=False):
+    """One energy test per test particle, every halo at once (halma_energy_test_batched):
+    bound iff 1/2|v - v_h|^2 - factor*G*S <= 0, S over the halo's sources (CSR host
+    offsets).  Returns (bound uint8 tensor, S float64 tensor or None, tests re-tested in
+    FP64)."""
+    import ctypes
+    from .device import dtype_flag
+    L = _lib_()
+    src_off = np.asarray(src_off, np.int64)
+    tst_off = np.asarray(tst_off, np.int64)
+    H = len(src_off) - 1
+    if len(tst_off) != H + 1:
+        raise ValueError("offset arrays must have the same length")
+    m, x, y, z = (to_dev(a, np.float32) for a in (m, x, y, z))
+    tx, ty, tz = (to_dev(a, np.float32) for a in (tx, ty, tz))
+    vx, vy, vz = _coords(vx, vy, vz)
+    hv = to_dev(np.asarray(halo_v, np.float64).reshape(-1, 3) if not isinstance(halo_v, torch.Tensor)
+                else halo_v.double().reshape(-1, 3), np.float64).contiguous()
+    if hv.shape[0] != H:
+        raise ValueError("halo_v must hold 3 values per halo")
+    nt = int(tx.numel())
+    bound = torch.zeros(max(nt, 1), dtype=torch.uint8, device=tx.device)
+    S = torch.zeros(max(nt, 1), dtype=torch.float64, device=tx.device) if with_pot else None
+    st = (ctypes.c_int64 * 1)()
+    if H > 0 and nt > 0:
+        so = _lib.i64_array(src_off)
+        to = _lib.i64_array(tst_off)
+        ws = workspace(L.halma_energy_test_workspace_size(H, so, to))
+        rc = L.halma_energy_test_batched(H, so, ptr(m), ptr(x), ptr(y), ptr(z), to, ptr(tx),
+                                         ptr(ty), ptr(tz), ptr(vx), ptr(vy), ptr(vz),
+                                         dtype_flag(vx), ptr(hv), float(factor), ptr(bound),
+                                         ptr(S), st, ptr(ws), ws.numel(), stream_ptr())
+        _lib.check(rc, "halma_energy_test_batched")
+    return bound[:nt], (S[:nt] if S is not None else None), int(st[0])
+
+
 def _coords(x, y, z):
     """Device coordinate tensors sharing one float dtype (float32 kept, else float64)."""
     from .device import float_tensor

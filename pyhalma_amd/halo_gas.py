@@ -12,7 +12,8 @@ the O(N^2) sums run on the GPU through ``pyhalma_amd.particle`` (libhalma).
 RPS and most_bound_particle keep the reference's sequence of source groups and its
 subsampling above BRUTE_FORCE_LIM (np.random.choice with replacement from the global
 numpy RNG, drawn in the same order), so with a seeded ``np.random`` they draw the same
-samples the reference would.  The AMR source-gathering functions of the same module
+samples the reference would.  ``RPS_batched`` is the whole-catalogue form of RPS (every
+halo in one call, every source summed, exact decisions).  The AMR source-gathering functions of the same module
 (patch_to_particles, AMRgrid_to_particles, st_gas_dm_particles_inside) need MASCLET
 grid data and are outside this package.
 """
@@ -154,3 +155,72 @@ def AMRgrid_to_particles(*args, **kwargs):
 
 def st_gas_dm_particles_inside(*args, **kwargs):
     raise NotImplementedError("source gathering (halo_gas.py:223-277) needs MASCLET data")
+
+
+def _source_csr(groups, H):
+    """Per-halo concatenation [group 0 | group 1 | ...] of CSR source groups (x, y, z,
+    mass, offsets); returns the source offsets (host) and the packed device arrays."""
+    import torch
+
+    from .device import to_dev
+    counts = np.zeros(H, np.int64)
+    for g in groups:
+        counts += np.diff(np.asarray(g[4], np.int64))
+    off = np.zeros(H + 1, np.int64)
+    off[1:] = np.cumsum(counts)
+    n = int(off[-1])
+    dev = to_dev(np.zeros(1, np.float32)).device
+    out = [torch.empty(max(n, 1), dtype=torch.float32, device=dev) for _ in range(4)]
+    base = torch.from_numpy(off[:-1].copy()).to(dev)
+    filled = torch.zeros(H, dtype=torch.int64, device=dev)
+    for g in groups:
+        goff = np.asarray(g[4], np.int64)
+        gcnt = torch.from_numpy(np.diff(goff)).to(dev)
+        ng = int(goff[-1] - goff[0])
+        if ng == 0:
+            continue
+        halo = torch.repeat_interleave(torch.arange(H, device=dev), gcnt)
+        local = torch.arange(ng, device=dev) - torch.from_numpy(goff[:-1] - goff[0]).to(dev)[halo]
+        dest = base[halo] + filled[halo] + local
+        for k in range(4):
+            out[k][dest] = to_dev(np.asarray(g[k])[goff[0]:goff[-1]], np.float32)
+        filled += gcnt
+    return off, out
+
+
+def RPS_batched(gas, gas_offsets, halo_v, dm=None, stars=None):
+    """RPS (halo_gas.py:285-492) for every halo at once.  ``gas`` = (x, y, z, vx, vy, vz,
+    mass, temp) in halo order with CSR host ``gas_offsets``; ``dm`` / ``stars`` =
+    (x, y, z, mass, offsets) or None; ``halo_v`` (H, 3) the haloes' velocities.  Sources
+    of halo h = its gas, then its DM, then its stars, all summed (no subsampling).
+    Returns (H, 4) float64: total gas mass, cold bound fraction, cold unbound mass, hot
+    unbound mass (cold iff T < 5e4 K), and the per-cell bound mask (device uint8)."""
+    import torch
+
+    from . import engine
+    from .device import to_dev
+    gas_offsets = np.asarray(gas_offsets, np.int64)
+    H = len(gas_offsets) - 1
+    gx, gy, gz, gvx, gvy, gvz, gm, gt = gas
+    groups = [(gx, gy, gz, gm, gas_offsets)] + [g for g in (dm, stars) if g is not None]
+    src_off, (sx, sy, sz, sm) = _source_csr(groups, H)
+    bound, _, _ = engine.energy_test_batched(src_off, sm, sx, sy, sz, gas_offsets, gx, gy, gz,
+                                             gvx, gvy, gvz, halo_v, 2.0)
+    mass = to_dev(gm, np.float64)
+    temp = to_dev(gt, np.float64)
+    dev = mass.device
+    halo = torch.repeat_interleave(torch.arange(H, device=dev),
+                                   torch.from_numpy(np.diff(gas_offsets)).to(dev))
+    b = bound.to(torch.bool)
+    cold = temp < 5 * 1e4
+
+    def seg(v):
+        return torch.zeros(H, dtype=torch.float64, device=dev).index_add(0, halo, v)
+
+    total = seg(mass)
+    cold_bound = seg(torch.where(cold & b, mass, torch.zeros_like(mass)))
+    unb_cold = seg(torch.where(cold & ~b, mass, torch.zeros_like(mass)))
+    unb_hot = seg(torch.where(~cold & ~b, mass, torch.zeros_like(mass)))
+    frac = torch.where(total != 0, cold_bound / torch.where(total != 0, total, 1.0),
+                       torch.zeros_like(total))
+    return torch.stack([total, frac, unb_cold, unb_hot], 1).cpu().numpy(), bound

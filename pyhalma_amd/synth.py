@@ -132,8 +132,11 @@ def nfw_radii(rng, n, c, rs):
 
 
 def gas_haloes(seed=20261205, n_haloes=500, nmin=1000, nmax=200000, scale=1.0,
-               box=40.0, m_cell=1e5):
-    """Batched gas-cell haloes.  Returns CSR-ordered arrays with ``offsets``."""
+               box=40.0, m_cell=1e5, dm_per_gas=0.0, st_per_gas=0.0):
+    """Batched gas-cell haloes.  Returns CSR-ordered arrays with ``offsets``.  With
+    dm_per_gas / st_per_gas > 0, also NFW dark-matter (5 m_cell each) and stellar
+    (0.1 m_cell) particles per halo, in their own CSR arrays (``dm``, ``stars``:
+    (x, y, z, mass, offsets)), drawn after all the gas so the gas is unchanged."""
     rng = np.random.default_rng(seed)
     nh = max(1, int(round(n_haloes * scale)))
     sizes = np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=nh))).astype(np.int64)
@@ -161,10 +164,27 @@ def gas_haloes(seed=20261205, n_haloes=500, nmin=1000, nmax=200000, scale=1.0,
     x = np.concatenate(xs)
     v = np.concatenate(vs)
     f32 = np.float32
-    return dict(x=x[:, 0].astype(f32), y=x[:, 1].astype(f32), z=x[:, 2].astype(f32),
-                vx=v[:, 0].astype(f32), vy=v[:, 1].astype(f32), vz=v[:, 2].astype(f32),
-                mass=np.concatenate(ms).astype(f32), temp=np.concatenate(ts).astype(f32),
-                offsets=offsets, bulk=np.array(bulks), sizes=sizes)
+    out = dict(x=x[:, 0].astype(f32), y=x[:, 1].astype(f32), z=x[:, 2].astype(f32),
+               vx=v[:, 0].astype(f32), vy=v[:, 1].astype(f32), vz=v[:, 2].astype(f32),
+               mass=np.concatenate(ms).astype(f32), temp=np.concatenate(ts).astype(f32),
+               offsets=offsets, bulk=np.array(bulks), sizes=sizes)
+    for key, per, mfac in (("dm", dm_per_gas, 5.0), ("stars", st_per_gas, 0.1)):
+        if per <= 0:
+            continue
+        cnt = np.maximum(1, (sizes * per).astype(np.int64))
+        off = np.zeros(nh + 1, np.int64)
+        off[1:] = np.cumsum(cnt)
+        ps = []
+        for k in range(nh):
+            n = int(cnt[k])
+            c = rng.uniform(5.0, 15.0)
+            rv = 0.05 * (sizes[k] / 1e4) ** (1.0 / 3.0) + 0.02
+            ps.append(nfw_radii(rng, n, c, rv / c)[:, None] * _unit_vectors(rng, n)
+                      + centres[k])
+        p = np.concatenate(ps)
+        out[key] = (p[:, 0].astype(f32), p[:, 1].astype(f32), p[:, 2].astype(f32),
+                    np.full(len(p), mfac * m_cell, f32), off)
+    return out
 
 
 def bridged_pairs(seed=20261206, n_pairs=4000, n_per_pair=2000, bridge_frac=0.05,
