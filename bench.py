@@ -50,6 +50,11 @@ def parse():
                     help="run one 256 MiB device copy first (scales profiler byte counters)")
     ap.add_argument("--cpu-frac", type=float, default=0.5,
                     help="fraction of the box (along x) used for the CPU baseline sample")
+    ap.add_argument("--workload", default="clumps",
+                    choices=["clumps", "gas", "bridged", "centring"],
+                    help="clumps (default): the headline 2.1M-particle step; the others are "
+                         "the gas batch, the bridged-pair FoF stress case and the "
+                         "potential-minimum centring at their full sizes")
     return ap.parse_args()
 
 
@@ -68,6 +73,13 @@ def main():
     import torch
 
     from pyhalma_amd import engine, synth
+
+    if args.workload != "clumps":
+        torch.cuda.init()
+        out = {"gas": bench_gas, "bridged": bench_bridged,
+               "centring": bench_centring}[args.workload](args)
+        print(json.dumps(out))
+        return
 
     torch.cuda.init()
     dev = torch.device("cuda", 0)
@@ -198,6 +210,161 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_frac)
     print(json.dumps(out))
+
+
+def _timed(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, r
+
+
+def _line(metric, value, unit, step_s, args, config, **extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+         "warmup": args.warmup, "ms_per_step": step_s * 1e3, "higher_is_better": True,
+         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+         "data": "synthetic (seeded)", "config": config}
+    d.update(extra)
+    return d
+
+
+def bench_gas(args):
+    """Gas batch: 500 NFW gas haloes of 1e3-2e5 cells, one exact energy test per cell
+    (factor 2, sources = the halo's cells, halo_gas.py:285-492 without subsampling),
+    every halo in one call.  Full-size parity: a random sample of cells is re-decided
+    with the FP64 fixed-order sum on the GPU."""
+    import numpy as np
+    import torch
+
+    from pyhalma_amd import engine, halo_gas, synth
+
+    t0 = time.time()
+    w = synth.gas_haloes()
+    off = w["offsets"]
+    H = len(off) - 1
+    n = int(off[-1])
+    pairs = float((np.diff(off).astype(np.float64) ** 2).sum())
+    log(f"gas workload: {H} haloes, {n} cells, {pairs:.3e} pairs ({time.time() - t0:.1f}s)")
+    dev = torch.device("cuda", 0)
+    g = [torch.from_numpy(w[k]).to(dev) for k in ("x", "y", "z", "vx", "vy", "vz", "mass",
+                                                   "temp")]
+    bulk = torch.from_numpy(w["bulk"]).to(dev)
+    step_s, (out, bound) = _timed(lambda: halo_gas.RPS_batched(g, off, bulk), args.steps,
+                                  args.warmup)
+    log(f"gas: {step_s * 1e3:.1f} ms/step")
+    # full-size parity on a sample: FP64 fixed-order sums of sampled cells
+    rng = np.random.default_rng(1)
+    sample = np.sort(rng.choice(n, 256, replace=False))
+    halo = np.searchsorted(off, sample, side="right") - 1
+    mism = 0
+    for i, h in zip(sample, halo):
+        a, b = off[h], off[h + 1]
+        S = engine.pot_sum_f64_ordered(g[6][a:b], g[0][a:b], g[1][a:b], g[2][a:b],
+                                       g[0][i:i + 1], g[1][i:i + 1], g[2][i:i + 1])
+        dv = np.array([w[k][i] for k in ("vx", "vy", "vz")], np.float64) - w["bulk"][h]
+        kin = 0.5 * ((dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2])
+        E = kin - 2.0 * engine_g() * float(S[0])
+        mism += int((E <= 0.0) != bool(bound[i].item()))
+    return _line("gas cells/s, exact batched energy test (500 NFW haloes, 1e3-2e5 cells)",
+                 n / step_s, "cells/s", step_s, args,
+                 {"workload": "gas batch: 500 NFW haloes, cells log-uniform 1e3-2e5, FP32, "
+                              "factor 2, sources = each halo's cells", "cells": n,
+                  "haloes": H, "pairs": pairs},
+                 pairs_per_s=pairs / step_s,
+                 achieved_tflops=pairs * FLOP_PER_PAIR / step_s / 1e12,
+                 sample_mismatches=mism, sample_size=int(len(sample)),
+                 bound_fraction=float(bound.float().mean().item()))
+
+
+def engine_g():
+    return 4.3 * 1e-3 * 1e-6
+
+
+def bench_bridged(args):
+    """Bridged stress case: 8M stars in 4000 Plummer pairs joined by a 5% bridge; FoF
+    labels + CSR groups + gather + reductions + iterated unbinding of the groups.  FoF
+    must merge each pair: every group of >= 50 members holds exactly one pair's
+    particles, checked against the generator's pair ids."""
+    import numpy as np
+    import torch
+
+    from pyhalma_amd import engine, synth
+
+    t0 = time.time()
+    w = synth.bridged_pairs()
+    n = len(w["x"])
+    ll = w["ll"]
+    log(f"bridged workload: {n} particles, ll={ll:.5f} ({time.time() - t0:.1f}s)")
+    dev = torch.device("cuda", 0)
+    D = {k: torch.from_numpy(w[k]).to(dev) for k in ("x", "y", "z", "vx", "vy", "vz", "mass")}
+
+    def step():
+        lab = engine.fof_labels(D["x"], D["y"], D["z"], ll)
+        groups = engine.groups_from_labels(lab, MINP)
+        md = engine.gather_members(groups, D["x"], D["y"], D["z"], D["mass"], D["vx"], D["vy"],
+                                   D["vz"])
+        red = engine.group_reduce(md.groups, md.x, md.y, md.z, md.m, md.vx, md.vy, md.vz)
+        ub = engine.unbind(md.groups, md.x, md.y, md.z, md.vx, md.vy, md.vz, md.m, factor=1.0)
+        return groups, red, ub
+
+    step_s, (groups, red, ub) = _timed(step, args.steps, args.warmup)
+    mem = groups.members[: groups.n_members].cpu().numpy()
+    offs = groups.offsets_host
+    pid = w["pair_id"][mem]
+    seg_min = np.minimum.reduceat(pid, offs[:-1]) if groups.n_groups else np.array([])
+    seg_max = np.maximum.reduceat(pid, offs[:-1]) if groups.n_groups else np.array([])
+    pure = int((seg_min == seg_max).sum())
+    pairs_hit = len(np.unique(seg_min[seg_min == seg_max]))
+    log(f"bridged: {step_s * 1e3:.1f} ms/step; groups={groups.n_groups} pure={pure} "
+        f"pairs found={pairs_hit}")
+    return _line("particles/s FoF+unbinding, 8M-particle bridged pairs", n / step_s,
+                 "particles/s", step_s, args,
+                 {"workload": "4000 Plummer pairs at 1-3 a joined by 5% bridges, 8M FP32, "
+                              "groups>=50, physical-mode unbinding", "particles": n,
+                  "groups": groups.n_groups, "passes": ub.stats["passes"]},
+                 groups_single_pair=pure, pairs_merged=pairs_hit,
+                 members=groups.n_members, pairs_evaluated=ub.stats["pairs"],
+                 bound_fraction=float(ub.bound.float().mean().item()))
+
+
+def bench_centring(args):
+    """Differentiable centring: 200 Hernquist haloes of 5e4-5e5 particles, seeds offset
+    by 0.1-0.5 a; autograd-driven descent on the softened potential (eps = 0.5 kpc) to
+    the potential-minimum centre of every halo at once."""
+    import numpy as np
+    import torch
+
+    from pyhalma_amd import centring, synth
+
+    t0 = time.time()
+    w = synth.hernquist_haloes()
+    off = w["offsets"]
+    n = int(off[-1])
+    log(f"centring workload: {len(off) - 1} haloes, {n} particles ({time.time() - t0:.1f}s)")
+    dev = torch.device("cuda", 0)
+    D = [torch.from_numpy(w[k]).to(dev) for k in ("x", "y", "z", "mass")]
+    eps = 5e-4
+    step_s, res = _timed(lambda: centring.refine_centres(*D, off, w["seeds"], eps,
+                                                         max_iter=100, tol=1e-5),
+                         args.steps, args.warmup)
+    c = res.centres.cpu().numpy()
+    err = np.linalg.norm(c - w["centres"], axis=1) / w["a"]
+    it = res.iters.cpu().numpy()
+    log(f"centring: {step_s * 1e3:.1f} ms/step; iterations max {it.max()} "
+        f"median {np.median(it)}; |c - c_true|/a median {np.median(err):.2e}")
+    return _line("particles/s potential-minimum centring (200 Hernquist haloes)", n / step_s,
+                 "particles/s", step_s, args,
+                 {"workload": "200 Hernquist haloes of 5e4-5e5, seeds 0.1-0.5 a off, FP32, "
+                              "eps 0.5 kpc", "particles": n, "haloes": len(off) - 1},
+                 iterations_max=int(it.max()), iterations_median=float(np.median(it)),
+                 offset_over_a_median=float(np.median(err)),
+                 offset_over_a_max=float(err.max()))
 
 
 def cpu_baseline(w, frac):

@@ -183,7 +183,10 @@ def _source_csr(groups, H):
         local = torch.arange(ng, device=dev) - torch.from_numpy(goff[:-1] - goff[0]).to(dev)[halo]
         dest = base[halo] + filled[halo] + local
         for k in range(4):
-            out[k][dest] = to_dev(np.asarray(g[k])[goff[0]:goff[-1]], np.float32)
+            col = g[k]
+            if not isinstance(col, torch.Tensor):
+                col = np.asarray(col)
+            out[k][dest] = to_dev(col[goff[0]:goff[-1]], np.float32)
         filled += gcnt
     return off, out
 

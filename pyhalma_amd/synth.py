@@ -123,7 +123,7 @@ def nfw_radii(rng, n, c, rs):
     target = rng.uniform(size=n) * _nfw_m(c)
     lo = np.zeros(n)
     hi = np.full(n, float(c))
-    for _ in range(60):
+    for _ in range(45):  # bracket below 1e-12 of c
         mid = 0.5 * (lo + hi)
         less = _nfw_m(mid) < target
         lo = np.where(less, mid, lo)
@@ -190,14 +190,19 @@ def gas_haloes(seed=20261205, n_haloes=500, nmin=1000, nmax=200000, scale=1.0,
 def bridged_pairs(seed=20261206, n_pairs=4000, n_per_pair=2000, bridge_frac=0.05,
                   box=40.0, scale=1.0):
     """Pairs of Plummer spheres joined by uniform cylinders.  The linking length is
-    0.35 a (a = sphere scale radius), which the bridge's axial density percolates."""
+    0.35 a (a = sphere scale radius), which the bridge's axial density percolates.
+    Velocities (from a second generator, so positions do not depend on them): each
+    sphere moves along the pair axis at -+v_rel/2 plus its virial dispersion, with
+    v_rel = f * sqrt(2 G M_pair / sep), f uniform in [0.5, 2], so some pairs are bound
+    and some fly apart; bridge particles move with the linear interpolation between the
+    two spheres."""
     rng = np.random.default_rng(seed)
     npairs = max(1, int(round(n_pairs * scale)))
     L = box * scale ** (1.0 / 3.0)
     nb = int(round(bridge_frac * n_per_pair))
     ns = (n_per_pair - nb) // 2
     a = 0.005 * (ns / 2e4) ** (1.0 / 3.0)
-    xs, pair_id = [], []
+    xs, pair_id, geo = [], [], []
     centres = rng.uniform(-L / 2, L / 2, size=(npairs, 3))
     for k in range(npairs):
         sep = rng.uniform(1.0, 3.0) * a
@@ -219,14 +224,29 @@ def bridged_pairs(seed=20261206, n_pairs=4000, n_per_pair=2000, bridge_frac=0.05
               + (rr * np.sin(ph))[:, None] * e2)
         xs += [p1, p2, pb]
         pair_id.append(np.full(2 * ns + nb, k))
+        geo.append((axis, sep, t))
     x = np.concatenate(xs)
     pid = np.concatenate(pair_id)
     perm = rng.permutation(len(x))
     x, pid = x[perm], pid[perm]
+    # velocities, in generation order, then the same permutation
+    vr = np.random.default_rng(seed + 7919)
+    m_p = 1e5
+    sig = plummer_sigma1d(ns * m_p, a)
+    vs = []
+    for k in range(npairs):
+        axis, sep, t = geo[k]
+        vrel = vr.uniform(0.5, 2.0) * np.sqrt(2.0 * G * (2 * ns + nb) * m_p / sep)
+        v1 = -0.5 * vrel * axis
+        v2 = 0.5 * vrel * axis
+        vs += [v1 + vr.normal(scale=sig, size=(ns, 3)), v2 + vr.normal(scale=sig, size=(ns, 3)),
+               v1 + t[:, None] * (v2 - v1) + vr.normal(scale=sig, size=(nb, 3))]
+    v = np.concatenate(vs)[perm]
     f32 = np.float32
     n = len(x)
     return dict(x=x[:, 0].astype(f32), y=x[:, 1].astype(f32), z=x[:, 2].astype(f32),
-                mass=np.full(n, 1e5, f32), ll=0.35 * a, pair_id=pid, a=a, box=L)
+                vx=v[:, 0].astype(f32), vy=v[:, 1].astype(f32), vz=v[:, 2].astype(f32),
+                mass=np.full(n, m_p, f32), ll=0.35 * a, pair_id=pid, a=a, box=L)
 
 
 def hernquist_positions(rng, n, a):
