@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparator")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in pair-kernel tiling instead of autotuning it on "
+                         "the workload's haloes before the timed steps")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (scales profiler byte counters)")
     ap.add_argument("--cpu-frac", type=float, default=0.5,
@@ -122,6 +125,18 @@ def main():
                            time_pot=time_pot)
         return lab, groups, red, peak, ub
 
+    tuning = None
+    if not args.no_autotune:
+        from pyhalma_amd import autotune
+        _, groups, _, _, _ = step()
+        md = engine.gather_members(groups, D["x"], D["y"], D["z"], D["mass"])
+        t_at = time.time()
+        setting, ms, trials = autotune.autotune(md.groups.offsets_host, md.m, md.x, md.y, md.z)
+        tuning = {"setting": setting, "first_pass_ms": ms, "trials": len(trials),
+                  "default_ms": trials[0][1] if trials else ms}
+        log(f"autotune: {setting} {ms:.3f} ms (default {tuning['default_ms']:.3f} ms, "
+            f"{len(trials)} trials, {time.time() - t_at:.1f}s)")
+        del md
     for i in range(args.warmup):
         step()
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -162,6 +177,7 @@ def main():
                    "particles": n, "groups": groups.n_groups,
                    "members": groups.n_members, "passes": ub.stats["passes"]},
         "roofline": roofline,
+        "tuning": tuning,
         # inputs handed over from (pageable) host memory instead of resident in HBM
         "pcie_inclusive": {"h2d_ms": h2d_s * 1e3, "h2d_bytes": h2d_bytes,
                            "value": n / (step_s + h2d_s)},
