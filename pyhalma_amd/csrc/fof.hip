@@ -8,20 +8,26 @@
 //    cell index), so a sub-cell is one vertex of the cell graph and needs no pair test;
 //    every link spans at most 2 sub-cells per axis (ll/s < 1.733);
 // 3. occupied sub-cells go into an open-addressing hash table (64-bit keys, vector
-//    atomics); counts, per-particle ranks and each cell's minimum particle index come
-//    from the same pass, aggregated per block in LDS;
-// 4. an exclusive scan of the table numbers the cells and gives their particle ranges;
-//    particles are scattered into cell order with their coordinates;
-// 5. one wavefront per cell looks up its 62 half-stencil neighbours (one lane each) and,
-//    for every neighbour not already in the same component, searches for one linking
-//    pair with 64 lanes in parallel (early exit) and hooks the two roots (lock-free
-//    union-find over cells: the larger root index is hooked under the smaller, so parents
-//    only decrease and no cycle can form; finds halve paths);
+//    atomics); each particle records its cell's slot, and a new cell sets its bit in a
+//    hashed occupancy bitmap;
+// 4. a stable radix sort of (slot, particle) lists the particles cell by cell (cells
+//    numbered in slot order, particles ascending within a cell); coordinates are
+//    gathered into that order;
+// 5. links, in two rounds (the 13 neighbours at Chebyshev distance 1, then the 49 at
+//    distance 2): one lane per (cell, half-stencil offset) skips empty neighbours (a
+//    clear occupancy bit, else the table) and pairs already joined, tests small pairs
+//    of cells itself, and lists larger pairs for one wavefront each (64 lanes over the
+//    larger cell, early exit); linked cells are joined in a lock-free union-find over
+//    cells (the larger root index is hooked under the smaller, so parents only decrease
+//    and no cycle can form; finds halve paths);
 // 6. roots are read back with atomic reads (see k_roots), each component's minimum
 //    particle index is taken over its cells, and every particle gets it as label.
 #include <math.h>
 
+#include <algorithm>
+
 #include "halma_common.h"
+#include "radix.h"
 #include "scan.h"
 #include "staging.h"
 
@@ -171,149 +177,115 @@ __global__ void k_fill_i32n(int* p, int64_t n, int v) {
     if (i < n) p[i] = v;
 }
 
-// Binning, aggregated per block: a block takes kBinChunk particles, finds (or inserts)
-// each particle's cell in the global table, and counts its particles per cell in an LDS
-// table first; one global atomic per distinct cell and block then reserves the block's
-// ranks (and folds the block's minimum particle index into the entry's `cell` field,
-// which holds the cell minimum until k_cells numbers the cells).  Dense haloes put 10^4
-// particles into one sub-cell, and one global atomic per particle on that entry would
-// serialise them all.
-constexpr int kBinThreads = 256;
-constexpr int kBinPerThread = 8;
-constexpr int kBinChunk = kBinThreads * kBinPerThread;
-constexpr int kBinLds = 2 * kBinChunk;  // LDS slots, load factor <= 1/2
+// Binning: every particle finds (or inserts) its sub-cell in the hash table and records
+// the slot; a stable radix sort of (slot, particle index) then lists the particles cell
+// by cell, ascending within a cell (so a cell's first particle is its minimum index).
+// A newly inserted cell also sets its bit in a hashed occupancy bitmap, which lets the
+// link phase skip the hash lookups of empty neighbour cells.
+// Occupancy bit of a cell: the low bits of its Morton code, so that the neighbourhood of
+// a cell falls on a few bitmap lines (cells far apart may share a bit: a false positive
+// only costs a table lookup).
+__device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__device__ __forceinline__ uint64_t occ_bit(uint64_t key, uint64_t bmask) {
+    const uint64_t m = (1ull << kCellBits) - 1;
+    return (spread3(key >> (2 * kCellBits)) << 2 | spread3((key >> kCellBits) & m) << 1 |
+            spread3(key & m)) & bmask;
+}
 
 template <typename T>
-__global__ __launch_bounds__(kBinThreads) void k_fof_bin(
+__global__ __launch_bounds__(256) void k_fof_keys(
     const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, int64_t n,
     const double* __restrict__ box, double inv_s, Entry* __restrict__ tab, uint64_t mask,
-    int* __restrict__ pslot, int* __restrict__ prank, int* __restrict__ err) {
-    __shared__ int l_slot[kBinLds];
-    __shared__ int l_cnt[kBinLds];
-    __shared__ int l_min[kBinLds];
-    for (int k = threadIdx.x; k < kBinLds; k += kBinThreads) {
-        l_slot[k] = -1;
-        l_cnt[k] = 0;
-        l_min[k] = INT32_MAX;
-    }
-    __syncthreads();
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kBinChunk + threadIdx.x;
-    int loc[kBinPerThread], rk[kBinPerThread];
-#pragma unroll
-    for (int j = 0; j < kBinPerThread; ++j) {
-        loc[j] = -1;
-        rk[j] = 0;
-        const int64_t i = base + j * kBinThreads;
-        if (i >= n) continue;
-        const double q[3] = {(static_cast<double>(x[i]) - box[0]) * inv_s,
-                             (static_cast<double>(y[i]) - box[1]) * inv_s,
-                             (static_cast<double>(z[i]) - box[2]) * inv_s};
-        uint64_t key = 0;
-        bool bad = false;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            bad |= !(q[d] >= 0.0 && q[d] < kCellLimit);  // also catches NaN
-            const uint64_t c = bad ? 0 : static_cast<uint64_t>(floor(q[d]));
-            key = (key << kCellBits) | c;
-        }
-        if (bad) {
-            atomicOr(err, 1);
-            key = 0;
-        }
-        uint64_t h = slot_of(key, mask);
-        bool full = false;
-        for (uint64_t probe = 0;; ++probe) {
-            const uint64_t k = __hip_atomic_load(&tab[h].key, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            if (k == key) break;
-            if (k == kEmpty) {
-                uint64_t expected = kEmpty;
-                if (__hip_atomic_compare_exchange_strong(&tab[h].key, &expected, key,
-                                                         __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT) ||
-                    expected == key)
-                    break;
-            }
-            if (probe > mask) {  // table full (cannot happen with capacity >= 2n)
-                full = true;
-                break;
-            }
-            h = (h + 1) & mask;
-        }
-        if (full) {
-            atomicOr(err, 2);
-            pslot[i] = 0;
-            prank[i] = 0;
-            continue;
-        }
-        pslot[i] = static_cast<int>(h);
-        unsigned l = (static_cast<unsigned>(h) * 0x9e3779b1u) >> 20;  // 12 bits
-        for (;;) {
-            const int old = atomicCAS(&l_slot[l], -1, static_cast<int>(h));
-            if (old == -1 || old == static_cast<int>(h)) break;
-            l = (l + 1) & (kBinLds - 1);
-        }
-        loc[j] = static_cast<int>(l);
-        rk[j] = atomicAdd(&l_cnt[l], 1);
-        atomicMin(&l_min[l], static_cast<int>(i));
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < kBinLds; k += kBinThreads) {
-        const int h = l_slot[k];
-        if (h >= 0) {
-            l_cnt[k] = atomicAdd(&tab[h].cnt, l_cnt[k]);
-            atomicMin(&tab[h].cell, l_min[k]);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kBinPerThread; ++j)
-        if (loc[j] >= 0) prank[base + j * kBinThreads] = l_cnt[loc[j]] + rk[j];
-}
-static_assert(kBinLds == 4096, "the LDS slot hash takes the top 12 bits");
-
-// packed (count << 32 | occupied) per table slot for one scan that yields both the
-// particle offset (high word) and the cell number (low word) of every occupied slot
-__global__ void k_slot_pack(const Entry* __restrict__ tab, int64_t cap, int64_t* __restrict__ v) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= cap) return;
-    const int c = tab[i].cnt;
-    v[i] = (static_cast<int64_t>(c) << 32) | (c > 0 ? 1 : 0);
-}
-
-__global__ void k_cells(Entry* __restrict__ tab, const int64_t* __restrict__ scan, int64_t cap,
-                        int* __restrict__ cell_start, int* __restrict__ cell_cnt,
-                        uint64_t* __restrict__ cell_key, int* __restrict__ cell_min,
-                        int* __restrict__ parent, int* __restrict__ comp_min) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= cap) return;
-    const int c = tab[i].cnt;
-    if (c <= 0) return;
-    const int64_t v = scan[i];
-    const int cell = static_cast<int>(v & 0xffffffff);
-    cell_min[cell] = tab[i].cell;  // the binning left the cell's minimum index here
-    tab[i].cell = cell;
-    cell_start[cell] = static_cast<int>(v >> 32);
-    cell_cnt[cell] = c;
-    cell_key[cell] = tab[i].key;
-    parent[cell] = cell;
-    comp_min[cell] = INT32_MAX;
-}
-
-template <typename T>
-__global__ void k_scatter(const T* __restrict__ x, const T* __restrict__ y,
-                          const T* __restrict__ z, int64_t n, const int* __restrict__ pslot,
-                          const int* __restrict__ prank, const int64_t* __restrict__ scan,
-                          T* __restrict__ sx, T* __restrict__ sy,
-                          T* __restrict__ sz, int* __restrict__ pcell) {
+    uint32_t* __restrict__ skey, uint32_t* __restrict__ bits, uint64_t bmask,
+    int* __restrict__ err) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t v = scan[pslot[i]];
-    const int64_t pos = (v >> 32) + prank[i];
-    sx[pos] = x[i];
-    sy[pos] = y[i];
-    sz[pos] = z[i];
-    pcell[i] = static_cast<int>(v & 0xffffffff);
+    const double q[3] = {(static_cast<double>(x[i]) - box[0]) * inv_s,
+                         (static_cast<double>(y[i]) - box[1]) * inv_s,
+                         (static_cast<double>(z[i]) - box[2]) * inv_s};
+    uint64_t key = 0;
+    bool bad = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        bad |= !(q[d] >= 0.0 && q[d] < kCellLimit);  // also catches NaN
+        const uint64_t c = bad ? 0 : static_cast<uint64_t>(floor(q[d]));
+        key = (key << kCellBits) | c;
+    }
+    if (bad) {
+        atomicOr(err, 1);
+        key = 0;
+    }
+    uint64_t h = slot_of(key, mask);
+    for (uint64_t probe = 0;; ++probe) {
+        // a plain load: a slot's key never changes once set, so a stale value can only
+        // be a stale "empty", which the compare-and-swap below corrects
+        const uint64_t k = tab[h].key;
+        if (k == key) break;
+        if (k == kEmpty) {
+            uint64_t expected = kEmpty;
+            if (__hip_atomic_compare_exchange_strong(&tab[h].key, &expected, key,
+                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) {
+                const uint64_t b = occ_bit(key, bmask);  // this thread created the cell
+                atomicOr(bits + (b >> 5), 1u << (b & 31));
+                break;
+            }
+            if (expected == key) break;
+        }
+        if (probe > mask) {  // table full (cannot happen with capacity >= 2n)
+            atomicOr(err, 2);
+            h = 0;
+            break;
+        }
+        h = (h + 1) & mask;
+    }
+    skey[i] = static_cast<uint32_t>(h);
+}
+
+__global__ void k_run_heads(const uint32_t* __restrict__ s, int64_t n, int* __restrict__ head) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j < n) head[j] = (j == 0 || s[j] != s[j - 1]) ? 1 : 0;
+}
+
+// per sorted position j: its cell c (cells numbered in slot order), the particle's
+// coordinates in cell order, and, at the first particle of a cell, the cell's records
+template <typename T>
+__global__ void k_cells(const uint32_t* __restrict__ skey, const int32_t* __restrict__ perm,
+                        const int64_t* __restrict__ before, const int* __restrict__ head,
+                        int64_t n, Entry* __restrict__ tab, const T* __restrict__ x,
+                        const T* __restrict__ y, const T* __restrict__ z, T* __restrict__ sx,
+                        T* __restrict__ sy, T* __restrict__ sz, int* __restrict__ pcell,
+                        int* __restrict__ cell_start, uint64_t* __restrict__ cell_key,
+                        int* __restrict__ cell_min, int* __restrict__ parent,
+                        int* __restrict__ comp_min) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int hd = head[j];
+    const int c = static_cast<int>(before[j] + hd - 1);
+    const int32_t i = perm[j];
+    sx[j] = x[i];
+    sy[j] = y[i];
+    sz[j] = z[i];
+    pcell[i] = c;
+    if (hd) {
+        const uint32_t slot = skey[j];
+        tab[slot].cell = c;
+        cell_start[c] = static_cast<int>(j);
+        cell_key[c] = tab[slot].key;
+        cell_min[c] = i;  // stable sort: the cell's smallest particle index comes first
+        parent[c] = c;
+        comp_min[c] = INT32_MAX;
+    }
+    if (j == n - 1) cell_start[c + 1] = static_cast<int>(n);
 }
 
 // cell number of a key, or -1
@@ -329,74 +301,180 @@ __device__ __forceinline__ int table_find(const Entry* __restrict__ tab, uint64_
     return -1;
 }
 
+// Read-only find with plain loads, for the "already joined?" test before a pair search.
+// Parent pointers only ever move to ancestors (hooks attach roots, halving skips a
+// level), so every node reached here, even through a stale value, is an ancestor of x
+// in the current forest: equal results prove the two cells are joined, and a stale
+// "different" only costs a pair search.
+__device__ __forceinline__ int find_ro(const int* P, int x) {
+    for (int p = P[x]; p != x; p = P[x]) x = p;
+    return x;
+}
+
+// The 62 half-stencil offsets (dx, dy, dz) of a cell: the 13 at Chebyshev distance 1
+// first, then the 49 at distance 2, each group in (dz, dy, dx) order.
+__constant__ signed char kNbr[62][3] = {
+    {1, 0, 0}, {-1, 1, 0}, {0, 1, 0}, {1, 1, 0}, {-1, -1, 1}, {0, -1, 1}, {1, -1, 1},
+    {-1, 0, 1}, {0, 0, 1}, {1, 0, 1}, {-1, 1, 1}, {0, 1, 1}, {1, 1, 1},
+    {2, 0, 0}, {-2, 1, 0}, {2, 1, 0}, {-2, 2, 0}, {-1, 2, 0}, {0, 2, 0}, {1, 2, 0},
+    {2, 2, 0}, {-2, -2, 1}, {-1, -2, 1}, {0, -2, 1}, {1, -2, 1}, {2, -2, 1}, {-2, -1, 1},
+    {2, -1, 1}, {-2, 0, 1}, {2, 0, 1}, {-2, 1, 1}, {2, 1, 1}, {-2, 2, 1}, {-1, 2, 1},
+    {0, 2, 1}, {1, 2, 1}, {2, 2, 1}, {-2, -2, 2}, {-1, -2, 2}, {0, -2, 2}, {1, -2, 2},
+    {2, -2, 2}, {-2, -1, 2}, {-1, -1, 2}, {0, -1, 2}, {1, -1, 2}, {2, -1, 2}, {-2, 0, 2},
+    {-1, 0, 2}, {0, 0, 2}, {1, 0, 2}, {2, 0, 2}, {-2, 1, 2}, {-1, 1, 2}, {0, 1, 2},
+    {1, 1, 2}, {2, 1, 2}, {-2, 2, 2}, {-1, 2, 2}, {0, 2, 2}, {1, 2, 2}, {2, 2, 2}};
+constexpr int kNear = 13;
+constexpr int kLaneTest = 64;  // pair tests a lane does alone
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_fof_link(
-    const Entry* __restrict__ tab, uint64_t mask,
-    const int* __restrict__ cell_start, const int* __restrict__ cell_cnt,
+__device__ __forceinline__ bool within(const T* sx, const T* sy, const T* sz, int a, int b,
+                                       double ll2) {
+    const double dx = static_cast<double>(sx[a]) - static_cast<double>(sx[b]);
+    const double dy = static_cast<double>(sy[a]) - static_cast<double>(sy[b]);
+    const double dz = static_cast<double>(sz[a]) - static_cast<double>(sz[b]);
+    return (dx * dx + dy * dy) + dz * dz <= ll2;
+}
+
+// Link candidates, one lane per (cell, neighbour offset) of one distance class (NEAR:
+// the 13 at Chebyshev distance 1; otherwise the 49 at distance 2, launched after every
+// distance-1 link is in the forest, so that dense regions are recognised as joined and
+// need no search).  A lane skips empty neighbours (occupancy bit, then the table) and
+// pairs already joined, tests small pairs of cells itself and hooks them when linked,
+// and hands larger pairs to k_fof_big through a list (one atomic per wave); a full list
+// makes the lane test the pair itself.
+template <typename T, bool NEAR>
+__global__ __launch_bounds__(256) void k_fof_cand(
+    const Entry* __restrict__ tab, uint64_t mask, const uint32_t* __restrict__ bits,
+    uint64_t bmask, const int* __restrict__ cell_start,
     const uint64_t* __restrict__ cell_key, const T* __restrict__ sx, const T* __restrict__ sy,
     const T* __restrict__ sz, const int64_t* __restrict__ totals, double ll2,
-    int* __restrict__ parent) {
+    int* __restrict__ parent, int2* __restrict__ big, int64_t big_cap,
+    unsigned long long* __restrict__ n_big) {
+    constexpr int K = NEAR ? kNear : 62 - kNear;
+    constexpr int K0 = NEAR ? 0 : kNear;
     const int lane = __lane_id();
-    const int64_t n_cells = totals[0] & 0xffffffff;
-    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
+    const int64_t total = (totals[0] & 0xffffffff) * K;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     const uint64_t cmask = (1ull << kCellBits) - 1;
-    for (int64_t c64 = wave0; c64 < n_cells; c64 += nwaves) {
-        const int c = static_cast<int>(c64);
-        const uint64_t key = cell_key[c];
-        const int64_t cx = static_cast<int64_t>((key >> (2 * kCellBits)) & cmask);
-        const int64_t cy = static_cast<int64_t>((key >> kCellBits) & cmask);
-        const int64_t cz = static_cast<int64_t>(key & cmask);
-        // lane k < 62 -> the k-th offset after the centre in (dz, dy, dx) order
-        int dcell = -1;
-        if (lane < 62) {
-            const int idx = 63 + lane;
-            const int64_t nz = cz + idx / 25 - 2;
-            const int64_t ny = cy + (idx / 5) % 5 - 2;
-            const int64_t nx = cx + idx % 5 - 2;
+    // the wave iterates uniformly (lanes past the end idle) so that ballots are whole
+    for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~(kWave - 1));
+         i0 < total; i0 += stride) {
+        const int64_t idx = i0 + lane;
+        bool push = false;
+        int c = 0, d = -1;
+        if (idx < total) {
+            c = static_cast<int>(idx / K);
+            const int k = K0 + static_cast<int>(idx % K);
+            const uint64_t key = cell_key[c];
+            const int64_t nx = static_cast<int64_t>((key >> (2 * kCellBits)) & cmask) + kNbr[k][0];
+            const int64_t ny = static_cast<int64_t>((key >> kCellBits) & cmask) + kNbr[k][1];
+            const int64_t nz = static_cast<int64_t>(key & cmask) + kNbr[k][2];
             if (nx >= 0 && ny >= 0 && nz >= 0 && nx <= static_cast<int64_t>(cmask) &&
                 ny <= static_cast<int64_t>(cmask) && nz <= static_cast<int64_t>(cmask)) {
                 const uint64_t nkey = (static_cast<uint64_t>(nx) << (2 * kCellBits)) |
                                       (static_cast<uint64_t>(ny) << kCellBits) |
                                       static_cast<uint64_t>(nz);
-                dcell = table_find(tab, mask, nkey);
+                const uint64_t b = occ_bit(nkey, bmask);
+                if ((bits[b >> 5] >> (b & 31)) & 1u) d = table_find(tab, mask, nkey);
             }
-        }
-        uint64_t found = __ballot(dcell >= 0);
-        const int sc = cell_start[c];
-        const int nc = cell_cnt[c];
-        while (found) {
-            const int l = __builtin_ctzll(found);
-            found &= found - 1;
-            const int d = __shfl(dcell, l, kWave);
-            if (uf_find(parent, c) == uf_find(parent, d)) continue;
-            const int sd = cell_start[d];
-            const int nd = cell_cnt[d];
-            // lanes over the larger cell, loop over the smaller one
-            const bool c_small = nc <= nd;
-            const int sA = c_small ? sc : sd, nA = c_small ? nc : nd;
-            const int sB = c_small ? sd : sc, nB = c_small ? nd : nc;
-            bool linked = false;
-            for (int b0 = 0; b0 < nB && !linked; b0 += kWave) {
-                const int b = b0 + lane;
-                const bool valid = b < nB;
-                const int bb = valid ? sB + b : sB;
-                const double xb = static_cast<double>(sx[bb]);
-                const double yb = static_cast<double>(sy[bb]);
-                const double zb = static_cast<double>(sz[bb]);
-                for (int a = 0; a < nA; ++a) {
-                    const double dx = static_cast<double>(sx[sA + a]) - xb;
-                    const double dy = static_cast<double>(sy[sA + a]) - yb;
-                    const double dz = static_cast<double>(sz[sA + a]) - zb;
-                    const double d2 = (dx * dx + dy * dy) + dz * dz;
-                    if (__any(valid && d2 <= ll2)) {
-                        linked = true;
-                        break;
-                    }
+            if (d >= 0 && find_ro(parent, c) != find_ro(parent, d)) {
+                const int sc = cell_start[c], nc = cell_start[c + 1] - sc;
+                const int sd = cell_start[d], nd = cell_start[d + 1] - sd;
+                if (static_cast<int64_t>(nc) * nd <= kLaneTest) {
+                    bool linked = false;
+                    for (int a = 0; a < nc && !linked; ++a)
+                        for (int bb = 0; bb < nd; ++bb)
+                            if (within(sx, sy, sz, sc + a, sd + bb, ll2)) {
+                                linked = true;
+                                break;
+                            }
+                    if (linked) uf_union(parent, c, d);
+                } else {
+                    push = true;
                 }
             }
-            if (linked && lane == 0) uf_union(parent, c, d);
         }
+        const uint64_t pm = __ballot(push);
+        if (pm) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(n_big, static_cast<unsigned long long>(__popcll(pm)));
+            base = __shfl(base, 0, kWave);
+            if (push) {
+                const int64_t at = static_cast<int64_t>(base + __popcll(pm & ((1ull << lane) - 1ull)));
+                if (at < big_cap) {
+                    big[at] = make_int2(c, d);
+                } else {  // list full: test here (correct, slower)
+                    const int sc = cell_start[c], nc = cell_start[c + 1] - sc;
+                    const int sd = cell_start[d], nd = cell_start[d + 1] - sd;
+                    bool linked = false;
+                    for (int a = 0; a < nc && !linked; ++a)
+                        for (int bb = 0; bb < nd; ++bb)
+                            if (within(sx, sy, sz, sc + a, sd + bb, ll2)) {
+                                linked = true;
+                                break;
+                            }
+                    if (linked) uf_union(parent, c, d);
+                }
+            }
+        }
+    }
+}
+
+// Large pairs of cells, one wavefront each: skipped if joined meanwhile, else lanes run
+// over the larger cell four blocks of 64 at a time (loads in flight together) and loop
+// over the smaller cell, stopping at the first pair within ll.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fof_big(
+    const int2* __restrict__ big, const unsigned long long* __restrict__ n_big,
+    int64_t big_cap, const int* __restrict__ cell_start, const T* __restrict__ sx,
+    const T* __restrict__ sy, const T* __restrict__ sz, double ll2, int* __restrict__ parent) {
+    constexpr int U = 4;
+    const int lane = __lane_id();
+    const int64_t nb = static_cast<int64_t>(n_big[0]) < big_cap
+                           ? static_cast<int64_t>(n_big[0]) : big_cap;
+    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
+    for (int64_t e = wave0; e < nb; e += nwaves) {
+        const int2 pr = big[e];
+        const int c = pr.x, d = pr.y;
+        if (find_ro(parent, c) == find_ro(parent, d)) continue;
+        const int sc = cell_start[c], nc = cell_start[c + 1] - sc;
+        const int sd = cell_start[d], nd = cell_start[d + 1] - sd;
+        const bool c_small = nc <= nd;
+        const int sA = c_small ? sc : sd, nA = c_small ? nc : nd;
+        const int sB = c_small ? sd : sc, nB = c_small ? nd : nc;
+        bool linked = false;
+        for (int b0 = 0; b0 < nB && !linked; b0 += U * kWave) {
+            double xb[U], yb[U], zb[U];
+            bool valid[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int b = b0 + u * kWave + lane;
+                valid[u] = b < nB;
+                const int bb = valid[u] ? sB + b : sB;
+                xb[u] = static_cast<double>(sx[bb]);
+                yb[u] = static_cast<double>(sy[bb]);
+                zb[u] = static_cast<double>(sz[bb]);
+            }
+            for (int a = 0; a < nA; ++a) {
+                const double xa = static_cast<double>(sx[sA + a]);
+                const double ya = static_cast<double>(sy[sA + a]);
+                const double za = static_cast<double>(sz[sA + a]);
+                bool hit = false;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const double dx = xa - xb[u];
+                    const double dy = ya - yb[u];
+                    const double dz = za - zb[u];
+                    hit |= valid[u] && (dx * dx + dy * dy) + dz * dz <= ll2;
+                }
+                if (__any(hit)) {
+                    linked = true;
+                    break;
+                }
+            }
+        }
+        if (linked && lane == 0) uf_union(parent, c, d);
     }
 }
 
@@ -444,14 +522,17 @@ struct FofWs {
     double* box_part;
     double* box;
     Entry* tab;
-    int64_t* scan;
+    uint32_t* bits;   // hashed occupancy bitmap of the cells
     int64_t* totals;
     void* scan_ws;
-    int* pslot;
-    int* prank;
+    void* sort_ws;
+    uint32_t* skey;   // slot of each particle, then sorted
+    uint32_t* skey_sorted;
+    int32_t* perm;    // particle of each sorted position
+    int* head;
+    int64_t* before;  // exclusive scan of head
     int* pcell;
-    int* cell_start;
-    int* cell_cnt;
+    int* cell_start;  // n_cells + 1 entries
     uint64_t* cell_key;
     int* parent;
     int* root;
@@ -460,9 +541,14 @@ struct FofWs {
     void* sx;
     void* sy;
     void* sz;
+    int2* big;        // large pairs of cells left for k_fof_big
+    unsigned long long* n_big;
+    int64_t big_cap;
     int* err;
     size_t bytes;
     int64_t cap;
+    int64_t nbits;
+    int slot_bits;
 };
 
 int64_t table_capacity(int64_t n) {
@@ -471,22 +557,36 @@ int64_t table_capacity(int64_t n) {
     return cap;
 }
 
+// occupancy bits: the power of two >= 16 n (>= 2^15, <= 2^31), so that at most one
+// empty neighbour in 16 passes the bit test
+int64_t bitmap_bits(int64_t n) {
+    int64_t b = int64_t{1} << 15;
+    while (b < 16 * n && b < (int64_t{1} << 31)) b <<= 1;
+    return b;
+}
+
 FofWs carve_fof(void* base, int64_t n, size_t elt) {
     Carver c(base);
     FofWs w;
     const int64_t nn = n > 0 ? n : 1;
     w.cap = table_capacity(n);
+    w.slot_bits = 0;
+    while ((int64_t{1} << w.slot_bits) < w.cap) ++w.slot_bits;
+    w.nbits = bitmap_bits(n);
     w.box_part = c.take<double>(kBoxBlocks * 6);
     w.box = c.take<double>(8);
     w.tab = c.take<Entry>(w.cap);
-    w.scan = c.take<int64_t>(w.cap);
+    w.bits = c.take<uint32_t>(w.nbits / 32);
     w.totals = c.take<int64_t>(4);
-    w.scan_ws = c.take<char>(scan_workspace_bytes(w.cap));
-    w.pslot = c.take<int>(nn);
-    w.prank = c.take<int>(nn);
+    w.scan_ws = c.take<char>(scan_workspace_bytes(nn));
+    w.sort_ws = c.take<char>(radix_workspace_bytes(nn));
+    w.skey = c.take<uint32_t>(nn);
+    w.skey_sorted = c.take<uint32_t>(nn);
+    w.perm = c.take<int32_t>(nn);
+    w.head = c.take<int>(nn);
+    w.before = c.take<int64_t>(nn);
     w.pcell = c.take<int>(nn);
-    w.cell_start = c.take<int>(nn);
-    w.cell_cnt = c.take<int>(nn);
+    w.cell_start = c.take<int>(nn + 1);
     w.cell_key = c.take<uint64_t>(nn);
     w.parent = c.take<int>(nn);
     w.root = c.take<int>(nn);
@@ -495,9 +595,18 @@ FofWs carve_fof(void* base, int64_t n, size_t elt) {
     w.sx = c.take<char>(nn * elt);
     w.sy = c.take<char>(nn * elt);
     w.sz = c.take<char>(nn * elt);
+    w.big_cap = 2 * nn;
+    w.big = c.take<int2>(w.big_cap);
+    w.n_big = c.take<unsigned long long>(4);
     w.err = c.take<int>(64);
     w.bytes = c.used;
     return w;
+}
+
+__global__ void k_clear_words(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[i] = 0u;
 }
 
 inline unsigned blocks_for(int64_t n, int t = 256) {
@@ -516,28 +625,47 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     HALMA_LAUNCH_CHECK();
     k_clear_table<<<blocks_for(w.cap), 256, 0, st>>>(w.tab, w.cap);
     HALMA_LAUNCH_CHECK();
+    k_clear_words<<<1024, 256, 0, st>>>(w.bits, w.nbits / 32);
+    HALMA_LAUNCH_CHECK();
     // sub-cell side: ll/sqrt(3) with a 1e-6 relative margin (see the file header)
     const double side = ll / sqrt(3.0) * (1.0 - 1e-6);
     const double inv_s = 1.0 / side;
-    k_fof_bin<T><<<blocks_for(n, kBinChunk), kBinThreads, 0, st>>>(
-        x, y, z, n, w.box, inv_s, w.tab, static_cast<uint64_t>(w.cap - 1), w.pslot, w.prank,
-        w.err);
+    const uint64_t bmask = static_cast<uint64_t>(w.nbits - 1);
+    k_fof_keys<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.box, inv_s, w.tab,
+                                                 static_cast<uint64_t>(w.cap - 1), w.skey,
+                                                 w.bits, bmask, w.err);
     HALMA_LAUNCH_CHECK();
-    k_slot_pack<<<blocks_for(w.cap), 256, 0, st>>>(w.tab, w.cap, w.scan);
-    HALMA_LAUNCH_CHECK();
-    int rc = exclusive_scan_i64(w.scan, w.scan, w.cap, w.totals, w.scan_ws, st);
+    int rc = radix_sort_pairs(w.skey, nullptr, w.skey_sorted, w.perm, n, w.slot_bits,
+                              w.sort_ws, st);
     if (rc) return rc;
-    k_cells<<<blocks_for(w.cap), 256, 0, st>>>(w.tab, w.scan, w.cap, w.cell_start, w.cell_cnt,
-                                               w.cell_key, w.cell_min, w.parent, w.comp_min);
+    k_run_heads<<<blocks_for(n), 256, 0, st>>>(w.skey_sorted, n, w.head);
     HALMA_LAUNCH_CHECK();
-    k_scatter<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.pslot, w.prank, w.scan,
-                                                static_cast<T*>(w.sx), static_cast<T*>(w.sy),
-                                                static_cast<T*>(w.sz), w.pcell);
+    rc = exclusive_scan_i32_to_i64(w.head, w.before, n, w.totals, w.scan_ws, st);
+    if (rc) return rc;
+    k_cells<T><<<blocks_for(n), 256, 0, st>>>(
+        w.skey_sorted, w.perm, w.before, w.head, n, w.tab, x, y, z, static_cast<T*>(w.sx),
+        static_cast<T*>(w.sy), static_cast<T*>(w.sz), w.pcell, w.cell_start, w.cell_key,
+        w.cell_min, w.parent, w.comp_min);
     HALMA_LAUNCH_CHECK();
-    k_fof_link<T><<<grid_cells, 256, 0, st>>>(
-        w.tab, static_cast<uint64_t>(w.cap - 1), w.cell_start, w.cell_cnt, w.cell_key,
-        static_cast<const T*>(w.sx), static_cast<const T*>(w.sy), static_cast<const T*>(w.sz),
-        w.totals, ll * ll, w.parent);
+    k_clear_words<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(w.n_big), 4);
+    HALMA_LAUNCH_CHECK();
+    const T* sx = static_cast<const T*>(w.sx);
+    const T* sy = static_cast<const T*>(w.sy);
+    const T* sz = static_cast<const T*>(w.sz);
+    const unsigned cand_grid = static_cast<unsigned>(std::max(1, device_cu_count()) * 32);
+    k_fof_cand<T, true><<<cand_grid, 256, 0, st>>>(
+        w.tab, static_cast<uint64_t>(w.cap - 1), w.bits, bmask, w.cell_start, w.cell_key, sx,
+        sy, sz, w.totals, ll * ll, w.parent, w.big, w.big_cap, w.n_big);
+    HALMA_LAUNCH_CHECK();
+    k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big, w.big_cap, w.cell_start, sx, sy,
+                                             sz, ll * ll, w.parent);
+    HALMA_LAUNCH_CHECK();
+    k_fof_cand<T, false><<<cand_grid, 256, 0, st>>>(
+        w.tab, static_cast<uint64_t>(w.cap - 1), w.bits, bmask, w.cell_start, w.cell_key, sx,
+        sy, sz, w.totals, ll * ll, w.parent, w.big, w.big_cap, w.n_big + 1);
+    HALMA_LAUNCH_CHECK();
+    k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big + 1, w.big_cap, w.cell_start, sx,
+                                             sy, sz, ll * ll, w.parent);
     HALMA_LAUNCH_CHECK();
     k_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.parent, w.root);
     HALMA_LAUNCH_CHECK();
