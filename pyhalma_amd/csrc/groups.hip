@@ -28,6 +28,8 @@ namespace halma {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kBig = 1024;                  // threads of the per-group kernels
+constexpr int kMaxWaves = kBig / kWave;     // LDS slots of the block reductions
 constexpr int kNodes = 50;
 constexpr int kCells = kNodes * kNodes * kNodes;
 constexpr int kTable = 4096;      // LDS hash slots per group
@@ -158,7 +160,8 @@ __device__ void block_dd_sum(double& hi, double& lo, double* sh) {
     if (threadIdx.x == 0) {
         hi = sh[0];
         lo = sh[1];
-        for (int k = 1; k < kThreads / kWave; ++k) dd_add_dd(hi, lo, sh[2 * k], sh[2 * k + 1]);
+        for (int k = 1; k < static_cast<int>(blockDim.x) / kWave; ++k)
+            dd_add_dd(hi, lo, sh[2 * k], sh[2 * k + 1]);
     }
     __syncthreads();
 }
@@ -170,7 +173,7 @@ __device__ double block_max(double v, double* sh) {
     if (__lane_id() == 0) sh[w] = v;
     __syncthreads();
     double r = sh[0];
-    for (int k = 1; k < kThreads / kWave; ++k) r = fmax(r, sh[k]);
+    for (int k = 1; k < static_cast<int>(blockDim.x) / kWave; ++k) r = fmax(r, sh[k]);
     __syncthreads();
     return r;
 }
@@ -184,7 +187,7 @@ __device__ int block_min_i(int v, int* sh) {
     if (__lane_id() == 0) sh[w] = v;
     __syncthreads();
     int r = sh[0];
-    for (int k = 1; k < kThreads / kWave; ++k) r = min(r, sh[k]);
+    for (int k = 1; k < static_cast<int>(blockDim.x) / kWave; ++k) r = min(r, sh[k]);
     __syncthreads();
     return r;
 }
@@ -196,7 +199,7 @@ __device__ int block_sum_i(int v, int* sh) {
     if (__lane_id() == 0) sh[w] = v;
     __syncthreads();
     int r = 0;
-    for (int k = 0; k < kThreads / kWave; ++k) r += sh[k];
+    for (int k = 0; k < static_cast<int>(blockDim.x) / kWave; ++k) r += sh[k];
     __syncthreads();
     return r;
 }
@@ -224,13 +227,13 @@ struct Src {
     int mass_t;
 };
 
-__global__ __launch_bounds__(kThreads) void k_group_reduce(Src s, const int64_t* __restrict__ off,
+__global__ __launch_bounds__(kBig) void k_group_reduce(Src s, const int64_t* __restrict__ off,
                                                            const int32_t* __restrict__ mem,
                                                            int64_t G,
                                                            const double* __restrict__ centres,
                                                            double* __restrict__ out) {
-    __shared__ double sh[2 * (kThreads / kWave)];
-    __shared__ int shi[kThreads / kWave];
+    __shared__ double sh[2 * kMaxWaves];
+    __shared__ int shi[kMaxWaves];
     __shared__ double cen[3];
     for (int64_t g = blockIdx.x; g < G; g += gridDim.x) {
         const int64_t b = off[g], e = off[g + 1];
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(kThreads) void k_group_reduce(Src s, const int64_t*
         for (int k = 0; k < 8; ++k) acc[k][0] = acc[k][1] = 0.0;
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         int gexp = 4096;
-        for (int64_t k = b + threadIdx.x; k < e; k += kThreads) {
+        for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
             const int64_t i = mem[k];
             const double m = ldf(s.m, s.mass_t, i);
             const double p[3] = {ldf(s.x, s.pos_t, i), ldf(s.y, s.pos_t, i), ldf(s.z, s.pos_t, i)};
@@ -286,7 +289,7 @@ __global__ __launch_bounds__(kThreads) void k_group_reduce(Src s, const int64_t*
         }
         __syncthreads();
         double r2 = -1.0;
-        for (int64_t k = b + threadIdx.x; k < e; k += kThreads) {
+        for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
             const int64_t i = mem[k];
             const double dx = cen[0] - ldf(s.x, s.pos_t, i);
             const double dy = cen[1] - ldf(s.y, s.pos_t, i);
@@ -402,7 +405,7 @@ template <class Get>
 __device__ int pick_peak(const PeakArgs& a, const Grid1* gr, int64_t g, int64_t b, int64_t e,
                          int nslots, Get get, double* shd, int* shi, int* cand) {
     double vmax = -INFINITY;
-    for (int k = threadIdx.x; k < nslots; k += kThreads) {
+    for (int k = threadIdx.x; k < nslots; k += blockDim.x) {
         int node;
         double v;
         if (get(k, node, v)) vmax = fmax(vmax, v);
@@ -414,13 +417,13 @@ __device__ int pick_peak(const PeakArgs& a, const Grid1* gr, int64_t g, int64_t 
     const double band = exact ? 0.0 : 2.0 * static_cast<double>(e - b) * 0x1p-53 * r[14];
     int first = INT32_MAX;
     int ncand = 0;
-    for (int k = threadIdx.x; k < nslots; k += kThreads) {
+    for (int k = threadIdx.x; k < nslots; k += blockDim.x) {
         int node;
         double v;
         if (get(k, node, v) && v >= vmax - 2.0 * band) {
             first = min(first, node);
             if (!exact) {
-                const int at = atomicAdd(&shi[kThreads / kWave], 1);
+                const int at = atomicAdd(&shi[kMaxWaves], 1);
                 if (at < kMaxCandidates) cand[at] = node;
             }
         }
@@ -428,7 +431,7 @@ __device__ int pick_peak(const PeakArgs& a, const Grid1* gr, int64_t g, int64_t 
     first = block_min_i(first, shi);
     if (exact) return first;
     __syncthreads();
-    ncand = shi[kThreads / kWave];
+    ncand = shi[kMaxWaves];
     if (ncand == 1) return first;
     __shared__ int winner;
     if (threadIdx.x == 0) {
@@ -449,29 +452,29 @@ __device__ int pick_peak(const PeakArgs& a, const Grid1* gr, int64_t g, int64_t 
     return winner;
 }
 
-__global__ __launch_bounds__(kThreads) void k_density_peak(PeakArgs a) {
+__global__ __launch_bounds__(kBig) void k_density_peak(PeakArgs a) {
     __shared__ int skey[kTable];
     __shared__ double sval[kTable];
-    __shared__ double shd[kThreads / kWave];
-    __shared__ int shi[kThreads / kWave + 1];
+    __shared__ double shd[kMaxWaves];
+    __shared__ int shi[kMaxWaves + 1];
     __shared__ int cand[kMaxCandidates];
     __shared__ int used;
     __shared__ int overflow;
     for (int64_t g = blockIdx.x; g < a.G; g += gridDim.x) {
-        for (int k = threadIdx.x; k < kTable; k += kThreads) {
+        for (int k = threadIdx.x; k < kTable; k += blockDim.x) {
             skey[k] = -1;
             sval[k] = 0.0;
         }
         if (threadIdx.x == 0) {
             used = 0;
             overflow = 0;
-            shi[kThreads / kWave] = 0;
+            shi[kMaxWaves] = 0;
         }
         __syncthreads();
         Grid1 gr[3];
         grids_of(a, g, gr);
         const int64_t b = a.off[g], e = a.off[g + 1];
-        for (int64_t k = b + threadIdx.x; k < e; k += kThreads) {
+        for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
             const int64_t i = a.mem[k];
             const int node = cell_of(a, gr, i);
             const double m = ldf(a.s.m, a.s.mass_t, i);
@@ -533,14 +536,14 @@ __global__ __launch_bounds__(kThreads) void k_dense_accum(PeakArgs a, int first)
 }
 
 __global__ __launch_bounds__(kThreads) void k_dense_pick(PeakArgs a, int first) {
-    __shared__ double shd[kThreads / kWave];
-    __shared__ int shi[kThreads / kWave + 1];
+    __shared__ double shd[kMaxWaves];
+    __shared__ int shi[kMaxWaves + 1];
     __shared__ int cand[kMaxCandidates];
     const int o = first + blockIdx.x;
     if (o >= *a.over_count) return;
     const int64_t g = a.over_list[o];
     const double* grid = a.dense + static_cast<int64_t>(blockIdx.x) * kCells;
-    if (threadIdx.x == 0) shi[kThreads / kWave] = 0;
+    if (threadIdx.x == 0) shi[kMaxWaves] = 0;
     __syncthreads();
     Grid1 gr[3];
     grids_of(a, g, gr);
@@ -715,7 +718,7 @@ int halma_group_reduce(int64_t n_groups, const int64_t* offsets, const int32_t* 
     if (vx && (!vy || !vz)) return set_error(HALMA_E_ARG, "velocities need vx, vy and vz");
     Src s = make_src(x, y, z, pos_dtype, vx, vy, vz, vel_dtype, m, mass_dtype);
     const unsigned grid = static_cast<unsigned>(n_groups < 65536 ? n_groups : 65536);
-    k_group_reduce<<<grid, kThreads, 0, static_cast<hipStream_t>(stream)>>>(
+    k_group_reduce<<<grid, kBig, 0, static_cast<hipStream_t>(stream)>>>(
         s, offsets, members, n_groups, centres, out);
     HALMA_LAUNCH_CHECK();
     return 0;
@@ -755,7 +758,7 @@ int halma_density_peak(int64_t n_groups, const int64_t* offsets, const int32_t* 
     k_fill_ints<<<1, 64, 0, st>>>(w.err, 1);
     HALMA_LAUNCH_CHECK();
     const unsigned grid = static_cast<unsigned>(n_groups < 65536 ? n_groups : 65536);
-    k_density_peak<<<grid, kThreads, 0, st>>>(a);
+    k_density_peak<<<grid, kBig, 0, st>>>(a);
     HALMA_LAUNCH_CHECK();
     long long* h = pinned_words(2);
     if (!h) return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
