@@ -67,6 +67,18 @@ int halma_pot_sum_f32(const float *m, const float *x, const float *y, const floa
  * batched kernels for the current tuning. */
 double halma_pot_error_bound(void);
 
+/* Self-potential of n_halo haloes with tests = sources (the unbinding's full passes):
+ * out[k] = sum over the other members j of k's halo of m_j / r_kj, with the same
+ * three-coordinate rule (which also drops j = k).  Members are the CSR ranges
+ * [offsets[h], offsets[h+1]) (HOST offsets) of the FP32 arrays m, x, y, z; out is FP64
+ * and indexed like them.  Every pair distance is computed once for both of its terms;
+ * the relative error is below halma_pot_sym_error_bound().  Stream-ordered. */
+size_t halma_pot_sum_self_workspace_size(int64_t n_halo, const int64_t *offsets);
+int halma_pot_sum_self_batched(int64_t n_halo, const int64_t *offsets, const float *m,
+                               const float *x, const float *y, const float *z, double *out,
+                               void *workspace, size_t workspace_bytes, void *stream);
+double halma_pot_sym_error_bound(void);
+
 /* Batched form: H independent haloes in CSR layout.  Halo h's sources are
  * [src_off[h], src_off[h+1]) of (sm, sx, sy, sz) and its tests are
  * [tst_off[h], tst_off[h+1]) of (tx, ty, tz); out_f64 is indexed like the tests.

@@ -31,6 +31,9 @@ SIGNATURES = {
     "halma_device_cu_count": (_c.c_int, []),
     "halma_pot_sum_workspace_size": (_size, [_i64, _i64]),
     "halma_soft_workspace_size": (_size, [_i64, _i64p, _i64p]),
+    "halma_pot_sum_self_workspace_size": (_size, [_i64, _i64p]),
+    "halma_pot_sum_self_batched": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _p, _p, _size, _p]),
+    "halma_pot_sym_error_bound": (_dbl, []),
     "halma_energy_test_workspace_size": (_size, [_i64, _i64p, _i64p]),
     "halma_energy_test_batched": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _i64p, _p, _p, _p,
                                              _p, _p, _p, _c.c_int, _p, _dbl, _p, _p, _i64p,

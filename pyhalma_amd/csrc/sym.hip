@@ -1,0 +1,373 @@
+// Self-potential of many haloes with the pair symmetry used: S_i = sum_{j != i} m_j / r_ij
+// over the halo's own members (the sum of particle_subroutines.f90:466-514 with tests =
+// sources, as the unbinding's full passes need), each pair distance computed once and
+// its two terms, m_j / r to S_i and m_i / r to S_j, accumulated together.
+//
+// A halo of n members is cut into tiles of kTile = 128 members and the tiles into blocks
+// of kSymK tiles.  A work item is a pair of blocks (A, B), A <= B.  One wavefront holds
+// the 128 members of one tile of A as tests in registers (2 per lane), streams the
+// members of B's tiles as wave-uniform sources, and accumulates
+//   - test side: FP32 sums of 8 terms, Kahan-added, per test (the tile of A);
+//   - source side: per source, the wave-wide sum of its terms over the tile's tests
+//     (a fixed reduction tree across the lanes), Kahan-added in the wave's LDS.
+// Within a diagonal item (A == B) only pairs of tiles a <= b are taken, and a tile
+// against itself is summed on the test side alone.  Every member then has kSymK-block
+// partial sums in nb + 1 slots (nb = blocks of its halo), each written by exactly one
+// item: slot s < I (its block) from the source side of item (s, I), slot I from the test
+// side of item (I, I), slots s > I from the test side of item (I, s), and slot nb from
+// the source side of item (I, I).  The slots are merged in FP64 in slot order.
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "halma_common.h"
+#include "potential.h"
+#include "staging.h"
+#include "sym.h"
+#include "tuning.h"
+
+namespace halma {
+
+namespace {
+
+constexpr int kTile = 2 * kWave;  // 128 members: two tests per lane
+
+__device__ __forceinline__ float inv_r(float4 q, float x, float y, float z) {
+    const float dx = q.x - x;
+    const float dy = q.y - y;
+    const float dz = q.z - z;
+    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float mn = fminf(fminf(fabsf(dx), fabsf(dy)), fabsf(dz));
+    return mn > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+}
+
+__device__ __forceinline__ void kahan(float& s, float& c, float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// v[0..7] per lane -> the sum over all 64 lanes of v[k], for k = source index of the lane:
+// k = 4*bit5 + 2*bit4 + bit3 of the lane id (every lane holds the total of its k).
+// Recursive halving (32, 16, 8) then a butterfly over the low three lane bits: a fixed
+// binary tree of depth 6 per total.
+__device__ __forceinline__ float reduce8(float v[8], int lane) {
+    const bool h5 = (lane & 32) != 0;
+    float w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float send = h5 ? v[k] : v[k + 4];
+        const float keep = h5 ? v[k + 4] : v[k];
+        w[k] = keep + __shfl_xor(send, 32, kWave);
+    }
+    const bool h4 = (lane & 16) != 0;
+    float u[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const float send = h4 ? w[k] : w[k + 2];
+        const float keep = h4 ? w[k + 2] : w[k];
+        u[k] = keep + __shfl_xor(send, 16, kWave);
+    }
+    const bool h3 = (lane & 8) != 0;
+    float t = (h3 ? u[1] : u[0]) + __shfl_xor(h3 ? u[0] : u[1], 8, kWave);
+    t += __shfl_xor(t, 4, kWave);
+    t += __shfl_xor(t, 2, kWave);
+    t += __shfl_xor(t, 1, kWave);
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
+                                                 const SymDesc* __restrict__ halos,
+                                                 const int* __restrict__ live_n,
+                                                 const int4* __restrict__ items, int n_items,
+                                                 const int64_t* __restrict__ n_items_dev,
+                                                 float2* __restrict__ part,
+                                                 int* __restrict__ head) {
+    // wave-private LDS accumulators of the source side: kSymK tiles x 128 (sum, comp)
+    __shared__ float lsum[4][kSymK * kTile];
+    __shared__ float lcmp[4][kSymK * kTile];
+    const int lane = __lane_id();
+    const int wv = threadIdx.x / kWave;
+    float* ls = lsum[wv];
+    float* lc = lcmp[wv];
+    const int n_waves = static_cast<int>(gridDim.x * (blockDim.x / kWave));
+    if (n_items_dev) {
+        const int64_t nd = uniform_load(n_items_dev);
+        n_items = static_cast<int>(nd < n_items ? nd : n_items);
+    }
+    int k = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * (blockDim.x / kWave) + wv));
+    for (bool first = true;; first = false) {
+        if (!first) {
+            int t = 0;
+            if (lane == 0) t = atomicAdd(head, 1);
+            k = n_waves + __builtin_amdgcn_readfirstlane(t);
+        }
+        if (k >= n_items) break;
+        const int4 it = uniform_load(items + k);
+        const SymDesc d = uniform_load(halos + it.x);
+        const int n = live_n ? __builtin_amdgcn_readfirstlane(uniform_load(live_n + it.x)) : d.n;
+        const int nt = (n + kTile - 1) / kTile;
+        const int nb = (nt + kSymK - 1) / kSymK;
+        const int A = it.y, B = it.z;
+        if (B >= nb || A > B) continue;
+        const float4* sp = src + d.off;
+        float2* pp = part + d.part_off;
+        const int64_t stride = d.n;  // slots are laid out for the planned size
+#pragma unroll
+        for (int e = 0; e < kSymK * kTile / kWave; ++e) {
+            ls[e * kWave + lane] = 0.0f;
+            lc[e * kWave + lane] = 0.0f;
+        }
+        for (int ta = 0; ta < kSymK; ++ta) {
+            const int a = A * kSymK + ta;
+            if (a >= nt) break;
+            float xi[2], yi[2], zi[2], mi[2], as[2], ac[2];
+            int ii[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                ii[r] = a * kTile + r * kWave + lane;
+                const float4 p = sp[min(ii[r], n - 1)];
+                xi[r] = p.x;
+                yi[r] = p.y;
+                zi[r] = p.z;
+                mi[r] = ii[r] < n ? p.w : 0.0f;  // padding lanes add nothing to sources
+                as[r] = 0.0f;
+                ac[r] = 0.0f;
+            }
+            for (int tb = 0; tb < kSymK; ++tb) {
+                const int b = B * kSymK + tb;
+                if (b >= nt) break;
+                if (A == B && b < a) continue;
+                const int j0 = b * kTile;
+                const int cnt = min(kTile, n - j0);
+                const bool diag = b == a;
+                for (int g = 0; g < cnt; g += 8) {
+                    float ps[2] = {0.0f, 0.0f};
+                    float pj[8];
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) {
+                        const int j = g + s;
+                        const float4 q = uniform_load(sp + j0 + min(j, cnt - 1));
+                        const float qw = j < cnt ? q.w : 0.0f;
+                        const float v0 = inv_r(q, xi[0], yi[0], zi[0]);
+                        const float v1 = inv_r(q, xi[1], yi[1], zi[1]);
+                        ps[0] = __builtin_fmaf(qw, v0, ps[0]);
+                        ps[1] = __builtin_fmaf(qw, v1, ps[1]);
+                        pj[s] = __builtin_fmaf(mi[1], v1, mi[0] * v0);
+                    }
+                    kahan(as[0], ac[0], ps[0]);
+                    kahan(as[1], ac[1], ps[1]);
+                    if (!diag) {
+                        const float tot = reduce8(pj, lane);
+                        const int s = (lane >> 3) & 7;
+                        if ((lane & 7) == 0 && g + s < cnt) {
+                            const int e = tb * kTile + g + s;
+                            float sv = ls[e], cv = lc[e];
+                            kahan(sv, cv, tot);
+                            ls[e] = sv;
+                            lc[e] = cv;
+                        }
+                    }
+                }
+            }
+            // test side of tile a from this item: slot B
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+                if (ii[r] < n) pp[static_cast<int64_t>(B) * stride + ii[r]] = make_float2(as[r], ac[r]);
+        }
+        // source side of B's tiles from this item: slot A, or slot nb on the diagonal
+        const int slot = A < B ? A : nb;
+        for (int tb = 0; tb < kSymK; ++tb) {
+            const int b = B * kSymK + tb;
+            if (b >= nt) break;
+            const int j0 = b * kTile;
+            const int cnt = min(kTile, n - j0);
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int e = r * kWave + lane;
+                if (e < cnt)
+                    pp[static_cast<int64_t>(slot) * stride + j0 + e] =
+                        make_float2(ls[tb * kTile + e], lc[tb * kTile + e]);
+            }
+        }
+    }
+}
+
+__global__ void k_sym_merge(const SymDesc* __restrict__ halos, const int* __restrict__ mhalo,
+                            int64_t n_tot, const int64_t* __restrict__ member_base,
+                            const float2* __restrict__ part, double* __restrict__ out) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n_tot) return;
+    const int h = mhalo[g];
+    const SymDesc d = halos[h];
+    const int64_t i = g - member_base[h];
+    const int nb = sym_blocks(d.n);
+    double s = 0.0;
+    for (int slot = 0; slot <= nb; ++slot) {
+        const float2 v = part[d.part_off + static_cast<int64_t>(slot) * d.n + i];
+        s += static_cast<double>(v.x) + static_cast<double>(v.y);
+    }
+    out[g] = s;
+}
+
+__global__ void k_sym_mhalo(const int64_t* __restrict__ base, int H, int* __restrict__ mh) {
+    const int h = blockIdx.x;
+    if (h >= H) return;
+    for (int64_t k = base[h] + threadIdx.x; k < base[h + 1]; k += blockDim.x) mh[k] = h;
+}
+
+__global__ void k_sym_pack(const float* __restrict__ m, const float* __restrict__ x,
+                           const float* __restrict__ y, const float* __restrict__ z, int64_t n,
+                           float4* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float4(x[i], y[i], z[i], m[i]);
+}
+
+__global__ void k_sym_clear(int* __restrict__ p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+struct SymWs {
+    float4* src4;
+    SymDesc* halos;
+    int4* items;
+    float2* part;
+    int* mhalo;
+    int64_t* base;
+    int* head;
+    size_t bytes;
+};
+
+SymWs carve_sym(void* b, const SymPlan& p, int64_t H) {
+    Carver c(b);
+    SymWs w;
+    w.src4 = c.take<float4>(std::max<int64_t>(p.n_members, 1));
+    w.halos = c.take<SymDesc>(std::max<int64_t>(H, 1));
+    w.items = c.take<int4>(std::max<int64_t>(p.n_items, 1));
+    w.part = c.take<float2>(std::max<int64_t>(p.n_part, 1));
+    w.mhalo = c.take<int>(std::max<int64_t>(p.n_members, 1));
+    w.base = c.take<int64_t>(H + 1);
+    w.head = c.take<int>(64);
+    w.bytes = c.used;
+    return w;
+}
+
+}  // namespace
+
+int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items) {
+    plan = SymPlan();
+    plan.halos.resize(H);
+    int64_t part = 0;
+    std::vector<int64_t> work(H);
+    for (int64_t h = 0; h < H; ++h) {
+        const int64_t n = off[h + 1] - off[h];
+        if (n < 0 || n > INT32_MAX) return set_error(HALMA_E_ARG, "halo %lld: bad offsets",
+                                                      static_cast<long long>(h));
+        SymDesc& d = plan.halos[h];
+        d.off = off[h] - off[0];
+        d.part_off = part;
+        d.n = static_cast<int32_t>(n);
+        d.nb = sym_blocks(d.n);
+        part += static_cast<int64_t>(d.nb + 1) * n;
+        work[h] = n;
+    }
+    plan.order.resize(H);
+    for (int64_t h = 0; h < H; ++h) plan.order[h] = static_cast<int32_t>(h);
+    std::stable_sort(plan.order.begin(), plan.order.end(),
+                     [&](int32_t a, int32_t b) { return work[a] > work[b]; });
+    int64_t n_items = 0;
+    for (int64_t h = 0; h < H; ++h) {
+        const int64_t nb = plan.halos[h].nb;
+        n_items += nb * (nb + 1) / 2;
+    }
+    if (n_items > INT32_MAX) return set_error(HALMA_E_LIMIT, "too many work items");
+    plan.n_items = n_items;
+    if (with_items) {
+        plan.items.reserve(static_cast<size_t>(n_items));
+        for (int32_t h : plan.order) {
+            const int nb = plan.halos[h].nb;
+            for (int B = 0; B < nb; ++B)
+                for (int A = 0; A <= B; ++A) plan.items.push_back(make_int4(h, A, B, 0));
+        }
+    }
+    plan.n_members = off[H] - off[0];
+    plan.n_part = part;
+    return 0;
+}
+
+int launch_pot_sym(const SymPlan& plan, const float4* src4, const SymDesc* d_halos,
+                   const int* live_n, const int4* d_items, const int64_t* n_items_dev,
+                   float2* part, int* head, hipStream_t st, bool reset_head) {
+    if (reset_head) {
+        k_sym_clear<<<1, 64, 0, st>>>(head);
+        HALMA_LAUNCH_CHECK();
+    }
+    const int n_items = static_cast<int>(plan.n_items);
+    if (n_items == 0) return 0;
+    int cus = device_cu_count();
+    if (cus <= 0) cus = 256;
+    const int grid = std::max(1, std::min(cus * 8, (n_items + 3) / 4));
+    k_pot_sym<<<grid, 256, 0, st>>>(src4, d_halos, live_n, d_items, n_items, n_items_dev, part,
+                                    head);
+    HALMA_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace halma
+
+using namespace halma;
+
+extern "C" {
+
+double halma_pot_sym_error_bound(void) { return pot_sym_error_bound(); }
+
+size_t halma_pot_sum_self_workspace_size(int64_t n_halo, const int64_t* offsets) {
+    if (n_halo < 0 || !offsets) return 0;
+    SymPlan plan;
+    if (plan_sym(n_halo, offsets, plan, false)) return 0;
+    return carve_sym(nullptr, plan, n_halo).bytes;
+}
+
+int halma_pot_sum_self_batched(int64_t n_halo, const int64_t* offsets, const float* m,
+                               const float* x, const float* y, const float* z, double* out,
+                               void* ws, size_t ws_bytes, void* stream) {
+    if (n_halo < 0 || !offsets) return set_error(HALMA_E_ARG, "bad halo count or offsets");
+    if (n_halo == 0) return 0;
+    for (int64_t h = 0; h < n_halo; ++h)
+        if (offsets[h + 1] < offsets[h]) return set_error(HALMA_E_ARG, "offsets must not decrease");
+    const int64_t n = offsets[n_halo] - offsets[0];
+    if (n == 0) return 0;
+    if (!m || !x || !y || !z || !out) return set_error(HALMA_E_ARG, "null pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    SymPlan plan;
+    int rc = plan_sym(n_halo, offsets, plan, true);
+    if (rc) return rc;
+    SymWs w = carve_sym(ws, plan, n_halo);
+    if (ws_bytes < w.bytes)
+        return set_error(HALMA_E_WORKSPACE, "workspace %zu < required %zu", ws_bytes, w.bytes);
+    const int64_t o0 = offsets[0];
+    k_sym_pack<<<static_cast<unsigned>((n + 255) / 256), 256, 0, st>>>(m + o0, x + o0, y + o0,
+                                                                       z + o0, n, w.src4);
+    HALMA_LAUNCH_CHECK();
+    std::vector<int64_t> base(n_halo + 1);
+    for (int64_t h = 0; h <= n_halo; ++h) base[h] = offsets[h] - o0;
+    rc = upload_async(w.halos, plan.halos.data(), plan.halos.size() * sizeof(SymDesc), st);
+    if (!rc) rc = upload_async(w.base, base.data(), base.size() * sizeof(int64_t), st);
+    if (!rc && plan.n_items > 0)
+        rc = upload_async(w.items, plan.items.data(), plan.items.size() * sizeof(int4), st);
+    if (rc) return rc;
+    k_sym_mhalo<<<static_cast<unsigned>(n_halo), 256, 0, st>>>(w.base, static_cast<int>(n_halo),
+                                                               w.mhalo);
+    HALMA_LAUNCH_CHECK();
+    rc = launch_pot_sym(plan, w.src4, w.halos, nullptr, w.items, nullptr, w.part, w.head, st,
+                        true);
+    if (rc) return rc;
+    k_sym_merge<<<static_cast<unsigned>((n + 255) / 256), 256, 0, st>>>(w.halos, w.mhalo, n,
+                                                                       w.base, w.part, out + o0);
+    HALMA_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // extern "C"

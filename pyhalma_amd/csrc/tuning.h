@@ -24,4 +24,10 @@ inline double pot_error_bound(int sub) {
     return (sub + 7.0) * u + 1e-11;
 }
 
+// The symmetric self-potential kernel: test side as above with sub-tiles of 8 terms;
+// source side per term 4.5u (differences, squares, v_rsq_f32), the lane's two-term
+// product sum 2u, the 6-level reduction tree across the wave 6u, Kahan 2u, + 0.5u:
+// 15u either way.
+inline double pot_sym_error_bound() { return pot_error_bound(8); }
+
 }  // namespace halma

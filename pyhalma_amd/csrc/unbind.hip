@@ -42,6 +42,7 @@
 #include "halma_common.h"
 #include "potential.h"
 #include "scan.h"
+#include "sym.h"
 #include "staging.h"
 #include "tuning.h"
 
@@ -226,6 +227,7 @@ __global__ void k_halo_sets(int H, const int32_t* __restrict__ order,
                             int32_t* __restrict__ mode, double* __restrict__ vcm,
                             int32_t* __restrict__ tst_len, int32_t* __restrict__ src_len,
                             int32_t* __restrict__ src_sel, int64_t* __restrict__ icnt,
+                            int64_t* __restrict__ icnt_sym,
                             unsigned long long* __restrict__ counters) {
     const int pos = blockIdx.x * blockDim.x + threadIdx.x;
     if (pos == 0) counters[0] = 0;
@@ -238,6 +240,7 @@ __global__ void k_halo_sets(int H, const int32_t* __restrict__ order,
         src_len[h] = 0;
         src_sel[h] = 0;
         icnt[pos] = 0;
+        icnt_sym[pos] = 0;
         return;
     }
     int run = 0, rrun = 0;
@@ -264,10 +267,14 @@ __global__ void k_halo_sets(int H, const int32_t* __restrict__ order,
     tst_len[h] = nt;
     src_len[h] = ns;
     src_sel[h] = full ? 0 : 1;
+    // full passes (tests = sources = the bound set) go to the symmetric kernel: block
+    // pairs A <= B of its live size; incremental ones to the chunked kernel
     const int ch = halos[h].chunk;
-    icnt[pos] = (nt > 0 && ns > 0)
+    icnt[pos] = (!full && nt > 0 && ns > 0)
                     ? static_cast<int64_t>((nt + tile - 1) / tile) * ((ns + ch - 1) / ch)
                     : 0;
+    const int64_t nbs = full ? sym_blocks(nt) : 0;
+    icnt_sym[pos] = nbs * (nbs + 1) / 2;
 }
 
 __global__ __launch_bounds__(kThreads) void k_chunk_write(
@@ -312,10 +319,13 @@ struct EnergyArgs {
     const int32_t* mode;
     const int32_t* cidx;
     const float2* part;
+    const SymDesc* sym;      // full passes: symmetric kernel's slots
+    const float2* sym_part;
     const double* vel;
     const double* vcm;
     double cg;         // factor * G
-    double bound_rel;  // FP32 relative error bound B of one pair sum
+    double bound_rel;  // FP32 relative error bound B of one pair sum (incremental passes)
+    double bound_sym;  // the same for the symmetric kernel (full passes)
     uint8_t* state;
     double* sacc;      // running S per member position
     double* eacc;      // absolute error bound of sacc
@@ -342,28 +352,41 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     const int64_t p = g - d.tst_off;
     const int n = a.n_bound[h];
     if (p >= n) return;
-    const int ns = a.n_src[h];
-    const int nch = (ns + d.chunk - 1) / d.chunk;
-    // chunk partials summed in chunk order; loads issued eight at a time
-    const float2* pp = a.part + d.part_off + p;
+    const bool full = a.mode[h] != 0;
+    // full pass: the symmetric kernel's nb + 1 slots (nb of the live set); incremental
+    // pass: the chunk partials; both summed in order, loads issued eight at a time
+    int nch;
+    int64_t stride;
+    const float2* pp;
+    if (full) {
+        const SymDesc sd = a.sym[h];
+        nch = sym_blocks(n) + 1;
+        stride = sd.n;
+        pp = a.sym_part + sd.part_off + p;
+    } else {
+        const int ns = a.n_src[h];
+        nch = (ns + d.chunk - 1) / d.chunk;
+        stride = d.stride;
+        pp = a.part + d.part_off + p;
+    }
     double sum = 0.0;
     int c = 0;
     for (; c + 8 <= nch; c += 8) {
         float2 q[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) q[u] = pp[static_cast<int64_t>(c + u) * d.stride];
+        for (int u = 0; u < 8; ++u) q[u] = pp[static_cast<int64_t>(c + u) * stride];
 #pragma unroll
         for (int u = 0; u < 8; ++u) sum += static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
     }
     for (; c < nch; ++c) {
-        const float2 q = pp[static_cast<int64_t>(c) * d.stride];
+        const float2 q = pp[static_cast<int64_t>(c) * stride];
         sum += static_cast<double>(q.x) + static_cast<double>(q.y);
     }
     const int64_t k = a.cidx[g];
     double S, err;
-    if (a.mode[h]) {
+    if (full) {
         S = sum;
-        err = a.bound_rel * sum;
+        err = a.bound_sym * sum;
     } else {
         S = a.sacc[k] - sum;
         err = a.eacc[k] + a.bound_rel * sum;
@@ -466,6 +489,33 @@ __global__ void k_item_fill(int H, const int32_t* __restrict__ order,
     }
 }
 
+// The pass's items of the symmetric kernel: block pairs (A, B), A <= B, of every
+// full-pass halo, in the plan's halo order, B-major (local index B(B+1)/2 + A).
+__global__ void k_sym_item_fill(int H, const int32_t* __restrict__ order,
+                                const int64_t* __restrict__ ioff,
+                                const int64_t* __restrict__ itotal, int64_t cap,
+                                int4* __restrict__ items, int* __restrict__ head) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *head = 0;
+    const int64_t total = itotal[0] < cap ? itotal[0] : cap;
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int lo = 0, hi = H - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ioff[mid] <= k)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int64_t local = k - ioff[lo];
+        int64_t B = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(local) + 1.0) - 1.0) / 2.0);
+        while (B * (B + 1) / 2 > local) --B;
+        while ((B + 1) * (B + 2) / 2 <= local) ++B;
+        const int64_t A = local - B * (B + 1) / 2;
+        items[k] = make_int4(order[lo], static_cast<int>(A), static_cast<int>(B), 0);
+    }
+}
+
 __global__ void k_final_mask(int64_t nm, const uint8_t* __restrict__ state,
                              uint8_t* __restrict__ bound) {
     const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -505,6 +555,16 @@ struct UnbWs {
     void* iscan_ws;
     float2* part;
     int* head;
+    // the symmetric kernel of the full passes
+    SymDesc* sym;     // per halo (in the tables region)
+    int4* sym_items;
+    int64_t* sym_icnt;
+    int64_t* sym_ioff;
+    int64_t* sym_itotal;
+    void* sym_scan_ws;
+    float2* sym_part;
+    int* sym_head;
+    int64_t sym_cap;
     int64_t* off;
     int2* chunks;     // compaction chunks {halo, chunk number}, halo by halo
     int32_t* cfirst;  // first chunk of each halo (H + 1)
@@ -513,7 +573,7 @@ struct UnbWs {
     int64_t n_chunks;
     // host tables (off, halos, order, chunks, cfirst) sit in one region, uploaded at once
     char* tables;
-    size_t t_off, t_halos, t_order, t_chunks, t_cfirst, t_bytes;
+    size_t t_off, t_halos, t_order, t_chunks, t_cfirst, t_sym, t_bytes;
     size_t bytes;
 };
 
@@ -523,7 +583,8 @@ int64_t count_chunks(int64_t H, const int64_t* h_off) {
     return n;
 }
 
-UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, int64_t n_chunks) {
+UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, const SymPlan& splan,
+                int64_t n_chunks) {
     Carver c(base);
     const int64_t n1 = nm > 0 ? nm : 1;
     const int64_t h1 = H > 0 ? H : 1;
@@ -540,6 +601,8 @@ UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, int64_t 
     t.take<int2>(c1);
     w.t_cfirst = t.used;
     t.take<int32_t>(h1 + 1);
+    w.t_sym = t.used;
+    t.take<SymDesc>(h1);
     w.t_bytes = t.used;
     w.tables = c.take<char>(w.t_bytes);
     w.off = reinterpret_cast<int64_t*>(w.tables + w.t_off);
@@ -547,6 +610,7 @@ UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, int64_t 
     w.order = reinterpret_cast<int32_t*>(w.tables + w.t_order);
     w.chunks = reinterpret_cast<int2*>(w.tables + w.t_chunks);
     w.cfirst = reinterpret_cast<int32_t*>(w.tables + w.t_cfirst);
+    w.sym = reinterpret_cast<SymDesc*>(w.tables + w.t_sym);
     w.n_chunks = n_chunks;
     w.ccnt = c.take<int32_t>(2 * c1);
     w.csum = c.take<double>(8 * c1);
@@ -580,6 +644,14 @@ UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, int64_t 
     w.iscan_ws = c.take<char>(scan_workspace_bytes(h1));
     w.part = c.take<float2>(std::max<int64_t>(plan.n_part, 1));
     w.head = c.take<int>(64);
+    w.sym_cap = std::max<int64_t>(splan.n_items, 1);
+    w.sym_items = c.take<int4>(w.sym_cap);
+    w.sym_icnt = c.take<int64_t>(h1);
+    w.sym_ioff = c.take<int64_t>(h1);
+    w.sym_itotal = c.take<int64_t>(4);
+    w.sym_scan_ws = c.take<char>(scan_workspace_bytes(h1));
+    w.sym_part = c.take<float2>(std::max<int64_t>(splan.n_part, 1));
+    w.sym_head = c.take<int>(64);
     w.bytes = c.used;
     return w;
 }
@@ -603,6 +675,7 @@ struct PassLaunch {
     int64_t H, nm;
     int grid_chunks, grid_retest;
     const PotPlan* plan;
+    const SymPlan* splan;
     UnbWs w;
     uint8_t* bound;
     EnergyArgs ea;
@@ -623,7 +696,7 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
     k_halo_sets<<<blocks_for(L.H), kThreads, 0, st>>>(
         static_cast<int>(L.H), w.order, w.halos, tile, w.cfirst, w.active, w.pass_no, w.ccnt,
         w.csum, w.n_bound, w.n_rem, w.mode, w.vcm, w.tst_len, w.src_len, w.src_sel, w.icnt,
-        w.counters);
+        w.sym_icnt, w.counters);
     HALMA_LAUNCH_CHECK();
     const int rc = exclusive_scan_i64(w.icnt, w.ioff, L.H, w.itotal, w.iscan_ws, st);
     if (rc) return rc;
@@ -633,6 +706,15 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
     k_item_fill<<<fill_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.ioff,
                                                  w.itotal, w.tst_len, tile, cap, w.items,
                                                  w.head);
+    HALMA_LAUNCH_CHECK();
+    const int rc2 = exclusive_scan_i64(w.sym_icnt, w.sym_ioff, L.H, w.sym_itotal, w.sym_scan_ws,
+                                       st);
+    if (rc2) return rc2;
+    const unsigned sym_grid = static_cast<unsigned>(
+        std::max<int64_t>(1, std::min<int64_t>(1024, (w.sym_cap + 255) / 256)));
+    k_sym_item_fill<<<sym_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.sym_ioff,
+                                                   w.sym_itotal, w.sym_cap, w.sym_items,
+                                                   w.sym_head);
     HALMA_LAUNCH_CHECK();
     if (nc > 0) {
         k_chunk_write<<<L.grid_chunks, kThreads, 0, st>>>(w.chunks, nc, w.off, w.active, w.ccnt,
@@ -650,8 +732,11 @@ int launch_pot(const PassLaunch& L, hipStream_t st) {
     dyn.src_sel = w.src_sel;
     dyn.src_alt = w.rem4;
     dyn.n_items = w.itotal;
-    // the queue head is cleared by k_item_fill in the prepare part, so that this part
-    // holds the pair kernel alone (its timing events bracket nothing else)
+    // the queue heads are cleared by the item fills in the prepare part, so that this
+    // part holds the pair kernels alone (its timing events bracket nothing else)
+    int rc = launch_pot_sym(*L.splan, w.src4, w.sym, w.n_bound, w.sym_items, w.sym_itotal,
+                            w.sym_part, w.sym_head, st, false);
+    if (rc) return rc;
     return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, dyn, w.items, w.part, w.head, st,
                           false);
 }
@@ -700,9 +785,12 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     PotPlan plan;
     int rc = plan_pot(H, rel.data(), rel.data(), plan, false);
     if (rc) return rc;
+    SymPlan splan;
+    rc = plan_sym(H, rel.data(), splan, false);
+    if (rc) return rc;
     const int64_t n_chunks = count_chunks(H, h_off);
     if (n_chunks > INT32_MAX) return set_error(HALMA_E_LIMIT, "too many members");
-    UnbWs w = carve_unb(ws, H, nm, plan, n_chunks);
+    UnbWs w = carve_unb(ws, H, nm, plan, splan, n_chunks);
     if (ws_bytes < w.bytes)
         return set_error(HALMA_E_WORKSPACE, "workspace %zu < required %zu", ws_bytes, w.bytes);
     std::vector<int2> chunks(static_cast<size_t>(n_chunks));
@@ -717,13 +805,14 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         }
         cfirst[H] = q;
     }
-    const UploadPiece pieces[5] = {
+    const UploadPiece pieces[6] = {
         {w.t_off, rel.data(), (H + 1) * sizeof(int64_t)},
         {w.t_halos, plan.halos.data(), plan.halos.size() * sizeof(HaloDesc)},
         {w.t_order, plan.order.data(), plan.order.size() * sizeof(int32_t)},
         {w.t_chunks, chunks.data(), chunks.size() * sizeof(int2)},
-        {w.t_cfirst, cfirst.data(), cfirst.size() * sizeof(int32_t)}};
-    rc = upload_pieces(w.tables, w.t_bytes, pieces, 5, st);
+        {w.t_cfirst, cfirst.data(), cfirst.size() * sizeof(int32_t)},
+        {w.t_sym, splan.halos.data(), splan.halos.size() * sizeof(SymDesc)}};
+    rc = upload_pieces(w.tables, w.t_bytes, pieces, 6, st);
     if (rc) return rc;
     const int32_t* mem = members + h_off[0];
     if (nm > 0) {
@@ -750,6 +839,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.grid_chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(n_chunks, cus * 8)));
     L.grid_retest = cus * 4;
     L.plan = &plan;
+    L.splan = &splan;
     L.w = w;
     L.bound = bound_out;
     L.ea.halos = w.halos;
@@ -759,10 +849,13 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.ea.mode = w.mode;
     L.ea.cidx = w.cidx;
     L.ea.part = w.part;
+    L.ea.sym = w.sym;
+    L.ea.sym_part = w.sym_part;
     L.ea.vel = w.vel;
     L.ea.vcm = w.vcm;
     L.ea.cg = factor * G;
     L.ea.bound_rel = pot_error_bound(plan.subtile);
+    L.ea.bound_sym = pot_sym_error_bound();
     L.ea.state = w.state;
     L.ea.sacc = w.sacc;
     L.ea.eacc = w.eacc;
@@ -867,7 +960,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     if (stats) {
         stats[0] = passes;
         stats[1] = band_total;
-        stats[2] = plan.n_items;
+        stats[2] = splan.n_items;
         stats[3] = plan.tests_per_lane;
         stats[4] = plan.subtile;
         stats[5] = static_cast<int64_t>(pot_ms * 1e6);  // ns
@@ -960,7 +1053,10 @@ size_t unbind_workspace_bytes(int64_t H, const int64_t* h_off) {
     for (int64_t h = 0; h <= H; ++h) rel[h] = h_off[h] - h_off[0];
     PotPlan plan;
     if (plan_pot(H, rel.data(), rel.data(), plan, false)) return 0;
-    return carve_unb(nullptr, H, h_off[H] - h_off[0], plan, count_chunks(H, h_off)).bytes;
+    SymPlan splan;
+    if (plan_sym(H, rel.data(), splan, false)) return 0;
+    return carve_unb(nullptr, H, h_off[H] - h_off[0], plan, splan, count_chunks(H, h_off))
+        .bytes;
 }
 
 int unbind_plain_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs in,

@@ -66,6 +66,29 @@ def pot_sum_batched(src_off, m, x, y, z, tst_off, tx, ty, tz):
     return out
 
 
+def pot_sym_error_bound():
+    """Rigorous relative error bound of the symmetric self-potential kernel."""
+    return float(_lib_().halma_pot_sym_error_bound())
+
+
+def pot_sum_self(offsets, m, x, y, z):
+    """Self-potential of every halo (CSR host offsets; tests = sources = the members),
+    each pair distance computed once for both terms.  FP64 output per member."""
+    L = _lib_()
+    offsets = np.asarray(offsets, np.int64)
+    H = len(offsets) - 1
+    m, x, y, z = (to_dev(a, np.float32) for a in (m, x, y, z))
+    out = torch.zeros(int(x.numel()), dtype=torch.float64, device=x.device)
+    if H <= 0:
+        return out
+    off = _lib.i64_array(offsets)
+    ws = workspace(L.halma_pot_sum_self_workspace_size(H, off))
+    rc = L.halma_pot_sum_self_batched(H, off, ptr(m), ptr(x), ptr(y), ptr(z), ptr(out), ptr(ws),
+                                      ws.numel(), stream_ptr())
+    _lib.check(rc, "halma_pot_sum_self_batched")
+    return out
+
+
 def pot_sum_f64_ordered(m, x, y, z, tx, ty, tz):
     """FP64 pair sums in the oracle's fixed summation order (bitwise reproducible)."""
     L = _lib_()

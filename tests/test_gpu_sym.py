@@ -1,0 +1,54 @@
+"""Symmetric self-potential kernel against the FP64 oracle: every member within the
+rigorous bound, on haloes of every tile/block shape (1 member, a partial tile, exact
+tiles and blocks, several blocks), with coincident coordinates (the skip rule on both
+sides of a pair), and against the unsymmetric batched kernel."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def test_pot_sym_shapes_vs_oracle(gpu):
+    from pyhalma_amd import engine
+
+    rng = np.random.default_rng(17)
+    sizes = [1, 2, 37, 128, 129, 511, 512, 513, 1024, 2047, 3000]
+    parts = []
+    for n in sizes:
+        p = rng.normal(scale=0.01, size=(n, 3)).astype(np.float32)
+        if n > 10:  # members sharing one coordinate value with another member
+            p[: n // 10, 0] = p[n // 10: 2 * (n // 10), 0]
+        parts.append(p)
+    P = np.concatenate(parts)
+    m = rng.uniform(1e4, 1e6, len(P)).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    got = engine.pot_sum_self(off, m, P[:, 0], P[:, 1], P[:, 2]).cpu().numpy()
+    B = engine.pot_sym_error_bound()
+    for h, n in enumerate(sizes):
+        a, b = off[h], off[h + 1]
+        want = oracle.pot_sum_f64(m[a:b], P[a:b, 0], P[a:b, 1], P[a:b, 2], P[a:b, 0],
+                                  P[a:b, 1], P[a:b, 2])
+        err = np.abs(got[a:b] - want)
+        assert np.all(err <= B * want + 1e-300), (n, float((err / np.maximum(want, 1e-300)).max()))
+    print(f"haloes={len(sizes)} members={len(P)} bound={B:.2e}", flush=True)
+
+
+def test_pot_sym_matches_batched_on_clumps(gpu):
+    from pyhalma_amd import engine, synth
+
+    w = synth.clumps(seed=8, scale=0.01)
+    lab = oracle.fof_labels_c(w["x"], w["y"], w["z"], w["ll"])
+    g = engine.groups_from_labels(lab.astype(np.int32), 50)
+    md = engine.gather_members(g, w["x"], w["y"], w["z"], w["mass"])
+    off = md.groups.offsets_host
+    x, y, z, m = (t.cpu().numpy() for t in (md.x, md.y, md.z, md.m))
+    s_sym = engine.pot_sum_self(off, m, x, y, z).cpu().numpy()
+    s_bat = engine.pot_sum_batched(off, m, x, y, z, off, x, y, z).cpu().numpy()
+    rel = np.abs(s_sym - s_bat) / s_bat
+    assert rel.max() <= engine.pot_sym_error_bound() + engine.pot_error_bound()
+    h = int(np.argmax(np.diff(off)))
+    a, b = off[h], off[h + 1]
+    want = oracle.pot_sum_f64(m[a:b], x[a:b], y[a:b], z[a:b], x[a:b], y[a:b], z[a:b])
+    assert np.all(np.abs(s_sym[a:b] - want) <= engine.pot_sym_error_bound() * want)
