@@ -30,11 +30,11 @@ if ROOT not in sys.path:
 METRIC = "particles/s FoF+unbinding at 2M stars, batched FP32 vs plain FP64 on same GPU"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (AMD published, packed FMA)
 FLOP_PER_PAIR = 11         # particle_subroutines.f90:502-505 as written
-# HBM bytes per k_pot_f32 launch on this workload, averaged over the 8 unbinding
-# launches of a step: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from two rocprofv3 --pmc
+# HBM bytes per k_pot_sym launch on this workload (the first unbinding pass, the only
+# full pass of a step): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from two rocprofv3 --pmc
 # passes, each scaled by what the counter reads for a 256 MiB device copy
 # (profiles/bench_step_2M_hbm_traffic.md)
-POT_TRAFFIC_BYTES = 63.8e6
+SYM_TRAFFIC_BYTES = None
 MINP = 50
 
 
@@ -142,12 +142,12 @@ def main():
         log(f"warmup {i + 1}/{args.warmup} done")
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    pot_ns = pot_launches = pairs = 0
+    acc = dict.fromkeys(("pot_ns", "pot_launches", "pairs", "sym_ns", "sym_launches",
+                         "full_pairs", "f32_launches"), 0)
     for i in range(args.steps):
         lab, groups, red, peak, ub = step(time_pot=True)
-        pot_ns += ub.stats["pot_ns"]
-        pot_launches += ub.stats["pot_launches"]
-        pairs += ub.stats["pairs"]
+        for k in acc:
+            acc[k] += ub.stats[k]
     torch.cuda.synchronize()
     t_end = time.perf_counter()
     step_s = (t_end - t_start) / args.steps
@@ -155,16 +155,29 @@ def main():
         f"members={groups.n_members} passes={ub.stats['passes']} "
         f"retested={ub.stats['retested']}")
 
-    # roofline of the dominant kernel (the FP32 pair kernel), from its own events
-    pot_s_per_launch = pot_ns * 1e-9 / max(pot_launches, 1)
-    pairs_per_launch = pairs / max(pot_launches, 1)
-    achieved = FLOP_PER_PAIR * pairs_per_launch / pot_s_per_launch / 1e12
+    # roofline of the dominant kernel (the symmetric pair kernel of the full passes),
+    # from HIP events around its own launches.  Algorithmic work: the reference's loop
+    # over the same pairs, 11 FLOP per ordered (test, source) pair; the kernel evaluates
+    # each unordered pair once and accumulates both terms, so the FP32 instructions it
+    # executes are about half of that count (executed_* below).
+    sym_s = acc["sym_ns"] * 1e-9 / max(acc["sym_launches"], 1)
+    pairs_per_launch = acc["full_pairs"] / max(acc["sym_launches"], 1)
+    achieved = FLOP_PER_PAIR * pairs_per_launch / sym_s / 1e12
+    f32_pairs = acc["pairs"] - acc["full_pairs"]
+    f32_s = (acc["pot_ns"] - acc["sym_ns"]) * 1e-9
     roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": POT_TRAFFIC_BYTES,
-                "kernel": "k_pot_f32 (persistent FP32 pair kernel)",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": SYM_TRAFFIC_BYTES,
+                "kernel": "k_pot_sym (symmetric FP32 pair kernel, full passes)",
                 "pairs_per_launch": int(pairs_per_launch),
-                "launch_ms": round(pot_s_per_launch * 1e3, 4)}
+                "launch_ms": round(sym_s * 1e3, 4),
+                "launches_per_step": acc["sym_launches"] / args.steps,
+                "executed_pair_evals_per_launch": int(pairs_per_launch // 2),
+                "chunked_kernel": {"kernel": "k_pot_f32 (incremental passes)",
+                                   "launches_per_step": acc["f32_launches"] / args.steps,
+                                   "ms_per_step": f32_s * 1e3 / args.steps,
+                                   "tflops": (FLOP_PER_PAIR * f32_pairs / f32_s / 1e12
+                                              if f32_s > 0 else None)}}
 
     out = {
         "metric": METRIC, "value": n / step_s, "unit": "particles/s", "n_gpus": 1,

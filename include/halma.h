@@ -192,13 +192,20 @@ int halma_density_peak(int64_t n_groups, const int64_t *offsets, const int32_t *
  * lies inside the rigorous error band are re-tested with an FP64 sum in the oracle's
  * fixed order.  Outputs (device): bound[k] per member position (1 bound, 0 not); iters
  * (n_halo int32, may be NULL) passes per halo; vcm (3 per halo, may be NULL); pot (per
- * member, the last S used, may be NULL).  stats (HOST, 8 int64, may be NULL): [0] passes,
- * [1] members re-tested in FP64, [2] queue items per pass, [3] tests per lane,
- * [4] sub-tile, [5] pair-kernel time in ns and [6] timed launches (with
- * HALMA_UNBIND_TIME_POT: hipEvents recorded around the pair kernel on its stream),
- * [7] pairs evaluated.  flags: HALMA_UNBIND_GRAPHS captures each part of a pass (prepare,
- * pair kernel, decisions) into a hipGraph replayed every pass; by default the parts are
- * plain launches (a few passes do not repay the capture).  Synchronises the stream once
+ * member, the last S used, may be NULL).  stats (HOST, 12 int64, may be NULL): [0] passes,
+ * [1] members re-tested in FP64, [2] symmetric-kernel queue items of the first pass,
+ * [3] tests per lane, [4] sub-tile, [5] pair-kernel time in ns and [6] timed passes
+ * (with HALMA_UNBIND_TIME_POT: hipEvents recorded on the stream around the pair
+ * kernels), [7] pairs of the reference's loop (n_bound x n_sources summed over haloes
+ * and passes), [8] symmetric-kernel time in ns (part of [5]), [9] its launches,
+ * [10] pairs of the full passes (part of [7]; the symmetric kernel evaluates each
+ * unordered pair once for two of them), [11] chunked-kernel launches.  Full passes
+ * (first pass, or more than a quarter of the live set removed) go to the symmetric
+ * kernel, incremental ones to the chunked kernel; each pass launches only the kernel
+ * that has work.  flags: HALMA_UNBIND_GRAPHS captures each part of a pass (prepare,
+ * symmetric kernel, chunked kernel, decisions) into a hipGraph replayed every pass (both
+ * kernels every pass); by default the parts are plain launches (a few passes do not
+ * repay the capture).  Synchronises the stream once
  * per pass.
  * ---------------------------------------------------------------------------------- */
 #define HALMA_UNBIND_GRAPHS 1

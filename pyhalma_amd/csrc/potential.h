@@ -28,6 +28,10 @@ struct PotPlan {
     int grid = 0;              // persistent workgroups (256 threads)
 };
 
+// The halo order of a plan: indices sorted by key, largest first, ties by index (a
+// stable descending order).  Keys are non-negative and below 2^31 * 2^31.
+std::vector<int32_t> order_desc(const std::vector<int64_t>& key);
+
 // Plans a batched launch from HOST offset arrays (H+1 entries each).  The item list is
 // built only with with_items (callers that generate items on the device need the count).
 // tests_per_lane > 0 overrides the tuned tile height (kernels with a fixed one).

@@ -49,32 +49,40 @@ __device__ __forceinline__ void kahan(float& s, float& c, float v) {
     s = t;
 }
 
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    // lanes whose source lies outside their row of 16 read 0 (never used below)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf,
+                                                      true));
+}
+
 // v[0..7] per lane -> the sum over all 64 lanes of v[k], for k = source index of the lane:
-// k = 4*bit5 + 2*bit4 + bit3 of the lane id (every lane holds the total of its k).
-// Recursive halving (32, 16, 8) then a butterfly over the low three lane bits: a fixed
-// binary tree of depth 6 per total.
-__device__ __forceinline__ float reduce8(float v[8], int lane) {
-    const bool h5 = (lane & 32) != 0;
+// k = 4*bit5 + 2*bit4 + bit3 of the lane id, valid in the lanes with lane % 8 == 0.
+// A fixed binary tree of depth 6 per total: recursive halving over lane bits 5 and 4 with
+// the CDNA4 half-wave and row swaps (v_permlane32_swap / v_permlane16_swap: each lane
+// ends with its half's pair sum of v[k] or v[k + 4], then of w[k] or w[k + 2]), over
+// bit 3 with a row rotation by 8, then lane + (lane + 4), + (lane + 2), + (lane + 1)
+// within the row (DPP row shifts).
+__device__ __forceinline__ float reduce8(const float v[8], int lane) {
     float w[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const float send = h5 ? v[k] : v[k + 4];
-        const float keep = h5 ? v[k + 4] : v[k];
-        w[k] = keep + __shfl_xor(send, 32, kWave);
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[k]),
+                                                        __float_as_uint(v[k + 4]), false, false);
+        w[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
     }
-    const bool h4 = (lane & 16) != 0;
     float u[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-        const float send = h4 ? w[k] : w[k + 2];
-        const float keep = h4 ? w[k + 2] : w[k];
-        u[k] = keep + __shfl_xor(send, 16, kWave);
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(w[k]),
+                                                        __float_as_uint(w[k + 2]), false, false);
+        u[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
     }
     const bool h3 = (lane & 8) != 0;
-    float t = (h3 ? u[1] : u[0]) + __shfl_xor(h3 ? u[0] : u[1], 8, kWave);
-    t += __shfl_xor(t, 4, kWave);
-    t += __shfl_xor(t, 2, kWave);
-    t += __shfl_xor(t, 1, kWave);
+    float t = (h3 ? u[1] : u[0]) + dpp_move<0x128>(h3 ? u[0] : u[1]);  // row_ror:8
+    t += dpp_move<0x104>(t);                                           // row_shl:4
+    t += dpp_move<0x102>(t);                                           // row_shl:2
+    t += dpp_move<0x101>(t);                                           // row_shl:1
     return t;
 }
 
@@ -273,10 +281,7 @@ int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items) {
         part += static_cast<int64_t>(d.nb + 1) * n;
         work[h] = n;
     }
-    plan.order.resize(H);
-    for (int64_t h = 0; h < H; ++h) plan.order[h] = static_cast<int32_t>(h);
-    std::stable_sort(plan.order.begin(), plan.order.end(),
-                     [&](int32_t a, int32_t b) { return work[a] > work[b]; });
+    plan.order = order_desc(work);
     int64_t n_items = 0;
     for (int64_t h = 0; h < H; ++h) {
         const int64_t nb = plan.halos[h].nb;

@@ -108,6 +108,13 @@ __global__ void k_init_halo(const int64_t* __restrict__ off, int H, int32_t* __r
     removed[h] = 0;
 }
 
+// A halo's pass is full (tests = sources = the live bound set, symmetric kernel) on
+// the first pass and whenever more than a quarter of the live set left in the previous
+// pass; otherwise incremental (the removed members' terms are subtracted).
+__device__ __forceinline__ bool pass_is_full(int pass_no, int run, int rrun) {
+    return pass_no == 0 || 4 * static_cast<int64_t>(rrun) > run;
+}
+
 __global__ void k_zero_counters(unsigned long long* __restrict__ c, int n) {
     if (static_cast<int>(threadIdx.x) < n) c[threadIdx.x] = 0;
 }
@@ -230,7 +237,11 @@ __global__ void k_halo_sets(int H, const int32_t* __restrict__ order,
                             int64_t* __restrict__ icnt_sym,
                             unsigned long long* __restrict__ counters) {
     const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos == 0) counters[0] = 0;
+    if (pos == 0) {
+        counters[0] = 0;
+        counters[5] = 0;  // next-pass work, counted by k_finish
+        counters[6] = 0;
+    }
     if (pos >= H) return;
     const int h = order[pos];
     if (!active[h]) {
@@ -258,7 +269,7 @@ __global__ void k_halo_sets(int H, const int32_t* __restrict__ order,
     const double M = acc[0][0] + acc[0][1];
     for (int d = 0; d < 3; ++d)
         vcm[3 * h + d] = M > 0.0 ? (acc[1 + d][0] + acc[1 + d][1]) / M : 0.0;
-    const int full = (*pass_no == 0 || 4 * static_cast<int64_t>(rrun) > run) ? 1 : 0;
+    const int full = pass_is_full(*pass_no, run, rrun) ? 1 : 0;
     n_bound[h] = run;
     n_rem[h] = rrun;
     mode[h] = full;
@@ -354,7 +365,9 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     if (p >= n) return;
     const bool full = a.mode[h] != 0;
     // full pass: the symmetric kernel's nb + 1 slots (nb of the live set); incremental
-    // pass: the chunk partials; both summed in order, loads issued eight at a time
+    // pass: the chunk partials; both summed in order, loads issued sixteen at a time
+    // (indices past the end re-read the last slot and add zero, which leaves the sum
+    // unchanged); the member's own loads are issued first
     int nch;
     int64_t stride;
     const float2* pp;
@@ -369,20 +382,20 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
         stride = d.stride;
         pp = a.part + d.part_off + p;
     }
-    double sum = 0.0;
-    int c = 0;
-    for (; c + 8 <= nch; c += 8) {
-        float2 q[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) q[u] = pp[static_cast<int64_t>(c + u) * stride];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) sum += static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
-    }
-    for (; c < nch; ++c) {
-        const float2 q = pp[static_cast<int64_t>(c) * stride];
-        sum += static_cast<double>(q.x) + static_cast<double>(q.y);
-    }
     const int64_t k = a.cidx[g];
+    const double kin = kinetic(a.vel, k, a.vcm + 3 * h);
+    double sum = 0.0;
+    for (int c = 0; c < nch; c += 16) {
+        float2 q[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            q[u] = pp[static_cast<int64_t>(min(c + u, nch - 1)) * stride];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const double t = static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
+            sum += c + u < nch ? t : 0.0;
+        }
+    }
     double S, err;
     if (full) {
         S = sum;
@@ -393,7 +406,6 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     }
     a.sacc[k] = S;
     a.eacc[k] = err;
-    const double kin = kinetic(a.vel, k, a.vcm + 3 * h);
     const double pot = a.cg * S;
     const double E = kin - pot;
     // FP32 bound, FP64 rounding of the updates and of E, and the oracle's own FP64
@@ -438,27 +450,58 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
     }
 }
 
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// End of a pass: per-halo iteration counts, the halo stays active while it removed
+// someone, and the counters (wave-aggregated atomics): [0] removed total, [3] pairs of
+// the reference's loop this pass (n_bound x n_src), [4] those of full passes,
+// [5] / [6] haloes with full / incremental work in the next pass (the rule of
+// k_halo_sets applied to the next pass's live and removed counts).
 __global__ void k_finish(int H, const int32_t* __restrict__ n_bound,
-                         const int32_t* __restrict__ n_src, int32_t* __restrict__ active,
-                         int32_t* __restrict__ iters, int32_t* __restrict__ removed,
-                         int32_t* __restrict__ pass_no, unsigned long long* __restrict__ counters) {
+                         const int32_t* __restrict__ n_src, const int32_t* __restrict__ mode,
+                         int32_t* __restrict__ active, int32_t* __restrict__ iters,
+                         int32_t* __restrict__ removed, int32_t* __restrict__ pass_no,
+                         unsigned long long* __restrict__ counters) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h == 0) {
         counters[1] = 0;  // the band list is rebuilt by the next pass
         *pass_no += 1;
     }
-    if (h >= H) return;
-    if (!active[h]) return;
-    if (n_bound[h] > 0) {
-        iters[h] += 1;
-        // pairs evaluated by the pair kernel this pass
-        atomicAdd(counters + 3, static_cast<unsigned long long>(n_bound[h]) *
-                                    static_cast<unsigned long long>(n_src[h]));
+    unsigned long long rem = 0, pairs = 0, full_pairs = 0;
+    bool next_full = false, next_incr = false;
+    if (h < H && active[h]) {
+        const int nb = n_bound[h];
+        const int r = removed[h];
+        if (nb > 0) {
+            iters[h] += 1;
+            pairs = static_cast<unsigned long long>(nb) *
+                    static_cast<unsigned long long>(n_src[h]);
+            if (mode[h]) full_pairs = pairs;
+        }
+        if (r == 0 || nb == 0) {
+            active[h] = 0;
+        } else if (nb - r > 0) {
+            if (pass_is_full(1, nb - r, r)) next_full = true;
+            else next_incr = true;
+        }
+        rem = static_cast<unsigned long long>(r);
+        removed[h] = 0;
     }
-    const int r = removed[h];
-    if (r == 0 || n_bound[h] == 0) active[h] = 0;
-    if (r) atomicAdd(counters + 0, static_cast<unsigned long long>(r));
-    removed[h] = 0;
+    rem = wave_sum_u64(rem);
+    pairs = wave_sum_u64(pairs);
+    full_pairs = wave_sum_u64(full_pairs);
+    const unsigned long long nf = __popcll(__ballot(next_full));
+    const unsigned long long ni = __popcll(__ballot(next_incr));
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (rem) atomicAdd(counters + 0, rem);
+        if (pairs) atomicAdd(counters + 3, pairs);
+        if (full_pairs) atomicAdd(counters + 4, full_pairs);
+        if (nf) atomicAdd(counters + 5, nf);
+        if (ni) atomicAdd(counters + 6, ni);
+    }
 }
 
 // The pass's item list: the items of the haloes with live work, in the plan's halo
@@ -679,11 +722,15 @@ struct PassLaunch {
     UnbWs w;
     uint8_t* bound;
     EnergyArgs ea;
+    // which pair kernel has work this pass (from the previous pass's k_finish; both
+    // when the parts are captured into graphs)
+    bool do_sym, do_f32;
 };
 
-// A pass in three parts (each captured into its own graph): prepare (compaction, live
-// lengths), the pair kernel, and the energy decisions.  Timing events, when requested,
-// are recorded on the stream between the parts, around the pair kernel alone.
+// A pass in four parts (each captured into its own graph): prepare (compaction, live
+// lengths, item lists), the symmetric kernel (full passes), the chunked kernel
+// (incremental passes), and the energy decisions.  Timing events, when requested, are
+// recorded on the stream between the parts, so that they bracket the pair kernels alone.
 int launch_pre(const PassLaunch& L, hipStream_t st) {
     const UnbWs& w = L.w;
     const int nc = static_cast<int>(w.n_chunks);
@@ -698,24 +745,28 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
         w.csum, w.n_bound, w.n_rem, w.mode, w.vcm, w.tst_len, w.src_len, w.src_sel, w.icnt,
         w.sym_icnt, w.counters);
     HALMA_LAUNCH_CHECK();
-    const int rc = exclusive_scan_i64(w.icnt, w.ioff, L.H, w.itotal, w.iscan_ws, st);
-    if (rc) return rc;
-    const int64_t cap = L.plan->n_items;
-    const unsigned fill_grid =
-        static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(1024, (cap + 255) / 256)));
-    k_item_fill<<<fill_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.ioff,
-                                                 w.itotal, w.tst_len, tile, cap, w.items,
-                                                 w.head);
-    HALMA_LAUNCH_CHECK();
-    const int rc2 = exclusive_scan_i64(w.sym_icnt, w.sym_ioff, L.H, w.sym_itotal, w.sym_scan_ws,
-                                       st);
-    if (rc2) return rc2;
-    const unsigned sym_grid = static_cast<unsigned>(
-        std::max<int64_t>(1, std::min<int64_t>(1024, (w.sym_cap + 255) / 256)));
-    k_sym_item_fill<<<sym_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.sym_ioff,
-                                                   w.sym_itotal, w.sym_cap, w.sym_items,
-                                                   w.sym_head);
-    HALMA_LAUNCH_CHECK();
+    if (L.do_f32) {
+        const int rc = exclusive_scan_i64(w.icnt, w.ioff, L.H, w.itotal, w.iscan_ws, st);
+        if (rc) return rc;
+        const int64_t cap = L.plan->n_items;
+        const unsigned fill_grid = static_cast<unsigned>(
+            std::max<int64_t>(1, std::min<int64_t>(1024, (cap + 255) / 256)));
+        k_item_fill<<<fill_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.ioff,
+                                                     w.itotal, w.tst_len, tile, cap, w.items,
+                                                     w.head);
+        HALMA_LAUNCH_CHECK();
+    }
+    if (L.do_sym) {
+        const int rc = exclusive_scan_i64(w.sym_icnt, w.sym_ioff, L.H, w.sym_itotal,
+                                          w.sym_scan_ws, st);
+        if (rc) return rc;
+        const unsigned sym_grid = static_cast<unsigned>(
+            std::max<int64_t>(1, std::min<int64_t>(1024, (w.sym_cap + 255) / 256)));
+        k_sym_item_fill<<<sym_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order,
+                                                       w.sym_ioff, w.sym_itotal, w.sym_cap,
+                                                       w.sym_items, w.sym_head);
+        HALMA_LAUNCH_CHECK();
+    }
     if (nc > 0) {
         k_chunk_write<<<L.grid_chunks, kThreads, 0, st>>>(w.chunks, nc, w.off, w.active, w.ccnt,
                                                           w.state, w.p4, w.src4, w.cidx, w.rem4);
@@ -724,7 +775,17 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
     return 0;
 }
 
-int launch_pot(const PassLaunch& L, hipStream_t st) {
+// The queue heads are cleared by the item fills in the prepare part, so that the two
+// pair-kernel parts hold one kernel each (their timing events bracket nothing else).
+int launch_pot_sym_part(const PassLaunch& L, hipStream_t st) {
+    if (!L.do_sym) return 0;
+    const UnbWs& w = L.w;
+    return launch_pot_sym(*L.splan, w.src4, w.sym, w.n_bound, w.sym_items, w.sym_itotal,
+                          w.sym_part, w.sym_head, st, false);
+}
+
+int launch_pot_f32_part(const PassLaunch& L, hipStream_t st) {
+    if (!L.do_f32) return 0;
     const UnbWs& w = L.w;
     DynSets dyn;
     dyn.tst_len = w.tst_len;
@@ -732,11 +793,6 @@ int launch_pot(const PassLaunch& L, hipStream_t st) {
     dyn.src_sel = w.src_sel;
     dyn.src_alt = w.rem4;
     dyn.n_items = w.itotal;
-    // the queue heads are cleared by the item fills in the prepare part, so that this
-    // part holds the pair kernels alone (its timing events bracket nothing else)
-    int rc = launch_pot_sym(*L.splan, w.src4, w.sym, w.n_bound, w.sym_items, w.sym_itotal,
-                            w.sym_part, w.sym_head, st, false);
-    if (rc) return rc;
     return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, dyn, w.items, w.part, w.head, st,
                           false);
 }
@@ -750,8 +806,8 @@ int launch_post(const PassLaunch& L, hipStream_t st) {
         HALMA_LAUNCH_CHECK();
     }
     k_finish<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.n_bound, w.src_len,
-                                                   w.active, w.iters, w.removed, w.pass_no,
-                                                   w.counters);
+                                                   w.mode, w.active, w.iters, w.removed,
+                                                   w.pass_no, w.counters);
     HALMA_LAUNCH_CHECK();
     return 0;
 }
@@ -774,20 +830,55 @@ int capture_part(PartFn fn, const PassLaunch& L, hipStream_t cs, hipGraph_t* g,
     return 0;
 }
 
+// The plans of the last haloes planned on this thread: the workspace query and the run
+// of one unbinding plan the same haloes, and so do repeated runs over one catalogue.
+// Keyed by the relative offsets, the tuning and the device's CU count.
+struct UnbPlans {
+    std::vector<int64_t> rel;
+    Tuning tun;
+    int cus = -1;
+    bool valid = false;
+    PotPlan plan;
+    SymPlan splan;
+};
+
+int unb_plans(int64_t H, const int64_t* h_off, const UnbPlans** out) {
+    thread_local UnbPlans c;
+    const int cus = device_cu_count();
+    const Tuning t = g_tuning;
+    bool hit = c.valid && c.cus == cus && static_cast<int64_t>(c.rel.size()) == H + 1 &&
+               c.tun.tests_per_lane == t.tests_per_lane && c.tun.subtile == t.subtile &&
+               c.tun.min_chunk == t.min_chunk && c.tun.items_per_wave == t.items_per_wave &&
+               c.tun.waves_per_cu == t.waves_per_cu;
+    for (int64_t h = 0; hit && h <= H; ++h) hit = c.rel[h] == h_off[h] - h_off[0];
+    if (!hit) {
+        c.valid = false;
+        c.rel.resize(H + 1);
+        for (int64_t h = 0; h <= H; ++h) c.rel[h] = h_off[h] - h_off[0];
+        int rc = plan_pot(H, c.rel.data(), c.rel.data(), c.plan, false);
+        if (rc) return rc;
+        rc = plan_sym(H, c.rel.data(), c.splan, false);
+        if (rc) return rc;
+        c.tun = t;
+        c.cus = cus;
+        c.valid = true;
+    }
+    *out = &c;
+    return 0;
+}
+
 int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs in,
                double factor, int max_iter, int flags, uint8_t* bound_out, int32_t* iters_out,
                double* vcm_out, double* s_out, int64_t* stats, void* ws, size_t ws_bytes,
                hipStream_t st) {
     const int64_t nm = h_off[H] - h_off[0];
     // every halo's tests are its own (live) members
-    std::vector<int64_t> rel(H + 1);
-    for (int64_t h = 0; h <= H; ++h) rel[h] = h_off[h] - h_off[0];
-    PotPlan plan;
-    int rc = plan_pot(H, rel.data(), rel.data(), plan, false);
+    const UnbPlans* P = nullptr;
+    int rc = unb_plans(H, h_off, &P);
     if (rc) return rc;
-    SymPlan splan;
-    rc = plan_sym(H, rel.data(), splan, false);
-    if (rc) return rc;
+    const std::vector<int64_t>& rel = P->rel;
+    const PotPlan& plan = P->plan;
+    const SymPlan& splan = P->splan;
     const int64_t n_chunks = count_chunks(H, h_off);
     if (n_chunks > INT32_MAX) return set_error(HALMA_E_LIMIT, "too many members");
     UnbWs w = carve_unb(ws, H, nm, plan, splan, n_chunks);
@@ -867,31 +958,33 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
 
     const bool use_graph = (flags & HALMA_UNBIND_GRAPHS) != 0;
     const bool timing = (flags & HALMA_UNBIND_TIME_POT) != 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (timing) {
-        HALMA_HIP(hipEventCreate(&ev0));
-        HALMA_HIP(hipEventCreate(&ev1));
-    }
-    double pot_ms = 0.0;
-    long long pot_launches = 0;
-    const PartFn parts[3] = {launch_pre, launch_pot, launch_post};
-    hipGraph_t graphs[3] = {nullptr, nullptr, nullptr};
-    hipGraphExec_t execs[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (timing)
+        for (auto& e : ev) HALMA_HIP(hipEventCreate(&e));
+    double pot_ms = 0.0, sym_ms = 0.0;
+    long long pot_launches = 0, sym_launches = 0, f32_launches = 0;
+    const PartFn parts[4] = {launch_pre, launch_pot_sym_part, launch_pot_f32_part, launch_post};
+    hipGraph_t graphs[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t execs[4] = {nullptr, nullptr, nullptr, nullptr};
     auto cleanup = [&]() {
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < 4; ++q) {
             if (execs[q]) (void)hipGraphExecDestroy(execs[q]);
             if (graphs[q]) (void)hipGraphDestroy(graphs[q]);
         }
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
     };
+    // the first pass is full for every halo; later passes launch the kernels that the
+    // previous pass's k_finish found work for (graphs replay both every pass)
+    L.do_sym = true;
+    L.do_f32 = use_graph;
     if (use_graph) {
         hipStream_t cs = capture_stream();
         if (!cs) {
             cleanup();
             return set_error(HALMA_E_STATE, "cannot create the capture stream");
         }
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < 4; ++q) {
             rc = capture_part(parts[q], L, cs, &graphs[q], &execs[q]);
             if (rc) {
                 cleanup();
@@ -906,6 +999,12 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         }
         return parts[q](L, st);
     };
+    auto record = [&](int i) -> int {
+        if (!timing) return 0;
+        return hipEventRecord(ev[i], st) == hipSuccess
+                   ? 0
+                   : set_error(HALMA_E_STATE, "event record failed");
+    };
     long long* host = pinned_words(8);
     if (!host) {
         cleanup();
@@ -916,14 +1015,14 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     int status = 0;
     for (; passes < max_iter; ++passes) {
         status = run_part(0);
-        if (!status && timing) status = hipEventRecord(ev0, st) == hipSuccess ? 0 :
-            set_error(HALMA_E_STATE, "event record failed");
+        if (!status) status = record(0);
         if (!status) status = run_part(1);
-        if (!status && timing) status = hipEventRecord(ev1, st) == hipSuccess ? 0 :
-            set_error(HALMA_E_STATE, "event record failed");
+        if (!status) status = record(1);
         if (!status) status = run_part(2);
+        if (!status) status = record(2);
+        if (!status) status = run_part(3);
         if (status) break;
-        const hipError_t e1 = hipMemcpyAsync(host, w.counters, 4 * sizeof(long long),
+        const hipError_t e1 = hipMemcpyAsync(host, w.counters, 7 * sizeof(long long),
                                              hipMemcpyDeviceToHost, st);
         const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(st) : e1;
         if (e2 != hipSuccess) {
@@ -931,19 +1030,27 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
             break;
         }
         band_total = host[2];
+        sym_launches += L.do_sym ? 1 : 0;
+        f32_launches += L.do_f32 ? 1 : 0;
         if (timing) {
-            float ms = 0.0f;
-            const hipError_t e4 = hipEventElapsedTime(&ms, ev0, ev1);
+            float ms_all = 0.0f, ms_sym = 0.0f;
+            hipError_t e4 = hipEventElapsedTime(&ms_all, ev[0], ev[2]);
+            if (e4 == hipSuccess) e4 = hipEventElapsedTime(&ms_sym, ev[0], ev[1]);
             if (e4 != hipSuccess) {
                 status = hip_fail(e4, "pair-kernel timing", __FILE__, __LINE__);
                 break;
             }
-            pot_ms += ms;
+            pot_ms += ms_all;
+            if (L.do_sym) sym_ms += ms_sym;
             ++pot_launches;
         }
         if (host[0] == 0) {
             ++passes;
             break;
+        }
+        if (!use_graph) {
+            L.do_sym = host[5] > 0;
+            L.do_f32 = host[6] > 0;
         }
         // the removed total is cleared by the next pass's k_halo_sets
     }
@@ -966,6 +1073,10 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         stats[5] = static_cast<int64_t>(pot_ms * 1e6);  // ns
         stats[6] = pot_launches;
         stats[7] = host[3];
+        stats[8] = static_cast<int64_t>(sym_ms * 1e6);  // ns
+        stats[9] = sym_launches;
+        stats[10] = host[4];
+        stats[11] = f32_launches;
     }
     return 0;
 }
@@ -1049,13 +1160,9 @@ __global__ void k_plain_energy(const int32_t* __restrict__ cnt, const int32_t* _
 }  // namespace
 
 size_t unbind_workspace_bytes(int64_t H, const int64_t* h_off) {
-    std::vector<int64_t> rel(H + 1);
-    for (int64_t h = 0; h <= H; ++h) rel[h] = h_off[h] - h_off[0];
-    PotPlan plan;
-    if (plan_pot(H, rel.data(), rel.data(), plan, false)) return 0;
-    SymPlan splan;
-    if (plan_sym(H, rel.data(), splan, false)) return 0;
-    return carve_unb(nullptr, H, h_off[H] - h_off[0], plan, splan, count_chunks(H, h_off))
+    const UnbPlans* P = nullptr;
+    if (unb_plans(H, h_off, &P)) return 0;
+    return carve_unb(nullptr, H, h_off[H] - h_off[0], P->plan, P->splan, count_chunks(H, h_off))
         .bytes;
 }
 

@@ -392,13 +392,14 @@ def unbind(groups, x, y, z, vx, vy, vz, m, factor=1.0, max_iter=100, use_graph=F
     H = groups.n_groups
     nm = groups.n_members
     dev = x.device
-    bound = torch.zeros(max(nm, 1), dtype=torch.uint8, device=dev)
-    iters = torch.zeros(max(H, 1), dtype=torch.int32, device=dev)
-    vcm = torch.zeros((max(H, 1), 3), dtype=torch.float64, device=dev)
+    # every entry of these is written by the call (no clearing launches)
+    bound = torch.empty(max(nm, 1), dtype=torch.uint8, device=dev)
+    iters = torch.empty(max(H, 1), dtype=torch.int32, device=dev)
+    vcm = torch.empty((max(H, 1), 3), dtype=torch.float64, device=dev)
     pot = (torch.full((max(nm, 1),), float("nan"), dtype=torch.float64, device=dev)
            if with_pot else None)
     import ctypes
-    st = (ctypes.c_int64 * 8)()
+    st = (ctypes.c_int64 * 12)()
     if H > 0:
         off = _lib.i64_array(groups.offsets_host)
         ws = workspace(L.halma_unbind_workspace_size(H, off))
@@ -412,7 +413,8 @@ def unbind(groups, x, y, z, vx, vy, vz, m, factor=1.0, max_iter=100, use_graph=F
         _lib.check(rc, "halma_unbind_batched")
     s = dict(passes=int(st[0]), retested=int(st[1]), items=int(st[2]),
              tests_per_lane=int(st[3]), subtile=int(st[4]), pot_ns=int(st[5]),
-             pot_launches=int(st[6]), pairs=int(st[7]))
+             pot_launches=int(st[6]), pairs=int(st[7]), sym_ns=int(st[8]),
+             sym_launches=int(st[9]), full_pairs=int(st[10]), f32_launches=int(st[11]))
     return UnbindResult(bound[:nm], iters[:H], vcm[:H], None if pot is None else pot[:nm], s)
 
 

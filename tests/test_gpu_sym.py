@@ -52,3 +52,74 @@ def test_pot_sym_matches_batched_on_clumps(gpu):
     a, b = off[h], off[h + 1]
     want = oracle.pot_sum_f64(m[a:b], x[a:b], y[a:b], z[a:b], x[a:b], y[a:b], z[a:b])
     assert np.all(np.abs(s_sym[a:b] - want) <= engine.pot_sym_error_bound() * want)
+
+
+def _max_chain(parts):
+    worst = 0
+    for p in parts:
+        for ax in range(3):
+            _, c = np.unique(p[:, ax], return_counts=True)
+            worst = max(worst, int(c.max()))
+    return worst
+
+
+def _colliding_pairs(p):
+    """Pairs of one halo with an equal coordinate (counted once)."""
+    n = len(p)
+    eq = np.zeros((n, n), bool)
+    for ax in range(3):
+        eq |= p[:, ax][:, None] == p[:, ax][None, :]
+    return int((np.triu(eq, 1)).sum())
+
+
+def test_pot_sym_collisions_vs_oracle(gpu):
+    """Dense equal coordinates (tens of thousands of pairs the reference's rule skips,
+    spread over every tile pair) and coincident members: every member within the bound
+    of the oracle, which applies the rule to every pair."""
+    from pyhalma_amd import engine
+
+    rng = np.random.default_rng(23)
+    sizes = [300, 700, 1500, 2600]
+    parts = []
+    for n in sizes:
+        p = rng.normal(scale=0.01, size=(n, 3))
+        p = (np.round(p / 1e-4) * 1e-4).astype(np.float32)
+        p[:20] = p[20:40]  # coincident members
+        parts.append(p)
+    assert _max_chain(parts) > 5
+    n_coll = sum(_colliding_pairs(p) for p in parts)
+    assert n_coll > 10000
+    P = np.concatenate(parts)
+    m = rng.uniform(1e4, 1e6, len(P)).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    got = engine.pot_sum_self(off, m, P[:, 0], P[:, 1], P[:, 2]).cpu().numpy()
+    B = engine.pot_sym_error_bound()
+    for h in range(len(sizes)):
+        a, b = off[h], off[h + 1]
+        want = oracle.pot_sum_f64(m[a:b], P[a:b, 0], P[a:b, 1], P[a:b, 2], P[a:b, 0],
+                                  P[a:b, 1], P[a:b, 2])
+        assert np.all(np.isfinite(got[a:b]))
+        assert np.all(np.abs(got[a:b] - want) <= B * want), h
+    print(f"colliding pairs={n_coll}", flush=True)
+
+
+def test_pot_sym_shared_plane_vs_oracle(gpu):
+    """A halo whose members all share one z value: the rule skips every pair of it (S = 0
+    exactly), next to an ordinary halo in the same call."""
+    from pyhalma_amd import engine
+
+    rng = np.random.default_rng(29)
+    n = 900
+    p = rng.normal(scale=0.01, size=(n, 3)).astype(np.float32)
+    p[:, 2] = np.float32(0.003)
+    q = rng.normal(scale=0.01, size=(500, 3)).astype(np.float32)
+    P = np.concatenate([p, q])
+    m = rng.uniform(1e4, 1e6, len(P)).astype(np.float32)
+    off = np.array([0, n, n + 500])
+    got = engine.pot_sum_self(off, m, P[:, 0], P[:, 1], P[:, 2]).cpu().numpy()
+    B = engine.pot_sym_error_bound()
+    want0 = oracle.pot_sum_f64(m[:n], *(P[:n, k] for k in range(3)), *(P[:n, k] for k in range(3)))
+    # every pair of the first halo shares z: the reference's rule skips them all
+    assert np.all(want0 == 0.0) and np.all(got[:n] == 0.0)
+    want1 = oracle.pot_sum_f64(m[n:], *(P[n:, k] for k in range(3)), *(P[n:, k] for k in range(3)))
+    assert np.all(np.abs(got[n:] - want1) <= B * want1)

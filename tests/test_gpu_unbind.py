@@ -49,6 +49,42 @@ def test_unbind_matches_oracle(gpu, factor):
           f"unbound={int((~bound).sum())} max_rel_pot_err={rel.max():.2e}", flush=True)
 
 
+def test_unbind_equal_coordinates(gpu):
+    """Members sharing a coordinate value with another member of their halo (and
+    coincident members), whose pairs the reference's rule skips: masks and pass counts
+    equal the oracle's."""
+    from pyhalma_amd import engine, synth
+
+    w = synth.clumps(seed=14, scale=0.01)
+    g = _groups(w)
+    rng = np.random.default_rng(14)
+    w = dict(w)
+    for k in ("x", "y", "z"):
+        w[k] = w[k].copy()
+    off = g.offsets_host
+    mem = g.members[: g.n_members].cpu().numpy()
+    n_set = 0
+    for h in range(g.n_groups):
+        idx = mem[off[h]:off[h + 1]]
+        n = len(idx)
+        src = rng.choice(idx, size=n // 8)
+        dst = rng.choice(idx, size=n // 8)
+        ax = rng.integers(0, 3, size=n // 8)
+        for s, d, a in zip(src, dst, ax):
+            k = "xyz"[a]
+            w[k][d] = w[k][s]
+        n_set += n // 8
+        for k in ("x", "y", "z"):  # two coincident members
+            w[k][idx[1]] = w[k][idx[0]]
+    r = engine.unbind(g, w["x"], w["y"], w["z"], w["vx"], w["vy"], w["vz"], w["mass"], 1.0)
+    bound, iters, _ = _oracle(g, w, 1.0)
+    got = r.bound.cpu().numpy().astype(bool)
+    print(f"haloes={g.n_groups} members={g.n_members} copied coordinates={n_set} "
+          f"unbound={int((~bound).sum())}", flush=True)
+    assert np.array_equal(got, bound), int((got != bound).sum())
+    assert np.array_equal(r.iters.cpu().numpy(), iters)
+
+
 def test_unbind_plummer_background(gpu):
     from pyhalma_amd import engine, synth
 
