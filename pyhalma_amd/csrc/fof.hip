@@ -177,6 +177,22 @@ __global__ void k_fill_i32n(int* p, int64_t n, int v) {
     if (i < n) p[i] = v;
 }
 
+// (x, y, z, 0) in one vector: float4 for FP32 coordinates, double4 for FP64
+template <typename T>
+struct Packed;
+template <>
+struct Packed<float> {
+    typedef float4 type;
+    __device__ static float4 make(float x, float y, float z) { return make_float4(x, y, z, 0.f); }
+};
+template <>
+struct Packed<double> {
+    typedef double4 type;
+    __device__ static double4 make(double x, double y, double z) {
+        return make_double4(x, y, z, 0.0);
+    }
+};
+
 // Binning: every particle finds (or inserts) its sub-cell in the hash table and records
 // the slot; a stable radix sort of (slot, particle index) then lists the particles cell
 // by cell, ascending within a cell (so a cell's first particle is its minimum index).
@@ -206,12 +222,15 @@ __global__ __launch_bounds__(256) void k_fof_keys(
     const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, int64_t n,
     const double* __restrict__ box, double inv_s, Entry* __restrict__ tab, uint64_t mask,
     uint32_t* __restrict__ skey, uint32_t* __restrict__ bits, uint64_t bmask,
-    int* __restrict__ err) {
+    int* __restrict__ err, typename Packed<T>::type* __restrict__ pk) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double q[3] = {(static_cast<double>(x[i]) - box[0]) * inv_s,
-                         (static_cast<double>(y[i]) - box[1]) * inv_s,
-                         (static_cast<double>(z[i]) - box[2]) * inv_s};
+    const T xi = x[i], yi = y[i], zi = z[i];
+    // packed copy for the cell-order gather (one line per particle instead of three)
+    pk[i] = Packed<T>::make(xi, yi, zi);
+    const double q[3] = {(static_cast<double>(xi) - box[0]) * inv_s,
+                         (static_cast<double>(yi) - box[1]) * inv_s,
+                         (static_cast<double>(zi) - box[2]) * inv_s};
     uint64_t key = 0;
     bool bad = false;
 #pragma unroll
@@ -261,8 +280,8 @@ __global__ void k_run_heads(const uint32_t* __restrict__ s, int64_t n, int* __re
 template <typename T>
 __global__ void k_cells(const uint32_t* __restrict__ skey, const int32_t* __restrict__ perm,
                         const int64_t* __restrict__ before, const int* __restrict__ head,
-                        int64_t n, Entry* __restrict__ tab, const T* __restrict__ x,
-                        const T* __restrict__ y, const T* __restrict__ z, T* __restrict__ sx,
+                        int64_t n, Entry* __restrict__ tab,
+                        const typename Packed<T>::type* __restrict__ pk, T* __restrict__ sx,
                         T* __restrict__ sy, T* __restrict__ sz, int* __restrict__ pcell,
                         int* __restrict__ cell_start, uint64_t* __restrict__ cell_key,
                         int* __restrict__ cell_min, int* __restrict__ parent,
@@ -272,9 +291,10 @@ __global__ void k_cells(const uint32_t* __restrict__ skey, const int32_t* __rest
     const int hd = head[j];
     const int c = static_cast<int>(before[j] + hd - 1);
     const int32_t i = perm[j];
-    sx[j] = x[i];
-    sy[j] = y[i];
-    sz[j] = z[i];
+    const typename Packed<T>::type p = pk[i];
+    sx[j] = p.x;
+    sy[j] = p.y;
+    sz[j] = p.z;
     pcell[i] = c;
     if (hd) {
         const uint32_t slot = skey[j];
@@ -541,6 +561,7 @@ struct FofWs {
     void* sx;
     void* sy;
     void* sz;
+    void* pk;         // (x, y, z, 0) per particle in input order
     int2* big;        // large pairs of cells left for k_fof_big
     unsigned long long* n_big;
     int64_t big_cap;
@@ -595,6 +616,7 @@ FofWs carve_fof(void* base, int64_t n, size_t elt) {
     w.sx = c.take<char>(nn * elt);
     w.sy = c.take<char>(nn * elt);
     w.sz = c.take<char>(nn * elt);
+    w.pk = c.take<char>(nn * 4 * elt);
     w.big_cap = 2 * nn;
     w.big = c.take<int2>(w.big_cap);
     w.n_big = c.take<unsigned long long>(4);
@@ -631,9 +653,10 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     const double side = ll / sqrt(3.0) * (1.0 - 1e-6);
     const double inv_s = 1.0 / side;
     const uint64_t bmask = static_cast<uint64_t>(w.nbits - 1);
+    auto* pk = static_cast<typename Packed<T>::type*>(w.pk);
     k_fof_keys<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.box, inv_s, w.tab,
                                                  static_cast<uint64_t>(w.cap - 1), w.skey,
-                                                 w.bits, bmask, w.err);
+                                                 w.bits, bmask, w.err, pk);
     HALMA_LAUNCH_CHECK();
     int rc = radix_sort_pairs(w.skey, nullptr, w.skey_sorted, w.perm, n, w.slot_bits,
                               w.sort_ws, st);
@@ -643,7 +666,7 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     rc = exclusive_scan_i32_to_i64(w.head, w.before, n, w.totals, w.scan_ws, st);
     if (rc) return rc;
     k_cells<T><<<blocks_for(n), 256, 0, st>>>(
-        w.skey_sorted, w.perm, w.before, w.head, n, w.tab, x, y, z, static_cast<T*>(w.sx),
+        w.skey_sorted, w.perm, w.before, w.head, n, w.tab, pk, static_cast<T*>(w.sx),
         static_cast<T*>(w.sy), static_cast<T*>(w.sz), w.pcell, w.cell_start, w.cell_key,
         w.cell_min, w.parent, w.comp_min);
     HALMA_LAUNCH_CHECK();

@@ -343,9 +343,16 @@ struct EnergyArgs {
     int32_t* removed;
     int32_t* band;
     unsigned long long* counters;  // [0] removed total, [1] band count, [2] band total,
-                                   // [3] pairs evaluated
+                                   // [3] pairs, [4] full-pass pairs, [5] / [6] haloes
+                                   // with full / incremental work next pass, [7] the
+                                   // re-test kernel's block ticket
     double* s_out;     // optional: S per member position (FP64)
     int64_t nm;
+    // the end of a pass (the re-test kernel's last block)
+    int H;
+    int32_t* active;
+    int32_t* iters;
+    int32_t* pass_no;
 };
 
 __device__ __forceinline__ double kinetic(const double* vel, int64_t k, const double* vc) {
@@ -423,10 +430,60 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     }
 }
 
-// one block per band member (grid-stride): the FP64 sum in the oracle's order decides
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// End of a pass for the haloes h0 + threadIdx.x (every thread of the block calls it):
+// per-halo iteration counts, the halo stays active while it removed someone, and the
+// counters (wave-aggregated atomics): [0] removed total, [3] pairs of the reference's loop
+// this pass (n_bound x n_src), [4] those of full passes, [5] / [6] haloes with full /
+// incremental work in the next pass (the rule of k_halo_sets applied to the next pass's
+// live and removed counts).
+__device__ void finish_haloes(const EnergyArgs& a, int h) {
+    unsigned long long rem = 0, pairs = 0, full_pairs = 0;
+    bool next_full = false, next_incr = false;
+    if (h < a.H && a.active[h]) {
+        const int nb = a.n_bound[h];
+        const int r = a.removed[h];
+        if (nb > 0) {
+            a.iters[h] += 1;
+            pairs = static_cast<unsigned long long>(nb) *
+                    static_cast<unsigned long long>(a.n_src[h]);
+            if (a.mode[h]) full_pairs = pairs;
+        }
+        if (r == 0 || nb == 0) {
+            a.active[h] = 0;
+        } else if (nb - r > 0) {
+            if (pass_is_full(1, nb - r, r)) next_full = true;
+            else next_incr = true;
+        }
+        rem = static_cast<unsigned long long>(r);
+        a.removed[h] = 0;
+    }
+    rem = wave_sum_u64(rem);
+    pairs = wave_sum_u64(pairs);
+    full_pairs = wave_sum_u64(full_pairs);
+    const unsigned long long nf = __popcll(__ballot(next_full));
+    const unsigned long long ni = __popcll(__ballot(next_incr));
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (rem) atomicAdd(a.counters + 0, rem);
+        if (pairs) atomicAdd(a.counters + 3, pairs);
+        if (full_pairs) atomicAdd(a.counters + 4, full_pairs);
+        if (nf) atomicAdd(a.counters + 5, nf);
+        if (ni) atomicAdd(a.counters + 6, ni);
+    }
+}
+
+// One block per band member (grid-stride): the FP64 sum in the oracle's order decides.
+// The last block to finish (a ticket taken after every block's re-tests) then ends the
+// pass for all haloes (finish_haloes) and clears the band list and the ticket.
 __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
                                                       const float4* __restrict__ src4) {
+    static_assert(kOrdWidth == kThreads, "one thread per halo in the closing loop");
     __shared__ double sh[kOrdWidth];
+    __shared__ int last;
     const int64_t nb = static_cast<int64_t>(a.counters[1]);
     for (int64_t w = blockIdx.x; w < nb; w += gridDim.x) {
         const int64_t g = a.band[w];
@@ -448,59 +505,19 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
             }
         }
     }
-}
-
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
-}
-
-// End of a pass: per-halo iteration counts, the halo stays active while it removed
-// someone, and the counters (wave-aggregated atomics): [0] removed total, [3] pairs of
-// the reference's loop this pass (n_bound x n_src), [4] those of full passes,
-// [5] / [6] haloes with full / incremental work in the next pass (the rule of
-// k_halo_sets applied to the next pass's live and removed counts).
-__global__ void k_finish(int H, const int32_t* __restrict__ n_bound,
-                         const int32_t* __restrict__ n_src, const int32_t* __restrict__ mode,
-                         int32_t* __restrict__ active, int32_t* __restrict__ iters,
-                         int32_t* __restrict__ removed, int32_t* __restrict__ pass_no,
-                         unsigned long long* __restrict__ counters) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h == 0) {
-        counters[1] = 0;  // the band list is rebuilt by the next pass
-        *pass_no += 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();  // this block's decisions before its ticket
+        last = atomicAdd(a.counters + 7, 1ull) == gridDim.x - 1 ? 1 : 0;
     }
-    unsigned long long rem = 0, pairs = 0, full_pairs = 0;
-    bool next_full = false, next_incr = false;
-    if (h < H && active[h]) {
-        const int nb = n_bound[h];
-        const int r = removed[h];
-        if (nb > 0) {
-            iters[h] += 1;
-            pairs = static_cast<unsigned long long>(nb) *
-                    static_cast<unsigned long long>(n_src[h]);
-            if (mode[h]) full_pairs = pairs;
-        }
-        if (r == 0 || nb == 0) {
-            active[h] = 0;
-        } else if (nb - r > 0) {
-            if (pass_is_full(1, nb - r, r)) next_full = true;
-            else next_incr = true;
-        }
-        rem = static_cast<unsigned long long>(r);
-        removed[h] = 0;
-    }
-    rem = wave_sum_u64(rem);
-    pairs = wave_sum_u64(pairs);
-    full_pairs = wave_sum_u64(full_pairs);
-    const unsigned long long nf = __popcll(__ballot(next_full));
-    const unsigned long long ni = __popcll(__ballot(next_incr));
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        if (rem) atomicAdd(counters + 0, rem);
-        if (pairs) atomicAdd(counters + 3, pairs);
-        if (full_pairs) atomicAdd(counters + 4, full_pairs);
-        if (nf) atomicAdd(counters + 5, nf);
-        if (ni) atomicAdd(counters + 6, ni);
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // every block's decisions visible to the closing block
+    for (int h0 = 0; h0 < a.H; h0 += kOrdWidth) finish_haloes(a, h0 + threadIdx.x);
+    if (threadIdx.x == 0) {
+        a.counters[1] = 0;  // the band list is rebuilt by the next pass
+        a.counters[7] = 0;
+        *a.pass_no += 1;
     }
 }
 
@@ -556,6 +573,85 @@ __global__ void k_sym_item_fill(int H, const int32_t* __restrict__ order,
         while ((B + 1) * (B + 2) / 2 <= local) ++B;
         const int64_t A = local - B * (B + 1) / 2;
         items[k] = make_int4(order[lo], static_cast<int>(A), static_cast<int>(B), 0);
+    }
+}
+
+// Both item lists of a pass in one launch, for at most kFuseH haloes: every block scans
+// the per-position counts of both lists in LDS (32-bit: the plans keep both totals below
+// 2^31) and fills its share of each list as k_item_fill and k_sym_item_fill do; block 0
+// publishes the totals and clears both queue heads.
+constexpr int kFuseH = 4096;
+constexpr int kFusePer = kFuseH / kThreads;
+
+__device__ __forceinline__ int last_le(const int* off, int H, int64_t k) {
+    int lo = 0, hi = H - 1;  // last position with off <= k (its count is > 0)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= k)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kThreads) void k_items_fused(
+    int H, const int32_t* __restrict__ order, const int64_t* __restrict__ icnt,
+    const int64_t* __restrict__ icnt_sym, const int32_t* __restrict__ tst_len, int tile,
+    int64_t cap, int64_t sym_cap, int4* __restrict__ items, int4* __restrict__ sym_items,
+    int64_t* __restrict__ itotal, int64_t* __restrict__ sym_itotal, int* __restrict__ head,
+    int* __restrict__ sym_head) {
+    __shared__ int off_a[kFuseH];
+    __shared__ int off_b[kFuseH];
+    __shared__ int shs[kThreads / kWave];
+    const int base = threadIdx.x * kFusePer;
+    int va[kFusePer], vb[kFusePer];
+    int sa = 0, sb = 0;
+#pragma unroll
+    for (int k = 0; k < kFusePer; ++k) {
+        const int p = base + k;
+        va[k] = p < H ? static_cast<int>(icnt[p]) : 0;
+        vb[k] = p < H ? static_cast<int>(icnt_sym[p]) : 0;
+        sa += va[k];
+        sb += vb[k];
+    }
+    int ea, eb;
+    const int ta = block_excl_scan(sa, ea, shs);
+    const int tb = block_excl_scan(sb, eb, shs);
+#pragma unroll
+    for (int k = 0; k < kFusePer; ++k) {
+        off_a[base + k] = ea;
+        off_b[base + k] = eb;
+        ea += va[k];
+        eb += vb[k];
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *itotal = ta;
+        *sym_itotal = tb;
+        *head = 0;
+        *sym_head = 0;
+    }
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    const int64_t t0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t na = ta < cap ? ta : cap;
+    for (int64_t k = t0; k < na; k += stride) {
+        const int pos = last_le(off_a, H, k);
+        const int h = order[pos];
+        const int64_t local = k - off_a[pos];
+        const int tiles = (tst_len[h] + tile - 1) / tile;
+        items[k] = make_int4(h, static_cast<int>(local % tiles), static_cast<int>(local / tiles),
+                             0);
+    }
+    const int64_t nb = tb < sym_cap ? tb : sym_cap;
+    for (int64_t k = t0; k < nb; k += stride) {
+        const int pos = last_le(off_b, H, k);
+        const int64_t local = k - off_b[pos];
+        int64_t B = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(local) + 1.0) - 1.0) / 2.0);
+        while (B * (B + 1) / 2 > local) --B;
+        while ((B + 1) * (B + 2) / 2 <= local) ++B;
+        const int64_t A = local - B * (B + 1) / 2;
+        sym_items[k] = make_int4(order[pos], static_cast<int>(A), static_cast<int>(B), 0);
     }
 }
 
@@ -745,27 +841,38 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
         w.csum, w.n_bound, w.n_rem, w.mode, w.vcm, w.tst_len, w.src_len, w.src_sel, w.icnt,
         w.sym_icnt, w.counters);
     HALMA_LAUNCH_CHECK();
-    if (L.do_f32) {
-        const int rc = exclusive_scan_i64(w.icnt, w.ioff, L.H, w.itotal, w.iscan_ws, st);
-        if (rc) return rc;
-        const int64_t cap = L.plan->n_items;
-        const unsigned fill_grid = static_cast<unsigned>(
-            std::max<int64_t>(1, std::min<int64_t>(1024, (cap + 255) / 256)));
-        k_item_fill<<<fill_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.ioff,
-                                                     w.itotal, w.tst_len, tile, cap, w.items,
-                                                     w.head);
+    if (L.H <= kFuseH) {
+        const int64_t most = std::max<int64_t>(L.plan->n_items, w.sym_cap);
+        const unsigned grid = static_cast<unsigned>(
+            std::max<int64_t>(1, std::min<int64_t>(1024, (most + 255) / 256)));
+        k_items_fused<<<grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.icnt,
+                                                  w.sym_icnt, w.tst_len, tile, L.plan->n_items,
+                                                  w.sym_cap, w.items, w.sym_items, w.itotal,
+                                                  w.sym_itotal, w.head, w.sym_head);
         HALMA_LAUNCH_CHECK();
-    }
-    if (L.do_sym) {
-        const int rc = exclusive_scan_i64(w.sym_icnt, w.sym_ioff, L.H, w.sym_itotal,
-                                          w.sym_scan_ws, st);
-        if (rc) return rc;
-        const unsigned sym_grid = static_cast<unsigned>(
-            std::max<int64_t>(1, std::min<int64_t>(1024, (w.sym_cap + 255) / 256)));
-        k_sym_item_fill<<<sym_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order,
-                                                       w.sym_ioff, w.sym_itotal, w.sym_cap,
-                                                       w.sym_items, w.sym_head);
-        HALMA_LAUNCH_CHECK();
+    } else {
+        if (L.do_f32) {
+            const int rc = exclusive_scan_i64(w.icnt, w.ioff, L.H, w.itotal, w.iscan_ws, st);
+            if (rc) return rc;
+            const int64_t cap = L.plan->n_items;
+            const unsigned fill_grid = static_cast<unsigned>(
+                std::max<int64_t>(1, std::min<int64_t>(1024, (cap + 255) / 256)));
+            k_item_fill<<<fill_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.ioff,
+                                                         w.itotal, w.tst_len, tile, cap, w.items,
+                                                         w.head);
+            HALMA_LAUNCH_CHECK();
+        }
+        if (L.do_sym) {
+            const int rc = exclusive_scan_i64(w.sym_icnt, w.sym_ioff, L.H, w.sym_itotal,
+                                              w.sym_scan_ws, st);
+            if (rc) return rc;
+            const unsigned sym_grid = static_cast<unsigned>(
+                std::max<int64_t>(1, std::min<int64_t>(1024, (w.sym_cap + 255) / 256)));
+            k_sym_item_fill<<<sym_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order,
+                                                           w.sym_ioff, w.sym_itotal, w.sym_cap,
+                                                           w.sym_items, w.sym_head);
+            HALMA_LAUNCH_CHECK();
+        }
     }
     if (nc > 0) {
         k_chunk_write<<<L.grid_chunks, kThreads, 0, st>>>(w.chunks, nc, w.off, w.active, w.ccnt,
@@ -802,12 +909,9 @@ int launch_post(const PassLaunch& L, hipStream_t st) {
     if (L.nm > 0) {
         k_energy<<<blocks_for(L.nm), kThreads, 0, st>>>(L.ea);
         HALMA_LAUNCH_CHECK();
-        k_retest<<<L.grid_retest, kOrdWidth, 0, st>>>(L.ea, w.src4);
-        HALMA_LAUNCH_CHECK();
     }
-    k_finish<<<blocks_for(L.H), kThreads, 0, st>>>(static_cast<int>(L.H), w.n_bound, w.src_len,
-                                                   w.mode, w.active, w.iters, w.removed,
-                                                   w.pass_no, w.counters);
+    // re-tests, then the end of the pass in the kernel's last block
+    k_retest<<<L.grid_retest, kOrdWidth, 0, st>>>(L.ea, w.src4);
     HALMA_LAUNCH_CHECK();
     return 0;
 }
@@ -928,7 +1032,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.nm = nm;
     const int cus = std::max(device_cu_count(), 1);
     L.grid_chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(n_chunks, cus * 8)));
-    L.grid_retest = cus * 4;
+    L.grid_retest = 64;  // one ticket per block closes the pass: few blocks
     L.plan = &plan;
     L.splan = &splan;
     L.w = w;
@@ -955,12 +1059,23 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.ea.counters = w.counters;
     L.ea.s_out = s_out;
     L.ea.nm = nm;
+    L.ea.H = static_cast<int>(H);
+    L.ea.active = w.active;
+    L.ea.iters = w.iters;
+    L.ea.pass_no = w.pass_no;
 
     const bool use_graph = (flags & HALMA_UNBIND_GRAPHS) != 0;
     const bool timing = (flags & HALMA_UNBIND_TIME_POT) != 0;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // Passes are launched one ahead of the host: pass p + 1 is queued before pass p's
+    // counters are read, so the GPU never waits for the host between passes.  Pass p's
+    // counters go to slot p % 2 of a pinned buffer, with slot events; when pass p removes
+    // nobody, the queued pass p + 1 finds no active halo and does nothing.
+    hipEvent_t ev[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    for (auto& d : done) HALMA_HIP(hipEventCreateWithFlags(&d, hipEventDisableTiming));
     if (timing)
-        for (auto& e : ev) HALMA_HIP(hipEventCreate(&e));
+        for (auto& q : ev)
+            for (auto& e : q) HALMA_HIP(hipEventCreate(&e));
     double pot_ms = 0.0, sym_ms = 0.0;
     long long pot_launches = 0, sym_launches = 0, f32_launches = 0;
     const PartFn parts[4] = {launch_pre, launch_pot_sym_part, launch_pot_f32_part, launch_post};
@@ -971,13 +1086,16 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
             if (execs[q]) (void)hipGraphExecDestroy(execs[q]);
             if (graphs[q]) (void)hipGraphDestroy(graphs[q]);
         }
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
+        for (auto& q : ev)
+            for (auto& e : q)
+                if (e) (void)hipEventDestroy(e);
+        for (auto& d : done)
+            if (d) (void)hipEventDestroy(d);
     };
-    // the first pass is full for every halo; later passes launch the kernels that the
-    // previous pass's k_finish found work for (graphs replay both every pass)
+    // the first pass is full for every halo (no incremental work); later passes launch
+    // both pair kernels, and the one without items returns at once
     L.do_sym = true;
-    L.do_f32 = use_graph;
+    L.do_f32 = true;
     if (use_graph) {
         hipStream_t cs = capture_stream();
         if (!cs) {
@@ -999,61 +1117,71 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         }
         return parts[q](L, st);
     };
-    auto record = [&](int i) -> int {
-        if (!timing) return 0;
-        return hipEventRecord(ev[i], st) == hipSuccess
-                   ? 0
-                   : set_error(HALMA_E_STATE, "event record failed");
-    };
-    long long* host = pinned_words(8);
+    long long* host = pinned_words(16);
     if (!host) {
         cleanup();
         return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
     }
-    int passes = 0;
-    long long band_total = 0;
-    int status = 0;
-    for (; passes < max_iter; ++passes) {
-        status = run_part(0);
-        if (!status) status = record(0);
-        if (!status) status = run_part(1);
-        if (!status) status = record(1);
-        if (!status) status = run_part(2);
-        if (!status) status = record(2);
-        if (!status) status = run_part(3);
-        if (status) break;
-        const hipError_t e1 = hipMemcpyAsync(host, w.counters, 7 * sizeof(long long),
-                                             hipMemcpyDeviceToHost, st);
-        const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(st) : e1;
-        if (e2 != hipSuccess) {
-            status = hip_fail(e2, "counter read-back", __FILE__, __LINE__);
-            break;
-        }
-        band_total = host[2];
-        sym_launches += L.do_sym ? 1 : 0;
-        f32_launches += L.do_f32 ? 1 : 0;
+    auto record = [&](hipEvent_t e) -> int {
+        return hipEventRecord(e, st) == hipSuccess
+                   ? 0
+                   : set_error(HALMA_E_STATE, "event record failed");
+    };
+    auto launch_pass = [&](int p) -> int {
+        const int q = p & 1;
+        L.do_f32 = use_graph || p > 0;
+        int r = run_part(0);
+        if (!r && timing) r = record(ev[q][0]);
+        if (!r) r = run_part(1);
+        if (!r && timing) r = record(ev[q][1]);
+        if (!r) r = run_part(2);
+        if (!r && timing) r = record(ev[q][2]);
+        if (!r) r = run_part(3);
+        if (r) return r;
+        const hipError_t e = hipMemcpyAsync(host + 8 * q, w.counters, 7 * sizeof(long long),
+                                            hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return hip_fail(e, "counter read-back", __FILE__, __LINE__);
+        return record(done[q]);
+    };
+    // waits for pass p and adds its pair-kernel times
+    auto collect = [&](int p) -> int {
+        const int q = p & 1;
+        const hipError_t e = hipEventSynchronize(done[q]);
+        if (e != hipSuccess) return hip_fail(e, "pass completion", __FILE__, __LINE__);
+        sym_launches += 1;
+        f32_launches += (use_graph || p > 0) ? 1 : 0;
         if (timing) {
             float ms_all = 0.0f, ms_sym = 0.0f;
-            hipError_t e4 = hipEventElapsedTime(&ms_all, ev[0], ev[2]);
-            if (e4 == hipSuccess) e4 = hipEventElapsedTime(&ms_sym, ev[0], ev[1]);
-            if (e4 != hipSuccess) {
-                status = hip_fail(e4, "pair-kernel timing", __FILE__, __LINE__);
-                break;
-            }
+            hipError_t e4 = hipEventElapsedTime(&ms_all, ev[q][0], ev[q][2]);
+            if (e4 == hipSuccess) e4 = hipEventElapsedTime(&ms_sym, ev[q][0], ev[q][1]);
+            if (e4 != hipSuccess) return hip_fail(e4, "pair-kernel timing", __FILE__, __LINE__);
             pot_ms += ms_all;
-            if (L.do_sym) sym_ms += ms_sym;
+            sym_ms += ms_sym;
             ++pot_launches;
         }
-        if (host[0] == 0) {
-            ++passes;
-            break;
+        return 0;
+    };
+    int passes = 0, launched = 0;
+    long long band_total = 0;
+    const long long* last = host;
+    int status = max_iter > 0 ? launch_pass(0) : 0;
+    if (max_iter > 0) launched = 1;
+    while (!status && passes < launched) {
+        const int p = passes;
+        if (launched < max_iter) {
+            status = launch_pass(launched);
+            if (status) break;
+            ++launched;
         }
-        if (!use_graph) {
-            L.do_sym = host[5] > 0;
-            L.do_f32 = host[6] > 0;
-        }
-        // the removed total is cleared by the next pass's k_halo_sets
+        status = collect(p);
+        if (status) break;
+        last = host + 8 * (p & 1);
+        band_total = last[2];
+        passes = p + 1;
+        if (last[0] == 0) break;  // converged: a queued pass, if any, does nothing
     }
+    // the queued pass after convergence: its (empty) kernels are counted like the others
+    if (!status && launched > passes) status = collect(passes);
     cleanup();
     if (status) return status;
     if (nm > 0) {
@@ -1072,10 +1200,10 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         stats[4] = plan.subtile;
         stats[5] = static_cast<int64_t>(pot_ms * 1e6);  // ns
         stats[6] = pot_launches;
-        stats[7] = host[3];
+        stats[7] = last[3];
         stats[8] = static_cast<int64_t>(sym_ms * 1e6);  // ns
         stats[9] = sym_launches;
-        stats[10] = host[4];
+        stats[10] = last[4];
         stats[11] = f32_launches;
     }
     return 0;
