@@ -225,47 +225,85 @@ __global__ __launch_bounds__(kThreads) void k_chunk_sums(
 // previous pass in an incremental one; none when inactive) and the halo's number of
 // pair-kernel items (test tiles x source chunks).  Also clears the removed-members
 // counter for the pass.
-__global__ void k_halo_sets(int H, const int32_t* __restrict__ order,
-                            const HaloDesc* __restrict__ halos, int tile,
-                            const int32_t* __restrict__ cfirst,
-                            const int32_t* __restrict__ active, const int32_t* __restrict__ pass_no,
-                            int32_t* __restrict__ ccnt, const double* __restrict__ csum,
-                            int32_t* __restrict__ n_bound, int32_t* __restrict__ n_rem,
-                            int32_t* __restrict__ mode, double* __restrict__ vcm,
-                            int32_t* __restrict__ tst_len, int32_t* __restrict__ src_len,
-                            int32_t* __restrict__ src_sel, int64_t* __restrict__ icnt,
-                            int64_t* __restrict__ icnt_sym,
-                            unsigned long long* __restrict__ counters) {
-    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos == 0) {
+__device__ __forceinline__ double shfl_xor_f64(double v, int m) {
+    return __shfl_xor(v, m, kWave);
+}
+
+// One wavefront per halo position: the lanes take the halo's chunks (lane l: chunks l,
+// l + 64, ...), turn the counts into output offsets with a wave prefix scan, and merge the
+// double-double sums by a fixed butterfly across the lanes.
+__global__ __launch_bounds__(kThreads) void k_halo_sets(
+    int H, const int32_t* __restrict__ order, const HaloDesc* __restrict__ halos, int tile,
+    const int32_t* __restrict__ cfirst, const int32_t* __restrict__ active,
+    const int32_t* __restrict__ pass_no, int32_t* __restrict__ ccnt,
+    const double* __restrict__ csum, int32_t* __restrict__ n_bound, int32_t* __restrict__ n_rem,
+    int32_t* __restrict__ mode, double* __restrict__ vcm, int32_t* __restrict__ tst_len,
+    int32_t* __restrict__ src_len, int32_t* __restrict__ src_sel, int64_t* __restrict__ icnt,
+    int64_t* __restrict__ icnt_sym, unsigned long long* __restrict__ counters) {
+    const int lane = __lane_id();
+    const int pos = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x +
+                                      threadIdx.x) / kWave);
+    if (pos == 0 && lane == 0) {
         counters[0] = 0;
-        counters[5] = 0;  // next-pass work, counted by k_finish
+        counters[5] = 0;  // next-pass work, counted when the pass closes
         counters[6] = 0;
     }
-    if (pos >= H) return;
+    if (pos >= H) return;  // wave-uniform
     const int h = order[pos];
     if (!active[h]) {
-        n_bound[h] = 0;
-        n_rem[h] = 0;
-        tst_len[h] = 0;
-        src_len[h] = 0;
-        src_sel[h] = 0;
-        icnt[pos] = 0;
-        icnt_sym[pos] = 0;
+        if (lane == 0) {
+            n_bound[h] = 0;
+            n_rem[h] = 0;
+            tst_len[h] = 0;
+            src_len[h] = 0;
+            src_sel[h] = 0;
+            icnt[pos] = 0;
+            icnt_sym[pos] = 0;
+        }
         return;
     }
-    int run = 0, rrun = 0;
+    const int c0 = cfirst[h], c1 = cfirst[h + 1];
+    int run = 0, rrun = 0;  // totals of the chunks before the current 64
     double acc[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    for (int c = cfirst[h]; c < cfirst[h + 1]; ++c) {
-        const int kc = ccnt[2 * c], gc = ccnt[2 * c + 1];
-        ccnt[2 * c] = run;  // counts become the chunk's output offsets
-        ccnt[2 * c + 1] = rrun;
-        run += kc;
-        rrun += gc;
-        for (int q = 0; q < 4; ++q)
-            dd_add_dd(acc[q][0], acc[q][1], csum[8 * static_cast<int64_t>(c) + 2 * q],
-                      csum[8 * static_cast<int64_t>(c) + 2 * q + 1]);
+    for (int cb = c0; cb < c1; cb += kWave) {
+        const int c = cb + lane;
+        const bool in = c < c1;
+        const int kc = in ? ccnt[2 * c] : 0, gc = in ? ccnt[2 * c + 1] : 0;
+        // inclusive prefix of the packed pair (kept in the low, removed in the high half
+        // of a 64-bit word: both stay below 2^31)
+        unsigned long long v = static_cast<unsigned long long>(kc) |
+                               (static_cast<unsigned long long>(gc) << 32);
+        for (int d = 1; d < kWave; d <<= 1) {
+            const unsigned long long u = __shfl_up(v, d, kWave);
+            if (lane >= d) v += u;
+        }
+        const int incl_k = static_cast<int>(v & 0xffffffffu), incl_g = static_cast<int>(v >> 32);
+        if (in) {
+            ccnt[2 * c] = run + incl_k - kc;  // counts become the chunk's output offsets
+            ccnt[2 * c + 1] = rrun + incl_g - gc;
+            for (int q = 0; q < 4; ++q)
+                dd_add_dd(acc[q][0], acc[q][1], csum[8 * static_cast<int64_t>(c) + 2 * q],
+                          csum[8 * static_cast<int64_t>(c) + 2 * q + 1]);
+        }
+        const unsigned long long tot = __shfl(v, kWave - 1, kWave);
+        run += static_cast<int>(tot & 0xffffffffu);
+        rrun += static_cast<int>(tot >> 32);
     }
+    for (int m = kWave / 2; m >= 1; m >>= 1)
+        for (int q = 0; q < 4; ++q) {
+            const double oh = shfl_xor_f64(acc[q][0], m), ol = shfl_xor_f64(acc[q][1], m);
+            // the lower lane of each pair adds the upper one's value: the same tree and
+            // operand order in every lane
+            if (lane & m)
+                dd_add_dd(acc[q][0], acc[q][1], oh, ol);
+            else {
+                double th = oh, tl = ol;
+                dd_add_dd(th, tl, acc[q][0], acc[q][1]);
+                acc[q][0] = th;
+                acc[q][1] = tl;
+            }
+        }
+    if (lane != 0) return;
     const double M = acc[0][0] + acc[0][1];
     for (int d = 0; d < 3; ++d)
         vcm[3 * h + d] = M > 0.0 ? (acc[1 + d][0] + acc[1 + d][1]) / M : 0.0;
@@ -836,7 +874,8 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
                                                          w.vel, w.m64, w.ccnt, w.csum);
         HALMA_LAUNCH_CHECK();
     }
-    k_halo_sets<<<blocks_for(L.H), kThreads, 0, st>>>(
+    k_halo_sets<<<static_cast<unsigned>((L.H + kThreads / kWave - 1) / (kThreads / kWave)),
+                  kThreads, 0, st>>>(
         static_cast<int>(L.H), w.order, w.halos, tile, w.cfirst, w.active, w.pass_no, w.ccnt,
         w.csum, w.n_bound, w.n_rem, w.mode, w.vcm, w.tst_len, w.src_len, w.src_sel, w.icnt,
         w.sym_icnt, w.counters);
