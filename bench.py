@@ -30,11 +30,11 @@ if ROOT not in sys.path:
 METRIC = "particles/s FoF+unbinding at 2M stars, batched FP32 vs plain FP64 on same GPU"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (AMD published, packed FMA)
 FLOP_PER_PAIR = 11         # particle_subroutines.f90:502-505 as written
-# HBM bytes per k_pot_sym launch on this workload (the first unbinding pass, the only
-# full pass of a step): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from two rocprofv3 --pmc
-# passes, each scaled by what the counter reads for a 256 MiB device copy
+# HBM bytes per k_pot_sym launch on this workload, averaged over the 9 launches of a step
+# (the first pass carries almost all of it): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from two
+# rocprofv3 --pmc passes, each scaled by what the counter reads for a 256 MiB device copy
 # (profiles/bench_step_2M_hbm_traffic.md)
-SYM_TRAFFIC_BYTES = None
+SYM_TRAFFIC_BYTES = 80.9e6
 MINP = 50
 
 

@@ -410,9 +410,7 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     if (p >= n) return;
     const bool full = a.mode[h] != 0;
     // full pass: the symmetric kernel's nb + 1 slots (nb of the live set); incremental
-    // pass: the chunk partials; both summed in order, loads issued sixteen at a time
-    // (indices past the end re-read the last slot and add zero, which leaves the sum
-    // unchanged); the member's own loads are issued first
+    // pass: the chunk partials; both summed in order, loads issued eight at a time
     int nch;
     int64_t stride;
     const float2* pp;
@@ -427,20 +425,20 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
         stride = d.stride;
         pp = a.part + d.part_off + p;
     }
-    const int64_t k = a.cidx[g];
-    const double kin = kinetic(a.vel, k, a.vcm + 3 * h);
     double sum = 0.0;
-    for (int c = 0; c < nch; c += 16) {
-        float2 q[16];
+    int c = 0;
+    for (; c + 8 <= nch; c += 8) {
+        float2 q[8];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
-            q[u] = pp[static_cast<int64_t>(min(c + u, nch - 1)) * stride];
+        for (int u = 0; u < 8; ++u) q[u] = pp[static_cast<int64_t>(c + u) * stride];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const double t = static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
-            sum += c + u < nch ? t : 0.0;
-        }
+        for (int u = 0; u < 8; ++u) sum += static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
     }
+    for (; c < nch; ++c) {
+        const float2 q = pp[static_cast<int64_t>(c) * stride];
+        sum += static_cast<double>(q.x) + static_cast<double>(q.y);
+    }
+    const int64_t k = a.cidx[g];
     double S, err;
     if (full) {
         S = sum;
@@ -451,6 +449,7 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     }
     a.sacc[k] = S;
     a.eacc[k] = err;
+    const double kin = kinetic(a.vel, k, a.vcm + 3 * h);
     const double pot = a.cg * S;
     const double E = kin - pot;
     // FP32 bound, FP64 rounding of the updates and of E, and the oracle's own FP64
