@@ -198,15 +198,16 @@ int halma_density_peak(int64_t n_groups, const int64_t *offsets, const int32_t *
  * (with HALMA_UNBIND_TIME_POT: hipEvents recorded on the stream around the pair
  * kernels), [7] pairs of the reference's loop (n_bound x n_sources summed over haloes
  * and passes), [8] symmetric-kernel time in ns (part of [5]), [9] its launches,
- * [10] pairs of the full passes (part of [7]; the symmetric kernel evaluates each
- * unordered pair once for two of them), [11] chunked-kernel launches.  Full passes
- * (first pass, or more than a quarter of the live set removed) go to the symmetric
- * kernel, incremental ones to the chunked kernel; each pass launches only the kernel
- * that has work.  flags: HALMA_UNBIND_GRAPHS captures each part of a pass (prepare,
- * symmetric kernel, chunked kernel, decisions) into a hipGraph replayed every pass (both
- * kernels every pass); by default the parts are plain launches (a few passes do not
- * repay the capture).  Synchronises the stream once
- * per pass.
+ * [10] pairs of the symmetric kernel's passes (part of [7]; it evaluates each unordered
+ * pair once for two of them), [11] chunked-kernel launches.  Full passes (first pass, or
+ * more than a quarter of the live set removed) go to the symmetric kernel for haloes of
+ * at most HALMA_TUNE_SYM_MAX members, to the chunked kernel otherwise; incremental passes
+ * go to the chunked kernel.  Passes are queued one ahead of the host: the counters of
+ * pass p are read back (pinned buffer, event) while pass p + 1 is already queued, so
+ * the call ends with one extra pass that finds no active halo.  flags:
+ * HALMA_UNBIND_GRAPHS captures each part of a pass (prepare, symmetric kernel, chunked
+ * kernel, decisions) into a hipGraph replayed every pass; by default the parts are plain
+ * launches (a few passes do not repay the capture).
  * ---------------------------------------------------------------------------------- */
 #define HALMA_UNBIND_GRAPHS 1
 #define HALMA_UNBIND_TIME_POT 2
@@ -286,12 +287,16 @@ int halma_pot_soft_batched(int64_t n_halo, const int64_t *src_off, const float *
  *   HALMA_TUNE_MIN_CHUNK      : smallest source chunk when a halo's source range is
  *                               split across waves (multiple of 64, >= 64)
  *   HALMA_TUNE_ITEMS_PER_WAVE : target queue items per resident wave
- *   HALMA_TUNE_WAVES_PER_CU   : persistent waves launched per compute unit (4..32) */
+ *   HALMA_TUNE_WAVES_PER_CU   : persistent waves launched per compute unit (4..32)
+ *   HALMA_TUNE_SYM_MAX        : largest halo whose full unbinding passes use the symmetric
+ *                               kernel (0..65536; its partial slots take (n/512 + 1) * n * 8
+ *                               bytes); larger haloes use the chunked kernel */
 #define HALMA_TUNE_TESTS_PER_LANE 1
 #define HALMA_TUNE_SUBTILE 2
 #define HALMA_TUNE_MIN_CHUNK 3
 #define HALMA_TUNE_ITEMS_PER_WAVE 4
 #define HALMA_TUNE_WAVES_PER_CU 5
+#define HALMA_TUNE_SYM_MAX 6
 int halma_tune_set(int key, int value);
 int halma_tune_get(int key);
 

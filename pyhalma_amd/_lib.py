@@ -79,6 +79,7 @@ TUNE_SUBTILE = 2
 TUNE_MIN_CHUNK = 3
 TUNE_ITEMS_PER_WAVE = 4
 TUNE_WAVES_PER_CU = 5
+TUNE_SYM_MAX = 6
 
 F32 = 0
 F64 = 1

@@ -3,6 +3,7 @@
 #include <stdio.h>
 
 #include "halma_common.h"
+#include "sym.h"
 #include "tuning.h"
 
 namespace halma {
@@ -80,6 +81,12 @@ int halma_tune_set(int key, int value) {
                                         "waves per CU must be a multiple of 4 in [4, 32]");
             g_tuning.waves_per_cu = value;
             return 0;
+        case HALMA_TUNE_SYM_MAX:
+            if (value < 0 || value > halma::kSymMaxMembers)
+                return halma::set_error(HALMA_E_ARG, "symmetric-kernel halo limit must be in "
+                                                     "[0, %d]", halma::kSymMaxMembers);
+            g_tuning.sym_max = value;
+            return 0;
         default:
             return halma::set_error(HALMA_E_ARG, "unknown tuning key %d", key);
     }
@@ -93,6 +100,7 @@ int halma_tune_get(int key) {
         case HALMA_TUNE_MIN_CHUNK: return g_tuning.min_chunk;
         case HALMA_TUNE_ITEMS_PER_WAVE: return g_tuning.items_per_wave;
         case HALMA_TUNE_WAVES_PER_CU: return g_tuning.waves_per_cu;
+        case HALMA_TUNE_SYM_MAX: return g_tuning.sym_max;
         default: return halma::set_error(HALMA_E_ARG, "unknown tuning key %d", key);
     }
 }

@@ -8,6 +8,10 @@ namespace halma {
 
 constexpr int kSymK = 4;  // tiles of 128 members per block
 
+// The partial slots of a halo take (nb + 1) * n * 8 bytes, quadratic in n: the
+// unbinding keeps haloes above this size on the chunked kernel (their plan has nb = 0).
+constexpr int kSymMaxMembers = 65536;
+
 // blocks of a halo of n members
 __host__ __device__ inline int sym_blocks(int n) {
     const int nt = (n + 127) / 128;
@@ -30,7 +34,10 @@ struct SymPlan {
 };
 
 // Plans from HOST member offsets (H+1 entries); the item list only with with_items.
-int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items);
+// With max_members != 0, haloes above that size (every halo when it is negative) get no
+// blocks (nb = 0, no slots).
+int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items,
+             int max_members = 0);
 
 // Persistent launch over the items (count plan.n_items, or *n_items_dev when given);
 // live_n (device, may be null) overrides each halo's member count.  head must hold 0

@@ -264,7 +264,7 @@ SymWs carve_sym(void* b, const SymPlan& p, int64_t H) {
 
 }  // namespace
 
-int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items) {
+int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items, int max_members) {
     plan = SymPlan();
     plan.halos.resize(H);
     int64_t part = 0;
@@ -277,7 +277,7 @@ int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items) {
         d.off = off[h] - off[0];
         d.part_off = part;
         d.n = static_cast<int32_t>(n);
-        d.nb = sym_blocks(d.n);
+        d.nb = (max_members != 0 && d.n > max_members) ? 0 : sym_blocks(d.n);
         part += static_cast<int64_t>(d.nb + 1) * n;
         work[h] = n;
     }

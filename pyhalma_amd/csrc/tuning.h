@@ -11,6 +11,7 @@ struct Tuning {
     int min_chunk = 512;      // smallest source chunk when a source range is split
     int items_per_wave = 8;   // queue items targeted per resident wave
     int waves_per_cu = 32;    // persistent waves per compute unit
+    int sym_max = 65536;      // largest halo on the symmetric kernel in the unbinding
 };
 
 extern Tuning g_tuning;

@@ -239,15 +239,12 @@ __global__ __launch_bounds__(kThreads) void k_halo_sets(
     const double* __restrict__ csum, int32_t* __restrict__ n_bound, int32_t* __restrict__ n_rem,
     int32_t* __restrict__ mode, double* __restrict__ vcm, int32_t* __restrict__ tst_len,
     int32_t* __restrict__ src_len, int32_t* __restrict__ src_sel, int64_t* __restrict__ icnt,
-    int64_t* __restrict__ icnt_sym, unsigned long long* __restrict__ counters) {
+    int64_t* __restrict__ icnt_sym, const SymDesc* __restrict__ sym,
+    unsigned long long* __restrict__ counters) {
     const int lane = __lane_id();
     const int pos = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x +
                                       threadIdx.x) / kWave);
-    if (pos == 0 && lane == 0) {
-        counters[0] = 0;
-        counters[5] = 0;  // next-pass work, counted when the pass closes
-        counters[6] = 0;
-    }
+    if (pos == 0 && lane == 0) counters[0] = 0;
     if (pos >= H) return;  // wave-uniform
     const int h = order[pos];
     if (!active[h]) {
@@ -317,12 +314,14 @@ __global__ __launch_bounds__(kThreads) void k_halo_sets(
     src_len[h] = ns;
     src_sel[h] = full ? 0 : 1;
     // full passes (tests = sources = the bound set) go to the symmetric kernel: block
-    // pairs A <= B of its live size; incremental ones to the chunked kernel
+    // pairs A <= B of its live size; incremental ones, and full ones of haloes planned
+    // without slots (above kSymMaxMembers), to the chunked kernel
+    const bool use_sym = full && sym[h].nb > 0;
     const int ch = halos[h].chunk;
-    icnt[pos] = (!full && nt > 0 && ns > 0)
+    icnt[pos] = (!use_sym && nt > 0 && ns > 0)
                     ? static_cast<int64_t>((nt + tile - 1) / tile) * ((ns + ch - 1) / ch)
                     : 0;
-    const int64_t nbs = full ? sym_blocks(nt) : 0;
+    const int64_t nbs = use_sym ? sym_blocks(nt) : 0;
     icnt_sym[pos] = nbs * (nbs + 1) / 2;
 }
 
@@ -381,8 +380,7 @@ struct EnergyArgs {
     int32_t* removed;
     int32_t* band;
     unsigned long long* counters;  // [0] removed total, [1] band count, [2] band total,
-                                   // [3] pairs, [4] full-pass pairs, [5] / [6] haloes
-                                   // with full / incremental work next pass, [7] the
+                                   // [3] pairs, [4] symmetric-kernel pairs, [7] the
                                    // re-test kernel's block ticket
     double* s_out;     // optional: S per member position (FP64)
     int64_t nm;
@@ -409,12 +407,14 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     const int n = a.n_bound[h];
     if (p >= n) return;
     const bool full = a.mode[h] != 0;
-    // full pass: the symmetric kernel's nb + 1 slots (nb of the live set); incremental
-    // pass: the chunk partials; both summed in order, loads issued eight at a time
+    const bool use_sym = full && a.sym[h].nb > 0;
+    // symmetric full pass: the kernel's nb + 1 slots (nb of the live set); chunked pass
+    // (incremental, or full for haloes planned without slots): the chunk partials; both
+    // summed in order, loads issued eight at a time
     int nch;
     int64_t stride;
     const float2* pp;
-    if (full) {
+    if (use_sym) {
         const SymDesc sd = a.sym[h];
         nch = sym_blocks(n) + 1;
         stride = sd.n;
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     double S, err;
     if (full) {
         S = sum;
-        err = a.bound_sym * sum;
+        err = (use_sym ? a.bound_sym : a.bound_rel) * sum;
     } else {
         S = a.sacc[k] - sum;
         err = a.eacc[k] + a.bound_rel * sum;
@@ -472,15 +472,12 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-// End of a pass for the haloes h0 + threadIdx.x (every thread of the block calls it):
-// per-halo iteration counts, the halo stays active while it removed someone, and the
-// counters (wave-aggregated atomics): [0] removed total, [3] pairs of the reference's loop
-// this pass (n_bound x n_src), [4] those of full passes, [5] / [6] haloes with full /
-// incremental work in the next pass (the rule of k_halo_sets applied to the next pass's
-// live and removed counts).
+// End of a pass for the halo h (every thread of the block calls it): per-halo iteration
+// counts, the halo stays active while it removed someone, and the counters
+// (wave-aggregated atomics): [0] removed total, [3] pairs of the reference's loop this
+// pass (n_bound x n_src), [4] those of the symmetric kernel's passes.
 __device__ void finish_haloes(const EnergyArgs& a, int h) {
     unsigned long long rem = 0, pairs = 0, full_pairs = 0;
-    bool next_full = false, next_incr = false;
     if (h < a.H && a.active[h]) {
         const int nb = a.n_bound[h];
         const int r = a.removed[h];
@@ -488,28 +485,19 @@ __device__ void finish_haloes(const EnergyArgs& a, int h) {
             a.iters[h] += 1;
             pairs = static_cast<unsigned long long>(nb) *
                     static_cast<unsigned long long>(a.n_src[h]);
-            if (a.mode[h]) full_pairs = pairs;
+            if (a.mode[h] && a.sym[h].nb > 0) full_pairs = pairs;  // symmetric kernel's
         }
-        if (r == 0 || nb == 0) {
-            a.active[h] = 0;
-        } else if (nb - r > 0) {
-            if (pass_is_full(1, nb - r, r)) next_full = true;
-            else next_incr = true;
-        }
+        if (r == 0 || nb == 0) a.active[h] = 0;
         rem = static_cast<unsigned long long>(r);
         a.removed[h] = 0;
     }
     rem = wave_sum_u64(rem);
     pairs = wave_sum_u64(pairs);
     full_pairs = wave_sum_u64(full_pairs);
-    const unsigned long long nf = __popcll(__ballot(next_full));
-    const unsigned long long ni = __popcll(__ballot(next_incr));
     if ((threadIdx.x & (kWave - 1)) == 0) {
         if (rem) atomicAdd(a.counters + 0, rem);
         if (pairs) atomicAdd(a.counters + 3, pairs);
         if (full_pairs) atomicAdd(a.counters + 4, full_pairs);
-        if (nf) atomicAdd(a.counters + 5, nf);
-        if (ni) atomicAdd(a.counters + 6, ni);
     }
 }
 
@@ -855,9 +843,6 @@ struct PassLaunch {
     UnbWs w;
     uint8_t* bound;
     EnergyArgs ea;
-    // which pair kernel has work this pass (from the previous pass's k_finish; both
-    // when the parts are captured into graphs)
-    bool do_sym, do_f32;
 };
 
 // A pass in four parts (each captured into its own graph): prepare (compaction, live
@@ -877,7 +862,7 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
                   kThreads, 0, st>>>(
         static_cast<int>(L.H), w.order, w.halos, tile, w.cfirst, w.active, w.pass_no, w.ccnt,
         w.csum, w.n_bound, w.n_rem, w.mode, w.vcm, w.tst_len, w.src_len, w.src_sel, w.icnt,
-        w.sym_icnt, w.counters);
+        w.sym_icnt, w.sym, w.counters);
     HALMA_LAUNCH_CHECK();
     if (L.H <= kFuseH) {
         const int64_t most = std::max<int64_t>(L.plan->n_items, w.sym_cap);
@@ -889,7 +874,7 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
                                                   w.sym_itotal, w.head, w.sym_head);
         HALMA_LAUNCH_CHECK();
     } else {
-        if (L.do_f32) {
+        {
             const int rc = exclusive_scan_i64(w.icnt, w.ioff, L.H, w.itotal, w.iscan_ws, st);
             if (rc) return rc;
             const int64_t cap = L.plan->n_items;
@@ -900,7 +885,7 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
                                                          w.head);
             HALMA_LAUNCH_CHECK();
         }
-        if (L.do_sym) {
+        {
             const int rc = exclusive_scan_i64(w.sym_icnt, w.sym_ioff, L.H, w.sym_itotal,
                                               w.sym_scan_ws, st);
             if (rc) return rc;
@@ -923,14 +908,12 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
 // The queue heads are cleared by the item fills in the prepare part, so that the two
 // pair-kernel parts hold one kernel each (their timing events bracket nothing else).
 int launch_pot_sym_part(const PassLaunch& L, hipStream_t st) {
-    if (!L.do_sym) return 0;
     const UnbWs& w = L.w;
     return launch_pot_sym(*L.splan, w.src4, w.sym, w.n_bound, w.sym_items, w.sym_itotal,
                           w.sym_part, w.sym_head, st, false);
 }
 
 int launch_pot_f32_part(const PassLaunch& L, hipStream_t st) {
-    if (!L.do_f32) return 0;
     const UnbWs& w = L.w;
     DynSets dyn;
     dyn.tst_len = w.tst_len;
@@ -991,7 +974,7 @@ int unb_plans(int64_t H, const int64_t* h_off, const UnbPlans** out) {
     bool hit = c.valid && c.cus == cus && static_cast<int64_t>(c.rel.size()) == H + 1 &&
                c.tun.tests_per_lane == t.tests_per_lane && c.tun.subtile == t.subtile &&
                c.tun.min_chunk == t.min_chunk && c.tun.items_per_wave == t.items_per_wave &&
-               c.tun.waves_per_cu == t.waves_per_cu;
+               c.tun.waves_per_cu == t.waves_per_cu && c.tun.sym_max == t.sym_max;
     for (int64_t h = 0; hit && h <= H; ++h) hit = c.rel[h] == h_off[h] - h_off[0];
     if (!hit) {
         c.valid = false;
@@ -999,7 +982,9 @@ int unb_plans(int64_t H, const int64_t* h_off, const UnbPlans** out) {
         for (int64_t h = 0; h <= H; ++h) c.rel[h] = h_off[h] - h_off[0];
         int rc = plan_pot(H, c.rel.data(), c.rel.data(), c.plan, false);
         if (rc) return rc;
-        rc = plan_sym(H, c.rel.data(), c.splan, false);
+        // haloes above the limit plan no slots (their full passes take the chunked kernel);
+        // a limit of 0 plans none at all
+        rc = plan_sym(H, c.rel.data(), c.splan, false, t.sym_max > 0 ? t.sym_max : -1);
         if (rc) return rc;
         c.tun = t;
         c.cus = cus;
@@ -1130,10 +1115,8 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         for (auto& d : done)
             if (d) (void)hipEventDestroy(d);
     };
-    // the first pass is full for every halo (no incremental work); later passes launch
-    // both pair kernels, and the one without items returns at once
-    L.do_sym = true;
-    L.do_f32 = true;
+    // every pass launches both pair kernels (full passes of haloes above the symmetric
+    // limit run on the chunked kernel); the one without items returns at once
     if (use_graph) {
         hipStream_t cs = capture_stream();
         if (!cs) {
@@ -1167,7 +1150,6 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     };
     auto launch_pass = [&](int p) -> int {
         const int q = p & 1;
-        L.do_f32 = use_graph || p > 0;
         int r = run_part(0);
         if (!r && timing) r = record(ev[q][0]);
         if (!r) r = run_part(1);
@@ -1176,7 +1158,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         if (!r && timing) r = record(ev[q][2]);
         if (!r) r = run_part(3);
         if (r) return r;
-        const hipError_t e = hipMemcpyAsync(host + 8 * q, w.counters, 7 * sizeof(long long),
+        const hipError_t e = hipMemcpyAsync(host + 8 * q, w.counters, 5 * sizeof(long long),
                                             hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return hip_fail(e, "counter read-back", __FILE__, __LINE__);
         return record(done[q]);
@@ -1187,7 +1169,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         const hipError_t e = hipEventSynchronize(done[q]);
         if (e != hipSuccess) return hip_fail(e, "pass completion", __FILE__, __LINE__);
         sym_launches += 1;
-        f32_launches += (use_graph || p > 0) ? 1 : 0;
+        f32_launches += 1;
         if (timing) {
             float ms_all = 0.0f, ms_sym = 0.0f;
             hipError_t e4 = hipEventElapsedTime(&ms_all, ev[q][0], ev[q][2]);

@@ -43,3 +43,16 @@ def test_error_string_without_gpu():
     assert lib.halma_tune_set(1, 3) == -1  # tests per lane must be 1, 2 or 4
     assert b"tests per lane" in lib.halma_last_error()
     assert lib.halma_version() == 10000
+
+
+def test_symmetric_limit_key_without_gpu():
+    from pyhalma_amd import _lib
+
+    lib = _lib.load()
+    key = _lib.TUNE_SYM_MAX
+    old = lib.halma_tune_get(key)
+    assert old == 65536
+    assert lib.halma_tune_set(key, 65537) == -1
+    assert b"symmetric-kernel halo limit" in lib.halma_last_error()
+    assert lib.halma_tune_set(key, 1000) == 0 and lib.halma_tune_get(key) == 1000
+    assert lib.halma_tune_set(key, old) == 0

@@ -85,6 +85,36 @@ def test_unbind_equal_coordinates(gpu):
     assert np.array_equal(r.iters.cpu().numpy(), iters)
 
 
+@pytest.mark.parametrize("limit", [0, 700])
+def test_unbind_full_passes_on_chunked_kernel(gpu, limit):
+    """Haloes above the symmetric-kernel limit (all of them at 0, the larger ones at
+    700 members) run their full passes on the chunked kernel; masks and pass counts are
+    the oracle's either way."""
+    from pyhalma_amd import _lib, engine, synth
+
+    w = synth.clumps(seed=13, scale=0.01)
+    g = _groups(w)
+    sizes = np.diff(g.offsets_host)
+    assert limit == 0 or (sizes.max() > limit and sizes.min() <= limit)
+    old = engine.tune_get(_lib.TUNE_SYM_MAX)
+    engine.tune_set(_lib.TUNE_SYM_MAX, limit)
+    try:
+        r = engine.unbind(g, w["x"], w["y"], w["z"], w["vx"], w["vy"], w["vz"], w["mass"],
+                          1.0, time_pot=True)
+    finally:
+        engine.tune_set(_lib.TUNE_SYM_MAX, old)
+    bound, iters, _ = _oracle(g, w, 1.0)
+    got = r.bound.cpu().numpy().astype(bool)
+    print(f"limit={limit} haloes={g.n_groups} sizes {sizes.min()}..{sizes.max()} "
+          f"symmetric pairs={r.stats['full_pairs']} all pairs={r.stats['pairs']}", flush=True)
+    assert np.array_equal(got, bound), int((got != bound).sum())
+    assert np.array_equal(r.iters.cpu().numpy(), iters)
+    if limit == 0:
+        assert r.stats["full_pairs"] == 0
+    else:
+        assert 0 < r.stats["full_pairs"] < r.stats["pairs"]
+
+
 def test_unbind_plummer_background(gpu):
     from pyhalma_amd import engine, synth
 
