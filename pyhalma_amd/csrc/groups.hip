@@ -227,13 +227,82 @@ struct Src {
     int mass_t;
 };
 
+// All of a group's block reductions in one LDS round: every wave reduces its eight
+// double-double sums (shuffle tree), three minima, three maxima and the grain exponent,
+// writes them to LDS, and thread q then folds quantity q over the waves in wave order (the
+// order of block_dd_sum / block_max), so the results are those of the separate
+// reductions with two barriers instead of forty-five.
+constexpr int kGrpQ = 8 * 2 + 6 + 1;
+
+__device__ void block_group_reduce(double (&acc)[8][2], double (&lo)[3], double (&hi)[3],
+                                   int& gexp, double* sh /* kGrpQ * kMaxWaves */) {
+    for (int k = 0; k < 8; ++k)
+        for (int s = 32; s >= 1; s >>= 1) {
+            const double h2 = __shfl_down(acc[k][0], s, kWave);
+            const double l2 = __shfl_down(acc[k][1], s, kWave);
+            dd_add_dd(acc[k][0], acc[k][1], h2, l2);
+        }
+    for (int d = 0; d < 3; ++d)
+        for (int s = 32; s >= 1; s >>= 1) {
+            lo[d] = fmin(lo[d], __shfl_down(lo[d], s, kWave));
+            hi[d] = fmax(hi[d], __shfl_down(hi[d], s, kWave));
+        }
+    for (int s = 32; s >= 1; s >>= 1) gexp = min(gexp, __shfl_down(gexp, s, kWave));
+    const int w = threadIdx.x / kWave;
+    const int nw = static_cast<int>(blockDim.x) / kWave;
+    __syncthreads();
+    if (__lane_id() == 0) {
+        double* o = sh + w * kGrpQ;
+        for (int k = 0; k < 8; ++k) {
+            o[2 * k] = acc[k][0];
+            o[2 * k + 1] = acc[k][1];
+        }
+        for (int d = 0; d < 3; ++d) {
+            o[16 + d] = lo[d];
+            o[19 + d] = hi[d];
+        }
+        o[22] = static_cast<double>(gexp);
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (q < 8) {
+        double h = sh[2 * q], l = sh[2 * q + 1];
+        for (int k = 1; k < nw; ++k) dd_add_dd(h, l, sh[k * kGrpQ + 2 * q], sh[k * kGrpQ + 2 * q + 1]);
+        sh[nw * kGrpQ + 2 * q] = h;
+        sh[nw * kGrpQ + 2 * q + 1] = l;
+    } else if (q < 11) {
+        double v = sh[16 + (q - 8)];
+        for (int k = 1; k < nw; ++k) v = fmin(v, sh[k * kGrpQ + 16 + (q - 8)]);
+        sh[nw * kGrpQ + 16 + (q - 8)] = v;
+    } else if (q < 14) {
+        double v = sh[19 + (q - 11)];
+        for (int k = 1; k < nw; ++k) v = fmax(v, sh[k * kGrpQ + 19 + (q - 11)]);
+        sh[nw * kGrpQ + 19 + (q - 11)] = v;
+    } else if (q == 14) {
+        double v = sh[22];
+        for (int k = 1; k < nw; ++k) v = fmin(v, sh[k * kGrpQ + 22]);
+        sh[nw * kGrpQ + 22] = v;
+    }
+    __syncthreads();
+    const double* r = sh + nw * kGrpQ;
+    for (int k = 0; k < 8; ++k) {
+        acc[k][0] = r[2 * k];
+        acc[k][1] = r[2 * k + 1];
+    }
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = r[16 + d];
+        hi[d] = r[19 + d];
+    }
+    gexp = static_cast<int>(r[22]);
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(kBig) void k_group_reduce(Src s, const int64_t* __restrict__ off,
                                                            const int32_t* __restrict__ mem,
                                                            int64_t G,
                                                            const double* __restrict__ centres,
                                                            double* __restrict__ out) {
-    __shared__ double sh[2 * kMaxWaves];
-    __shared__ int shi[kMaxWaves];
+    __shared__ double sh[kGrpQ * (kMaxWaves + 1)];
     __shared__ double cen[3];
     for (int64_t g = blockIdx.x; g < G; g += gridDim.x) {
         const int64_t b = off[g], e = off[g + 1];
@@ -260,13 +329,9 @@ __global__ __launch_bounds__(kBig) void k_group_reduce(Src s, const int64_t* __r
             dd_add(acc[7][0], acc[7][1], fabs(m));
             gexp = min(gexp, mass_grain_exp(m));
         }
-        for (int k = 0; k < 8; ++k) block_dd_sum(acc[k][0], acc[k][1], sh);
-        double bl[3], bh[3];
-        for (int d = 0; d < 3; ++d) {
-            bl[d] = block_min(lo[d], sh);
-            bh[d] = block_max(hi[d], sh);
-        }
-        gexp = block_min_i(gexp, shi);
+        block_group_reduce(acc, lo, hi, gexp, sh);
+        const double* bl = lo;
+        const double* bh = hi;
         if (threadIdx.x == 0) {
             double* o = out + g * kRedFields;
             const double M = acc[0][0] + acc[0][1];
