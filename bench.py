@@ -129,10 +129,13 @@ def main():
     if not args.no_autotune:
         from pyhalma_amd import autotune
         _, groups, _, _, _ = step()
-        md = engine.gather_members(groups, D["x"], D["y"], D["z"], D["mass"])
+        md = engine.gather_members(groups, D["x"], D["y"], D["z"], D["mass"], D["vx"], D["vy"],
+                                   D["vz"])
         t_at = time.time()
-        setting, ms, trials = autotune.autotune(md.groups.offsets_host, md.m, md.x, md.y, md.z)
-        tuning = {"setting": setting, "first_pass_ms": ms, "trials": len(trials),
+        # the chunked kernel's knobs, timed on the step's own unbinding
+        setting, ms, trials = autotune.autotune_unbind(md.groups, md.x, md.y, md.z, md.vx,
+                                                       md.vy, md.vz, md.m, factor=1.0, reps=5)
+        tuning = {"setting": setting, "unbind_ms": ms, "trials": len(trials),
                   "default_ms": trials[0][1] if trials else ms}
         log(f"autotune: {setting} {ms:.3f} ms (default {tuning['default_ms']:.3f} ms, "
             f"{len(trials)} trials, {time.time() - t_at:.1f}s)")
