@@ -54,10 +54,11 @@ def parse():
     ap.add_argument("--cpu-frac", type=float, default=0.5,
                     help="fraction of the box (along x) used for the CPU baseline sample")
     ap.add_argument("--workload", default="clumps",
-                    choices=["clumps", "gas", "bridged", "centring"],
+                    choices=["clumps", "gas", "bridged", "centring", "substructure"],
                     help="clumps (default): the headline 2.1M-particle step; the others are "
-                         "the gas batch, the bridged-pair FoF stress case and the "
-                         "potential-minimum centring at their full sizes")
+                         "the gas batch, the bridged-pair FoF stress case, the "
+                         "potential-minimum centring and the substructure splitting of the "
+                         "bridged pairs at their full sizes")
     return ap.parse_args()
 
 
@@ -79,8 +80,8 @@ def main():
 
     if args.workload != "clumps":
         torch.cuda.init()
-        out = {"gas": bench_gas, "bridged": bench_bridged,
-               "centring": bench_centring}[args.workload](args)
+        out = {"gas": bench_gas, "bridged": bench_bridged, "centring": bench_centring,
+               "substructure": bench_substructure}[args.workload](args)
         print(json.dumps(out))
         return
 
@@ -363,6 +364,62 @@ def bench_bridged(args):
                  groups_single_pair=pure, pairs_merged=pairs_hit,
                  members=groups.n_members, pairs_evaluated=ub.stats["pairs"],
                  bound_fraction=float(ub.bound.float().mean().item()))
+
+
+def bench_substructure(args):
+    """Substructure splitting (pyHALMA.py:703-766) of the bridged-pair workload: FoF
+    labels + CSR groups at ll, then two sub-linking levels (ll/2, ll/4) over all groups
+    at once, minp_sub 25, distance factor 0.75, minp 50.  CPU baseline: the oracle's
+    restatement of the same block (C cell-list FoF per group and level) on the first
+    groups of the same catalogue."""
+    import numpy as np
+    import torch
+
+    from pyhalma_amd import engine, synth
+
+    t0 = time.time()
+    w = synth.bridged_pairs()
+    n = len(w["x"])
+    ll = float(w["ll"])
+    min_ll = ll / 4.0 * 0.99
+    log(f"substructure workload: {n} particles, ll={ll:.5f} ({time.time() - t0:.1f}s)")
+    dev = torch.device("cuda", 0)
+    D = {k: torch.from_numpy(w[k]).to(dev) for k in ("x", "y", "z", "mass")}
+
+    def step():
+        lab = engine.fof_labels(D["x"], D["y"], D["z"], ll)
+        groups = engine.groups_from_labels(lab, MINP)
+        sub = engine.split_substructure(groups, D["x"], D["y"], D["z"], D["mass"], ll,
+                                        minp_sub=25, dec_factor=2.0, min_ll=min_ll,
+                                        dist_factor=0.75, minp=MINP)
+        return groups, sub
+
+    step_s, (groups, sub) = _timed(step, args.steps, args.warmup)
+    log(f"substructure: {step_s * 1e3:.1f} ms/step; groups in={groups.n_groups} "
+        f"out={sub.groups.n_groups} found={sub.n_found}")
+    extra = {}
+    if not args.no_cpu:
+        import oracle
+
+        k = 40
+        gl = groups.to_arrays()[:k]
+        ns = int(sum(len(g) for g in gl))
+        tc = time.perf_counter()
+        _, cnt = oracle.split_substructure(gl, w["x"], w["y"], w["z"], w["mass"], ll, 25, 2.0,
+                                           min_ll, 0.75, MINP)
+        cs = time.perf_counter() - tc
+        extra["cpu_baseline"] = {"value": ns / cs, "unit": "members/s", "cores": 1,
+                                 "kind": "port", "seconds": cs,
+                                 "sample": f"the first {k} groups ({ns} members, {cnt} "
+                                           "substructures): the oracle's restatement, C "
+                                           "cell-list FoF per group and level"}
+    return _line("particles/s FoF + substructure splitting, 8M-particle bridged pairs",
+                 n / step_s, "particles/s", step_s, args,
+                 {"workload": "4000 Plummer pairs at 1-3 a joined by 5% bridges, 8M FP32, "
+                              "FoF at ll = 0.35 a, sub-FoF at ll/2 and ll/4",
+                  "particles": n, "groups_in": groups.n_groups,
+                  "groups_out": sub.groups.n_groups}, substructures=sub.n_found,
+                 members_in=groups.n_members, **extra)
 
 
 def bench_centring(args):

@@ -178,6 +178,21 @@ int halma_density_peak(int64_t n_groups, const int64_t *offsets, const int32_t *
                        const void *m, int mass_dtype, double ll, const double *reduced,
                        double *out, void *workspace, size_t workspace_bytes, void *stream);
 
+/* half_mass_radius (halo_properties.py:366-392) for every group: the distance to the
+ * centre of the last member, members by increasing distance, before the running mass
+ * first exceeds M / 2 (the smallest distance if the first member already exceeds it,
+ * the largest if none does; 0 for an empty group).  Groups are the CSR ranges
+ * offsets[g]..offsets[g+1] (device, offsets[0] = 0, n_members = offsets[G]) of
+ * `members`; `reduced` is halma_group_reduce's output for the same groups (M and the
+ * centre).  Equal distances keep member order (the reference's np.argsort leaves them
+ * unordered).  Used by the substructure splitting of pyHALMA.py:703-766. */
+size_t halma_half_mass_radius_workspace_size(int64_t n_members);
+int halma_half_mass_radius(int64_t n_groups, int64_t n_members, const int64_t *offsets,
+                           const int32_t *members, const void *x, const void *y,
+                           const void *z, int pos_dtype, const void *m, int mass_dtype,
+                           const double *reduced, double *out, void *workspace,
+                           size_t workspace_bytes, void *stream);
+
 /* ------------------------------------------------------------------------------------
  * Batched iterative unbinding.  The energy test of halo_properties.py:342-359 (stars,
  * factor = FACTOR_V^2) and halo_gas.py:456-476 (gas, factor = 2):

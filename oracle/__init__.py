@@ -35,6 +35,11 @@ from .reductions import (  # noqa: F401
     density_peak,
     linspace_ref,
 )
+from .substructure import (  # noqa: F401
+    half_mass_radius,
+    split_substructure,
+    sub_fof,
+)
 from .unbind import (  # noqa: F401
     G_CONST,
     binding_energy,

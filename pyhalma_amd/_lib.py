@@ -62,6 +62,9 @@ SIGNATURES = {
     "halma_density_peak_workspace_size": (_size, [_i64]),
     "halma_density_peak": (_c.c_int, [_i64, _p, _p, _p, _p, _p, _c.c_int, _p, _c.c_int, _dbl,
                                       _p, _p, _p, _size, _p]),
+    "halma_half_mass_radius_workspace_size": (_size, [_i64]),
+    "halma_half_mass_radius": (_c.c_int, [_i64, _i64, _p, _p, _p, _p, _p, _c.c_int, _p,
+                                          _c.c_int, _p, _p, _p, _size, _p]),
     "halma_unbind_workspace_size": (_size, [_i64, _i64p]),
     "halma_unbind_batched": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _c.c_int, _p, _p, _p,
                                         _c.c_int, _p, _c.c_int, _dbl, _c.c_int, _c.c_int, _p,

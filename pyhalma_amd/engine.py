@@ -372,6 +372,143 @@ def density_peak(groups, x, y, z, m, ll, reduced=None):
     return out
 
 
+def half_mass_radius(groups, x, y, z, m, reduced=None):
+    """half_mass_radius (halo_properties.py:366-392) of every group: float64 (G,)."""
+    from .device import dtype_flag
+    L = _lib_()
+    x, y, z = _coords(x, y, z)
+    m = _vec(m)
+    G = groups.n_groups
+    out = torch.empty(max(G, 1), dtype=torch.float64, device=x.device)
+    if G == 0:
+        return out[:0]
+    if reduced is None:
+        reduced = group_reduce(groups, x, y, z, m)
+    nm = groups.n_members
+    ws = workspace(L.halma_half_mass_radius_workspace_size(nm))
+    rc = L.halma_half_mass_radius(G, nm, ptr(groups.offsets), ptr(groups.members), ptr(x),
+                                  ptr(y), ptr(z), dtype_flag(x), ptr(m), dtype_flag(m),
+                                  ptr(reduced), ptr(out), ptr(ws), ws.numel(), stream_ptr())
+    _lib.check(rc, "halma_half_mass_radius")
+    return out[:G]
+
+
+class Substructure:
+    """Result of split_substructure: the groups after splitting (CSR ``groups``: the
+    original groups in order, emptied ones dropped, then the new groups, all filtered by
+    ``minp``), the number of substructures found, and for each output group the index of
+    the original group it came from and whether it is a new (split-off) group."""
+
+    def __init__(self, groups, n_found, parent, is_new):
+        self.groups = groups
+        self.n_found = n_found
+        self.parent = parent    # numpy int64 per output group
+        self.is_new = is_new    # numpy bool per output group
+
+    def to_lists(self):
+        return self.groups.to_lists()
+
+
+def split_substructure(groups, x, y, z, m, ll, minp_sub=25, dec_factor=2.0, min_ll=0.5e-3,
+                       dist_factor=0.75, minp=50):
+    """The substructure (bridge) splitting of pyHALMA.py:703-766 for every group at once.
+
+    For sub-linking lengths ll / dec_factor^k above ``min_ll``: the FoF groups (at least
+    ``minp_sub`` members) of each group's remaining members that hold between 1 % and
+    60 % of them and whose centre of mass lies further than ``dist_factor`` times the
+    group's half-mass radius from the group's centre become new groups and leave the
+    group.  A group left with under 30 % of its members is emptied.  Output order: the
+    original groups, then the new ones by original group and discovery order; groups
+    under ``minp`` members are dropped (pyHALMA.py:758-764).  One batched FoF per level
+    runs over the members of all groups together: members of different groups are more
+    than ``ll`` apart, so no sub-group crosses two groups.
+    """
+    x, y, z = _coords(x, y, z)
+    m = _vec(m)
+    dev = x.device
+    H = groups.n_groups
+    cur = groups.members[: groups.n_members].to(torch.int64)   # particle indices
+    cur_off = np.asarray(groups.offsets_host, np.int64).copy()
+    start = np.diff(cur_off)
+    new_mem = []   # per level: particle indices of the new groups, in sub-group order
+    new_tab = []   # rows (halo, level, order, level start, length, tensor id)
+    sub_ll = ll / dec_factor
+    level = 0
+    while sub_ll > min_ll and cur.numel() > 0:
+        sizes = np.diff(cur_off)
+        sizes_d = torch.from_numpy(sizes).to(dev)
+        halo_of_pos = torch.repeat_interleave(torch.arange(H, device=dev), sizes_d)
+        cg = Groups(cur.to(torch.int32), torch.from_numpy(cur_off).to(dev), cur_off)
+        red = group_reduce(cg, x, y, z, m)
+        r12 = half_mass_radius(cg, x, y, z, m, red)
+        md = gather_members(cg, x, y, z, m)
+        lab = fof_labels(md.x, md.y, md.z, sub_ll)
+        sub = groups_from_labels(lab, minp_sub)
+        if sub.n_groups > 0:
+            soh = np.asarray(sub.offsets_host, np.int64)
+            soff = torch.from_numpy(soh).to(dev)
+            smem = sub.members[: sub.n_members].to(torch.int64)     # concat positions
+            h = halo_of_pos[smem[soff[:-1]]]
+            ssize = torch.from_numpy(np.diff(soh)).to(dev).to(torch.float64)
+            frac = ssize / sizes_d.to(torch.float64)[h]
+            rs = group_reduce(Groups(sub.members, soff, soh), md.x, md.y, md.z, md.m)
+            dist = torch.sqrt((red[h, 1] - rs[:, 1]) ** 2 + (red[h, 2] - rs[:, 2]) ** 2
+                              + (red[h, 3] - rs[:, 3]) ** 2)
+            sel = (frac < 0.6) & (frac > 0.01) & (dist > dist_factor * r12[h])
+            sel_h = sel.cpu().numpy()
+            if sel_h.any():
+                sub_of_pos = torch.repeat_interleave(
+                    torch.arange(sub.n_groups, device=dev), torch.from_numpy(np.diff(soh)).to(dev))
+                taken = smem[sel[sub_of_pos]]                       # concat positions, in order
+                keep = torch.ones(cur.numel(), dtype=torch.bool, device=dev)
+                keep[taken] = False
+                idx = np.nonzero(sel_h)[0]
+                lens = np.diff(soh)[idx]
+                starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+                hh = h.cpu().numpy()[idx]
+                for q in range(idx.size):
+                    new_tab.append((int(hh[q]), level, q, int(starts[q]), int(lens[q]),
+                                    len(new_mem)))
+                new_mem.append(cur[taken])
+                kept_per = torch.zeros(H, dtype=torch.int64, device=dev)
+                kept_per.index_add_(0, halo_of_pos, keep.to(torch.int64))
+                cur = cur[keep]
+                cur_off = np.concatenate([[0], np.cumsum(kept_per.cpu().numpy())]).astype(
+                    np.int64)
+        sub_ll = sub_ll / dec_factor
+        level += 1
+    # assemble: original groups (emptied below 30 %), then the new ones
+    sizes = np.diff(cur_off)
+    # pyHALMA.py:752: len(part_list) / starting_npart < 0.3 empties the group
+    ok = (start > 0) & (sizes >= minp)
+    ok &= np.where(start > 0, sizes / np.maximum(start, 1) >= 0.3, False)
+    sizes_d = torch.from_numpy(sizes).to(dev)
+    halo_of_pos = torch.repeat_interleave(torch.arange(H, device=dev), sizes_d)
+    parts = [cur[torch.from_numpy(ok).to(dev)[halo_of_pos]]] if cur.numel() else []
+    lens = list(sizes[ok])
+    parent = list(np.nonzero(ok)[0])
+    is_new = [False] * len(lens)
+    new_tab.sort(key=lambda t: (t[0], t[1], t[2]))
+    rows = [t for t in new_tab if t[4] >= minp]
+    if rows:
+        base = np.concatenate([[0], np.cumsum([t.numel() for t in new_mem])]).astype(np.int64)
+        allnew = torch.cat(new_mem)
+        st = np.array([base[t[5]] + t[3] for t in rows], np.int64)
+        ln = np.array([t[4] for t in rows], np.int64)
+        gather = np.repeat(st - np.concatenate([[0], np.cumsum(ln)[:-1]]), ln) + np.arange(ln.sum())
+        parts.append(allnew[torch.from_numpy(gather).to(dev)])
+        lens += list(ln)
+        parent += [t[0] for t in rows]
+        is_new += [True] * len(rows)
+    off = np.concatenate([[0], np.cumsum(np.asarray(lens, np.int64))]).astype(np.int64)
+    members = torch.cat(parts) if parts else torch.zeros(0, dtype=torch.int64, device=dev)
+    out = Groups(members.to(torch.int32) if members.numel() else
+                 torch.zeros(1, dtype=torch.int32, device=dev),
+                 torch.from_numpy(off).to(dev), off)
+    return Substructure(out, len(new_tab), np.asarray(parent, np.int64),
+                        np.asarray(is_new, bool))
+
+
 class UnbindResult:
     def __init__(self, bound, iters, vcm, pot, stats):
         self.bound = bound      # uint8 device tensor, member order

@@ -90,6 +90,19 @@ def furthest_particle(cx, cy, cz, part_list, st_x, st_y, st_z):
     return float(r[7])
 
 
+def half_mass_radius(cx, cy, cz, M, part_list, st_x, st_y, st_z, st_mass):
+    """halo_properties.py:366-392 (the np.min of an empty list raises, as there)."""
+    p = np.asarray(part_list)
+    if p.size == 0:
+        raise ValueError("zero-size array to reduction operation minimum which has no "
+                         "identity")
+    groups, sub = _one_group(part_list, [st_x, st_y, st_z, st_mass])
+    red = torch.zeros((1, 16), dtype=torch.float64, device=groups.members.device)
+    red[0, 0] = float(M)
+    red[0, 1], red[0, 2], red[0, 3] = float(cx), float(cy), float(cz)
+    return float(engine.half_mass_radius(groups, *sub, reduced=red)[0].item())
+
+
 def density_peak(part_list, st_x, st_y, st_z, st_mass, ll):
     p = np.asarray(part_list)
     if p.size == 0:
