@@ -193,6 +193,40 @@ int halma_half_mass_radius(int64_t n_groups, int64_t n_members, const int64_t *o
                            const double *reduced, double *out, void *workspace,
                            size_t workspace_bytes, void *stream);
 
+/* Phase-space cleaning (halo_properties.py:113-278: calc_cell / clean_cell, driver
+ * pyHALMA.py:789-843), one round for many groups.  The set S of the round is given as
+ * n_set members (particle indices `set_members`, group `set_groups`, groups' members
+ * contiguous and in ascending position).  halma_clean_cells puts every member of S in the
+ * nearest node of its group's grid per axis (node k of group g: grid_start[g] for k = 0,
+ * grid_g1[g] for k = 1, grid_start[g] + k * grid_d[g] after, grid_n[g] nodes: the values of
+ * np.arange(-(R + ll), R + ll, 2 ll)) about centre[3 g..] (first node among equal
+ * distances; max_grid_n (host) >= every grid_n of the set, at most 2^21), writes the three
+ * node indices to ijk (3 per member of S, device, S order), and the per-cell
+ * statistics to cell_stats (6 doubles per cell, cells in (group, cell) order: mass-
+ * weighted velocity (3), mass, member count, dispersion sqrt(sum |v_cell - v|^2 / 3 / q));
+ * *n_cells (HOST) their number.  Sums are double-double (the reference's member-order FP64
+ * sums agree to a few ulp); v * m is formed in FP32 when both arrays are FP32, as numba
+ * does.  Synchronises the stream.  halma_clean_decide, called next with the same set and
+ * workspace, tests every member of S whose group has phase 0 (first round: |v_cell - v| /
+ * sigma counts only in cells of more than two members) or 1 (later rounds: two or more):
+ * a member of a cell with at most q_fil members, or with the scaled deviation >= fac[g] *
+ * sig_fil, is removed: control[set_target[k]] = 0 and cleaned[g] += 1. */
+size_t halma_clean_workspace_size(int64_t n_set);
+int halma_clean_cells(int64_t n_set, int64_t n_groups, const int32_t *set_members,
+                      const int32_t *set_groups, const void *x, const void *y, const void *z,
+                      int pos_dtype, const void *vx, const void *vy, const void *vz,
+                      int vel_dtype, const void *m, int mass_dtype, const double *centre,
+                      const double *grid_start, const double *grid_g1, const double *grid_d,
+                      const int32_t *grid_n, int32_t max_grid_n, int32_t *ijk,
+                      double *cell_stats, int64_t *n_cells, void *workspace,
+                      size_t workspace_bytes, void *stream);
+int halma_clean_decide(int64_t n_set, const int32_t *set_members, const int32_t *set_groups,
+                       const int64_t *set_target, const void *vx, const void *vy,
+                       const void *vz, int vel_dtype, const double *cell_stats,
+                       const int32_t *phase, const double *fac, double q_fil, double sig_fil,
+                       uint8_t *control, int32_t *cleaned, void *workspace,
+                       size_t workspace_bytes, void *stream);
+
 /* ------------------------------------------------------------------------------------
  * Batched iterative unbinding.  The energy test of halo_properties.py:342-359 (stars,
  * factor = FACTOR_V^2) and halo_gas.py:456-476 (gas, factor = 2):

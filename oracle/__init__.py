@@ -35,6 +35,7 @@ from .reductions import (  # noqa: F401
     density_peak,
     linspace_ref,
 )
+from .cleaning import phase_space_clean  # noqa: F401
 from .substructure import (  # noqa: F401
     half_mass_radius,
     split_substructure,

@@ -65,6 +65,12 @@ SIGNATURES = {
     "halma_half_mass_radius_workspace_size": (_size, [_i64]),
     "halma_half_mass_radius": (_c.c_int, [_i64, _i64, _p, _p, _p, _p, _p, _c.c_int, _p,
                                           _c.c_int, _p, _p, _p, _size, _p]),
+    "halma_clean_workspace_size": (_size, [_i64]),
+    "halma_clean_cells": (_c.c_int, [_i64, _i64, _p, _p, _p, _p, _p, _c.c_int, _p, _p, _p,
+                                     _c.c_int, _p, _c.c_int, _p, _p, _p, _p, _p, _c.c_int,
+                                     _p, _p, _i64p, _p, _size, _p]),
+    "halma_clean_decide": (_c.c_int, [_i64, _p, _p, _p, _p, _p, _p, _c.c_int, _p, _p, _p,
+                                      _dbl, _dbl, _p, _p, _p, _size, _p]),
     "halma_unbind_workspace_size": (_size, [_i64, _i64p]),
     "halma_unbind_batched": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _c.c_int, _p, _p, _p,
                                         _c.c_int, _p, _c.c_int, _dbl, _c.c_int, _c.c_int, _p,

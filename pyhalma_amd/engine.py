@@ -509,6 +509,195 @@ def split_substructure(groups, x, y, z, m, ll, minp_sub=25, dec_factor=2.0, min_
                         np.asarray(is_new, bool))
 
 
+class CleanResult:
+    """Result of phase_space_clean for G input groups (member order of the input Groups).
+
+    control_clean: bool per member, clean_cell's control array; control: the driver's
+    (all members kept where ``psc`` is off or the group has fewer than ``min_part``
+    members).  clean: numpy (G, 5) float64, clean_cell's (cx, cy, cz, M, R).  n_cell: the
+    first grid's node count per axis.  rounds: test rounds per group.  which: device
+    int32 (n_members, 3), the node indices of the last calc_cell, group g's rows at
+    offsets[g] .. offsets[g] + which_len[g].  kept: the groups with at least ``minp``
+    members left (CSR over the kept members), kept_index their input indices, and their
+    recomputed centre (K, 3), mass, rmax, npart and n_cell (pyHALMA.py:823-843)."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+    def which_of(self, g):
+        a = int(self.offsets_host[g])
+        return self.which[a: a + int(self.which_len[g])].cpu().numpy()
+
+
+def _grid(R, ll):
+    """np.arange(-(R + ll), R + ll, 2 ll) per group as (start, second node, delta, length):
+    node k is start (k = 0), the second node (k = 1), start + k * delta (k >= 2)."""
+    stop = R + ll
+    start = -stop
+    step = 2.0 * ll
+    n = np.ceil((stop - start) / step).astype(np.int64)
+    g1 = start + step
+    return start, g1, g1 - start, n
+
+
+def _centres(red, counts):
+    """(centre (G, 3), M, R) of group_reduce rows, zeros for groups without mass (the
+    reference's center_of_mass returns zeros when M <= 0, furthest_particle 0 when empty)."""
+    M = red[:, 0].copy()
+    c = red[:, 1:4].copy()
+    R = red[:, 7].copy()
+    nomass = ~(M > 0)
+    c[nomass] = 0.0
+    M[nomass] = 0.0
+    R[counts == 0] = 0.0
+    return c, M, R
+
+
+def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=True,
+                      min_part=50, minp=50):
+    """The phase-space cleaning of pyHALMA.py:789-843 (calc_cell / clean_cell,
+    halo_properties.py:113-278) for every group at once.
+
+    Each round works on the set S of members still flagged of every group not finished:
+    one group_reduce gives the centres and radii of the grids (np.arange(-(R + ll),
+    R + ll, 2 ll) about the centre of mass), halma_clean_cells the cell statistics and
+    halma_clean_decide the removals.  The per-group bookkeeping (remaining, the
+    fraction removed, fac_sig: 2, 1.5, then x 0.75 with a floor of 1) is the reference's,
+    quirks included: in rounds after the first the decision about the ip-th member of S
+    clears entry ip of the group's control array.  A group that removes all of its
+    members in the first round ends with zeros, as the reference's final step does with
+    no members.  A cell with more than two members whose members share one velocity
+    has sigma 0: its members then score inf (removed) or nan (kept) where the
+    reference's compiled division raises.
+    """
+    from .device import dtype_flag
+    L = _lib_()
+    if not ll > 0:
+        raise ValueError("ll must be positive")
+    x, y, z = _coords(x, y, z)
+    vx, vy, vz = _coords(vx, vy, vz)
+    m = _vec(m)
+    dev = x.device
+    H = groups.n_groups
+    off = np.asarray(groups.offsets_host, np.int64)
+    sizes = np.diff(off)
+    nm = groups.n_members
+    mem = groups.members[:nm]
+    if H == 0:
+        none = torch.zeros(0, dtype=torch.bool, device=dev)
+        return CleanResult(control=none, control_clean=none, clean=np.zeros((0, 5)),
+                           n_cell=np.zeros(0, np.int64), rounds=np.zeros(0, np.int64),
+                           which=torch.zeros((0, 3), dtype=torch.int32, device=dev),
+                           which_len=np.zeros(0, np.int64), offsets_host=off, kept=groups,
+                           kept_index=np.zeros(0, np.int64), centre=np.zeros((0, 3)),
+                           mass=np.zeros(0), rmax=np.zeros(0), npart=np.zeros(0, np.int64),
+                           kept_n_cell=np.zeros(0, np.int64))
+    off_d = torch.from_numpy(off).to(dev)
+    grp_of = torch.repeat_interleave(torch.arange(H, device=dev),
+                                     torch.from_numpy(sizes).to(dev))  # int64
+    control = torch.ones(nm, dtype=torch.uint8, device=dev)
+    phase = np.zeros(H, np.int32)  # 0 first round, 1 later round, 2 final grid, 3 done
+    fac = np.full(H, 2.0)
+    remaining = sizes.copy()
+    rounds = np.zeros(H, np.int64)
+    n_cell = np.zeros(H, np.int64)
+    clean = np.zeros((H, 5))
+    which = torch.zeros((max(nm, 1), 3), dtype=torch.int32, device=dev)
+    which_len = np.zeros(H, np.int64)
+    ws = workspace(L.halma_clean_workspace_size(nm))
+    ijk = torch.empty((max(nm, 1), 3), dtype=torch.int32, device=dev)
+    stats = torch.empty(6 * max(nm, 1), dtype=torch.float64, device=dev)
+    cleaned = torch.zeros(max(H, 1), dtype=torch.int32, device=dev)
+    ncells = ctypes_i64()
+    pf, vf, mf = dtype_flag(x), dtype_flag(vx), dtype_flag(m)
+    while (phase < 3).any():
+        live = torch.from_numpy(phase < 3).to(dev)
+        sidx = torch.nonzero(control.bool() & live[grp_of]).squeeze(1)
+        ns = int(sidx.numel())
+        s_mem = mem[sidx]
+        sg = grp_of[sidx]
+        s_grp = sg.to(torch.int32)
+        counts = torch.bincount(sg, minlength=H)
+        counts_h = counts.cpu().numpy().astype(np.int64)
+        s_off = np.concatenate([[0], np.cumsum(counts_h)]).astype(np.int64)
+        s_off_d = torch.from_numpy(s_off).to(dev)
+        target = off_d[sg] + (torch.arange(ns, device=dev) - s_off_d[sg])
+        red = group_reduce(Groups(s_mem if ns else torch.zeros(1, dtype=torch.int32, device=dev),
+                                  s_off_d, s_off), x, y, z, m).cpu().numpy()
+        c, M, R = _centres(red, counts_h)
+        start, g1, d, n = _grid(R, ll)
+        first = phase == 0
+        n_cell[first] = n[first]
+        if ns:
+            used = counts_h > 0
+            max_n = int(n[used].max())
+            dv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+            cen_d, st_d, g1_d, d_d = dv(c), dv(start), dv(g1), dv(d)
+            n_d = dv(np.clip(n, 1, None).astype(np.int32))
+            rc = L.halma_clean_cells(ns, H, ptr(s_mem), ptr(s_grp), ptr(x), ptr(y), ptr(z), pf,
+                                     ptr(vx), ptr(vy), ptr(vz), vf, ptr(m), mf, ptr(cen_d),
+                                     ptr(st_d), ptr(g1_d), ptr(d_d), ptr(n_d), max_n, ptr(ijk),
+                                     ptr(stats), ncells, ptr(ws), ws.numel(), stream_ptr())
+            _lib.check(rc, "halma_clean_cells")
+            cleaned.zero_()
+            phase_d, fac_d = dv(phase), dv(fac)  # held until the kernel has read them
+            rc = L.halma_clean_decide(ns, ptr(s_mem), ptr(s_grp), ptr(target), ptr(vx), ptr(vy),
+                                      ptr(vz), vf, ptr(stats), ptr(phase_d), ptr(fac_d),
+                                      float(q_fil), float(sig_fil), ptr(control), ptr(cleaned),
+                                      ptr(ws), ws.numel(), stream_ptr())
+            _lib.check(rc, "halma_clean_decide")
+            cl = cleaned[:H].cpu().numpy().astype(np.int64)
+        else:
+            cl = np.zeros(H, np.int64)
+        tested = (phase == 0) | (phase == 1)
+        rounds[tested] += 1
+        # groups ending this round keep its grid's node indices
+        ends = phase == 2
+        p0 = phase == 0
+        p1 = phase == 1
+        remaining[p0] = sizes[p0] - cl[p0]
+        remaining[p1] -= cl[p1]
+        zero = p1 & (remaining == 0)
+        ends |= zero
+        if ns and ends.any():
+            pick = torch.from_numpy(ends).to(dev)[sg]
+            which[target[pick]] = ijk[:ns][pick]
+        which_len[ends] = counts_h[ends]
+        fin = phase == 2
+        clean[fin] = np.column_stack([c, M, R])[fin]
+        phase[ends] = 3
+        with np.errstate(divide="ignore", invalid="ignore"):
+            frac = np.where(remaining != 0, cl / np.where(remaining != 0, remaining, 1), 0.0)
+        go0 = p0 & (remaining > 0) & (frac > 0.1)
+        phase[p0] = np.where(go0[p0], 1, 2)
+        fac[p0] = 2.0 * 0.75
+        live1 = p1 & ~zero
+        phase[live1] = np.where(frac[live1] > 0.1, 1, 2)
+        fac[live1] = np.maximum(1.0, fac[live1] * 0.75)
+    control_clean = control.bool()
+    keep_all = np.ones(H, bool) if not psc else sizes < min_part
+    final = control_clean | torch.from_numpy(keep_all).to(dev)[grp_of]
+    kidx = torch.nonzero(final).squeeze(1)
+    kcount = torch.bincount(grp_of[kidx], minlength=H).cpu().numpy().astype(np.int64)
+    koff = np.concatenate([[0], np.cumsum(kcount)]).astype(np.int64)
+    kmem = mem[kidx]
+    kg = Groups(kmem if kmem.numel() else torch.zeros(1, dtype=torch.int32, device=dev),
+                torch.from_numpy(koff).to(dev), koff)
+    kred = group_reduce(kg, x, y, z, m).cpu().numpy() if H else np.zeros((0, 16))
+    kc, kM, kR = _centres(kred, kcount)
+    ok = kcount >= minp
+    okd = torch.from_numpy(ok).to(dev)
+    sel = okd[grp_of[kidx]]
+    out_mem = kmem[sel]
+    out_off = np.concatenate([[0], np.cumsum(kcount[ok])]).astype(np.int64)
+    kept = Groups(out_mem if out_mem.numel() else torch.zeros(1, dtype=torch.int32, device=dev),
+                  torch.from_numpy(out_off).to(dev), out_off)
+    return CleanResult(control=final, control_clean=control_clean, clean=clean, n_cell=n_cell,
+                       rounds=rounds, which=which, which_len=which_len, offsets_host=off,
+                       kept=kept, kept_index=np.nonzero(ok)[0], centre=kc[ok], mass=kM[ok],
+                       rmax=kR[ok], npart=kcount[ok], kept_n_cell=n_cell[ok])
+
+
 class UnbindResult:
     def __init__(self, bound, iters, vcm, pot, stats):
         self.bound = bound      # uint8 device tensor, member order
