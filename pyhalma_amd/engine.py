@@ -553,8 +553,54 @@ def _centres(red, counts):
     return c, M, R
 
 
+def _clean_cells(L, ns, H, s_mem, s_grp, x, y, z, vx, vy, vz, m, c, grid, ijk, stats, ws):
+    """halma_clean_cells over the set (s_mem, s_grp) with per-group centres c (H, 3) and
+    grids (start, second node, delta, length); returns the number of cells and the device
+    copies of the per-group arguments (to be held until the stream has used them)."""
+    from .device import dtype_flag
+    dev = x.device
+    start, g1, d, n = grid
+    used = np.zeros(H, bool)
+    used[torch.unique(s_grp).cpu().numpy()] = True
+    max_n = int(n[used].max())
+    dv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    held = (dv(c), dv(start), dv(g1), dv(d), dv(np.clip(n, 1, 2 ** 30).astype(np.int32)))
+    ncells = ctypes_i64()
+    rc = L.halma_clean_cells(ns, H, ptr(s_mem), ptr(s_grp), ptr(x), ptr(y), ptr(z),
+                             dtype_flag(x), ptr(vx), ptr(vy), ptr(vz), dtype_flag(vx), ptr(m),
+                             dtype_flag(m), *(ptr(t) for t in held), max_n, ptr(ijk), ptr(stats),
+                             ncells, ptr(ws), ws.numel(), stream_ptr())
+    _lib.check(rc, "halma_clean_cells")
+    return int(ncells.value), held
+
+
+def cell_statistics(x, y, z, vx, vy, vz, m, members, centre, grid_start, grid_g1, grid_d,
+                    grid_n):
+    """calc_cell (halo_properties.py:113-159) of one group: node indices (n, 3) int32 and
+    the occupied cells' statistics (cells, 6): mass-weighted velocity (3), mass, member
+    count, dispersion; cells in increasing (ix n + iy) n + iz order."""
+    L = _lib_()
+    x, y, z = _coords(x, y, z)
+    vx, vy, vz = _coords(vx, vy, vz)
+    m = _vec(m)
+    dev = x.device
+    members = to_dev(members, np.int32)
+    ns = int(members.numel())
+    ijk = torch.empty((max(ns, 1), 3), dtype=torch.int32, device=dev)
+    stats = torch.empty(6 * max(ns, 1), dtype=torch.float64, device=dev)
+    if ns == 0:
+        return ijk[:0], stats[:0].reshape(0, 6)
+    ws = workspace(L.halma_clean_workspace_size(ns))
+    grp = torch.zeros(ns, dtype=torch.int32, device=dev)
+    grid = tuple(np.array([v]) for v in (grid_start, grid_g1, grid_d)) + (
+        np.array([int(grid_n)], np.int64),)
+    nc, _ = _clean_cells(L, ns, 1, members, grp, x, y, z, vx, vy, vz, m,
+                         np.asarray(centre, np.float64).reshape(1, 3), grid, ijk, stats, ws)
+    return ijk[:ns], stats[: 6 * nc].reshape(nc, 6)
+
+
 def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=True,
-                      min_part=50, minp=50):
+                      min_part=50, minp=50, centres=None, radii=None):
     """The phase-space cleaning of pyHALMA.py:789-843 (calc_cell / clean_cell,
     halo_properties.py:113-278) for every group at once.
 
@@ -568,7 +614,9 @@ def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=Tr
     members in the first round ends with zeros, as the reference's final step does with
     no members.  A cell with more than two members whose members share one velocity
     has sigma 0: its members then score inf (removed) or nan (kept) where the
-    reference's compiled division raises.
+    reference's compiled division raises.  ``centres`` (G, 3) and ``radii`` (G,), when
+    given, replace the first round's centre of mass and furthest-member distance (the
+    driver's own values, which clean_cell receives from its caller).
     """
     from .device import dtype_flag
     L = _lib_()
@@ -608,8 +656,11 @@ def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=Tr
     ijk = torch.empty((max(nm, 1), 3), dtype=torch.int32, device=dev)
     stats = torch.empty(6 * max(nm, 1), dtype=torch.float64, device=dev)
     cleaned = torch.zeros(max(H, 1), dtype=torch.int32, device=dev)
-    ncells = ctypes_i64()
-    pf, vf, mf = dtype_flag(x), dtype_flag(vx), dtype_flag(m)
+    vf = dtype_flag(vx)
+    given = None
+    if centres is not None:
+        given = (np.asarray(centres, np.float64).reshape(H, 3).copy(),
+                 np.asarray(radii, np.float64).reshape(H).copy())
     while (phase < 3).any():
         live = torch.from_numpy(phase < 3).to(dev)
         sidx = torch.nonzero(control.bool() & live[grp_of]).squeeze(1)
@@ -625,21 +676,17 @@ def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=Tr
         red = group_reduce(Groups(s_mem if ns else torch.zeros(1, dtype=torch.int32, device=dev),
                                   s_off_d, s_off), x, y, z, m).cpu().numpy()
         c, M, R = _centres(red, counts_h)
-        start, g1, d, n = _grid(R, ll)
+        if given is not None:
+            c, R = given  # the caller's first grid (every group is in its first round)
+            given = None
+        grid = _grid(R, ll)
         first = phase == 0
-        n_cell[first] = n[first]
+        n_cell[first] = grid[3][first]
         if ns:
-            used = counts_h > 0
-            max_n = int(n[used].max())
-            dv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
-            cen_d, st_d, g1_d, d_d = dv(c), dv(start), dv(g1), dv(d)
-            n_d = dv(np.clip(n, 1, None).astype(np.int32))
-            rc = L.halma_clean_cells(ns, H, ptr(s_mem), ptr(s_grp), ptr(x), ptr(y), ptr(z), pf,
-                                     ptr(vx), ptr(vy), ptr(vz), vf, ptr(m), mf, ptr(cen_d),
-                                     ptr(st_d), ptr(g1_d), ptr(d_d), ptr(n_d), max_n, ptr(ijk),
-                                     ptr(stats), ncells, ptr(ws), ws.numel(), stream_ptr())
-            _lib.check(rc, "halma_clean_cells")
+            _, held = _clean_cells(L, ns, H, s_mem, s_grp, x, y, z, vx, vy, vz, m, c, grid, ijk,
+                                   stats, ws)
             cleaned.zero_()
+            dv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
             phase_d, fac_d = dv(phase), dv(fac)  # held until the kernel has read them
             rc = L.halma_clean_decide(ns, ptr(s_mem), ptr(s_grp), ptr(target), ptr(vx), ptr(vy),
                                       ptr(vz), vf, ptr(stats), ptr(phase_d), ptr(fac_d),

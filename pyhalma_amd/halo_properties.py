@@ -8,6 +8,9 @@ Same names, arguments and return values as the reference (scalars are Python flo
   furthest_particle(cx, cy, cz, part_list, st_x, st_y, st_z)    :98-111
   density_peak(part_list, st_x, st_y, st_z, st_mass, ll)        :518-556
   escape_velocity_unbinding_fortran(...)                        :282-361
+  half_mass_radius(cx, cy, cz, M, part_list, st_x, ..., st_mass) :366-392
+  calc_cell(cx, cy, cz, grid, part_list, ..., sig3D_cell)        :113-159
+  clean_cell(cx, cy, cz, M, RRHH, grid, part_list, ...)          :162-278
 
 The reference's numba ``prange`` sums have no fixed order; here they are FP64
 double-double sums (within about an ulp of the correctly rounded value).  density_peak
@@ -15,9 +18,9 @@ is bit-exact to the reference's serial loop (see libhalma's halma_density_peak).
 Each call here handles one group; the batched forms over all groups at once are
 ``pyhalma_amd.engine.group_reduce`` / ``density_peak`` / ``unbind``.
 
-The remaining functions of the reference module (phase-space cleaning, half-mass radii,
-angular momentum, morphology, star formation, dispersions, shapes, Sersic fits, SMBH)
-are halo-property calculations outside the accelerated path.
+The remaining functions of the reference module (angular momentum, morphology, star
+formation, dispersions, shapes, Sersic fits, SMBH) are halo-property calculations
+outside the accelerated path.
 """
 import numpy as np
 import torch
@@ -166,3 +169,70 @@ def escape_velocity_unbinding_fortran(rete, L, ncoarse, grid_data, gas_data, mas
     # rounding, and the GPU sum is compensated
     return escape_velocity_unbinding(part_list, st_x, st_y, st_z, st_vx, st_vy, st_vz,
                                      st_mass, vx, vy, vz, factor_v, extra_sources=sources)
+
+
+def _arange_grid(grid):
+    """(start, second node, delta, length) of a grid made by np.arange (node k = start,
+    the second node, start + k * delta); ValueError for any other grid."""
+    grid = np.asarray(grid, dtype=np.float64).ravel()
+    n = grid.size
+    if n == 0:
+        raise ValueError("attempt to get argmin of an empty sequence")
+    start = float(grid[0])
+    g1 = float(grid[1]) if n > 1 else start + 1.0
+    d = g1 - start
+    nodes = start + np.arange(n, dtype=np.float64) * d
+    nodes[0] = start
+    if n > 1:
+        nodes[1] = g1
+    if not (d > 0 and np.array_equal(nodes, grid)):
+        raise ValueError("grid must be an increasing np.arange grid")
+    return start, g1, d, n
+
+
+def calc_cell(cx, cy, cz, grid, part_list, st_x, st_y, st_z, st_vx, st_vy, st_vz, st_mass,
+              vcm_cell, mass_cell, quantas_cell, sig3D_cell):
+    """halo_properties.py:113-159.  The cell arrays are filled for the occupied cells (the
+    reference adds into them; every caller passes them zero-filled) and returned with the
+    per-member node indices (int32).  The grid must be an np.arange grid, as the
+    reference's callers build it."""
+    start, g1, d, n = _arange_grid(grid)
+    groups, sub = _one_group(part_list, [st_x, st_y, st_z, st_vx, st_vy, st_vz, st_mass])
+    npart = groups.n_members
+    ijk, st = engine.cell_statistics(*sub, groups.members[:npart], (cx, cy, cz), start, g1, d,
+                                     n)
+    ijk = ijk.cpu().numpy()
+    st = st.cpu().numpy()
+    if npart:
+        keys = np.unique((ijk[:, 0].astype(np.int64) * n + ijk[:, 1]) * n + ijk[:, 2])
+        ix, rem = np.divmod(keys, n * n)
+        iy, iz = np.divmod(rem, n)
+        vcm_cell[ix, iy, iz, :] = st[:, 0:3]
+        mass_cell[ix, iy, iz] = st[:, 3]
+        quantas_cell[ix, iy, iz] = st[:, 4]
+        sig3D_cell[ix, iy, iz] = st[:, 5]
+    wx, wy, wz = (np.ascontiguousarray(ijk[:, a]) for a in range(3))
+    return vcm_cell, mass_cell, quantas_cell, sig3D_cell, wx, wy, wz
+
+
+def clean_cell(cx, cy, cz, M, RRHH, grid, part_list, st_x, st_y, st_z, st_vx, st_vy, st_vz,
+               st_mass, vcm_cell, mass_cell, quantas_cell, sig3D_cell, ll, q_fil, sig_fil,
+               which_cell_x, which_cell_y, which_cell_z):
+    """halo_properties.py:162-278: (cx, cy, cz, M, R, control (int32), which_cell_x, _y,
+    _z).  The first round's cells are recomputed on the GPU from (cx, cy, cz) and the
+    grid, which must be np.arange(-(RRHH + ll), RRHH + ll, 2 ll) as the reference's
+    driver builds it; the cell arrays and node indices passed in (calc_cell's output for
+    that grid) are not read."""
+    want = engine._grid(np.array([float(RRHH)]), float(ll))
+    start, g1, d, n = _arange_grid(grid)
+    if n != int(want[3][0]) or start != want[0][0] or (n > 1 and g1 != want[1][0]):
+        raise ValueError("grid must be np.arange(-(RRHH + ll), RRHH + ll, 2 ll)")
+    groups, sub = _one_group(part_list, [st_x, st_y, st_z, st_vx, st_vy, st_vz, st_mass])
+    res = engine.phase_space_clean(groups, *sub, float(ll), float(q_fil), float(sig_fil),
+                                   centres=[(float(cx), float(cy), float(cz))],
+                                   radii=[float(RRHH)])
+    control = res.control_clean.cpu().numpy().astype(np.int32)
+    w = res.which_of(0)
+    wx, wy, wz = (np.ascontiguousarray(w[:, a]) for a in range(3))
+    ccx, ccy, ccz, cM, cR = (float(v) for v in res.clean[0])
+    return ccx, ccy, ccz, cM, cR, control, wx, wy, wz

@@ -131,3 +131,55 @@ def test_fof_clumps_vs_oracle(gpu):
     res = engine.phase_space_clean(g, *args, ll / 16, 1.0, 3.0)
     per, kept = oracle.phase_space_clean(g.to_arrays(), *args, ll / 16, 1.0, 3.0)
     _check(res, per, kept)
+
+
+def test_wide_grid_uses_64_bit_cell_keys(gpu):
+    # one member 3000 ll from a dense core: about 3000 nodes per axis, 2.7e10 cells, so the
+    # cell key (ix n + iy) n + iz needs more than 32 bits
+    from pyhalma_amd import engine
+    rng = np.random.default_rng(7)
+    core = rng.normal(size=(3000, 3)) * 0.3
+    P = np.concatenate([core, [[3000.0, 1.0, -1.0]]])
+    V = rng.normal(size=(len(P), 3)) * 10
+    V[:200] += 400.0
+    M = rng.uniform(1, 2, len(P))
+    groups = [np.arange(len(P))]
+    args = (P[:, 0], P[:, 1], P[:, 2], V[:, 0], V[:, 1], V[:, 2], M)
+    res = engine.phase_space_clean(_dev_groups(groups), *args, 1.0, 1.0, 3.0)
+    per, kept = oracle.phase_space_clean(groups, *args, 1.0, 1.0, 3.0)
+    assert per[0]["n_cell"] ** 3 > 2 ** 32
+    _check(res, per, kept)
+
+
+def test_drop_in_calc_cell_and_clean_cell(gpu):
+    from oracle import cleaning
+    from pyhalma_amd import halo_properties as hp
+    P, V, M, groups = _contaminated(5, k=3)
+    args = (P[:, 0], P[:, 1], P[:, 2], V[:, 0], V[:, 1], V[:, 2], M)
+    ll = 0.3
+    for part_list in groups:
+        part_list = part_list[::-1].copy()  # any member order
+        cx, cy, cz, mass = oracle.center_of_mass(part_list, P[:, 0], P[:, 1], P[:, 2], M)
+        R = oracle.furthest_particle(cx, cy, cz, part_list, P[:, 0], P[:, 1], P[:, 2])
+        grid = np.arange(-(R + ll), R + ll, 2 * ll)
+        n = len(grid)
+        arrs = (np.zeros((n, n, n, 3)), np.zeros((n, n, n)), np.zeros((n, n, n)),
+                np.zeros((n, n, n)))
+        vcm, mc, qc, sc, wx, wy, wz = hp.calc_cell(cx, cy, cz, grid, part_list, *args, *arrs)
+        cells, which = cleaning.calc_cell(cx, cy, cz, grid, part_list, *args)
+        assert (np.stack([wx, wy, wz], 1) == which).all() and wx.dtype == np.int32
+        assert int((qc > 0).sum()) == len(cells)
+        for (ix, iy, iz), (vc, cm, q, sig) in cells.items():
+            assert qc[ix, iy, iz] == q
+            assert _close(mc[ix, iy, iz], cm) and _close(sc[ix, iy, iz], sig, 1e-12)
+            for a in range(3):
+                assert _close(vcm[ix, iy, iz, a], vc[a], 1e-12)
+        got = hp.clean_cell(cx, cy, cz, mass, R, grid, part_list, *args, vcm, mc, qc, sc, ll,
+                            1.0, 3.0, wx, wy, wz)
+        want = cleaning.clean_cell(part_list, *args, cells, which, ll, 1.0, 3.0)
+        assert (got[5] == want[5]).all() and got[5].dtype == np.int32
+        assert (np.stack(got[6:9], 1) == want[6]).all()
+        for q in range(5):
+            assert _close(got[q], want[q])
+    with pytest.raises(ValueError):
+        hp.calc_cell(0.0, 0.0, 0.0, np.array([0.0, 1.0, 3.0]), groups[0], *args, *arrs)
