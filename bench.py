@@ -54,11 +54,13 @@ def parse():
     ap.add_argument("--cpu-frac", type=float, default=0.5,
                     help="fraction of the box (along x) used for the CPU baseline sample")
     ap.add_argument("--workload", default="clumps",
-                    choices=["clumps", "gas", "bridged", "centring", "substructure"],
+                    choices=["clumps", "gas", "bridged", "centring", "substructure",
+                             "cleaning"],
                     help="clumps (default): the headline 2.1M-particle step; the others are "
                          "the gas batch, the bridged-pair FoF stress case, the "
-                         "potential-minimum centring and the substructure splitting of the "
-                         "bridged pairs at their full sizes")
+                         "potential-minimum centring, the substructure splitting of the "
+                         "bridged pairs and the phase-space cleaning of the clumps at their "
+                         "full sizes")
     return ap.parse_args()
 
 
@@ -81,7 +83,8 @@ def main():
     if args.workload != "clumps":
         torch.cuda.init()
         out = {"gas": bench_gas, "bridged": bench_bridged, "centring": bench_centring,
-               "substructure": bench_substructure}[args.workload](args)
+               "substructure": bench_substructure,
+               "cleaning": bench_cleaning}[args.workload](args)
         print(json.dumps(out))
         return
 
@@ -420,6 +423,61 @@ def bench_substructure(args):
                   "particles": n, "groups_in": groups.n_groups,
                   "groups_out": sub.groups.n_groups}, substructures=sub.n_found,
                  members_in=groups.n_members, **extra)
+
+
+def bench_cleaning(args):
+    """Phase-space cleaning (pyHALMA.py:789-843) of the headline clump catalogue: FoF
+    labels + CSR groups at ll, then calc_cell / clean_cell of every group at once with
+    the reference's parameter file values (cells of 2 ll, sig_fil 3, q_fil 1, groups
+    of at least 50 members cleaned, minp 50).  CPU baseline: the oracle's restatement
+    of the same block on the first groups of the same catalogue."""
+    import numpy as np
+    import torch
+
+    from pyhalma_amd import engine, synth
+
+    t0 = time.time()
+    w = synth.clumps(seed=args.seed)
+    n = len(w["x"])
+    ll = float(w["ll"])
+    log(f"cleaning workload: {n} particles, ll={ll:.5f} ({time.time() - t0:.1f}s)")
+    dev = torch.device("cuda", 0)
+    keys = ("x", "y", "z", "vx", "vy", "vz", "mass")
+    D = {k: torch.from_numpy(w[k]).to(dev) for k in keys}
+
+    def step():
+        lab = engine.fof_labels(D["x"], D["y"], D["z"], ll)
+        groups = engine.groups_from_labels(lab, MINP)
+        res = engine.phase_space_clean(groups, *(D[k] for k in keys), ll, 1.0, 3.0,
+                                       psc=True, min_part=50, minp=MINP)
+        return groups, res
+
+    step_s, (groups, res) = _timed(step, args.steps, args.warmup)
+    removed = int(groups.n_members - int(res.control_clean.sum().item()))
+    log(f"cleaning: {step_s * 1e3:.1f} ms/step; groups {groups.n_groups} kept "
+        f"{res.kept.n_groups}; members removed {removed}; rounds max {res.rounds.max()}")
+    extra = {}
+    if not args.no_cpu:
+        import oracle
+
+        k = 40
+        gl = groups.to_arrays()[:k]
+        ns = int(sum(len(g) for g in gl))
+        tc = time.perf_counter()
+        oracle.phase_space_clean(gl, *(w[q] for q in keys), ll, 1.0, 3.0)
+        cs = time.perf_counter() - tc
+        extra["cpu_baseline"] = {"value": ns / cs, "unit": "members/s", "cores": 1,
+                                 "kind": "port", "seconds": cs,
+                                 "sample": f"the first {k} groups ({ns} members): the "
+                                           "oracle's restatement (sparse cells, one group "
+                                           "at a time)"}
+    return _line("particles/s FoF + phase-space cleaning, 2.1M-particle clumps",
+                 n / step_s, "particles/s", step_s, args,
+                 {"workload": "1000 Plummer clumps in a 40 Mpc box, 2.1M FP32, FoF at ll, "
+                              "cells of 2 ll, sig_fil 3, q_fil 1",
+                  "particles": n, "groups": groups.n_groups, "kept": res.kept.n_groups},
+                 members_in=groups.n_members, members_removed=removed,
+                 max_rounds=int(res.rounds.max()) if len(res.rounds) else 0, **extra)
 
 
 def bench_centring(args):

@@ -212,6 +212,20 @@ int halma_half_mass_radius(int64_t n_groups, int64_t n_members, const int64_t *o
  * a member of a cell with at most q_fil members, or with the scaled deviation >= fac[g] *
  * sig_fil, is removed: control[set_target[k]] = 0 and cleaned[g] += 1. */
 size_t halma_clean_workspace_size(int64_t n_set);
+/* The set S of a round (clean_cell's clean_particles = part_list[control != 0] of every
+ * live group, halo_properties.py:209 and :263): over the CSR groups offsets (device, G+1,
+ * offsets[0] = 0, n_members = offsets[G]), member position k of group g is in S when
+ * live[g] and (control[k] or force[g]) (device uint8 arrays; force may be NULL).  S in
+ * position order: set_members = members[k] (the position k itself when members is NULL),
+ * set_groups = g, set_target = offsets[g] + the member's rank in its group's part of S
+ * (the control entry that clean_cell clears for it); set_offsets (device, G+1) the CSR
+ * offsets of S per group; *n_set (HOST) its size.  Synchronises the stream. */
+size_t halma_clean_select_workspace_size(int64_t n_members);
+int halma_clean_select(int64_t n_groups, int64_t n_members, const int64_t *offsets,
+                       const int32_t *members, const uint8_t *control, const uint8_t *live,
+                       const uint8_t *force, int32_t *set_members, int32_t *set_groups,
+                       int64_t *set_target, int64_t *set_offsets, int64_t *n_set,
+                       void *workspace, size_t workspace_bytes, void *stream);
 int halma_clean_cells(int64_t n_set, int64_t n_groups, const int32_t *set_members,
                       const int32_t *set_groups, const void *x, const void *y, const void *z,
                       int pos_dtype, const void *vx, const void *vy, const void *vz,

@@ -65,6 +65,9 @@ SIGNATURES = {
     "halma_half_mass_radius_workspace_size": (_size, [_i64]),
     "halma_half_mass_radius": (_c.c_int, [_i64, _i64, _p, _p, _p, _p, _p, _c.c_int, _p,
                                           _c.c_int, _p, _p, _p, _size, _p]),
+    "halma_clean_select_workspace_size": (_size, [_i64]),
+    "halma_clean_select": (_c.c_int, [_i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64p,
+                                      _p, _size, _p]),
     "halma_clean_workspace_size": (_size, [_i64]),
     "halma_clean_cells": (_c.c_int, [_i64, _i64, _p, _p, _p, _p, _p, _c.c_int, _p, _p, _p,
                                      _c.c_int, _p, _c.c_int, _p, _p, _p, _p, _p, _c.c_int,

@@ -553,15 +553,15 @@ def _centres(red, counts):
     return c, M, R
 
 
-def _clean_cells(L, ns, H, s_mem, s_grp, x, y, z, vx, vy, vz, m, c, grid, ijk, stats, ws):
+def _clean_cells(L, ns, H, s_mem, s_grp, x, y, z, vx, vy, vz, m, c, grid, used, ijk, stats,
+                 ws):
     """halma_clean_cells over the set (s_mem, s_grp) with per-group centres c (H, 3) and
-    grids (start, second node, delta, length); returns the number of cells and the device
-    copies of the per-group arguments (to be held until the stream has used them)."""
+    grids (start, second node, delta, length); ``used`` marks the groups with members in
+    the set.  Returns the number of cells and the device copies of the per-group
+    arguments (to be held until the stream has used them)."""
     from .device import dtype_flag
     dev = x.device
     start, g1, d, n = grid
-    used = np.zeros(H, bool)
-    used[torch.unique(s_grp).cpu().numpy()] = True
     max_n = int(n[used].max())
     dv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
     held = (dv(c), dv(start), dv(g1), dv(d), dv(np.clip(n, 1, 2 ** 30).astype(np.int32)))
@@ -595,8 +595,20 @@ def cell_statistics(x, y, z, vx, vy, vz, m, members, centre, grid_start, grid_g1
     grid = tuple(np.array([v]) for v in (grid_start, grid_g1, grid_d)) + (
         np.array([int(grid_n)], np.int64),)
     nc, _ = _clean_cells(L, ns, 1, members, grp, x, y, z, vx, vy, vz, m,
-                         np.asarray(centre, np.float64).reshape(1, 3), grid, ijk, stats, ws)
+                         np.asarray(centre, np.float64).reshape(1, 3), grid, np.ones(1, bool),
+                         ijk, stats, ws)
     return ijk[:ns], stats[: 6 * nc].reshape(nc, 6)
+
+
+def _select(L, H, goff, nm, members, control, live, force, out, ws):
+    """halma_clean_select into out = (set_members, set_groups, set_target, set_offsets);
+    returns the set's size."""
+    n = ctypes_i64()
+    rc = L.halma_clean_select(H, nm, ptr(goff), ptr(members), ptr(control), ptr(live),
+                              ptr(force), *(ptr(t) for t in out), n, ptr(ws), ws.numel(),
+                              stream_ptr())
+    _lib.check(rc, "halma_clean_select")
+    return int(n.value)
 
 
 def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=True,
@@ -604,19 +616,21 @@ def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=Tr
     """The phase-space cleaning of pyHALMA.py:789-843 (calc_cell / clean_cell,
     halo_properties.py:113-278) for every group at once.
 
-    Each round works on the set S of members still flagged of every group not finished:
-    one group_reduce gives the centres and radii of the grids (np.arange(-(R + ll),
-    R + ll, 2 ll) about the centre of mass), halma_clean_cells the cell statistics and
-    halma_clean_decide the removals.  The per-group bookkeeping (remaining, the
-    fraction removed, fac_sig: 2, 1.5, then x 0.75 with a floor of 1) is the reference's,
-    quirks included: in rounds after the first the decision about the ip-th member of S
-    clears entry ip of the group's control array.  A group that removes all of its
-    members in the first round ends with zeros, as the reference's final step does with
-    no members.  A cell with more than two members whose members share one velocity
-    has sigma 0: its members then score inf (removed) or nan (kept) where the
-    reference's compiled division raises.  ``centres`` (G, 3) and ``radii`` (G,), when
-    given, replace the first round's centre of mass and furthest-member distance (the
-    driver's own values, which clean_cell receives from its caller).
+    Each round works on the set S of members still flagged of every group not finished
+    (halma_clean_select): one group_reduce gives the centres and radii of the grids
+    (np.arange(-(R + ll), R + ll, 2 ll) about the centre of mass), halma_clean_cells the
+    cell statistics and halma_clean_decide the removals.  The members' data are gathered
+    once into member order when the seven arrays share one dtype, so the rounds read
+    them contiguously.  The per-group bookkeeping (remaining, the fraction removed,
+    fac_sig: 2, 1.5, then x 0.75 with a floor of 1) is the reference's, quirks included:
+    in rounds after the first the decision about the ip-th member of S clears entry ip
+    of the group's control array.  A group that removes all of its members in the first
+    round ends with zeros, as the reference's final step does with no members.  A cell
+    with more than two members whose members share one velocity has sigma 0: its
+    members then score inf (removed) or nan (kept) where the reference's compiled
+    division raises.  ``centres`` (G, 3) and ``radii`` (G,), when given, replace the
+    first round's centre of mass and furthest-member distance (the driver's own values,
+    which clean_cell receives from its caller).
     """
     from .device import dtype_flag
     L = _lib_()
@@ -640,10 +654,21 @@ def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=Tr
                            kept_index=np.zeros(0, np.int64), centre=np.zeros((0, 3)),
                            mass=np.zeros(0), rmax=np.zeros(0), npart=np.zeros(0, np.int64),
                            kept_n_cell=np.zeros(0, np.int64))
-    off_d = torch.from_numpy(off).to(dev)
-    grp_of = torch.repeat_interleave(torch.arange(H, device=dev),
-                                     torch.from_numpy(sizes).to(dev))  # int64
-    control = torch.ones(nm, dtype=torch.uint8, device=dev)
+    goff = torch.from_numpy(off).to(dev)
+    if x.dtype == vx.dtype == m.dtype and nm > 0:
+        md = gather_members(groups, x, y, z, m, vx, vy, vz)
+        A = (md.x, md.y, md.z, md.vx, md.vy, md.vz, md.m)
+        by = None          # the set holds member positions into the gathered arrays
+    else:
+        A = (x, y, z, vx, vy, vz, m)
+        by = mem           # the set holds particle indices
+    control = torch.ones(max(nm, 1), dtype=torch.uint8, device=dev)
+    sel = (torch.empty(max(nm, 1), dtype=torch.int32, device=dev),
+           torch.empty(max(nm, 1), dtype=torch.int32, device=dev),
+           torch.empty(max(nm, 1), dtype=torch.int64, device=dev),
+           torch.empty(H + 1, dtype=torch.int64, device=dev))
+    s_mem, s_grp, target, s_off = sel
+    sel_ws = workspace(L.halma_clean_select_workspace_size(nm))
     phase = np.zeros(H, np.int32)  # 0 first round, 1 later round, 2 final grid, 3 done
     fac = np.full(H, 2.0)
     remaining = sizes.copy()
@@ -652,29 +677,26 @@ def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=Tr
     clean = np.zeros((H, 5))
     which = torch.zeros((max(nm, 1), 3), dtype=torch.int32, device=dev)
     which_len = np.zeros(H, np.int64)
+    zero_ended = np.zeros(H, bool)  # emptied by a later round: clean_cell returned zeros
     ws = workspace(L.halma_clean_workspace_size(nm))
     ijk = torch.empty((max(nm, 1), 3), dtype=torch.int32, device=dev)
     stats = torch.empty(6 * max(nm, 1), dtype=torch.float64, device=dev)
     cleaned = torch.zeros(max(H, 1), dtype=torch.int32, device=dev)
-    vf = dtype_flag(vx)
+    vf = dtype_flag(A[3])
     given = None
     if centres is not None:
         given = (np.asarray(centres, np.float64).reshape(H, 3).copy(),
                  np.asarray(radii, np.float64).reshape(H).copy())
+
+    def dv(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
     while (phase < 3).any():
-        live = torch.from_numpy(phase < 3).to(dev)
-        sidx = torch.nonzero(control.bool() & live[grp_of]).squeeze(1)
-        ns = int(sidx.numel())
-        s_mem = mem[sidx]
-        sg = grp_of[sidx]
-        s_grp = sg.to(torch.int32)
-        counts = torch.bincount(sg, minlength=H)
-        counts_h = counts.cpu().numpy().astype(np.int64)
-        s_off = np.concatenate([[0], np.cumsum(counts_h)]).astype(np.int64)
-        s_off_d = torch.from_numpy(s_off).to(dev)
-        target = off_d[sg] + (torch.arange(ns, device=dev) - s_off_d[sg])
-        red = group_reduce(Groups(s_mem if ns else torch.zeros(1, dtype=torch.int32, device=dev),
-                                  s_off_d, s_off), x, y, z, m).cpu().numpy()
+        live = dv((phase < 3).astype(np.uint8))
+        ns = _select(L, H, goff, nm, by, control, live, None, sel, sel_ws)
+        s_off_h = s_off.cpu().numpy()
+        counts_h = np.diff(s_off_h)
+        red = group_reduce(Groups(s_mem, s_off, s_off_h), A[0], A[1], A[2], A[6]).cpu().numpy()
         c, M, R = _centres(red, counts_h)
         if given is not None:
             c, R = given  # the caller's first grid (every group is in its first round)
@@ -683,15 +705,15 @@ def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=Tr
         first = phase == 0
         n_cell[first] = grid[3][first]
         if ns:
-            _, held = _clean_cells(L, ns, H, s_mem, s_grp, x, y, z, vx, vy, vz, m, c, grid, ijk,
+            _, held = _clean_cells(L, ns, H, s_mem, s_grp, *A, c, grid, counts_h > 0, ijk,
                                    stats, ws)
+            which.index_copy_(0, target[:ns], ijk[:ns])
             cleaned.zero_()
-            dv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
             phase_d, fac_d = dv(phase), dv(fac)  # held until the kernel has read them
-            rc = L.halma_clean_decide(ns, ptr(s_mem), ptr(s_grp), ptr(target), ptr(vx), ptr(vy),
-                                      ptr(vz), vf, ptr(stats), ptr(phase_d), ptr(fac_d),
-                                      float(q_fil), float(sig_fil), ptr(control), ptr(cleaned),
-                                      ptr(ws), ws.numel(), stream_ptr())
+            rc = L.halma_clean_decide(ns, ptr(s_mem), ptr(s_grp), ptr(target), ptr(A[3]),
+                                      ptr(A[4]), ptr(A[5]), vf, ptr(stats), ptr(phase_d),
+                                      ptr(fac_d), float(q_fil), float(sig_fil), ptr(control),
+                                      ptr(cleaned), ptr(ws), ws.numel(), stream_ptr())
             _lib.check(rc, "halma_clean_decide")
             cl = cleaned[:H].cpu().numpy().astype(np.int64)
         else:
@@ -705,10 +727,8 @@ def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=Tr
         remaining[p0] = sizes[p0] - cl[p0]
         remaining[p1] -= cl[p1]
         zero = p1 & (remaining == 0)
+        zero_ended |= zero
         ends |= zero
-        if ns and ends.any():
-            pick = torch.from_numpy(ends).to(dev)[sg]
-            which[target[pick]] = ijk[:ns][pick]
         which_len[ends] = counts_h[ends]
         fin = phase == 2
         clean[fin] = np.column_stack([c, M, R])[fin]
@@ -721,24 +741,31 @@ def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=Tr
         live1 = p1 & ~zero
         phase[live1] = np.where(frac[live1] > 0.1, 1, 2)
         fac[live1] = np.maximum(1.0, fac[live1] * 0.75)
-    control_clean = control.bool()
+    # the driver (pyHALMA.py:819-843): every member kept where psc is off or the group is
+    # small; centre, mass and radius again over the kept members; groups of minp or more.
+    # A group that ended on its final grid keeps that grid's set, whose centre, mass and
+    # radius are already known (same set, same reduction); the others are reduced again.
     keep_all = np.ones(H, bool) if not psc else sizes < min_part
-    final = control_clean | torch.from_numpy(keep_all).to(dev)[grp_of]
-    kidx = torch.nonzero(final).squeeze(1)
-    kcount = torch.bincount(grp_of[kidx], minlength=H).cpu().numpy().astype(np.int64)
-    koff = np.concatenate([[0], np.cumsum(kcount)]).astype(np.int64)
-    kmem = mem[kidx]
-    kg = Groups(kmem if kmem.numel() else torch.zeros(1, dtype=torch.int32, device=dev),
-                torch.from_numpy(koff).to(dev), koff)
-    kred = group_reduce(kg, x, y, z, m).cpu().numpy() if H else np.zeros((0, 16))
-    kc, kM, kR = _centres(kred, kcount)
+    force = dv(keep_all.astype(np.uint8))
+    control_clean = control[:nm].bool()
+    kc, kM, kR = clean[:, 0:3].copy(), clean[:, 3].copy(), clean[:, 4].copy()
+    kcount = which_len.copy()
+    redo = keep_all | zero_ended
+    if redo.any():
+        ns = _select(L, H, goff, nm, by, control, dv(redo.astype(np.uint8)), force, sel,
+                     sel_ws)
+        cnt = np.diff(s_off.cpu().numpy())
+        kred = group_reduce(Groups(s_mem, s_off, np.concatenate([[0], np.cumsum(cnt)])),
+                            A[0], A[1], A[2], A[6]).cpu().numpy()
+        rcen, rM, rR = _centres(kred, cnt)
+        kc[redo], kM[redo], kR[redo], kcount[redo] = rcen[redo], rM[redo], rR[redo], cnt[redo]
     ok = kcount >= minp
-    okd = torch.from_numpy(ok).to(dev)
-    sel = okd[grp_of[kidx]]
-    out_mem = kmem[sel]
+    grp_of = torch.repeat_interleave(torch.arange(H, device=dev),
+                                     torch.from_numpy(sizes).to(dev))
+    final = control_clean | force.bool()[grp_of]
+    ns = _select(L, H, goff, nm, mem, control, dv(ok.astype(np.uint8)), force, sel, sel_ws)
     out_off = np.concatenate([[0], np.cumsum(kcount[ok])]).astype(np.int64)
-    kept = Groups(out_mem if out_mem.numel() else torch.zeros(1, dtype=torch.int32, device=dev),
-                  torch.from_numpy(out_off).to(dev), out_off)
+    kept = Groups(s_mem[:max(ns, 1)].clone(), torch.from_numpy(out_off).to(dev), out_off)
     return CleanResult(control=final, control_clean=control_clean, clean=clean, n_cell=n_cell,
                        rounds=rounds, which=which, which_len=which_len, offsets_host=off,
                        kept=kept, kept_index=np.nonzero(ok)[0], centre=kc[ok], mass=kM[ok],
