@@ -55,12 +55,12 @@ def parse():
                     help="fraction of the box (along x) used for the CPU baseline sample")
     ap.add_argument("--workload", default="clumps",
                     choices=["clumps", "gas", "bridged", "centring", "substructure",
-                             "cleaning"],
+                             "cleaning", "progenitors"],
                     help="clumps (default): the headline 2.1M-particle step; the others are "
                          "the gas batch, the bridged-pair FoF stress case, the "
                          "potential-minimum centring, the substructure splitting of the "
-                         "bridged pairs and the phase-space cleaning of the clumps at their "
-                         "full sizes")
+                         "bridged pairs, the phase-space cleaning and the progenitor "
+                         "matching of the clumps at their full sizes")
     return ap.parse_args()
 
 
@@ -84,7 +84,8 @@ def main():
         torch.cuda.init()
         out = {"gas": bench_gas, "bridged": bench_bridged, "centring": bench_centring,
                "substructure": bench_substructure,
-               "cleaning": bench_cleaning}[args.workload](args)
+               "cleaning": bench_cleaning,
+               "progenitors": bench_progenitors}[args.workload](args)
         print(json.dumps(out))
         return
 
@@ -478,6 +479,71 @@ def bench_cleaning(args):
                   "particles": n, "groups": groups.n_groups, "kept": res.kept.n_groups},
                  members_in=groups.n_members, members_removed=removed,
                  max_rounds=int(res.rounds.max()) if len(res.rounds) else 0, **extra)
+
+
+def bench_progenitors(args):
+    """Progenitor matching (pyHALMA.py:1211-1266) on the headline clump catalogue: the
+    new haloes are the FoF groups at ll (1001 haloes, 1.91M members), the old ones the
+    FoF groups of the same particles at 0.6 ll (fragments: several progenitors per new
+    halo); particle IDs are a random permutation offset by 10^9, masses the groups'
+    summed masses.  One step = halma_progenitors over all new haloes (inputs resident).
+    CPU baseline: the oracle's restatement (np.isin per (new, old) halo pair, as the
+    reference) on the first new haloes."""
+    import numpy as np
+    import torch
+
+    from pyhalma_amd import engine, synth
+
+    t0 = time.time()
+    w = synth.clumps(seed=args.seed)
+    n = len(w["x"])
+    ll = float(w["ll"])
+    dev = torch.device("cuda", 0)
+    D = {k: torch.from_numpy(w[k]).to(dev) for k in ("x", "y", "z", "mass")}
+    rng = np.random.default_rng(args.seed)
+    oripa = rng.permutation(n).astype(np.int64) + 10 ** 9
+    _, new = engine.fof_groups(D["x"], D["y"], D["z"], ll, MINP)
+    _, old = engine.fof_groups(D["x"], D["y"], D["z"], 0.6 * ll, MINP)
+    ids_d = torch.from_numpy(oripa).to(dev)
+    new_mass = engine.group_reduce(new, D["x"], D["y"], D["z"], D["mass"])[:, 0].cpu().numpy()
+    omm = engine.group_reduce(old, D["x"], D["y"], D["z"], D["mass"])[:, 0].cpu().numpy()
+    old_ids = ids_d[old.members[: old.n_members].long()]
+    old_npart = old.sizes()
+    log(f"progenitor workload: {new.n_groups} new haloes ({new.n_members} members), "
+        f"{old.n_groups} old haloes ({old.n_members} members) ({time.time() - t0:.1f}s)")
+
+    def step():
+        return engine.progenitors(new, ids_d, new_mass, old_npart, old_ids, omm)
+
+    step_s, (pro1, pro2, nmerg, mtype) = _timed(step, args.steps, args.warmup)
+    log(f"progenitors: {step_s * 1e3:.2f} ms/step; with a progenitor {int((pro1 > 0).sum())}; "
+        f"n_mergers max {int(nmerg.max())}; types {np.unique(mtype).tolist()}")
+    extra = {}
+    if not args.no_cpu:
+        import oracle
+
+        k = 8
+        groups = new.to_arrays()[:k]
+        old_lists = [oripa[g] for g in old.to_arrays()]
+        tc = time.perf_counter()
+        want = oracle.progenitors(groups, oripa, new_mass[:k], old_lists, omm)
+        cs = time.perf_counter() - tc
+        same = all((a[:k] == b).all() for a, b in zip((pro1, pro2, nmerg, mtype), want))
+        ns = int(sum(len(g) for g in groups))
+        extra["cpu_baseline"] = {"value": ns / cs, "unit": "new members/s", "cores": 1,
+                                 "kind": "port", "seconds": cs,
+                                 "sample": f"the first {k} new haloes ({ns} members) against "
+                                           f"all {old.n_groups} old haloes: the oracle's "
+                                           "restatement, np.isin per halo pair"}
+        extra["sample_matches_oracle"] = bool(same)
+    return _line("new members/s progenitor matching, 2.1M-particle clumps",
+                 new.n_members / step_s, "new members/s", step_s, args,
+                 {"workload": "FoF groups at ll (new) against FoF groups at 0.6 ll (old) of "
+                              "1000 Plummer clumps, 2.1M FP32, random int64 IDs",
+                  "new_haloes": new.n_groups, "old_haloes": old.n_groups,
+                  "new_members": new.n_members, "old_members": old.n_members},
+                 with_progenitor=int((pro1 > 0).sum()), max_mergers=int(nmerg.max()),
+                 **extra)
 
 
 def bench_centring(args):

@@ -36,6 +36,7 @@ from .reductions import (  # noqa: F401
     linspace_ref,
 )
 from .cleaning import phase_space_clean  # noqa: F401
+from .progenitors import main_progenitors, progenitors  # noqa: F401
 from .substructure import (  # noqa: F401
     half_mass_radius,
     split_substructure,

@@ -772,6 +772,51 @@ def phase_space_clean(groups, x, y, z, vx, vy, vz, m, ll, q_fil, sig_fil, psc=Tr
                        rmax=kR[ok], npart=kcount[ok], kept_n_cell=n_cell[ok])
 
 
+def progenitors(new_groups, oripa, masses, old_npart, old_oripas, omm):
+    """The merger section of pyHALMA.py:1211-1266 for every new halo at once
+    (halma_progenitors): new haloes ``new_groups`` (particle indices) with per-particle
+    IDs ``oripa`` (st_oripa) and masses; old haloes as the checkpoint holds them
+    (pyHALMA.py:419-428): members per halo ``old_npart``, their IDs concatenated
+    ``old_oripas``, masses ``omm``.  Returns int32 numpy arrays (pro1, pro2, n_mergers,
+    merger_type)."""
+    L = _lib_()
+    H = new_groups.n_groups
+    out = [np.zeros(H, np.int32) for _ in range(4)]
+    if H == 0:
+        return tuple(out)
+    nm = new_groups.n_members
+    ids = to_dev(oripa, np.int64)
+    new_ids = ids[new_groups.members[:nm].long()] if nm else ids[:0]
+    new_mass = to_dev(np.asarray(masses, np.float64).reshape(H), np.float64)
+    old_npart = np.asarray(old_npart, np.int64).ravel()
+    G = old_npart.size
+    old_off = to_dev(np.concatenate([[0], np.cumsum(old_npart)]).astype(np.int64), np.int64)
+    no = int(old_npart.sum())
+    if no == 0:
+        old_ids = new_ids[:0]
+    elif isinstance(old_oripas, torch.Tensor):
+        old_ids = to_dev(old_oripas.reshape(-1)[:no], np.int64)
+    else:
+        old_ids = to_dev(np.asarray(old_oripas).ravel()[:no], np.int64)
+    old_mass = to_dev(np.asarray(omm, np.float64).ravel()[:G], np.float64) if G else new_mass
+    dev = new_mass.device
+    res = [torch.empty(H, dtype=torch.int32, device=dev) for _ in range(4)]
+    max_pairs = nm
+    n_pairs = ctypes_i64()
+    for _ in range(2):
+        ws = workspace(L.halma_progenitors_workspace_size(nm, no, max_pairs))
+        rc = L.halma_progenitors(H, nm, ptr(new_groups.offsets), ptr(new_ids), ptr(new_mass), G,
+                                 no, ptr(old_off), ptr(old_ids), ptr(old_mass), max_pairs,
+                                 *(ptr(r) for r in res), n_pairs, ptr(ws), ws.numel(),
+                                 stream_ptr())
+        if rc == _lib.E_LIMIT and n_pairs.value > max_pairs:
+            max_pairs = int(n_pairs.value)  # IDs repeated across old haloes: size again
+            continue
+        _lib.check(rc, "halma_progenitors")
+        break
+    return tuple(r.cpu().numpy() for r in res)
+
+
 class UnbindResult:
     def __init__(self, bound, iters, vcm, pot, stats):
         self.bound = bound      # uint8 device tensor, member order
