@@ -246,24 +246,23 @@ int halma_clean_decide(int64_t n_set, const int32_t *set_members, const int32_t 
  * new_offsets[0] = 0) over new_ids (device int64, the particle IDs of each halo's
  * members, st_oripa[halo]) and their masses new_mass (device FP64, n_new); old haloes
  * likewise (the previous iteration's oripas_before and omm).  An old halo intersects a
- * new one when they share an ID (np.in1d(oripas, ooripas) of :1230-1234).  Outputs
- * (device int32, n_new each): pro1 / pro2 (1-based old index of the most and second
- * most massive intersecting old halo, 0 for none; equal masses rank the higher old index
- * first), n_mergers (intersecting old haloes), merger_type (-1 when 1.2 * new mass <
- * pro1's mass; with two or more intersections 2 when 1/20 < second / first mass < 1/3,
- * 3 otherwise -- the reference's type 1 is overwritten by its following if / else; 0
- * without intersections or with one).  max_pairs bounds the (new member, old member)
- * ID matches (n_new_members when the old haloes are disjoint); *n_pairs (HOST) receives
- * their number, also when HALMA_E_LIMIT reports that it exceeds max_pairs, so a caller
- * can size the workspace again.  Synchronises the stream. */
-size_t halma_progenitors_workspace_size(int64_t n_new_members, int64_t n_old_members,
-                                        int64_t max_pairs);
+ * new one when they share an ID (np.in1d(oripas, ooripas) of :1230-1234); an ID may be
+ * listed in several old haloes.  Outputs (device int32, n_new each): pro1 / pro2 (1-based
+ * old index of the most and second most massive intersecting old halo, 0 for none; equal
+ * masses rank the higher old index first), n_mergers (intersecting old haloes),
+ * merger_type (-1 when 1.2 * new mass < pro1's mass; with two or more intersections 2
+ * when 1/20 < second / first mass < 1/3, 3 otherwise -- the reference's type 1 is
+ * overwritten by its following if / else; 0 without intersections or with one).  At most
+ * 2^30-1 members per catalogue; IDs may not span the whole int64 range.  Synchronises
+ * the stream. */
+size_t halma_progenitors_workspace_size(int64_t n_new, int64_t n_new_members, int64_t n_old,
+                                        int64_t n_old_members);
 int halma_progenitors(int64_t n_new, int64_t n_new_members, const int64_t *new_offsets,
                       const int64_t *new_ids, const double *new_mass, int64_t n_old,
                       int64_t n_old_members, const int64_t *old_offsets,
-                      const int64_t *old_ids, const double *old_mass, int64_t max_pairs,
-                      int32_t *pro1, int32_t *pro2, int32_t *n_mergers, int32_t *merger_type,
-                      int64_t *n_pairs, void *workspace, size_t workspace_bytes, void *stream);
+                      const int64_t *old_ids, const double *old_mass, int32_t *pro1,
+                      int32_t *pro2, int32_t *n_mergers, int32_t *merger_type,
+                      void *workspace, size_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------------------------------------------
  * Batched iterative unbinding.  The energy test of halo_properties.py:342-359 (stars,

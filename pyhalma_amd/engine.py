@@ -801,19 +801,11 @@ def progenitors(new_groups, oripa, masses, old_npart, old_oripas, omm):
     old_mass = to_dev(np.asarray(omm, np.float64).ravel()[:G], np.float64) if G else new_mass
     dev = new_mass.device
     res = [torch.empty(H, dtype=torch.int32, device=dev) for _ in range(4)]
-    max_pairs = nm
-    n_pairs = ctypes_i64()
-    for _ in range(2):
-        ws = workspace(L.halma_progenitors_workspace_size(nm, no, max_pairs))
-        rc = L.halma_progenitors(H, nm, ptr(new_groups.offsets), ptr(new_ids), ptr(new_mass), G,
-                                 no, ptr(old_off), ptr(old_ids), ptr(old_mass), max_pairs,
-                                 *(ptr(r) for r in res), n_pairs, ptr(ws), ws.numel(),
-                                 stream_ptr())
-        if rc == _lib.E_LIMIT and n_pairs.value > max_pairs:
-            max_pairs = int(n_pairs.value)  # IDs repeated across old haloes: size again
-            continue
-        _lib.check(rc, "halma_progenitors")
-        break
+    ws = workspace(L.halma_progenitors_workspace_size(H, nm, G, no))
+    rc = L.halma_progenitors(H, nm, ptr(new_groups.offsets), ptr(new_ids), ptr(new_mass), G, no,
+                             ptr(old_off), ptr(old_ids), ptr(old_mass), *(ptr(r) for r in res),
+                             ptr(ws), ws.numel(), stream_ptr())
+    _lib.check(rc, "halma_progenitors")
     return tuple(r.cpu().numpy() for r in res)
 
 

@@ -63,8 +63,8 @@ def test_random_catalogues_vs_oracle(gpu, spread):
 
 
 def test_ids_repeated_across_old_haloes(gpu):
-    # the old catalogue lists some IDs in two haloes (as overlapping domains can): more
-    # ID matches than new members, so the call sizes its workspace again
+    # the old catalogue lists some IDs in two haloes (as overlapping domains can): each
+    # such ID links the new halo to both
     old = [np.array([1, 2, 3, 4]), np.array([1, 2, 3, 4, 5]), np.array([6, 7])]
     got, want = _both([np.array([1, 2, 3, 4, 6])], [10.0], old, [5.0, 5.0, 1.0])
     assert got == want
@@ -72,8 +72,8 @@ def test_ids_repeated_across_old_haloes(gpu):
     assert got[0] == [2] and got[1] == [1] and got[2] == [3]
 
 
-def test_many_haloes_two_pass_pair_sort(gpu):
-    # 2^17 new and 2^16 + 10 old haloes: 17 + 17 bits do not fit one 32-bit pair key
+def test_many_haloes(gpu):
+    # 2^17 one-member new haloes, each the continuation of one of 2^16 + 10 old haloes
     from pyhalma_amd import engine
     n_old = 2 ** 16 + 10
     n_new = 2 ** 17
@@ -94,3 +94,14 @@ def test_empty_catalogues(gpu):
     g = _groups([2, 1])
     out = engine.progenitors(g, np.array([4, 5, 6]), [1.0, 2.0], [], np.zeros(0), [])
     assert all((a == 0).all() for a in out)
+
+
+def test_more_progenitors_than_the_lds_set(gpu):
+    # one new halo made of 3000 one-member old haloes (beyond the 2048 distinct old haloes
+    # a block keeps in LDS), beside ordinary ones
+    rng = np.random.default_rng(9)
+    old = [np.array([i]) for i in range(3000)] + [np.array([5000, 5001]), np.array([6000])]
+    omm = rng.uniform(1.0, 10.0, len(old))
+    new = [np.arange(3000)[::-1].copy(), np.array([5000, 6000]), np.array([7000])]
+    got, want = _both(new, [20000.0, 3.0, 1.0], old, omm)
+    assert got == want and got[2][0] == 3000
