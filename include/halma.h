@@ -241,6 +241,27 @@ int halma_clean_decide(int64_t n_set, const int32_t *set_members, const int32_t 
                        uint8_t *control, int32_t *cleaned, void *workspace,
                        size_t workspace_bytes, void *stream);
 
+/* Particles within a sphere, many spheres at once: the DM and star KD-tree queries of
+ * st_gas_dm_particles_inside (halo_gas.py:255, :269, query_ball_point: distance <= R,
+ * strict = 0) or parallel_inside (halo_gas.py:213-215, sqrt(d2) < R, strict = 1), and
+ * the dark-matter mass inside a sphere (pyHALMA.py:1172-1185).  Points x, y, z (device,
+ * pos_dtype) are binned in cells of side `cell` (<= 0: chosen from the bounding box;
+ * coarsened when the grid would exceed 4 cells per point); centres (device, 3 per sphere,
+ * FP64), radii (device FP64).  Outputs: offsets (device, n_centres+1) and indices (device;
+ * NULL to count only, max_found is then not checked): the indices of the points of sphere h at offsets[h]..,
+ * ascending when `sorted`; mass_sums (device FP64 per sphere, may be NULL): the
+ * double-double sum of m over each sphere's points; *n_found (HOST) the total.  Returns
+ * HALMA_E_LIMIT (with *n_found set) when the total exceeds max_found, so a caller can
+ * size the workspace again.  FP64 predicate (dx^2 + dy^2) + dz^2 against R^2.
+ * Synchronises the stream. */
+size_t halma_ball_query_workspace_size(int64_t n_points, int64_t n_centres, int64_t max_found);
+int halma_ball_query(int64_t n_points, const void *x, const void *y, const void *z,
+                     int pos_dtype, const void *m, int mass_dtype, int64_t n_centres,
+                     const double *centres, const double *radii, int strict, double cell,
+                     int sorted, int64_t max_found, int64_t *offsets, int32_t *indices,
+                     double *mass_sums, int64_t *n_found, void *workspace,
+                     size_t workspace_bytes, void *stream);
+
 /* Progenitor matching (the merger section of pyHALMA.py:1211-1266, main_progenitors_finder
  * :1219-1257) for every new halo at once.  New haloes: CSR new_offsets (device, n_new+1,
  * new_offsets[0] = 0) over new_ids (device int64, the particle IDs of each halo's

@@ -65,6 +65,10 @@ SIGNATURES = {
     "halma_half_mass_radius_workspace_size": (_size, [_i64]),
     "halma_half_mass_radius": (_c.c_int, [_i64, _i64, _p, _p, _p, _p, _p, _c.c_int, _p,
                                           _c.c_int, _p, _p, _p, _size, _p]),
+    "halma_ball_query_workspace_size": (_size, [_i64, _i64, _i64]),
+    "halma_ball_query": (_c.c_int, [_i64, _p, _p, _p, _c.c_int, _p, _c.c_int, _i64, _p, _p,
+                                    _c.c_int, _dbl, _c.c_int, _i64, _p, _p, _p, _i64p, _p,
+                                    _size, _p]),
     "halma_progenitors_workspace_size": (_size, [_i64, _i64, _i64, _i64]),
     "halma_progenitors": (_c.c_int, [_i64, _i64, _p, _p, _p, _i64, _i64, _p, _p, _p, _p, _p,
                                      _p, _p, _p, _size, _p]),
@@ -98,6 +102,7 @@ TUNE_SYM_MAX = 6
 
 F32 = 0
 F64 = 1
+E_LIMIT = -3
 GROUP_FIELDS = 16
 UNBIND_GRAPHS = 1
 UNBIND_TIME_POT = 2

@@ -809,6 +809,46 @@ def progenitors(new_groups, oripa, masses, old_npart, old_oripas, omm):
     return tuple(r.cpu().numpy() for r in res)
 
 
+def ball_query(x, y, z, centres, radii, strict=False, m=None, cell=0.0, sorted=True,
+               count_only=False):
+    """The points within radii[h] of centres[h] for every sphere h (halma_ball_query): the
+    KD-tree query_ball_point of halo_gas.py:255,269 and pyHALMA.py:1179 (distance <= R),
+    or parallel_inside of halo_gas.py:213-215 (``strict``: distance < R).  Returns
+    (offsets (numpy int64, H+1), indices (int32 device tensor, ascending per sphere when
+    ``sorted``; None when ``count_only``), mass sums (numpy FP64 per sphere, or None
+    without ``m``))."""
+    from .device import dtype_flag
+    L = _lib_()
+    x, y, z = _coords(x, y, z)
+    dev = x.device
+    c = to_dev(np.asarray(centres, np.float64).reshape(-1, 3), np.float64)
+    H = int(c.shape[0])
+    r = to_dev(np.asarray(radii, np.float64).reshape(H), np.float64)
+    n = int(x.numel())
+    mm = _vec(m) if m is not None else None
+    sums = torch.zeros(max(H, 1), dtype=torch.float64, device=dev) if m is not None else None
+    off = torch.zeros(H + 1, dtype=torch.int64, device=dev)
+    found = ctypes_i64()
+    max_found = 0 if count_only else max(n, 1)
+    for _ in range(2):
+        idx = None if count_only else torch.empty(max(max_found, 1), dtype=torch.int32,
+                                                  device=dev)
+        ws = workspace(L.halma_ball_query_workspace_size(n, H, max_found))
+        rc = L.halma_ball_query(n, ptr(x), ptr(y), ptr(z), dtype_flag(x), ptr(mm),
+                                dtype_flag(mm) if mm is not None else _lib.F64, H, ptr(c), ptr(r),
+                                int(bool(strict)), float(cell), int(bool(sorted)), max_found,
+                                ptr(off), ptr(idx), ptr(sums), found, ptr(ws), ws.numel(),
+                                stream_ptr())
+        if rc == _lib.E_LIMIT and not count_only and found.value > max_found:
+            max_found = int(found.value)  # more hits than points: size the output again
+            continue
+        _lib.check(rc, "halma_ball_query")
+        break
+    F = int(found.value)
+    return (off.cpu().numpy(), None if count_only else idx[:F],
+            sums[:H].cpu().numpy() if sums is not None else None)
+
+
 class UnbindResult:
     def __init__(self, bound, iters, vcm, pot, stats):
         self.bound = bound      # uint8 device tensor, member order
