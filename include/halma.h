@@ -243,7 +243,7 @@ int halma_clean_decide(int64_t n_set, const int32_t *set_members, const int32_t 
 
 /* Particles within a sphere, many spheres at once: the DM and star KD-tree queries of
  * st_gas_dm_particles_inside (halo_gas.py:255, :269, query_ball_point: distance <= R,
- * strict = 0) or parallel_inside (halo_gas.py:213-215, sqrt(d2) < R, strict = 1), and
+ * strict = 0) or parallel_inside (halo_gas.py:218-220, sqrt(d2) < R, strict = 1), and
  * the dark-matter mass inside a sphere (pyHALMA.py:1172-1185).  Points x, y, z (device,
  * pos_dtype) are binned in cells of side `cell` (<= 0: chosen from the bounding box;
  * coarsened when the grid would exceed 4 cells per point); centres (device, 3 per sphere,
@@ -261,6 +261,35 @@ int halma_ball_query(int64_t n_points, const void *x, const void *y, const void 
                      int sorted, int64_t max_found, int64_t *offsets, int32_t *indices,
                      double *mass_sums, int64_t *n_found, void *workspace,
                      size_t workspace_bytes, void *stream);
+
+/* Gas particles from AMR cells for many queries (halo_gas.py:9-52 patch_to_particles,
+ * :56-141 AMRgrid_to_particles, and the sphere test and mass scaling of :236-246).
+ * Patches (device arrays, n_patches each): level, nx, ny, nz, rx, ry, rz (the centre of the
+ * patch's first parent cell), cell_offsets (n_patches+1: patch p's cells at
+ * cell_offsets[p] + (ix ny + iy) nz + iz); per cell (device): delta, vx, vy, vz, temp
+ * (gas_dtype) and the cr0amr / solapst flags (uint8).  res / res3 (HOST, n_levels): the
+ * cell side (L/ncoarse)/2^l of each level and its cube as the caller computes it.  For
+ * query h (boxes: device FP64, 6 each: x lo, x hi, y lo, y hi, z lo, z hi) every cell of a
+ * level >= 1 patch whose centre x0 + ix res (x0 = rx - res/2) is strictly inside the box
+ * per axis, with cr0amr and solapst set -- and, when centres (device FP64, 3 each) is not
+ * NULL, sqrt((dx^2 + dy^2) + dz^2) < radii[h] -- gives a particle: x, y, z,
+ * v * vel_scale (3), (1 + delta) rho_B res3 mass_scale, temp, written to out (device FP64,
+ * 8 arrays of max_found: quantity q of particle k at out[q max_found + k]) in the
+ * reference's order (patches ascending, then ix, iy, iz) at offsets[h] (device,
+ * n_queries+1).  *n_found (HOST) the total; HALMA_E_LIMIT when it exceeds max_found (out
+ * NULL: count only).  Synchronises the stream. */
+size_t halma_amr_particles_workspace_size(int64_t n_queries);
+int halma_amr_particles(int64_t n_patches, const int32_t *level, const int32_t *nx,
+                        const int32_t *ny, const int32_t *nz, const double *rx,
+                        const double *ry, const double *rz, const int64_t *cell_offsets,
+                        const void *delta, const void *vx, const void *vy, const void *vz,
+                        const void *temp, int gas_dtype, const uint8_t *cr0amr,
+                        const uint8_t *solapst, int n_levels, const double *res,
+                        const double *res3, double rho_B, double mass_scale,
+                        double vel_scale, int64_t n_queries, const double *boxes,
+                        const double *centres, const double *radii, int64_t max_found,
+                        int64_t *offsets, double *out, int64_t *n_found, void *workspace,
+                        size_t workspace_bytes, void *stream);
 
 /* Progenitor matching (the merger section of pyHALMA.py:1211-1266, main_progenitors_finder
  * :1219-1257) for every new halo at once.  New haloes: CSR new_offsets (device, n_new+1,

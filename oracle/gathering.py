@@ -1,7 +1,7 @@
 """Source gathering oracle -- test infrastructure only.
 
 Restates the source gathering of the gas path (halo_gas.py:9-52 patch_to_particles,
-:56-141 AMRgrid_to_particles, :213-215 parallel_inside, :218-277
+:56-141 AMRgrid_to_particles, :218-220 parallel_inside, :223-277
 st_gas_dm_particles_inside) and the dark-matter mass inside a sphere of
 pyHALMA.py:1172-1185:
 
@@ -16,6 +16,9 @@ pyHALMA.py:1172-1185:
     used here as the oracle, with a brute-force check beside it; the indices are
     returned in ascending order (the KD-tree's own order for a single point is
     unspecified).
+
+``res ** 3`` is Python's float power; the reference's compiled ``patch_res**3`` may round
+its last bit differently (masses agree to 1 ulp there, parity unpinned below that).
 
 The reference selects the patches with masclet_framework's which_patches_inside_box
 and numbers their levels with create_vector_levels (masclet_framework, unpinned, not
@@ -50,7 +53,7 @@ def patch_to_particles(res, rx, ry, rz, nx, ny, nz, delta, cr0amr, solapst, vx, 
                 if cr0amr[ix, iy, iz] and solapst[ix, iy, iz]:
                     vals = (x, y, z, float(vx[ix, iy, iz]), float(vy[ix, iy, iz]),
                             float(vz[ix, iy, iz]),
-                            (1 + float(delta[ix, iy, iz])) * rho_B * (res * res * res),
+                            (1 + float(delta[ix, iy, iz])) * rho_B * res ** 3,
                             float(temp[ix, iy, iz]))
                     for a, v in zip(out, vals):
                         a.append(v)
@@ -80,7 +83,7 @@ def amr_to_particles(L, ncoarse, levels, rx, ry, rz, gas, R, cx, cy, cz, rho_B):
 
 
 def inside(x, y, z, R, cx, cy, cz):
-    """halo_gas.py:213-215."""
+    """halo_gas.py:218-220."""
     return np.sqrt((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2) < R
 
 
@@ -102,7 +105,7 @@ def ball_brute(x, y, z, cx, cy, cz, R):
 
 def gather_sources(rete, L, ncoarse, levels, rx, ry, rz, gas, dm, st, cx, cy, cz, R, rho_B,
                    mass_to_sun):
-    """halo_gas.py:218-277 for one halo: (gas x, y, z, vx, vy, vz, mass, temp, dm x, y, z,
+    """halo_gas.py:223-277 for one halo: (gas x, y, z, vx, vy, vz, mass, temp, dm x, y, z,
     mass, st x, y, z, mass, oripa); dm = (x, y, z, mass), st = (x, y, z, mass, oripa) in
     code units as masclet reads them."""
     g = amr_to_particles(L, ncoarse, levels, rx, ry, rz, gas, R, cx, cy, cz, rho_B)

@@ -69,6 +69,10 @@ SIGNATURES = {
     "halma_ball_query": (_c.c_int, [_i64, _p, _p, _p, _c.c_int, _p, _c.c_int, _i64, _p, _p,
                                     _c.c_int, _dbl, _c.c_int, _i64, _p, _p, _p, _i64p, _p,
                                     _size, _p]),
+    "halma_amr_particles_workspace_size": (_size, [_i64]),
+    "halma_amr_particles": (_c.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                       _c.c_int, _p, _p, _c.c_int, _p, _p, _dbl, _dbl, _dbl,
+                                       _i64, _p, _p, _p, _i64, _p, _p, _i64p, _p, _size, _p]),
     "halma_progenitors_workspace_size": (_size, [_i64, _i64, _i64, _i64]),
     "halma_progenitors": (_c.c_int, [_i64, _i64, _p, _p, _p, _i64, _i64, _p, _p, _p, _p, _p,
                                      _p, _p, _p, _size, _p]),

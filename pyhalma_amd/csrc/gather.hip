@@ -1,5 +1,5 @@
 // Source gathering for many haloes at once: the particles within a radius of each centre
-// (halo_gas.py:218-277 queries its KD-trees one halo at a time, pyHALMA.py:1172-1185 sums
+// (halo_gas.py:223-277 queries its KD-trees one halo at a time, pyHALMA.py:1172-1185 sums
 // the dark-matter mass inside a sphere the same way).
 //
 // The points are binned once into a uniform grid (cell keys (iz ny + iy) nx + ix, stable
@@ -12,7 +12,7 @@
 // pair of stable radix passes puts every centre's indices in ascending order.
 //
 // Predicates: (dx^2 + dy^2) + dz^2 <= R^2 (the KD-tree's query_ball_point) or
-// sqrt((dx^2 + dy^2) + dz^2) < R (parallel_inside, halo_gas.py:213-215), in FP64.
+// sqrt((dx^2 + dy^2) + dz^2) < R (parallel_inside, halo_gas.py:218-220), in FP64.
 #include <math.h>
 
 #include <algorithm>
@@ -379,6 +379,249 @@ BallWs carve_ball(void* base, int64_t n, int64_t H, int64_t max_found) {
     return w;
 }
 
+// ---------------------------------------------------------------- AMR gas cells ----
+// Gas particles from AMR cells (halo_gas.py:9-141, with the sphere test and mass scaling of
+// :236-246 when asked): for every query, every level >= 1 patch, the cells whose centre
+// x0 + ix res (x0 = rx - res/2) lies strictly inside the query's box per axis, that are
+// neither refined nor overlapped (cr0amr and solapst) and, with a sphere, lie within
+// sqrt(d2) < R.  One block per halo culls the patches
+// 256 at a time (a box-overlap test, widened by a cell against rounding), then walks the
+// candidate patches in order, an (ix, iy) row of cells per wavefront, lanes along iz --
+// the reference's loop order, kept by the row-batch scan of the emitting kernel.
+constexpr int kMaxLevels = 40;
+
+struct AmrPatches {
+    int64_t P;
+    const int32_t* level;
+    const int32_t* nx;
+    const int32_t* ny;
+    const int32_t* nz;
+    const double* rx;
+    const double* ry;
+    const double* rz;
+    const int64_t* cell_off;
+    double res[kMaxLevels];   // cell side per level
+    double res3[kMaxLevels];  // its cube, as the caller computes it
+    int n_levels;
+};
+
+struct AmrCells {
+    const void* delta;
+    const void* vx;
+    const void* vy;
+    const void* vz;
+    const void* temp;
+    int t;
+    const uint8_t* cr0amr;
+    const uint8_t* solapst;
+    double rho_B;
+    double mass_scale;  // rete^3 after the sphere test, 1 without
+    double vel_scale;   // 3e5 (c = 1 units to km/s), 1 for the raw patch values
+};
+
+struct PatchClip {
+    int64_t ix0, ix1, iy0, iy1, iz0, iz1, base;
+    int32_t ny, nz;
+    double x0, y0, z0, res, res3;
+    __device__ int64_t rows() const {
+        return (ix1 < ix0 || iy1 < iy0 || iz1 < iz0) ? 0 : (ix1 - ix0 + 1) * (iy1 - iy0 + 1);
+    }
+};
+
+// cell indices k with x0 + k res possibly strictly inside (lo, hi), widened by one
+// (k0 > k1 when none; non-finite bounds give none)
+__device__ __forceinline__ void clip_axis(double x0, double res, int32_t n, double lo, double hi,
+                                          int64_t& k0, int64_t& k1) {
+    const double a = ceil((lo - x0) / res) - 1.0;
+    const double b = floor((hi - x0) / res) + 1.0;
+    const double top = static_cast<double>(n - 1);
+    if (!isfinite(a) || !isfinite(b) || n < 1 || b < 0.0 || a > top) {
+        k0 = 1;
+        k1 = 0;
+        return;
+    }
+    k0 = a < 0.0 ? 0 : static_cast<int64_t>(a);
+    k1 = b > top ? n - 1 : static_cast<int64_t>(b);
+}
+
+__device__ PatchClip clip_patch(const AmrPatches& pa, int64_t p, const double* box) {
+    PatchClip c;
+    c.ix0 = 1;
+    c.ix1 = 0;
+    c.iy0 = c.iz0 = 0;
+    c.iy1 = c.iz1 = 0;
+    const int l = pa.level[p];
+    if (l < 1 || l >= pa.n_levels || !(pa.res[l] > 0.0)) return c;
+    c.res = pa.res[l];
+    c.res3 = pa.res3[l];
+    c.x0 = pa.rx[p] - c.res / 2;
+    c.y0 = pa.ry[p] - c.res / 2;
+    c.z0 = pa.rz[p] - c.res / 2;
+    c.ny = pa.ny[p];
+    c.nz = pa.nz[p];
+    c.base = pa.cell_off[p];
+    clip_axis(c.x0, c.res, pa.nx[p], box[0], box[1], c.ix0, c.ix1);
+    clip_axis(c.y0, c.res, c.ny, box[2], box[3], c.iy0, c.iy1);
+    clip_axis(c.z0, c.res, c.nz, box[4], box[5], c.iz0, c.iz1);
+    return c;
+}
+
+__device__ __forceinline__ bool amr_hit(const PatchClip& c, const AmrCells& g, const double* box,
+                                        const double* cen, double R, int64_t ix, int64_t iy,
+                                        int64_t iz, int64_t& cell, double& x, double& y,
+                                        double& z) {  // cen null: the box alone
+    x = c.x0 + static_cast<double>(ix) * c.res;
+    y = c.y0 + static_cast<double>(iy) * c.res;
+    z = c.z0 + static_cast<double>(iz) * c.res;
+    if (!(x > box[0] && x < box[1] && y > box[2] && y < box[3] && z > box[4] && z < box[5]))
+        return false;
+    cell = c.base + (ix * c.ny + iy) * c.nz + iz;
+    if (!(g.cr0amr[cell] && g.solapst[cell])) return false;
+    if (!cen) return true;
+    const double dx = x - cen[0], dy = y - cen[1], dz = z - cen[2];
+    return sqrt((dx * dx + dy * dy) + dz * dz) < R;
+}
+
+// the patches of a halo's box, in order, kThreads at a time into an LDS list
+template <typename F>
+__device__ void for_candidate_patches(const AmrPatches& pa, const double* box, int32_t* list,
+                                      int* nlist, F&& f) {
+    for (int64_t p0 = 0; p0 < pa.P; p0 += blockDim.x) {
+        const int64_t p = p0 + threadIdx.x;
+        const bool cand = p < pa.P && clip_patch(pa, p, box).rows() > 0;
+        const unsigned long long mask = __ballot(cand);
+        const int w = threadIdx.x / kWave, lane = __lane_id();
+        __shared__ int wcnt[kWavesPerBlock];
+        if (lane == 0) wcnt[w] = __popcll(mask);
+        __syncthreads();
+        int base = 0;
+        for (int v = 0; v < w; ++v) base += wcnt[v];
+        if (cand) {
+            const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+            list[base + __popcll(mask & below)] = static_cast<int32_t>(p);
+        }
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int v = 0; v < kWavesPerBlock; ++v) t += wcnt[v];
+            *nlist = t;
+        }
+        __syncthreads();
+        const int n = *nlist;
+        for (int i = 0; i < n; ++i) f(static_cast<int64_t>(list[i]));
+        __syncthreads();
+    }
+}
+
+struct AmrHalo {
+    const double* box;     // 6 per query: x lo, x hi, y lo, y hi, z lo, z hi
+    const double* centre;  // 3 per query, null: no sphere test
+    const double* radius;
+};
+
+__global__ __launch_bounds__(kThreads) void k_amr_count(int64_t H, AmrPatches pa, AmrCells g,
+                                                        AmrHalo hq, int64_t* __restrict__ count) {
+    __shared__ int32_t list[kThreads];
+    __shared__ int nlist;
+    __shared__ int64_t shc[kWavesPerBlock];
+    const int w = threadIdx.x / kWave;
+    for (int64_t h = blockIdx.x; h < H; h += gridDim.x) {
+        const double* cen = hq.centre ? hq.centre + 3 * h : nullptr;
+        const double R = hq.centre ? hq.radius[h] : 0.0;
+        const double* box = hq.box + 6 * h;
+        int64_t cnt = 0;
+        if (!cen || R >= 0.0)
+            for_candidate_patches(pa, box, list, &nlist, [&](int64_t p) {
+                const PatchClip c = clip_patch(pa, p, box);
+                const int64_t niy = c.iy1 - c.iy0 + 1;
+                for (int64_t r = w; r < c.rows(); r += kWavesPerBlock) {
+                    const int64_t ix = c.ix0 + r / niy, iy = c.iy0 + r % niy;
+                    for (int64_t iz = c.iz0 + __lane_id(); iz <= c.iz1; iz += kWave) {
+                        int64_t cell;
+                        double x, y, z;
+                        if (amr_hit(c, g, box, cen, R, ix, iy, iz, cell, x, y, z)) ++cnt;
+                    }
+                }
+            });
+        for (int s = kWave / 2; s >= 1; s >>= 1) cnt += __shfl_down(cnt, s, kWave);
+        if (__lane_id() == 0) shc[w] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t t = 0;
+            for (int v = 0; v < kWavesPerBlock; ++v) t += shc[v];
+            count[h] = t;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_amr_emit(int64_t H, AmrPatches pa, AmrCells g,
+                                                       AmrHalo hq, const int64_t* __restrict__ off,
+                                                       int64_t stride, double* __restrict__ out) {
+    __shared__ int32_t list[kThreads];
+    __shared__ int nlist;
+    __shared__ int64_t shc[kWavesPerBlock];
+    const int w = threadIdx.x / kWave;
+    const int lane = __lane_id();
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+    for (int64_t h = blockIdx.x; h < H; h += gridDim.x) {
+        const double* cen = hq.centre ? hq.centre + 3 * h : nullptr;
+        const double R = hq.centre ? hq.radius[h] : 0.0;
+        const double* box = hq.box + 6 * h;
+        int64_t running = off[h];
+        if (cen && !(R >= 0.0)) continue;  // block-uniform
+        for_candidate_patches(pa, box, list, &nlist, [&](int64_t p) {
+            const PatchClip c = clip_patch(pa, p, box);
+            const int64_t niy = c.iy1 - c.iy0 + 1;
+            const int64_t nrows = c.rows();
+            for (int64_t r0 = 0; r0 < nrows; r0 += kWavesPerBlock) {
+                const int64_t r = r0 + w;
+                const bool live = r < nrows;
+                const int64_t ix = live ? c.ix0 + r / niy : 0, iy = live ? c.iy0 + r % niy : 0;
+                int64_t cnt = 0;
+                if (live)
+                    for (int64_t z0 = c.iz0; z0 <= c.iz1; z0 += kWave) {
+                        const int64_t iz = z0 + lane;
+                        int64_t cell;
+                        double x, y, z;
+                        const bool hit = iz <= c.iz1 && amr_hit(c, g, box, cen, R, ix, iy, iz,
+                                                                cell, x, y, z);
+                        cnt += __popcll(__ballot(hit));
+                    }
+                if (lane == 0) shc[w] = cnt;
+                __syncthreads();
+                int64_t base = running;
+                for (int v = 0; v < w; ++v) base += shc[v];
+                int64_t total = 0;
+                for (int v = 0; v < kWavesPerBlock; ++v) total += shc[v];
+                if (live)
+                    for (int64_t z0 = c.iz0; z0 <= c.iz1; z0 += kWave) {
+                        const int64_t iz = z0 + lane;
+                        int64_t cell = 0;
+                        double x = 0, y = 0, z = 0;
+                        const bool hit = iz <= c.iz1 && amr_hit(c, g, box, cen, R, ix, iy, iz,
+                                                                cell, x, y, z);
+                        const unsigned long long mask = __ballot(hit);
+                        if (hit) {
+                            const int64_t o = base + __popcll(mask & below);
+                            out[o] = x;
+                            out[stride + o] = y;
+                            out[2 * stride + o] = z;
+                            out[3 * stride + o] = ldf(g.vx, g.t, cell) * g.vel_scale;
+                            out[4 * stride + o] = ldf(g.vy, g.t, cell) * g.vel_scale;
+                            out[5 * stride + o] = ldf(g.vz, g.t, cell) * g.vel_scale;
+                            out[6 * stride + o] = (1.0 + ldf(g.delta, g.t, cell)) * g.rho_B *
+                                                  c.res3 * g.mass_scale;
+                            out[7 * stride + o] = ldf(g.temp, g.t, cell);
+                        }
+                        base += __popcll(mask);
+                    }
+                running += total;
+                __syncthreads();
+            }
+        });
+    }
+}
+
 }  // namespace
 
 }  // namespace halma
@@ -500,6 +743,80 @@ int halma_ball_query(int64_t n_points, const void* x, const void* y, const void*
         k_gather_i32<<<blocks_for(F), kThreads, 0, st>>>(w.raw, w.p1, F, indices);
         HALMA_LAUNCH_CHECK();
     }
+    return 0;
+}
+
+size_t halma_amr_particles_workspace_size(int64_t n_haloes) {
+    if (n_haloes < 0) return 0;
+    Carver c(nullptr);
+    c.take<int64_t>(n_haloes + 1);
+    c.take<int64_t>(2);
+    c.take<char>(scan_workspace_bytes(n_haloes + 1));
+    return c.used;
+}
+
+int halma_amr_particles(int64_t n_patches, const int32_t* level, const int32_t* nx,
+                        const int32_t* ny, const int32_t* nz, const double* rx, const double* ry,
+                        const double* rz, const int64_t* cell_offsets, const void* delta,
+                        const void* vx, const void* vy, const void* vz, const void* temp,
+                        int gas_dtype, const uint8_t* cr0amr, const uint8_t* solapst,
+                        int n_levels, const double* res, const double* res3, double rho_B,
+                        double mass_scale, double vel_scale, int64_t n_haloes,
+                        const double* boxes, const double* centres, const double* radii,
+                        int64_t max_found, int64_t* offsets, double* out, int64_t* n_found,
+                        void* ws, size_t ws_bytes, void* stream) {
+    if (n_patches < 0 || n_haloes < 0 || max_found < 0 || !n_found || n_levels < 0 ||
+        n_levels > kMaxLevels)
+        return set_error(HALMA_E_ARG, "bad argument (at most %d levels)", kMaxLevels);
+    *n_found = 0;
+    if (n_haloes == 0) return 0;
+    if (!boxes || (centres && !radii) || !offsets || (n_levels > 0 && (!res || !res3)) ||
+        (n_patches > 0 && (!level || !nx || !ny || !nz || !rx || !ry || !rz || !cell_offsets ||
+                           !delta || !vx || !vy || !vz || !temp || !cr0amr || !solapst)))
+        return set_error(HALMA_E_ARG, "null pointer");
+    if (gas_dtype != HALMA_F32 && gas_dtype != HALMA_F64) return set_error(HALMA_E_ARG, "bad dtype");
+    if (n_patches > INT32_MAX || n_haloes > INT32_MAX)
+        return set_error(HALMA_E_LIMIT, "at most 2^31-1 patches and haloes");
+    Carver cv(ws);
+    int64_t* count = cv.take<int64_t>(n_haloes + 1);
+    int64_t* total = cv.take<int64_t>(2);
+    void* sws = cv.take<char>(scan_workspace_bytes(n_haloes + 1));
+    if (ws_bytes < cv.used)
+        return set_error(HALMA_E_WORKSPACE, "workspace %zu < required %zu", ws_bytes, cv.used);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    AmrPatches pa{};
+    pa.P = n_patches;
+    pa.level = level;
+    pa.nx = nx;
+    pa.ny = ny;
+    pa.nz = nz;
+    pa.rx = rx;
+    pa.ry = ry;
+    pa.rz = rz;
+    pa.cell_off = cell_offsets;
+    pa.n_levels = n_levels;
+    for (int l = 0; l < n_levels; ++l) {
+        pa.res[l] = res[l];
+        pa.res3[l] = res3[l];
+    }
+    AmrCells g{delta, vx, vy, vz, temp, gas_dtype, cr0amr, solapst, rho_B, mass_scale, vel_scale};
+    AmrHalo hq{boxes, centres, radii};
+    const unsigned hb = static_cast<unsigned>(std::min<int64_t>(n_haloes, 65536));
+    k_amr_count<<<hb, kThreads, 0, st>>>(n_haloes, pa, g, hq, count);
+    HALMA_LAUNCH_CHECK();
+    int rc = exclusive_scan_i64(count, offsets, n_haloes + 1, total, sws, st);
+    if (rc) return rc;
+    HALMA_HIP(hipMemcpyAsync(n_found, offsets + n_haloes, sizeof(int64_t), hipMemcpyDeviceToHost,
+                             st));
+    HALMA_HIP(hipStreamSynchronize(st));
+    const int64_t F = *n_found;
+    if (!out) return 0;  // counting only
+    if (F > max_found)
+        return set_error(HALMA_E_LIMIT, "%lld gas cells found exceed max_found = %lld",
+                         static_cast<long long>(F), static_cast<long long>(max_found));
+    if (F == 0) return 0;
+    k_amr_emit<<<hb, kThreads, 0, st>>>(n_haloes, pa, g, hq, offsets, max_found, out);
+    HALMA_LAUNCH_CHECK();
     return 0;
 }
 

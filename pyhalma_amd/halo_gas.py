@@ -6,6 +6,8 @@ the O(N^2) sums run on the GPU through ``pyhalma_amd.particle`` (libhalma).
   brute_force_binding_energy_fortran          halo_gas.py:164-187
   serial_brute_force_binding_energy_fortran   halo_gas.py:191-215
   parallel_inside                             halo_gas.py:218-220
+  patch_to_particles, AMRgrid_to_particles    halo_gas.py:9-141
+  st_gas_dm_particles_inside                  halo_gas.py:223-277
   RPS                                         halo_gas.py:285-492
   most_bound_particle                         halo_gas.py:498-634
 
@@ -13,9 +15,10 @@ RPS and most_bound_particle keep the reference's sequence of source groups and i
 subsampling above BRUTE_FORCE_LIM (np.random.choice with replacement from the global
 numpy RNG, drawn in the same order), so with a seeded ``np.random`` they draw the same
 samples the reference would.  ``RPS_batched`` is the whole-catalogue form of RPS (every
-halo in one call, every source summed, exact decisions).  The AMR source-gathering functions of the same module
-(patch_to_particles, AMRgrid_to_particles, st_gas_dm_particles_inside) need MASCLET
-grid data and are outside this package.
+halo in one call, every source summed, exact decisions).  The source gathering of the
+same module (patch_to_particles, AMRgrid_to_particles, st_gas_dm_particles_inside,
+halo_gas.py:9-141, :223-277) runs on the GPU through ``pyhalma_amd.gathering``, whose
+``gather_sources`` does it for every halo at once.
 """
 import numpy as np
 
@@ -145,16 +148,54 @@ def most_bound_particle(gas_x, gas_y, gas_z, gas_mass, dm_x, dm_y, dm_z, dm_mass
     return st_x[k], st_y[k], st_z[k], st_oripa[k]
 
 
-def patch_to_particles(*args, **kwargs):
-    raise NotImplementedError("AMR source gathering (halo_gas.py:9-52) needs MASCLET data")
+_GAS_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "mass", "temp")
 
 
-def AMRgrid_to_particles(*args, **kwargs):
-    raise NotImplementedError("AMR source gathering (halo_gas.py:56-141) needs MASCLET data")
+def patch_to_particles(patch_res, patch_rx, patch_ry, patch_rz, patch_nx, patch_ny, patch_nz,
+                       patch_delta, patch_cr0amr, patch_solapst, patch_vx, patch_vy, patch_vz,
+                       patch_temp, box, rho_B):
+    """halo_gas.py:9-52 on the GPU: the (x, y, z, vx, vy, vz, mass, temp) arrays of the
+    patch's unrefined, non-overlapped cells strictly inside the box."""
+    from .gathering import AmrGrid, amr_particles
+    nx, ny, nz = int(patch_nx), int(patch_ny), int(patch_nz)
+    cells = [[np.asarray(a)[:nx, :ny, :nz]] for a in (patch_delta, patch_vx, patch_vy,
+                                                       patch_vz, patch_temp, patch_cr0amr,
+                                                       patch_solapst)]
+    # one level-1 patch whose cell side is patch_res: L / ncoarse = 2 patch_res
+    grid = AmrGrid(2.0 * float(patch_res), 1, [1], [nx], [ny], [nz], [patch_rx], [patch_ry],
+                   [patch_rz], cells)
+    _, g = amr_particles(grid, rho_B, [np.asarray(box, np.float64)], vel_scale=1.0)
+    return tuple(g[k].cpu().numpy() for k in _GAS_FIELDS)
 
 
-def st_gas_dm_particles_inside(*args, **kwargs):
-    raise NotImplementedError("source gathering (halo_gas.py:223-277) needs MASCLET data")
+def AMRgrid_to_particles(L, ncoarse, grid_data, gas_data, Rrps, cx, cy, cz, rho_B):
+    """halo_gas.py:56-141 on the GPU (the snapshot is uploaded once per grid_data /
+    gas_data pair, see ``pyhalma_amd.gathering``)."""
+    from .gathering import amr_particles, cached_grid
+    grid = cached_grid(L, ncoarse, grid_data, gas_data)
+    box = np.array([cx - Rrps, cx + Rrps, cy - Rrps, cy + Rrps, cz - Rrps, cz + Rrps])
+    _, g = amr_particles(grid, rho_B, [box])
+    return tuple(g[k].cpu().numpy() for k in _GAS_FIELDS)
+
+
+def st_gas_dm_particles_inside(rete, L, ncoarse, grid_data, gas_data, masclet_dm_data,
+                               masclet_st_data, st_kdtree, dm_kdtree, cx, cy, cz, R, rho_B):
+    """halo_gas.py:223-277 on the GPU.  The KD-trees are not used: the dark matter and
+    stars within R come from a sphere query of masclet_dm_data / masclet_st_data (same
+    set; indices in ascending order, where the KD-tree leaves the order open).  Masses
+    in Msun with masclet's mass unit ``gathering.MASS_TO_SUN``."""
+    from .gathering import cached_grid, gather_sources
+    grid = cached_grid(L, ncoarse, grid_data, gas_data)
+    dm = tuple(masclet_dm_data[k] for k in (0, 1, 2, 3))
+    st = tuple(masclet_st_data[k] for k in (0, 1, 2, 6, 9))
+    out = gather_sources(grid, rho_B, rete, dm, st, [[cx, cy, cz]], [R])
+    gas = tuple(out["gas"][k].cpu().numpy() for k in _GAS_FIELDS)
+    di = out["dm_index"].cpu().numpy()
+    si = out["st_index"].cpu().numpy()
+    dm_xyz = tuple(np.asarray(masclet_dm_data[k])[di] for k in (0, 1, 2))
+    st_xyz = tuple(np.asarray(masclet_st_data[k])[si] for k in (0, 1, 2))
+    return (gas + dm_xyz + (out["dm_mass"].cpu().numpy(),) + st_xyz
+            + (out["st_mass"].cpu().numpy(), out["st_oripa"].cpu().numpy()))
 
 
 def _source_csr(groups, H):
