@@ -55,12 +55,12 @@ def parse():
                     help="fraction of the box (along x) used for the CPU baseline sample")
     ap.add_argument("--workload", default="clumps",
                     choices=["clumps", "gas", "bridged", "centring", "substructure",
-                             "cleaning", "progenitors"],
+                             "cleaning", "progenitors", "sources"],
                     help="clumps (default): the headline 2.1M-particle step; the others are "
                          "the gas batch, the bridged-pair FoF stress case, the "
                          "potential-minimum centring, the substructure splitting of the "
-                         "bridged pairs, the phase-space cleaning and the progenitor "
-                         "matching of the clumps at their full sizes")
+                         "bridged pairs, and the phase-space cleaning, progenitor "
+                         "matching and source gathering of the clumps at full size")
     return ap.parse_args()
 
 
@@ -85,7 +85,8 @@ def main():
         out = {"gas": bench_gas, "bridged": bench_bridged, "centring": bench_centring,
                "substructure": bench_substructure,
                "cleaning": bench_cleaning,
-               "progenitors": bench_progenitors}[args.workload](args)
+               "progenitors": bench_progenitors,
+               "sources": bench_sources}[args.workload](args)
         print(json.dumps(out))
         return
 
@@ -483,12 +484,14 @@ def bench_cleaning(args):
 
 def bench_progenitors(args):
     """Progenitor matching (pyHALMA.py:1211-1266) on the headline clump catalogue: the
-    new haloes are the FoF groups at ll (1001 haloes, 1.91M members), the old ones the
-    FoF groups of the same particles at 0.6 ll (fragments: several progenitors per new
-    halo); particle IDs are a random permutation offset by 10^9, masses the groups'
-    summed masses.  One step = halma_progenitors over all new haloes (inputs resident).
-    CPU baseline: the oracle's restatement (np.isin per (new, old) halo pair, as the
-    reference) on the first new haloes."""
+    new haloes are the FoF groups at ll (1001 haloes, 1.91M members).  The old catalogue
+    splits each new halo's members into 1-4 progenitors (Dirichlet fractions), leaves
+    10 % of them out of every old halo, and joins the first progenitor of 5 % of the
+    haloes to their neighbour's (old haloes that break apart); particle IDs are a random
+    permutation offset by 10^9, masses the summed particle masses.  One step =
+    halma_progenitors over all new haloes (inputs resident).  CPU baseline: the oracle's
+    restatement (np.isin per (new, old) halo pair, as the reference) on the first new
+    haloes."""
     import numpy as np
     import torch
 
@@ -503,14 +506,29 @@ def bench_progenitors(args):
     rng = np.random.default_rng(args.seed)
     oripa = rng.permutation(n).astype(np.int64) + 10 ** 9
     _, new = engine.fof_groups(D["x"], D["y"], D["z"], ll, MINP)
-    _, old = engine.fof_groups(D["x"], D["y"], D["z"], 0.6 * ll, MINP)
     ids_d = torch.from_numpy(oripa).to(dev)
     new_mass = engine.group_reduce(new, D["x"], D["y"], D["z"], D["mass"])[:, 0].cpu().numpy()
-    omm = engine.group_reduce(old, D["x"], D["y"], D["z"], D["mass"])[:, 0].cpu().numpy()
-    old_ids = ids_d[old.members[: old.n_members].long()]
-    old_npart = old.sizes()
-    log(f"progenitor workload: {new.n_groups} new haloes ({new.n_members} members), "
-        f"{old.n_groups} old haloes ({old.n_members} members) ({time.time() - t0:.1f}s)")
+    H = new.n_groups
+    mem = new.members[: new.n_members].cpu().numpy().astype(np.int64)
+    halo_of = np.repeat(np.arange(H), new.sizes())
+    k = rng.integers(1, 5, H)
+    frac = rng.dirichlet(np.full(4, 0.7), H) * (np.arange(4)[None, :] < k[:, None])
+    cum = np.cumsum(frac / frac.sum(1, keepdims=True), 1)
+    part = (rng.random(mem.size)[:, None] > cum[halo_of]).sum(1).clip(0, 3)
+    label = halo_of * 4 + part
+    joined = np.nonzero(rng.random(H - 1) < 0.05)[0] + 1  # first progenitor -> neighbour's
+    label[np.isin(halo_of, joined) & (part == 0)] -= 4
+    label[rng.random(mem.size) < 0.1] = -1                 # not in any old halo
+    keep = label >= 0
+    order = np.argsort(label[keep], kind="stable")
+    old_members = mem[keep][order]
+    _, old_npart = np.unique(label[keep], return_counts=True)
+    omm = np.bincount(np.unique(label[keep], return_inverse=True)[1],
+                      weights=w["mass"][mem[keep]].astype(np.float64))
+    old_ids = ids_d[torch.from_numpy(old_members).to(dev)]
+    old_lists = np.split(oripa[old_members], np.cumsum(old_npart)[:-1])
+    log(f"progenitor workload: {H} new haloes ({new.n_members} members), "
+        f"{old_npart.size} old haloes ({old_members.size} members) ({time.time() - t0:.1f}s)")
 
     def step():
         return engine.progenitors(new, ids_d, new_mass, old_npart, old_ids, omm)
@@ -524,7 +542,6 @@ def bench_progenitors(args):
 
         k = 8
         groups = new.to_arrays()[:k]
-        old_lists = [oripa[g] for g in old.to_arrays()]
         tc = time.perf_counter()
         want = oracle.progenitors(groups, oripa, new_mass[:k], old_lists, omm)
         cs = time.perf_counter() - tc
@@ -533,17 +550,105 @@ def bench_progenitors(args):
         extra["cpu_baseline"] = {"value": ns / cs, "unit": "new members/s", "cores": 1,
                                  "kind": "port", "seconds": cs,
                                  "sample": f"the first {k} new haloes ({ns} members) against "
-                                           f"all {old.n_groups} old haloes: the oracle's "
+                                           f"all {old_npart.size} old haloes: the oracle's "
                                            "restatement, np.isin per halo pair"}
         extra["sample_matches_oracle"] = bool(same)
     return _line("new members/s progenitor matching, 2.1M-particle clumps",
                  new.n_members / step_s, "new members/s", step_s, args,
-                 {"workload": "FoF groups at ll (new) against FoF groups at 0.6 ll (old) of "
-                              "1000 Plummer clumps, 2.1M FP32, random int64 IDs",
-                  "new_haloes": new.n_groups, "old_haloes": old.n_groups,
-                  "new_members": new.n_members, "old_members": old.n_members},
+                 {"workload": "FoF groups at ll of 1000 Plummer clumps (2.1M FP32) against "
+                              "an old catalogue of 1-4 progenitors each, 10 % unassigned, 5 % "
+                              "joined to a neighbour; random int64 IDs",
+                  "new_haloes": H, "old_haloes": int(old_npart.size),
+                  "new_members": new.n_members, "old_members": int(old_members.size)},
                  with_progenitor=int((pro1 > 0).sum()), max_mergers=int(nmerg.max()),
+                 merger_types={str(t): int((mtype == t).sum()) for t in np.unique(mtype)},
                  **extra)
+
+
+def bench_sources(args):
+    """Source gathering of the gas path (st_gas_dm_particles_inside, halo_gas.py:223-277)
+    for every halo of the headline clump catalogue at once: the 1001 FoF groups' centres,
+    radius 3 half-mass radii; stars = the 2.1M clump particles, dark matter = 4000 Plummer
+    particles around each group plus a 2M uniform field (6M), gas = a synthetic AMR
+    snapshot of two 16^3 patches (levels 6 and 7) around each group (8.2M cells).  One step
+    = gas particles from the AMR cells in every sphere + the dark matter and stars in
+    every sphere (ascending indices, masses in Msun).  CPU baseline: the oracle's
+    restatement on the first haloes (scipy KD-trees built once, as the reference does;
+    the AMR loops in pure Python, which the reference compiles with numba)."""
+    import numpy as np
+    import torch
+
+    from pyhalma_amd import engine, gathering, synth
+
+    t0 = time.time()
+    w = synth.clumps(seed=args.seed)
+    ll = float(w["ll"])
+    dev = torch.device("cuda", 0)
+    D = {k: torch.from_numpy(w[k]).to(dev) for k in ("x", "y", "z", "mass")}
+    _, groups = engine.fof_groups(D["x"], D["y"], D["z"], ll, MINP)
+    red = engine.group_reduce(groups, D["x"], D["y"], D["z"], D["mass"])
+    r12 = engine.half_mass_radius(groups, D["x"], D["y"], D["z"], D["mass"], red)
+    centres = red[:, 1:4].cpu().numpy()
+    radii = 3.0 * r12.cpu().numpy()
+    H = len(radii)
+    dm = synth.dark_matter_around(centres)
+    amr = synth.amr_around(centres)
+    grid = gathering.AmrGrid(amr["L"], amr["ncoarse"], amr["levels"], amr["nx"], amr["ny"],
+                             amr["nz"], amr["rx"], amr["ry"], amr["rz"], amr["cells"])
+    rng = np.random.default_rng(args.seed)
+    oripa = rng.permutation(len(w["x"])).astype(np.int32)
+    DM = tuple(torch.from_numpy(dm[k]).to(dev) for k in ("x", "y", "z", "mass"))
+    ST = (D["x"], D["y"], D["z"], D["mass"] * 1e-17, torch.from_numpy(oripa).to(dev))
+    rete, rho_B = 0.5, 2.7e10
+    log(f"source workload: {H} haloes, {len(dm['x'])} DM, {len(w['x'])} stars, "
+        f"{grid.n_cells} AMR cells in {grid.n_patches} patches ({time.time() - t0:.1f}s)")
+
+    def step():
+        return gathering.gather_sources(grid, rho_B, rete, DM, ST, centres, radii)
+
+    step_s, out = _timed(step, args.steps, args.warmup)
+    n_gas = int(out["gas_offsets"][-1])
+    n_dm = int(out["dm_offsets"][-1])
+    n_st = int(out["st_offsets"][-1])
+    log(f"sources: {step_s * 1e3:.2f} ms/step; gas {n_gas}, dm {n_dm}, stars {n_st}")
+    extra = {}
+    if not args.no_cpu:
+        from scipy.spatial import KDTree
+
+        import oracle.gathering as og
+
+        k = 10
+        dm_tree = KDTree(np.array((dm["x"], dm["y"], dm["z"])).T.astype(np.float64))
+        st_tree = KDTree(np.array((w["x"], w["y"], w["z"])).T.astype(np.float64))
+        tc = time.perf_counter()
+        same = True
+        for h in range(k):
+            c, R = centres[h], radii[h]
+            g = og.amr_to_particles(amr["L"], amr["ncoarse"], amr["levels"], amr["rx"],
+                                    amr["ry"], amr["rz"], amr["cells"], R, *c, rho_B)
+            keep = og.inside(g[0], g[1], g[2], R, *c)
+            di = sorted(dm_tree.query_ball_point(c, R))
+            si = sorted(st_tree.query_ball_point(c, R))
+            a, b = out["gas_offsets"][h], out["gas_offsets"][h + 1]
+            same &= bool(np.array_equal(out["gas"]["x"][a:b].cpu().numpy(), g[0][keep]))
+            a, b = out["dm_offsets"][h], out["dm_offsets"][h + 1]
+            same &= out["dm_index"][a:b].cpu().tolist() == di
+            a, b = out["st_offsets"][h], out["st_offsets"][h + 1]
+            same &= out["st_index"][a:b].cpu().tolist() == si
+        cs = time.perf_counter() - tc
+        extra["cpu_baseline"] = {"value": k / cs, "unit": "haloes/s", "cores": 1,
+                                 "kind": "port", "seconds": cs,
+                                 "sample": f"the first {k} haloes: AMR cells by the oracle's "
+                                           "pure-Python loops, dark matter and stars by "
+                                           "scipy KD-trees built once (untimed)"}
+        extra["sample_matches_oracle"] = same
+    return _line("haloes/s source gathering (gas from AMR cells, dark matter, stars)",
+                 H / step_s, "haloes/s", step_s, args,
+                 {"workload": "1001 clump haloes, R = 3 half-mass radii; 2.1M stars, 6M dark "
+                              "matter, 8.2M AMR cells (2 patches of 16^3 per halo)",
+                  "haloes": H, "dm_particles": len(dm["x"]), "stars": len(w["x"]),
+                  "amr_cells": grid.n_cells},
+                 gas_found=n_gas, dm_found=n_dm, stars_found=n_st, **extra)
 
 
 def bench_centring(args):

@@ -6,10 +6,11 @@
 // radix sort, packed FP64 copies in cell order, dense cell start offsets), so the points
 // of one (iy, iz) row of cells between two ix are contiguous.  One block per centre walks
 // the rows of the cells that can hold a point within R (the box of the sphere widened by
-// one cell against rounding) with a wavefront per row: a first kernel counts the hits
-// (and sums their masses in double-double), a scan gives each centre its output range,
-// a second kernel writes the hits row by row at wave-scanned offsets, and an optional
-// pair of stable radix passes puts every centre's indices in ascending order.
+// one cell against rounding), each wavefront a contiguous range of rows: a first kernel
+// counts the hits per wavefront (and sums their masses in double-double), a scan gives
+// each centre its output range, a second kernel writes every wavefront's hits after the
+// counts of the wavefronts before it (hits in row order), and an optional pair of stable
+// radix passes puts every centre's indices in ascending order.
 //
 // Predicates: (dx^2 + dy^2) + dz^2 <= R^2 (the KD-tree's query_ball_point) or
 // sqrt((dx^2 + dy^2) + dz^2) < R (parallel_inside, halo_gas.py:218-220), in FP64.
@@ -26,7 +27,7 @@ namespace halma {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRB = 256;                   // reduction blocks of the bounding box
+constexpr int kRB = 2048;                  // reduction blocks of the bounding box
 constexpr int kWavesPerBlock = kThreads / kWave;
 
 inline unsigned blocks_for(int64_t n) {
@@ -143,12 +144,18 @@ __global__ void k_zero_out(int64_t H, int64_t* __restrict__ off, double* __restr
     if (mass && h < H) mass[h] = 0.0;
 }
 
-__global__ void k_pack(Pts p, const int32_t* __restrict__ perm, int64_t n,
-                       double4* __restrict__ pk) {
+// packed FP64 copies in input order (coalesced) ...
+__global__ void k_pack(Pts p, int64_t n, double4* __restrict__ tmp) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tmp[i] = make_double4(ldf(p.x, p.t, i), ldf(p.y, p.t, i), ldf(p.z, p.t, i), 0.0);
+}
+
+// ... then into cell order, one 32-byte record per point
+__global__ void k_pack_sorted(const double4* __restrict__ tmp, const int32_t* __restrict__ perm,
+                              int64_t n, double4* __restrict__ pk) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int64_t i = perm[j];
-    pk[j] = make_double4(ldf(p.x, p.t, i), ldf(p.y, p.t, i), ldf(p.z, p.t, i), 0.0);
+    if (j < n) pk[j] = tmp[perm[j]];
 }
 
 struct Query {
@@ -206,12 +213,22 @@ __device__ __forceinline__ void row_range(const Grid& g, const CellRows& cr, con
     b = start[cell_key(g, cr.ix1, iy, iz) + 1];
 }
 
-// one block per centre: number of hits (and their mass, double-double)
+// the rows of a centre's cell box are split into kWavesPerBlock contiguous ranges, one
+// per wavefront (the same split in both kernels)
+__device__ __forceinline__ void wave_rows(int64_t nrows, int w, int64_t& r0, int64_t& r1) {
+    const int64_t per = (nrows + kWavesPerBlock - 1) / kWavesPerBlock;
+    r0 = w * per < nrows ? w * per : nrows;
+    r1 = r0 + per < nrows ? r0 + per : nrows;
+}
+
+// one block per centre: hits per wavefront's rows (wcount, kWavesPerBlock per centre), their
+// total, and their mass (double-double, fixed order)
 __global__ __launch_bounds__(kThreads) void k_ball_count(int64_t H, Grid g, Query q,
                                                          const int32_t* __restrict__ start,
                                                          const int32_t* __restrict__ perm,
                                                          const double4* __restrict__ pk,
                                                          int64_t* __restrict__ count,
+                                                         int64_t* __restrict__ wcount,
                                                          double* __restrict__ mass) {
     __shared__ int64_t shc[kWavesPerBlock];
     __shared__ double shm[2 * kWavesPerBlock];
@@ -220,9 +237,11 @@ __global__ __launch_bounds__(kThreads) void k_ball_count(int64_t H, Grid g, Quer
         const double* c = q.centre + 3 * h;
         const double R = q.radius[h];
         const CellRows cr = rows_of(g, c, R);
+        int64_t r0, r1;
+        wave_rows(cr.rows(), w, r0, r1);
         int64_t cnt = 0;
         double mh = 0.0, ml = 0.0;
-        for (int64_t r = w; r < cr.rows(); r += kWavesPerBlock) {
+        for (int64_t r = r0; r < r1; ++r) {
             int64_t a, b;
             row_range(g, cr, start, r, a, b);
             for (int64_t j = a + __lane_id(); j < b; j += kWave)
@@ -240,6 +259,7 @@ __global__ __launch_bounds__(kThreads) void k_ball_count(int64_t H, Grid g, Quer
         }
         if (__lane_id() == 0) {
             shc[w] = cnt;
+            wcount[kWavesPerBlock * h + w] = cnt;
             shm[2 * w] = mh;
             shm[2 * w + 1] = ml;
         }
@@ -258,15 +278,15 @@ __global__ __launch_bounds__(kThreads) void k_ball_count(int64_t H, Grid g, Quer
     }
 }
 
-// one block per centre: the hits' indices at off[h], rows in order, kWavesPerBlock rows
-// at a time (a wave per row: count, then a block scan of the rows' counts, then write)
+// one block per centre: every wavefront writes the hits of its rows, in row order, after
+// the counts of the wavefronts before it (rows in order -> hits in row order)
 __global__ __launch_bounds__(kThreads) void k_ball_emit(int64_t H, Grid g, Query q,
                                                         const int32_t* __restrict__ start,
                                                         const int32_t* __restrict__ perm,
                                                         const double4* __restrict__ pk,
                                                         const int64_t* __restrict__ off,
+                                                        const int64_t* __restrict__ wcount,
                                                         int32_t* __restrict__ out) {
-    __shared__ int64_t shc[kWavesPerBlock];
     const int w = threadIdx.x / kWave;
     const int lane = __lane_id();
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
@@ -274,24 +294,13 @@ __global__ __launch_bounds__(kThreads) void k_ball_emit(int64_t H, Grid g, Query
         const double* c = q.centre + 3 * h;
         const double R = q.radius[h];
         const CellRows cr = rows_of(g, c, R);
-        const int64_t nrows = cr.rows();
-        int64_t running = off[h];
-        for (int64_t r0 = 0; r0 < nrows; r0 += kWavesPerBlock) {
-            const int64_t r = r0 + w;
-            int64_t a = 0, b = 0;
-            if (r < nrows) row_range(g, cr, start, r, a, b);
-            int64_t cnt = 0;
-            for (int64_t j0 = a; j0 < b; j0 += kWave) {
-                const int64_t j = j0 + lane;
-                const bool hit = j < b && is_hit(pk[j], c, R, q.strict);
-                cnt += __popcll(__ballot(hit));
-            }
-            if (lane == 0) shc[w] = cnt;
-            __syncthreads();
-            int64_t base = running;
-            for (int v = 0; v < w; ++v) base += shc[v];
-            int64_t total = 0;
-            for (int v = 0; v < kWavesPerBlock; ++v) total += shc[v];
+        int64_t r0, r1;
+        wave_rows(cr.rows(), w, r0, r1);
+        int64_t base = off[h];
+        for (int v = 0; v < w; ++v) base += wcount[kWavesPerBlock * h + v];
+        for (int64_t r = r0; r < r1; ++r) {
+            int64_t a, b;
+            row_range(g, cr, start, r, a, b);
             for (int64_t j0 = a; j0 < b; j0 += kWave) {
                 const int64_t j = j0 + lane;
                 const bool hit = j < b && is_hit(pk[j], c, R, q.strict);
@@ -299,8 +308,6 @@ __global__ __launch_bounds__(kThreads) void k_ball_emit(int64_t H, Grid g, Query
                 if (hit) out[base + __popcll(mask & below)] = perm[j];
                 base += __popcll(mask);
             }
-            running += total;
-            __syncthreads();
         }
     }
 }
@@ -337,7 +344,9 @@ struct BallWs {
     uint32_t* skey;
     int32_t* perm;
     double4* pk;
+    double4* tmp;
     int32_t* start;
+    int64_t* wcount;
     int64_t* count;
     int64_t* total;
     int32_t* raw;
@@ -364,6 +373,8 @@ BallWs carve_ball(void* base, int64_t n, int64_t H, int64_t max_found) {
     w.skey = c.take<uint32_t>(n1);
     w.perm = c.take<int32_t>(n1);
     w.pk = c.take<double4>(n1);
+    w.tmp = c.take<double4>(n1);
+    w.wcount = c.take<int64_t>(kWavesPerBlock * h1);
     w.start = c.take<int32_t>(w.max_cells + 1);
     w.count = c.take<int64_t>(h1 + 1);
     w.total = c.take<int64_t>(2);
@@ -386,8 +397,9 @@ BallWs carve_ball(void* base, int64_t n, int64_t H, int64_t max_found) {
 // neither refined nor overlapped (cr0amr and solapst) and, with a sphere, lie within
 // sqrt(d2) < R.  One block per halo culls the patches
 // 256 at a time (a box-overlap test, widened by a cell against rounding), then walks the
-// candidate patches in order, an (ix, iy) row of cells per wavefront, lanes along iz --
-// the reference's loop order, kept by the row-batch scan of the emitting kernel.
+// candidate patches in order, the cells of each patch's clip box a block-width at a time
+// in (ix, iy, iz) order -- the reference's loop order, kept by the chunk scan of the
+// emitting kernel.
 constexpr int kMaxLevels = 40;
 
 struct AmrPatches {
@@ -518,6 +530,19 @@ struct AmrHalo {
     const double* radius;
 };
 
+// cell q (0 .. cells()) of a patch's clip box in (ix, iy, iz) order
+__device__ __forceinline__ void clip_cell(const PatchClip& c, int64_t q, int64_t& ix, int64_t& iy,
+                                          int64_t& iz) {
+    const int64_t niz = c.iz1 - c.iz0 + 1, niy = c.iy1 - c.iy0 + 1;
+    iz = c.iz0 + q % niz;
+    iy = c.iy0 + (q / niz) % niy;
+    ix = c.ix0 + q / (niz * niy);
+}
+
+__device__ __forceinline__ int64_t clip_cells(const PatchClip& c) {
+    return c.rows() * (c.iz1 - c.iz0 + 1);
+}
+
 __global__ __launch_bounds__(kThreads) void k_amr_count(int64_t H, AmrPatches pa, AmrCells g,
                                                         AmrHalo hq, int64_t* __restrict__ count) {
     __shared__ int32_t list[kThreads];
@@ -532,14 +557,12 @@ __global__ __launch_bounds__(kThreads) void k_amr_count(int64_t H, AmrPatches pa
         if (!cen || R >= 0.0)
             for_candidate_patches(pa, box, list, &nlist, [&](int64_t p) {
                 const PatchClip c = clip_patch(pa, p, box);
-                const int64_t niy = c.iy1 - c.iy0 + 1;
-                for (int64_t r = w; r < c.rows(); r += kWavesPerBlock) {
-                    const int64_t ix = c.ix0 + r / niy, iy = c.iy0 + r % niy;
-                    for (int64_t iz = c.iz0 + __lane_id(); iz <= c.iz1; iz += kWave) {
-                        int64_t cell;
-                        double x, y, z;
-                        if (amr_hit(c, g, box, cen, R, ix, iy, iz, cell, x, y, z)) ++cnt;
-                    }
+                const int64_t nc = clip_cells(c);
+                for (int64_t q = threadIdx.x; q < nc; q += blockDim.x) {
+                    int64_t ix, iy, iz, cell;
+                    double x, y, z;
+                    clip_cell(c, q, ix, iy, iz);
+                    if (amr_hit(c, g, box, cen, R, ix, iy, iz, cell, x, y, z)) ++cnt;
                 }
             });
         for (int s = kWave / 2; s >= 1; s >>= 1) cnt += __shfl_down(cnt, s, kWave);
@@ -554,6 +577,8 @@ __global__ __launch_bounds__(kThreads) void k_amr_count(int64_t H, AmrPatches pa
     }
 }
 
+// the same traversal, the clip box's cells a block-width at a time in (ix, iy, iz) order:
+// each chunk's hits are written after the chunk's earlier hits (ballot + wave counts)
 __global__ __launch_bounds__(kThreads) void k_amr_emit(int64_t H, AmrPatches pa, AmrCells g,
                                                        AmrHalo hq, const int64_t* __restrict__ off,
                                                        int64_t stride, double* __restrict__ out) {
@@ -571,50 +596,35 @@ __global__ __launch_bounds__(kThreads) void k_amr_emit(int64_t H, AmrPatches pa,
         if (cen && !(R >= 0.0)) continue;  // block-uniform
         for_candidate_patches(pa, box, list, &nlist, [&](int64_t p) {
             const PatchClip c = clip_patch(pa, p, box);
-            const int64_t niy = c.iy1 - c.iy0 + 1;
-            const int64_t nrows = c.rows();
-            for (int64_t r0 = 0; r0 < nrows; r0 += kWavesPerBlock) {
-                const int64_t r = r0 + w;
-                const bool live = r < nrows;
-                const int64_t ix = live ? c.ix0 + r / niy : 0, iy = live ? c.iy0 + r % niy : 0;
-                int64_t cnt = 0;
-                if (live)
-                    for (int64_t z0 = c.iz0; z0 <= c.iz1; z0 += kWave) {
-                        const int64_t iz = z0 + lane;
-                        int64_t cell;
-                        double x, y, z;
-                        const bool hit = iz <= c.iz1 && amr_hit(c, g, box, cen, R, ix, iy, iz,
-                                                                cell, x, y, z);
-                        cnt += __popcll(__ballot(hit));
-                    }
-                if (lane == 0) shc[w] = cnt;
+            const int64_t nc = clip_cells(c);
+            for (int64_t q0 = 0; q0 < nc; q0 += blockDim.x) {
+                const int64_t q = q0 + threadIdx.x;
+                int64_t ix = 0, iy = 0, iz = 0, cell = 0;
+                double x = 0, y = 0, z = 0;
+                bool hit = false;
+                if (q < nc) {
+                    clip_cell(c, q, ix, iy, iz);
+                    hit = amr_hit(c, g, box, cen, R, ix, iy, iz, cell, x, y, z);
+                }
+                const unsigned long long mask = __ballot(hit);
+                if (lane == 0) shc[w] = __popcll(mask);
                 __syncthreads();
                 int64_t base = running;
                 for (int v = 0; v < w; ++v) base += shc[v];
                 int64_t total = 0;
                 for (int v = 0; v < kWavesPerBlock; ++v) total += shc[v];
-                if (live)
-                    for (int64_t z0 = c.iz0; z0 <= c.iz1; z0 += kWave) {
-                        const int64_t iz = z0 + lane;
-                        int64_t cell = 0;
-                        double x = 0, y = 0, z = 0;
-                        const bool hit = iz <= c.iz1 && amr_hit(c, g, box, cen, R, ix, iy, iz,
-                                                                cell, x, y, z);
-                        const unsigned long long mask = __ballot(hit);
-                        if (hit) {
-                            const int64_t o = base + __popcll(mask & below);
-                            out[o] = x;
-                            out[stride + o] = y;
-                            out[2 * stride + o] = z;
-                            out[3 * stride + o] = ldf(g.vx, g.t, cell) * g.vel_scale;
-                            out[4 * stride + o] = ldf(g.vy, g.t, cell) * g.vel_scale;
-                            out[5 * stride + o] = ldf(g.vz, g.t, cell) * g.vel_scale;
-                            out[6 * stride + o] = (1.0 + ldf(g.delta, g.t, cell)) * g.rho_B *
-                                                  c.res3 * g.mass_scale;
-                            out[7 * stride + o] = ldf(g.temp, g.t, cell);
-                        }
-                        base += __popcll(mask);
-                    }
+                if (hit) {
+                    const int64_t o = base + __popcll(mask & below);
+                    out[o] = x;
+                    out[stride + o] = y;
+                    out[2 * stride + o] = z;
+                    out[3 * stride + o] = ldf(g.vx, g.t, cell) * g.vel_scale;
+                    out[4 * stride + o] = ldf(g.vy, g.t, cell) * g.vel_scale;
+                    out[5 * stride + o] = ldf(g.vz, g.t, cell) * g.vel_scale;
+                    out[6 * stride + o] =
+                        (1.0 + ldf(g.delta, g.t, cell)) * g.rho_B * c.res3 * g.mass_scale;
+                    out[7 * stride + o] = ldf(g.temp, g.t, cell);
+                }
                 running += total;
                 __syncthreads();
             }
@@ -698,7 +708,9 @@ int halma_ball_query(int64_t n_points, const void* x, const void* y, const void*
         if (rc) return rc;
         k_starts<<<blocks_for(ncell + 1), kThreads, 0, st>>>(w.skey, n_points, ncell, w.start);
         HALMA_LAUNCH_CHECK();
-        k_pack<<<blocks_for(n_points), kThreads, 0, st>>>(p, w.perm, n_points, w.pk);
+        k_pack<<<blocks_for(n_points), kThreads, 0, st>>>(p, n_points, w.tmp);
+        HALMA_LAUNCH_CHECK();
+        k_pack_sorted<<<blocks_for(n_points), kThreads, 0, st>>>(w.tmp, w.perm, n_points, w.pk);
         HALMA_LAUNCH_CHECK();
     } else {
         g.cs = 1.0;
@@ -708,7 +720,7 @@ int halma_ball_query(int64_t n_points, const void* x, const void* y, const void*
     const unsigned hb = static_cast<unsigned>(std::min<int64_t>(n_centres, 65536));
     if (n_points > 0) {
         k_ball_count<<<hb, kThreads, 0, st>>>(n_centres, g, q, w.start, w.perm, w.pk, w.count,
-                                              mass_sums);
+                                              w.wcount, mass_sums);
         HALMA_LAUNCH_CHECK();
         int rc = exclusive_scan_i64(w.count, offsets, n_centres + 1, w.total, w.sws, st);
         if (rc) return rc;
@@ -727,7 +739,7 @@ int halma_ball_query(int64_t n_points, const void* x, const void* y, const void*
     if (F > INT32_MAX) return set_error(HALMA_E_LIMIT, "at most 2^31-1 points found");
     if (F == 0) return 0;
     k_ball_emit<<<hb, kThreads, 0, st>>>(n_centres, g, q, w.start, w.perm, w.pk, offsets,
-                                         sorted ? w.raw : indices);
+                                         w.wcount, sorted ? w.raw : indices);
     HALMA_LAUNCH_CHECK();
     if (sorted) {  // ascending indices per centre: by index, then stably by centre
         k_entry_centre<<<hb, kThreads, 0, st>>>(offsets, n_centres, w.cid);

@@ -21,6 +21,11 @@ from .device import ptr, stream_ptr, to_dev, workspace
 MASS_TO_SUN = 9.1717e18
 
 
+def _native(a):
+    """A device tensor of ``a`` in its own dtype (tensors stay where they are on the GPU)."""
+    return to_dev(a if isinstance(a, torch.Tensor) else np.asarray(a), None)
+
+
 def levels_of(npatch):
     """Level of every patch: [0] + [1] * npatch[1] + [2] * npatch[2] + ..."""
     npatch = np.asarray(npatch, np.int64)
@@ -147,9 +152,9 @@ def gather_sources(grid, rho_B, rete, dm, st, centres, radii, mass_to_sun=MASS_T
         off, idx, _ = engine.ball_query(src[0], src[1], src[2], c, r, cell=cell)
         out[f"{name}_offsets"] = off
         out[f"{name}_index"] = idx
-        mass = to_dev(np.asarray(src[3]), None)  # scaled in its own precision, as numpy
+        mass = _native(src[3])  # scaled in its own precision, as numpy scales it
         out[f"{name}_mass"] = mass[idx.long()] * mass_to_sun
-    out["st_oripa"] = to_dev(np.asarray(st[4]), None)[out["st_index"].long()]
+    out["st_oripa"] = _native(st[4])[out["st_index"].long()]
     return out
 
 
@@ -161,7 +166,7 @@ def dark_matter_mass(dm_x, dm_y, dm_z, dm_mass, centres, radii, mass_to_sun=MASS
     (distance <= R, the KD-tree's query_ball_point).  numpy's np.sum of the float32
     values rounds each pairwise partial sum to float32 instead: the two agree to the
     float32 summation error (parity to that tolerance)."""
-    m = to_dev(np.asarray(dm_mass), None) * mass_to_sun
+    m = _native(dm_mass) * mass_to_sun
     _, _, sums = engine.ball_query(dm_x, dm_y, dm_z, centres, radii, m=m, cell=cell,
                                    count_only=True)
     return sums

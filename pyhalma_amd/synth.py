@@ -282,3 +282,50 @@ def hernquist_haloes(seed=20261207, n_haloes=200, nmin=50000, nmax=500000, scale
     return dict(x=x[:, 0].astype(f32), y=x[:, 1].astype(f32), z=x[:, 2].astype(f32),
                 mass=np.full(n, 1e5, f32), offsets=offsets, seeds=np.array(seeds),
                 centres=np.array(centres), a=np.array(scales), sizes=sizes)
+
+
+def dark_matter_around(centres, seed=20261208, per_halo=4000, n_field=2000000, box=40.0,
+                       a=0.02):
+    """Dark-matter particles: a Plummer sphere of scale ``a`` (Mpc) and ``per_halo``
+    particles around each centre plus a uniform field, FP32, masses 1e-12 code units."""
+    rng = np.random.default_rng(seed)
+    centres = np.asarray(centres, np.float64)
+    parts = [plummer_positions(rng, per_halo, a) + c for c in centres]
+    parts.append(rng.uniform(-box / 2, box / 2, size=(n_field, 3)))
+    x = np.concatenate(parts)
+    x = x[rng.permutation(len(x))].astype(np.float32)
+    m = rng.uniform(0.8e-12, 1.2e-12, len(x)).astype(np.float32)
+    return dict(x=x[:, 0].copy(), y=x[:, 1].copy(), z=x[:, 2].copy(), mass=m)
+
+
+def amr_around(centres, seed=20261209, L=40.0, ncoarse=128, levels=(6, 7), cells=16,
+               refined=0.15, overlapped=0.05):
+    """A MASCLET-like AMR snapshot: for every centre one patch per level in ``levels``
+    of ``cells`` cells per axis centred on it (parent-cell aligned), FP32 gas fields
+    (delta lognormal, velocities in c = 1 units, temperatures 1e4-1e7 K) and random
+    refined / overlapped flags.  Returns the AmrGrid constructor arguments (level-0 base
+    patch included, empty)."""
+    rng = np.random.default_rng(seed)
+    base = L / ncoarse
+    lev, nx, rx, ry, rz = [0], [1], [0.0], [0.0], [0.0]
+    fields = [[np.zeros((1, 1, 1), np.float32)] for _ in range(5)] + \
+        [[np.zeros((1, 1, 1), bool)] for _ in range(2)]
+    for c in np.asarray(centres, np.float64):
+        for level in levels:
+            res = base / 2 ** level
+            corner = np.floor((c - cells * res / 2) / (2 * res)) * 2 * res + res
+            lev.append(level)
+            nx.append(cells)
+            rx.append(corner[0])
+            ry.append(corner[1])
+            rz.append(corner[2])
+            shp = (cells, cells, cells)
+            fields[0].append(np.exp(rng.normal(1.0, 1.0, shp)).astype(np.float32) - 1.0)
+            for k in (1, 2, 3):
+                fields[k].append(rng.normal(scale=5e-4, size=shp).astype(np.float32))
+            fields[4].append(np.exp(rng.uniform(np.log(1e4), np.log(1e7), shp)).astype(np.float32))
+            fields[5].append(rng.random(shp) >= refined)
+            fields[6].append(rng.random(shp) >= overlapped)
+    n = np.array(nx)
+    return dict(L=L, ncoarse=ncoarse, levels=np.array(lev), nx=n, ny=n, nz=n,
+                rx=np.array(rx), ry=np.array(ry), rz=np.array(rz), cells=fields)
