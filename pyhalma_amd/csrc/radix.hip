@@ -1,8 +1,9 @@
 // Stable LSD radix sort (8-bit digits).  Per pass: a per-block digit histogram, one
 // exclusive scan over the digit-major [digit][block] table, then a stable scatter in
-// which each block walks its tile in order, 256 keys per round; ranks among equal digits
-// inside a wavefront come from eight ballots (lanes with the same digit), and the
-// wavefronts of a round are offset by their predecessors' counts through LDS.
+// which each block ranks its tile's keys by digit (256 keys per round; ranks among equal
+// digits inside a wavefront from eight ballots, wavefronts of a round offset by their
+// predecessors' counts through LDS), sorts the tile by digit in LDS and writes each
+// digit's run to its place, consecutive threads to consecutive addresses.
 #include "radix.h"
 #include "scan.h"
 
@@ -30,18 +31,47 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t* __restr
     hist[static_cast<int64_t>(threadIdx.x) * nb + blockIdx.x] = h[threadIdx.x];
 }
 
+// exclusive scan of one int per thread over the block (256 threads)
+__device__ int block_excl_scan(int v, int* sh /* kWaves */) {
+    const int lane = __lane_id(), w = threadIdx.x / kWave;
+    int x = v;
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int y = __shfl_up(x, d, kWave);
+        if (lane >= d) x += y;
+    }
+    if (lane == kWave - 1) sh[w] = x;
+    __syncthreads();
+    int before = 0;
+    for (int k = 0; k < w; ++k) before += sh[k];
+    __syncthreads();
+    return before + x - v;
+}
+
+// Stable scatter of one tile: every key's rank among the tile's keys of its digit (rounds
+// in order, wavefronts in order, lanes by ballot), then the tile sorted by digit in LDS,
+// then written out digit run by digit run, consecutive threads to consecutive addresses.
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const uint32_t* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int shift,
     int64_t nb, const int64_t* __restrict__ offs, uint32_t* __restrict__ keys_out,
     int32_t* __restrict__ vals_out) {
     __shared__ int wcnt[kWaves][256];
+    __shared__ int tcnt[256];
+    __shared__ int loff[256];
+    __shared__ int scan_sh[kWaves];
     __shared__ int64_t run[256];
+    __shared__ uint32_t sk[kTile];
+    __shared__ int32_t sv[kTile];
     const int tid = threadIdx.x;
     const int lane = __lane_id();
     const int w = tid / kWave;
     run[tid] = offs[static_cast<int64_t>(tid) * nb + blockIdx.x];
+    tcnt[tid] = 0;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile;
+    uint32_t kk[kRounds];
+    int32_t vv[kRounds];
+    int rk[kRounds];
+#pragma unroll
     for (int r = 0; r < kRounds; ++r) {
 #pragma unroll
         for (int k = 0; k < kWaves; ++k) wcnt[k][tid] = 0;
@@ -64,21 +94,45 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
             peers &= bit ? m : ~m;
         }
         const int rank = __popcll(peers & lt);
-        const bool last = valid && (peers >> lane) == 1ull;
-        if (last) wcnt[w][digit] = rank + 1;
+        if (valid && (peers >> lane) == 1ull) wcnt[w][digit] = rank + 1;
         __syncthreads();
+        int lr = -1;
         if (valid) {
-            int64_t dest = run[digit] + rank;
-            for (int k = 0; k < w; ++k) dest += wcnt[k][digit];
-            keys_out[dest] = key;
-            vals_out[dest] = val;
+            lr = tcnt[digit] + rank;
+            for (int k = 0; k < w; ++k) lr += wcnt[k][digit];
         }
+        kk[r] = key;
+        vv[r] = val;
+        rk[r] = lr;
         __syncthreads();
         int add = 0;
 #pragma unroll
         for (int k = 0; k < kWaves; ++k) add += wcnt[k][tid];
-        run[tid] += add;
-        __syncthreads();
+        tcnt[tid] += add;
+    }
+    __syncthreads();
+    loff[tid] = block_excl_scan(tcnt[tid], scan_sh);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r)
+        if (rk[r] >= 0) {
+            const int pos = loff[(kk[r] >> shift) & 0xff] + rk[r];
+            sk[pos] = kk[r];
+            sv[pos] = vv[r];
+        }
+    __syncthreads();
+    const int64_t left = n - base;
+    const int tile_n = left < kTile ? static_cast<int>(left) : kTile;
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int j = r * kThreads + tid;
+        if (j < tile_n) {
+            const uint32_t key = sk[j];
+            const int d = (key >> shift) & 0xff;
+            const int64_t dest = run[d] + (j - loff[d]);
+            keys_out[dest] = key;
+            vals_out[dest] = sv[j];
+        }
     }
 }
 
