@@ -291,6 +291,29 @@ int halma_amr_particles(int64_t n_patches, const int32_t *level, const int32_t *
                         int64_t *offsets, double *out, int64_t *n_found, void *workspace,
                         size_t workspace_bytes, void *stream);
 
+/* halo_shape (particle_subroutines.f90:160-214) of many groups: offsets (device, G+1)
+ * over member-ordered FP32 x, y, z (relative to each group's centre, as
+ * halo_properties.py:852-858 passes them) and masses; out (device, 3 FP32 per group): the
+ * square roots of the eigenvalues of sum(m r r^T) / sum(m), largest first.  The FP32
+ * terms are summed in double-double (the reference: FP32 OpenMP reduction) and the
+ * eigenvalues come from the file's Jacobi variant in FP64. */
+int halma_halo_shape(int64_t n_groups, const int64_t *offsets, const float *x, const float *y,
+                     const float *z, const float *mass, float *out, void *stream);
+
+/* sigma_projections (particle_subroutines.f90:217-461) of one halo: FP32 member arrays
+ * (device, n; velocities relative to the halo's, as halo_properties.py:791-793 passes
+ * them), grid (device, n_cell FP32 nodes), centre, R05 per axis and ll; out (device, 5
+ * FP32): SIG_1D_x_05, SIG_1D_y_05, SIG_1D_z_05, V_sigma, lambda.  Per-particle sums are
+ * FP64 rounded to FP32 (the reference: FP32 OpenMP reductions); the sums over cells are
+ * sequential FP32 in the reference's loop order. */
+size_t halma_sigma_projections_workspace_size(int64_t n, int n_cell);
+int halma_sigma_projections(int64_t n, const float *x, const float *y, const float *z,
+                            const float *vx, const float *vy, const float *vz,
+                            const float *mass, const float *grid, int n_cell, float cx,
+                            float cy, float cz, float R05x, float R05y, float R05z, float ll,
+                            float *out, void *workspace, size_t workspace_bytes,
+                            void *stream);
+
 /* Progenitor matching (the merger section of pyHALMA.py:1211-1266, main_progenitors_finder
  * :1219-1257) for every new halo at once.  New haloes: CSR new_offsets (device, n_new+1,
  * new_offsets[0] = 0) over new_ids (device int64, the particle IDs of each halo's

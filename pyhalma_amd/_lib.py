@@ -21,6 +21,7 @@ _p = _c.c_void_p
 _i64 = _c.c_int64
 _i32 = _c.c_int32
 _dbl = _c.c_double
+_flt = _c.c_float
 _size = _c.c_size_t
 _i64p = _c.POINTER(_c.c_int64)
 
@@ -65,6 +66,11 @@ SIGNATURES = {
     "halma_half_mass_radius_workspace_size": (_size, [_i64]),
     "halma_half_mass_radius": (_c.c_int, [_i64, _i64, _p, _p, _p, _p, _p, _c.c_int, _p,
                                           _c.c_int, _p, _p, _p, _size, _p]),
+    "halma_halo_shape": (_c.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p]),
+    "halma_sigma_projections_workspace_size": (_size, [_i64, _c.c_int]),
+    "halma_sigma_projections": (_c.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _c.c_int,
+                                           _flt, _flt, _flt, _flt, _flt, _flt, _flt, _p, _p,
+                                           _size, _p]),
     "halma_ball_query_workspace_size": (_size, [_i64, _i64, _i64]),
     "halma_ball_query": (_c.c_int, [_i64, _p, _p, _p, _c.c_int, _p, _c.c_int, _i64, _p, _p,
                                     _c.c_int, _dbl, _c.c_int, _i64, _p, _p, _p, _i64p, _p,

@@ -813,7 +813,7 @@ def ball_query(x, y, z, centres, radii, strict=False, m=None, cell=0.0, sorted=T
                count_only=False):
     """The points within radii[h] of centres[h] for every sphere h (halma_ball_query): the
     KD-tree query_ball_point of halo_gas.py:255,269 and pyHALMA.py:1179 (distance <= R),
-    or parallel_inside of halo_gas.py:213-215 (``strict``: distance < R).  Returns
+    or parallel_inside of halo_gas.py:218-220 (``strict``: distance < R).  Returns
     (offsets (numpy int64, H+1), indices (int32 device tensor, ascending per sphere when
     ``sorted``; None when ``count_only``), mass sums (numpy FP64 per sphere, or None
     without ``m``))."""
@@ -847,6 +847,40 @@ def ball_query(x, y, z, centres, radii, strict=False, m=None, cell=0.0, sorted=T
     F = int(found.value)
     return (off.cpu().numpy(), None if count_only else idx[:F],
             sums[:H].cpu().numpy() if sums is not None else None)
+
+
+def halo_shapes(offsets, x, y, z, mass):
+    """halo_shape (particle_subroutines.f90:160-214) of many groups at once
+    (halma_halo_shape): offsets (G+1) over member-ordered FP32 coordinates relative to
+    each group's centre and masses.  Returns (G, 3) float32 device tensor."""
+    L = _lib_()
+    off = to_dev(np.asarray(offsets, np.int64), np.int64)
+    G = int(off.numel()) - 1
+    arrs = [to_dev(np.asarray(a, np.float32) if not isinstance(a, torch.Tensor) else a,
+                   np.float32) for a in (x, y, z, mass)]
+    out = torch.zeros((max(G, 1), 3), dtype=torch.float32, device=off.device)
+    rc = L.halma_halo_shape(G, ptr(off), *(ptr(a) for a in arrs), ptr(out), stream_ptr())
+    _lib.check(rc, "halma_halo_shape")
+    return out[:G]
+
+
+def sigma_projections(grid, x, y, z, vx, vy, vz, mass, cx, cy, cz, R05x, R05y, R05z, ll):
+    """sigma_projections (particle_subroutines.f90:217-461) of one halo
+    (halma_sigma_projections): FP32 member arrays (velocities relative to the halo's),
+    the grid of cell nodes; returns numpy float32 (SIG_1D_x_05, SIG_1D_y_05,
+    SIG_1D_z_05, V_sigma, lambda)."""
+    L = _lib_()
+    g = to_dev(np.asarray(grid, np.float32), np.float32)
+    arrs = [to_dev(np.asarray(a, np.float32), np.float32) for a in (x, y, z, vx, vy, vz, mass)]
+    n = int(arrs[0].numel())
+    nc = int(g.numel())
+    out = torch.zeros(5, dtype=torch.float32, device=g.device)
+    ws = workspace(L.halma_sigma_projections_workspace_size(n, nc))
+    f = [float(np.float32(v)) for v in (cx, cy, cz, R05x, R05y, R05z, ll)]
+    rc = L.halma_sigma_projections(n, *(ptr(a) for a in arrs), ptr(g), nc, *f, ptr(out),
+                                   ptr(ws), ws.numel(), stream_ptr())
+    _lib.check(rc, "halma_sigma_projections")
+    return out.cpu().numpy()
 
 
 class UnbindResult:

@@ -19,8 +19,10 @@ accumulation; its relative error is below ``pyhalma_amd.engine.pot_error_bound()
 (the reference's plain FP32 loop has no such bound).
 
 The same Fortran module also holds ``halo_shape`` and ``sigma_projections``
-(particle_subroutines.f90:160-461), halo-property routines outside the accelerated
-path; calling them here raises NotImplementedError.
+(particle_subroutines.f90:160-461), which halo_properties.py:805,861 reach through the
+same attribute path; both run on the GPU here (libhalma's halma_halo_shape and
+halma_sigma_projections), with the reference's FP32 per-operation arithmetic and FP64
+sums where the reference sums in FP32 through OpenMP reductions.
 """
 import numpy as np
 
@@ -73,16 +75,33 @@ class _FortranModuleParticle:
                     total_y, total_z, ntest, test_x, test_y, test_z)
 
     @staticmethod
-    def halo_shape(*args, **kwargs):
-        raise NotImplementedError(
-            "particle.halo_shape (particle_subroutines.f90:160-214) is a halo-property "
-            "routine outside the accelerated FoF/reduction/unbinding path")
+    def halo_shape(ncore, npart, x, y, z, mass):
+        """particle_subroutines.f90:160-214 -> float32[3]: square roots of the eigenvalues of
+        the mass-weighted second-moment tensor of the (centred) points, largest first."""
+        del ncore
+        n = int(npart)
+        arrs = [_as_f32(a, n, name, "halo_shape")
+                for a, name in ((x, "x"), (y, "y"), (z, "z"), (mass, "mass"))]
+        return engine.halo_shapes(np.array([0, n], np.int64), *arrs).cpu().numpy()[0]
 
     @staticmethod
-    def sigma_projections(*args, **kwargs):
-        raise NotImplementedError(
-            "particle.sigma_projections (particle_subroutines.f90:217-461) is a "
-            "halo-property routine outside the accelerated FoF/reduction/unbinding path")
+    def sigma_projections(ncore, npart, grid, n_cell, part_list, st_x, st_y, st_z, st_vx,
+                          st_vy, st_vz, st_mass, cx, cy, cz, R05x, R05y, R05z, ll):
+        """particle_subroutines.f90:217-461 -> (SIG_1D_x_05, SIG_1D_y_05, SIG_1D_z_05,
+        V_sigma, lambda); part_list is 1-based, as f2py passes it."""
+        del ncore
+        n = int(npart)
+        nc = int(n_cell)
+        g = _as_f32(grid, nc, "grid", "sigma_projections")
+        pl = np.asarray(part_list).ravel()
+        if pl.shape[0] != n:
+            raise ValueError(f"sigma_projections: 0-th dimension of part_list must be fixed "
+                             f"to {n} but got {pl.shape[0]}")
+        idx = pl.astype(np.int64) - 1
+        arrs = [np.ascontiguousarray(np.asarray(a, np.float32)[idx])
+                for a in (st_x, st_y, st_z, st_vx, st_vy, st_vz, st_mass)]
+        out = engine.sigma_projections(g, *arrs, cx, cy, cz, R05x, R05y, R05z, ll)
+        return tuple(float(v) for v in out)
 
 
 particle = _FortranModuleParticle()
