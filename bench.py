@@ -72,7 +72,8 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the halo-finding path would shard haloes across GPUs; see DESIGN.md)",
+              "(the halo-finding path would shard FoF by spatial slabs and haloes by size "
+              "across GPUs)",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
