@@ -355,6 +355,20 @@ __device__ __forceinline__ bool within(const T* sx, const T* sy, const T* sz, in
     return (dx * dx + dy * dy) + dz * dz <= ll2;
 }
 
+// Every cell pointed straight at its root (between the candidate passes, when no union
+// runs): the read-only root checks of the next pass then take one or two steps instead
+// of walking the chains that hooking by index builds in dense regions.  Concurrent
+// writers store each cell's own root, so a reader sees either its old parent or the root.
+__global__ void k_flatten(const int64_t* __restrict__ totals, int* __restrict__ parent) {
+    const int64_t ncell = totals[0] & 0xffffffff;
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int r = static_cast<int>(c);
+        for (int p = parent[r]; p != r; p = parent[r]) r = p;
+        parent[c] = r;
+    }
+}
+
 // Link candidates, one lane per (cell, neighbour offset) of one distance class (NEAR:
 // the 13 at Chebyshev distance 1; otherwise the 49 at distance 2, launched after every
 // distance-1 link is in the forest, so that dense regions are recognised as joined and
@@ -682,6 +696,8 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     HALMA_LAUNCH_CHECK();
     k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big, w.big_cap, w.cell_start, sx, sy,
                                              sz, ll * ll, w.parent);
+    HALMA_LAUNCH_CHECK();
+    k_flatten<<<grid_cells, 256, 0, st>>>(w.totals, w.parent);
     HALMA_LAUNCH_CHECK();
     k_fof_cand<T, false><<<cand_grid, 256, 0, st>>>(
         w.tab, static_cast<uint64_t>(w.cap - 1), w.bits, bmask, w.cell_start, w.cell_key, sx,
