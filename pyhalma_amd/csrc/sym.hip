@@ -86,6 +86,28 @@ __device__ __forceinline__ float reduce8(const float v[8], int lane) {
     return t;
 }
 
+// Sources q[g .. g+7] of a tile (cnt of them exist) against the lane's two tests: the
+// test-side partial sums ps[r] (8 terms, source order) and the per-source pair terms
+// pj[s] = m_i0 / r + m_i1 / r.  A full group (g + 8 <= cnt, wave-uniform) needs no
+// padding select; the tail group clamps its loads and gives padding sources no mass.
+template <bool kFull>
+__device__ __forceinline__ void pair_group(const float4* __restrict__ q0, int g, int cnt,
+                                           const float xi[2], const float yi[2],
+                                           const float zi[2], const float mi[2], float ps[2],
+                                           float pj[8]) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int j = g + s;
+        const float4 q = uniform_load(q0 + (kFull ? j : min(j, cnt - 1)));
+        const float qw = (kFull || j < cnt) ? q.w : 0.0f;
+        const float v0 = inv_r(q, xi[0], yi[0], zi[0]);
+        const float v1 = inv_r(q, xi[1], yi[1], zi[1]);
+        ps[0] = __builtin_fmaf(qw, v0, ps[0]);
+        ps[1] = __builtin_fmaf(qw, v1, ps[1]);
+        pj[s] = __builtin_fmaf(mi[1], v1, mi[0] * v0);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
                                                  const SymDesc* __restrict__ halos,
                                                  const int* __restrict__ live_n,
@@ -154,17 +176,10 @@ __global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
                 for (int g = 0; g < cnt; g += 8) {
                     float ps[2] = {0.0f, 0.0f};
                     float pj[8];
-#pragma unroll
-                    for (int s = 0; s < 8; ++s) {
-                        const int j = g + s;
-                        const float4 q = uniform_load(sp + j0 + min(j, cnt - 1));
-                        const float qw = j < cnt ? q.w : 0.0f;
-                        const float v0 = inv_r(q, xi[0], yi[0], zi[0]);
-                        const float v1 = inv_r(q, xi[1], yi[1], zi[1]);
-                        ps[0] = __builtin_fmaf(qw, v0, ps[0]);
-                        ps[1] = __builtin_fmaf(qw, v1, ps[1]);
-                        pj[s] = __builtin_fmaf(mi[1], v1, mi[0] * v0);
-                    }
+                    if (g + 8 <= cnt)
+                        pair_group<true>(sp + j0, g, cnt, xi, yi, zi, mi, ps, pj);
+                    else
+                        pair_group<false>(sp + j0, g, cnt, xi, yi, zi, mi, ps, pj);
                     kahan(as[0], ac[0], ps[0]);
                     kahan(as[1], ac[1], ps[1]);
                     if (!diag) {
