@@ -227,10 +227,17 @@ __global__ void k_sym_merge(const SymDesc* __restrict__ halos, const int* __rest
     const SymDesc d = halos[h];
     const int64_t i = g - member_base[h];
     const int nb = sym_blocks(d.n);
+    // the nb + 1 slots in order, loads eight at a time in flight
+    const float2* pp = part + d.part_off + i;
     double s = 0.0;
-    for (int slot = 0; slot <= nb; ++slot) {
-        const float2 v = part[d.part_off + static_cast<int64_t>(slot) * d.n + i];
-        s += static_cast<double>(v.x) + static_cast<double>(v.y);
+    for (int c = 0; c <= nb; c += 8) {
+        float2 q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            q[u] = c + u <= nb ? pp[static_cast<int64_t>(c + u) * d.n] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c + u <= nb) s += static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
     }
     out[g] = s;
 }

@@ -425,18 +425,17 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
         stride = d.stride;
         pp = a.part + d.part_off + p;
     }
+    // every batch (the last one too) has its loads in flight together: a member pays
+    // ceil(nch / 8) memory round trips, never one per slot of a ragged tail
     double sum = 0.0;
-    int c = 0;
-    for (; c + 8 <= nch; c += 8) {
+    for (int c = 0; c < nch; c += 8) {
         float2 q[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) q[u] = pp[static_cast<int64_t>(c + u) * stride];
+        for (int u = 0; u < 8; ++u)
+            q[u] = c + u < nch ? pp[static_cast<int64_t>(c + u) * stride] : make_float2(0.0f, 0.0f);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) sum += static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
-    }
-    for (; c < nch; ++c) {
-        const float2 q = pp[static_cast<int64_t>(c) * stride];
-        sum += static_cast<double>(q.x) + static_cast<double>(q.y);
+        for (int u = 0; u < 8; ++u)
+            if (c + u < nch) sum += static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
     }
     const int64_t k = a.cidx[g];
     double S, err;
@@ -457,13 +456,24 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     const double band = a.cg * err * 1.01 + 8.0 * 0x1p-53 * (kin + fabs(pot)) +
                         1e-10 * fabs(pot);
     if (a.s_out) a.s_out[k] = S;
-    if (fabs(E) <= band) {
+    const bool in_band = fabs(E) <= band;
+    const bool gone = !in_band && E > 0.0;
+    if (in_band) {
         const unsigned long long at = atomicAdd(a.counters + 1, 1ull);
         a.band[at] = static_cast<int32_t>(g);
         atomicAdd(a.counters + 2, 1ull);
-    } else if (E > 0.0) {
+    } else if (gone) {
         a.state[k] = kJustGone;
-        atomicAdd(a.removed + h, 1);
+    }
+    // removals per halo, one atomic per halo present in the wave (members of a halo are
+    // contiguous, so a wave spans few haloes)
+    unsigned long long pending = __ballot(gone);
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const int hl = __shfl(h, leader, kWave);
+        const unsigned long long same = __ballot(gone && h == hl);
+        if (__lane_id() == leader) atomicAdd(a.removed + hl, __popcll(same));
+        pending &= ~same;
     }
 }
 
