@@ -21,8 +21,10 @@ namespace halma {
 // acc + m_j / r_ij for one source s = (x, y, z, m) and one test, with the reference's
 // rule that a pair counts only if x_j /= x_i .and. y_j /= y_i .and. z_j /= z_i: a
 // difference is zero exactly when the coordinates are equal (FP32 subtraction keeps
-// denormals), so the pair counts iff the smallest |difference| is positive (one
-// v_min3_f32 with |.| modifiers).  The mass multiply and the add are one FMA.
+// denormals), so the pair counts iff the smallest |difference| is not zero (one
+// v_min3_f32 with |.| modifiers; a NaN difference, like the reference's /= on NaN,
+// counts: the minimum skips NaN operands and is NaN only when all three are NaN).  The
+// mass multiply and the add are one FMA.
 __device__ __forceinline__ float pair_acc(const float4 s, float xi, float yi, float zi,
                                           float acc) {
     const float dx = s.x - xi;
@@ -30,7 +32,7 @@ __device__ __forceinline__ float pair_acc(const float4 s, float xi, float yi, fl
     const float dz = s.z - zi;
     const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
     const float mn = fminf(fminf(fabsf(dx), fabsf(dy)), fabsf(dz));
-    const float inv = mn > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+    const float inv = mn != 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
     return __builtin_fmaf(s.w, inv, acc);
 }
 

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_soft(const float4* __restrict__ src,
                 const float dz = (p.z - t.hz) - t.lz;
                 const float mn = fminf(fminf(fabsf(dx), fabsf(dy)), fabsf(dz));
                 const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + e2;
-                const float inv = mn > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+                const float inv = mn != 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
                 if (ACC) {
                     const float w = p.w * (inv * inv * inv);
                     ps[0] = __builtin_fmaf(w, dx, ps[0]);

@@ -39,7 +39,7 @@ __device__ __forceinline__ float inv_r(float4 q, float x, float y, float z) {
     const float dz = q.z - z;
     const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
     const float mn = fminf(fminf(fabsf(dx), fabsf(dy)), fabsf(dz));
-    return mn > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+    return mn != 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;  // the rule of pair_acc
 }
 
 __device__ __forceinline__ void kahan(float& s, float& c, float v) {

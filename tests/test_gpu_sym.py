@@ -123,3 +123,31 @@ def test_pot_sym_shared_plane_vs_oracle(gpu):
     assert np.all(want0 == 0.0) and np.all(got[:n] == 0.0)
     want1 = oracle.pot_sum_f64(m[n:], *(P[n:, k] for k in range(3)), *(P[n:, k] for k in range(3)))
     assert np.all(np.abs(got[n:] - want1) <= B * want1)
+
+
+def test_pair_kernels_nan_coordinates(gpu):
+    """A member with a NaN coordinate: the reference's /= is true for NaN, so its pairs
+    count and every sum of its halo is NaN, also when all three coordinates are NaN (no
+    difference is zero).  The symmetric kernel and the chunked batched kernel give NaN
+    exactly where the oracle does, and the clean halo of the same call within the bound."""
+    from pyhalma_amd import engine
+
+    rng = np.random.default_rng(31)
+    sizes = [700, 600, 500]
+    P = rng.normal(scale=0.01, size=(sum(sizes), 3)).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    P[off[1] + 5, 0] = np.nan           # one coordinate of a member of halo 1
+    P[off[2] + 9, :] = np.nan           # all three of a member of halo 2
+    m = rng.uniform(1e4, 1e6, len(P)).astype(np.float32)
+    want = np.concatenate([
+        oracle.pot_sum_f64(m[a:b], *(P[a:b, k] for k in range(3)), *(P[a:b, k] for k in range(3)))
+        for a, b in zip(off[:-1], off[1:])])
+    assert np.all(np.isfinite(want[:off[1]])) and np.all(np.isnan(want[off[1]:]))
+    sym = engine.pot_sum_self(off, m, P[:, 0], P[:, 1], P[:, 2]).cpu().numpy()
+    bat = engine.pot_sum_batched(off, m, P[:, 0], P[:, 1], P[:, 2], off, P[:, 0], P[:, 1],
+                                 P[:, 2])
+    bat = bat.cpu().numpy() if hasattr(bat, "cpu") else np.asarray(bat)
+    for got, bound in ((sym, engine.pot_sym_error_bound()), (bat, engine.pot_error_bound())):
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        a, b = off[0], off[1]
+        assert np.all(np.abs(got[a:b] - want[a:b]) <= bound * want[a:b])
