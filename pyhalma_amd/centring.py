@@ -33,6 +33,30 @@ def _cols(p):
     return p[:, 0].contiguous(), p[:, 1].contiguous(), p[:, 2].contiguous()
 
 
+_src_cache = {}
+
+
+def _src_cols(src_pos):
+    """FP32 columns of the source positions, kept for the next call on the same unchanged
+    tensor (the centring iterations pass one source array to every forward and backward).
+    The key holds the storage address, shape, strides, dtype and the version counter
+    (bumped by every in-place change); the entry holds the tensor itself, so its storage
+    cannot be freed and reused under the same address while it is cached."""
+    p = src_pos.detach()
+    key = (p.device, p.data_ptr(), tuple(p.shape), p.stride(), p.dtype, p._version)
+    hit = _src_cache.get("src")
+    if hit is not None and hit[0] == key:
+        return hit[2]
+    cols = _cols(p.float())
+    _src_cache["src"] = (key, p, cols)
+    return cols
+
+
+def clear_cache():
+    """Drop the cached source columns (frees their device memory)."""
+    _src_cache.clear()
+
+
 class SoftPotential(torch.autograd.Function):
     """S (float64, one per test) for tests (T, 3) float64 against the sources of their
     halo: src_pos (N, 3), src_m (N,), CSR host offsets src_off / tst_off (H+1 each).
@@ -42,7 +66,7 @@ class SoftPotential(torch.autograd.Function):
     @staticmethod
     def forward(ctx, tests, src_pos, src_m, src_off, tst_off, eps, aux=None):
         tx, ty, tz = _cols(tests.detach().double())
-        sx, sy, sz = _cols(src_pos.detach().float())
+        sx, sy, sz = _src_cols(src_pos)
         out = engine.pot_soft_batched(src_off, src_m.detach().float(), sx, sy, sz, tst_off,
                                       tx, ty, tz, eps)["pot"]
         ctx.save_for_backward(tests, src_pos, src_m)
@@ -55,7 +79,7 @@ class SoftPotential(torch.autograd.Function):
         g = g.detach().double().contiguous()
         gt = gp = gm = None
         tx, ty, tz = _cols(tests.detach().double())
-        sx, sy, sz = _cols(src_pos.detach().float())
+        sx, sy, sz = _src_cols(src_pos)
         if ctx.needs_input_grad[0]:
             r = engine.pot_soft_batched(ctx.src_off, src_m.detach().float(), sx, sy, sz,
                                         ctx.tst_off, tx, ty, tz, ctx.eps, pot=False,
@@ -134,4 +158,5 @@ def refine_centres(x, y, z, m, offsets, seeds, eps, max_iter=200, tol=1e-6):
         if not bool(active.any()):
             break
     S = soft_potential(c, pos, mt, offsets, tst_off, eps).detach()
+    clear_cache()  # the columns of this call's source array
     return CentringResult(c.detach(), S, iters, step_len)

@@ -159,10 +159,17 @@ __global__ void k_pot_merge(const HaloDesc* __restrict__ halos, const int* __res
     const HaloDesc d = halos[tst_halo[g]];
     const int64_t i = g - d.tst_off;
     const int nch = d.src_len > 0 ? (d.src_len + d.chunk - 1) / d.chunk : 0;
+    // chunk order, the loads eight at a time in flight
+    const float2* pp = part + d.part_off + i;
     double s = 0.0;
-    for (int c = 0; c < nch; ++c) {
-        const float2 p = part[d.part_off + static_cast<int64_t>(c) * d.stride + i];
-        s += static_cast<double>(p.x) + static_cast<double>(p.y);
+    for (int c = 0; c < nch; c += 8) {
+        float2 q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            q[u] = c + u < nch ? pp[static_cast<int64_t>(c + u) * d.stride] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c + u < nch) s += static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
     }
     if (out_f64) out_f64[g] = s;
     if (out_f32) out_f32[g] = static_cast<float>(s);

@@ -253,18 +253,23 @@ def RPS_batched(gas, gas_offsets, halo_v, dm=None, stars=None):
     mass = to_dev(gm, np.float64)
     temp = to_dev(gt, np.float64)
     dev = mass.device
-    halo = torch.repeat_interleave(torch.arange(H, device=dev),
-                                   torch.from_numpy(np.diff(gas_offsets)).to(dev))
     b = bound.to(torch.bool)
     cold = temp < 5 * 1e4
+    zero = torch.zeros_like(mass)
+    # the four per-halo sums with the library's group reduction over the CSR ranges
+    # (double-double, fixed order, no atomics on the few hundred per-halo totals)
+    n = int(mass.numel())
+    grp = engine.Groups(torch.arange(max(n, 1), dtype=torch.int32, device=dev),
+                        to_dev(gas_offsets, np.int64), gas_offsets)
+    cx, cy, cz = engine._coords(gx, gy, gz)
 
     def seg(v):
-        return torch.zeros(H, dtype=torch.float64, device=dev).index_add(0, halo, v)
+        return engine.group_reduce(grp, cx, cy, cz, v)[:, 0]
 
     total = seg(mass)
-    cold_bound = seg(torch.where(cold & b, mass, torch.zeros_like(mass)))
-    unb_cold = seg(torch.where(cold & ~b, mass, torch.zeros_like(mass)))
-    unb_hot = seg(torch.where(~cold & ~b, mass, torch.zeros_like(mass)))
+    cold_bound = seg(torch.where(cold & b, mass, zero))
+    unb_cold = seg(torch.where(cold & ~b, mass, zero))
+    unb_hot = seg(torch.where(~cold & ~b, mass, zero))
     frac = torch.where(total != 0, cold_bound / torch.where(total != 0, total, 1.0),
                        torch.zeros_like(total))
     return torch.stack([total, frac, unb_cold, unb_hot], 1).cpu().numpy(), bound

@@ -202,3 +202,35 @@ def test_refine_centres_reaches_fp64_minimum(gpu):
         pts = c[h] + np.stack([gx.ravel(), gy.ravel(), gz.ravel()], 1)
         Sg = oracle.pot_soft_f64(*args, pts[:, 0], pts[:, 1], pts[:, 2], EPS)
         assert Sc >= Sg.max() * (1 - 1e-12)
+
+
+def test_soft_potential_source_cache_follows_changes(gpu):
+    """The FP32 source columns kept between calls are reused only for the unchanged
+    array: after an in-place change of the positions, and for a different array, the sums
+    equal those of a fresh call."""
+    import torch
+
+    from pyhalma_amd import centring
+
+    rng = np.random.default_rng(41)
+    n = 3000
+    pos = torch.tensor(rng.normal(scale=0.01, size=(n, 3)), dtype=torch.float32,
+                       device="cuda")
+    m = torch.tensor(rng.uniform(1e4, 1e6, n), dtype=torch.float32, device="cuda")
+    off = np.array([0, 1000, n])
+    tests = torch.tensor(rng.normal(scale=0.005, size=(2, 3)), dtype=torch.float64,
+                         device="cuda")
+    t_off = np.array([0, 1, 2])
+    eps = 5e-4
+    a = centring.soft_potential(tests, pos, m, off, t_off, eps)
+    pos[7, 0] += 0.003  # in place: the version counter moves
+    b = centring.soft_potential(tests, pos, m, off, t_off, eps)
+    centring.clear_cache()
+    fresh = centring.soft_potential(tests, pos, m, off, t_off, eps)
+    assert torch.equal(b, fresh) and not torch.equal(a, b)
+    other = pos.clone()
+    other[2000, 1] -= 0.002
+    c = centring.soft_potential(tests, other, m, off, t_off, eps)
+    centring.clear_cache()
+    assert torch.equal(c, centring.soft_potential(tests, other, m, off, t_off, eps))
+    centring.clear_cache()
