@@ -9,8 +9,10 @@
 // (hi, lo) pair, dx = (x_j - hi) - lo, so a centre can move by less than an FP32 ulp
 // of its box coordinate.  The same persistent work queue as the unsoftened kernel
 // (halo, 64-test tile, source chunk), FP32 sub-tiles of 8 terms Kahan-added, chunk
-// partials merged in FP64 in chunk order.  Nothing of the N^2 pair terms is stored:
-// the gradient kernel recomputes every term.
+// partials merged in FP64 (in chunk order by a thread per test; for calls with few tests
+// by a wavefront per test, lanes over strided chunks then a fixed butterfly: the same
+// result on every run).  Nothing of the N^2 pair terms is stored: the gradient kernel
+// recomputes every term.
 #include <math.h>
 
 #include <algorithm>
@@ -130,6 +132,53 @@ __global__ void k_soft_tst_halo(const HaloDesc* __restrict__ halos, int n_halo,
 __global__ void k_soft_clear(int* __restrict__ p) {
     if (threadIdx.x == 0) *p = 0;
 }
+
+// The same merge with one wavefront per test, for calls with few tests (a centre per
+// halo against hundreds of source chunks): lane l adds chunks l, l + 64, ... in order,
+// then a butterfly over the lanes (lanes a and a ^ m add the same two values, so every
+// lane ends with the same total, run after run).
+template <bool ACC>
+__global__ __launch_bounds__(256) void k_soft_merge_wave(
+    const HaloDesc* __restrict__ halos, const int* __restrict__ tst_halo, int64_t n_tst,
+    const float2* __restrict__ part, int64_t n_part, double* __restrict__ pot,
+    double* __restrict__ acc, double* __restrict__ wsum) {
+    const int64_t g = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const int lane = __lane_id();
+    if (g >= n_tst) return;  // wave-uniform
+    const HaloDesc d = halos[tst_halo[g]];
+    const int64_t p = g - d.tst_off;
+    const int nch = d.src_len > 0 ? (d.src_len + d.chunk - 1) / d.chunk : 0;
+    constexpr int NQ = ACC ? 4 : 1;
+    double s[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) s[q] = 0.0;
+    for (int c = lane; c < nch; c += kWave) {
+        const int64_t o = d.part_off + static_cast<int64_t>(c) * d.stride + p;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const float2 v = part[q * n_part + o];
+            s[q] += static_cast<double>(v.x) + static_cast<double>(v.y);
+        }
+    }
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m >>= 1)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) s[q] += __shfl_xor(s[q], m, kWave);
+    if (lane != 0) return;
+    if (ACC) {
+        if (acc) {
+            acc[3 * g + 0] = s[0];
+            acc[3 * g + 1] = s[1];
+            acc[3 * g + 2] = s[2];
+        }
+        if (wsum) wsum[g] = s[NQ - 1];
+    } else if (pot) {
+        pot[g] = s[0];
+    }
+}
+
+// tests per call up to which the merge takes a wavefront per test
+constexpr int64_t kWaveMergeTests = 16384;
 
 // FP64 merge of the chunk partials of every test (chunk order); out planes as in k_soft
 template <bool ACC>
@@ -281,14 +330,24 @@ int halma_pot_soft_batched(int64_t n_halo, const int64_t* src_off, const float* 
                                                     n_items, e2, w.part, plan.n_part, w.head);
             HALMA_LAUNCH_CHECK();
         }
-        if (want_acc)
-            k_soft_merge<true><<<mblocks, 256, 0, st>>>(
-                w.halos, w.tst_halo, n_tst, w.part, plan.n_part, nullptr,
-                acc ? acc + 3 * t0 : nullptr, wsum ? wsum + t0 : nullptr);
-        else
+        double* acc_o = acc ? acc + 3 * t0 : nullptr;
+        double* wsum_o = wsum ? wsum + t0 : nullptr;
+        if (n_tst <= kWaveMergeTests) {
+            const unsigned wblocks = static_cast<unsigned>((n_tst * kWave + 255) / 256);
+            if (want_acc)
+                k_soft_merge_wave<true><<<wblocks, 256, 0, st>>>(
+                    w.halos, w.tst_halo, n_tst, w.part, plan.n_part, nullptr, acc_o, wsum_o);
+            else
+                k_soft_merge_wave<false><<<wblocks, 256, 0, st>>>(
+                    w.halos, w.tst_halo, n_tst, w.part, plan.n_part, pot + t0, nullptr, nullptr);
+        } else if (want_acc) {
+            k_soft_merge<true><<<mblocks, 256, 0, st>>>(w.halos, w.tst_halo, n_tst, w.part,
+                                                        plan.n_part, nullptr, acc_o, wsum_o);
+        } else {
             k_soft_merge<false><<<mblocks, 256, 0, st>>>(w.halos, w.tst_halo, n_tst, w.part,
                                                          plan.n_part, pot + t0, nullptr,
                                                          nullptr);
+        }
         HALMA_LAUNCH_CHECK();
     }
     return 0;
