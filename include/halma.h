@@ -438,6 +438,16 @@ int halma_pot_soft_batched(int64_t n_halo, const int64_t *src_off, const float *
                            const double *tz, double eps, double *pot, double *acc,
                            double *wsum, void *workspace, size_t workspace_bytes,
                            void *stream);
+/* The same with the sources packed by the caller: src_xyzm holds (x, y, z, m) per source
+ * as 4 consecutive floats (16-byte aligned), indexed like sm, sx, sy, sz above.  For
+ * callers that evaluate the same sources many times (the centring iterations): the
+ * sources are not repacked on every call.  Same workspace size, results bit-identical
+ * to halma_pot_soft_batched on the same values. */
+int halma_pot_soft_batched_packed(int64_t n_halo, const int64_t *src_off,
+                                  const float *src_xyzm, const int64_t *tst_off,
+                                  const double *tx, const double *ty, const double *tz,
+                                  double eps, double *pot, double *acc, double *wsum,
+                                  void *workspace, size_t workspace_bytes, void *stream);
 
 /* Tuning knobs of the FP32 pair kernels (set by the autotuner; read back with get).
  *   HALMA_TUNE_TESTS_PER_LANE : test particles per lane (1, 2, 4)

@@ -52,8 +52,26 @@ def _src_cols(src_pos):
     return cols
 
 
+def _key(t):
+    t = t.detach()
+    return (t.device, t.data_ptr(), tuple(t.shape), t.stride(), t.dtype, t._version)
+
+
+def _src_packed(src_pos, src_m):
+    """The sources as (x, y, z, m) FP32 rows for halma_pot_soft_batched_packed, kept like
+    the columns above (keyed on both arrays, holding both)."""
+    key = (_key(src_pos), _key(src_m))
+    hit = _src_cache.get("packed")
+    if hit is not None and hit[0] == key:
+        return hit[2]
+    packed = torch.cat([src_pos.detach().float(), src_m.detach().float()[:, None]],
+                       1).contiguous()
+    _src_cache["packed"] = (key, (src_pos.detach(), src_m.detach()), packed)
+    return packed
+
+
 def clear_cache():
-    """Drop the cached source columns (frees their device memory)."""
+    """Drop the cached source columns and packed sources (frees their device memory)."""
     _src_cache.clear()
 
 
@@ -66,9 +84,8 @@ class SoftPotential(torch.autograd.Function):
     @staticmethod
     def forward(ctx, tests, src_pos, src_m, src_off, tst_off, eps, aux=None):
         tx, ty, tz = _cols(tests.detach().double())
-        sx, sy, sz = _src_cols(src_pos)
-        out = engine.pot_soft_batched(src_off, src_m.detach().float(), sx, sy, sz, tst_off,
-                                      tx, ty, tz, eps)["pot"]
+        out = engine.pot_soft_batched(src_off, None, None, None, None, tst_off, tx, ty, tz,
+                                      eps, packed=_src_packed(src_pos, src_m))["pot"]
         ctx.save_for_backward(tests, src_pos, src_m)
         ctx.src_off, ctx.tst_off, ctx.eps, ctx.aux = src_off, tst_off, float(eps), aux
         return out
@@ -79,15 +96,16 @@ class SoftPotential(torch.autograd.Function):
         g = g.detach().double().contiguous()
         gt = gp = gm = None
         tx, ty, tz = _cols(tests.detach().double())
-        sx, sy, sz = _src_cols(src_pos)
         if ctx.needs_input_grad[0]:
-            r = engine.pot_soft_batched(ctx.src_off, src_m.detach().float(), sx, sy, sz,
-                                        ctx.tst_off, tx, ty, tz, ctx.eps, pot=False,
-                                        acc=True, wsum=ctx.aux is not None)
+            r = engine.pot_soft_batched(ctx.src_off, None, None, None, None, ctx.tst_off, tx,
+                                        ty, tz, ctx.eps, pot=False, acc=True,
+                                        wsum=ctx.aux is not None,
+                                        packed=_src_packed(src_pos, src_m))
             gt = (g[:, None] * r["acc"]).to(tests.dtype)
             if ctx.aux is not None:
                 ctx.aux["wsum"] = r["wsum"]
         if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
+            sx, sy, sz = _src_cols(src_pos)
             # dS_i/dx_j = m_j (t_i - x_j)(r^2+eps^2)^-3/2 and dS_i/dm_j = (r^2+eps^2)^-1/2:
             # the same sums with the tests as sources (masses g_i) and the sources as tests
             r = engine.pot_soft_batched(ctx.tst_off, g.float(), tx.float(), ty.float(),

@@ -268,11 +268,17 @@ size_t halma_soft_workspace_size(int64_t n_halo, const int64_t* src_off,
     return carve_soft(nullptr, plan).bytes;
 }
 
-int halma_pot_soft_batched(int64_t n_halo, const int64_t* src_off, const float* sm,
-                           const float* sx, const float* sy, const float* sz,
-                           const int64_t* tst_off, const double* tx, const double* ty,
-                           const double* tz, double eps, double* pot, double* acc,
-                           double* wsum, void* ws, size_t ws_bytes, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// The body of both entries: sources as separate arrays (packed here into the
+// workspace) or, with src_xyzm, already packed by the caller as (x, y, z, m) float4.
+int soft_batched(int64_t n_halo, const int64_t* src_off, const float* sm, const float* sx,
+                 const float* sy, const float* sz, const float* src_xyzm,
+                 const int64_t* tst_off, const double* tx, const double* ty, const double* tz,
+                 double eps, double* pot, double* acc, double* wsum, void* ws,
+                 size_t ws_bytes, void* stream) {
     if (n_halo < 0 || !src_off || !tst_off)
         return set_error(HALMA_E_ARG, "halma_pot_soft_batched: bad halo count or offsets");
     if (!(eps >= 0.0) || !isfinite(eps)) return set_error(HALMA_E_ARG, "eps must be >= 0");
@@ -283,8 +289,11 @@ int halma_pot_soft_batched(int64_t n_halo, const int64_t* src_off, const float* 
     const int64_t n_src = src_off[n_halo] - src_off[0];
     const int64_t n_tst = tst_off[n_halo] - tst_off[0];
     if (n_tst == 0) return 0;
-    if (!tx || !ty || !tz || (n_src > 0 && (!sm || !sx || !sy || !sz)))
+    if (!tx || !ty || !tz ||
+        (n_src > 0 && !src_xyzm && (!sm || !sx || !sy || !sz)))
         return set_error(HALMA_E_ARG, "halma_pot_soft_batched: null pointer");
+    if (src_xyzm && reinterpret_cast<uintptr_t>(src_xyzm) % alignof(float4) != 0)
+        return set_error(HALMA_E_ARG, "packed sources must be 16-byte aligned");
     hipStream_t st = static_cast<hipStream_t>(stream);
     PotPlan plan;
     rc = plan_pot(n_halo, src_off, tst_off, plan, true, 1);
@@ -297,7 +306,10 @@ int halma_pot_soft_batched(int64_t n_halo, const int64_t* src_off, const float* 
         d.src_off -= s0;
         d.tst_off -= t0;
     }
-    if (n_src > 0) {
+    const float4* src4 = w.src4;
+    if (src_xyzm) {
+        src4 = reinterpret_cast<const float4*>(src_xyzm) + s0;
+    } else if (n_src > 0) {
         k_soft_pack<<<static_cast<unsigned>((n_src + 255) / 256), 256, 0, st>>>(
             sm + s0, sx + s0, sy + s0, sz + s0, n_src, w.src4);
         HALMA_LAUNCH_CHECK();
@@ -323,10 +335,10 @@ int halma_pot_soft_batched(int64_t n_halo, const int64_t* src_off, const float* 
         HALMA_LAUNCH_CHECK();
         if (n_items > 0) {
             if (want_acc)
-                k_soft<true><<<grid, 256, 0, st>>>(w.src4, txo, tyo, tzo, w.halos, w.items,
+                k_soft<true><<<grid, 256, 0, st>>>(src4, txo, tyo, tzo, w.halos, w.items,
                                                    n_items, e2, w.part, plan.n_part, w.head);
             else
-                k_soft<false><<<grid, 256, 0, st>>>(w.src4, txo, tyo, tzo, w.halos, w.items,
+                k_soft<false><<<grid, 256, 0, st>>>(src4, txo, tyo, tzo, w.halos, w.items,
                                                     n_items, e2, w.part, plan.n_part, w.head);
             HALMA_LAUNCH_CHECK();
         }
@@ -351,6 +363,30 @@ int halma_pot_soft_batched(int64_t n_halo, const int64_t* src_off, const float* 
         HALMA_LAUNCH_CHECK();
     }
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int halma_pot_soft_batched(int64_t n_halo, const int64_t* src_off, const float* sm,
+                           const float* sx, const float* sy, const float* sz,
+                           const int64_t* tst_off, const double* tx, const double* ty,
+                           const double* tz, double eps, double* pot, double* acc,
+                           double* wsum, void* ws, size_t ws_bytes, void* stream) {
+    return soft_batched(n_halo, src_off, sm, sx, sy, sz, nullptr, tst_off, tx, ty, tz, eps,
+                        pot, acc, wsum, ws, ws_bytes, stream);
+}
+
+int halma_pot_soft_batched_packed(int64_t n_halo, const int64_t* src_off,
+                                  const float* src_xyzm, const int64_t* tst_off,
+                                  const double* tx, const double* ty, const double* tz,
+                                  double eps, double* pot, double* acc, double* wsum, void* ws,
+                                  size_t ws_bytes, void* stream) {
+    if (!src_xyzm && n_halo > 0 && src_off && src_off[n_halo] > src_off[0])
+        return set_error(HALMA_E_ARG, "halma_pot_soft_batched_packed: null sources");
+    return soft_batched(n_halo, src_off, nullptr, nullptr, nullptr, nullptr, src_xyzm, tst_off,
+                        tx, ty, tz, eps, pot, acc, wsum, ws, ws_bytes, stream);
 }
 
 }  // extern "C"

@@ -114,10 +114,12 @@ def pot_sum_f64_plain(m, x, y, z, tx, ty, tz):
 
 
 def pot_soft_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, eps, pot=True, acc=False,
-                     wsum=False):
+                     wsum=False, packed=None):
     """Softened sums per halo (CSR host offsets; see halma_pot_soft_batched):
     pot_i = sum_j m_j (r^2+eps^2)^-1/2, acc_i = d pot_i / d t_i (n_test x 3),
-    wsum_i = sum_j m_j (r^2+eps^2)^-3/2.  Sources FP32, tests FP64.  Returns a dict of
+    wsum_i = sum_j m_j (r^2+eps^2)^-3/2.  Sources FP32, tests FP64.  ``packed``: the
+    sources already as an (N, 4) contiguous FP32 device tensor of (x, y, z, m) rows
+    (halma_pot_soft_batched_packed; m, x, y, z are then not read).  Returns a dict of
     float64 device tensors for the requested outputs."""
     L = _lib_()
     src_off = np.asarray(src_off, np.int64)
@@ -125,7 +127,14 @@ def pot_soft_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, eps, pot=True, ac
     H = len(src_off) - 1
     if len(tst_off) != H + 1:
         raise ValueError("offset arrays must have the same length")
-    m, x, y, z = (to_dev(a, np.float32) for a in (m, x, y, z))
+    if packed is not None:
+        if not isinstance(packed, torch.Tensor) or packed.dim() != 2 or packed.shape[1] != 4:
+            raise ValueError("packed sources must be an (N, 4) tensor of (x, y, z, m) rows")
+        if int(packed.shape[0]) < int(src_off[-1]):  # rows src_off[0] .. src_off[-1] - 1
+            raise ValueError("packed sources shorter than the source offsets")
+        packed = to_dev(packed, np.float32)  # contiguous FP32 on the device
+    else:
+        m, x, y, z = (to_dev(a, np.float32) for a in (m, x, y, z))
     tx, ty, tz = (to_dev(a, np.float64) for a in (tx, ty, tz))
     nt = int(tx.numel())
     dev = tx.device
@@ -141,6 +150,13 @@ def pot_soft_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, eps, pot=True, ac
     so = _lib.i64_array(src_off)
     to = _lib.i64_array(tst_off)
     ws = workspace(L.halma_soft_workspace_size(H, so, to))
+    if packed is not None:
+        rc = L.halma_pot_soft_batched_packed(H, so, ptr(packed), to, ptr(tx), ptr(ty), ptr(tz),
+                                             float(eps), ptr(out.get("pot")),
+                                             ptr(out.get("acc")), ptr(out.get("wsum")),
+                                             ptr(ws), ws.numel(), stream_ptr())
+        _lib.check(rc, "halma_pot_soft_batched_packed")
+        return out
     rc = L.halma_pot_soft_batched(H, so, ptr(m), ptr(x), ptr(y), ptr(z), to, ptr(tx), ptr(ty),
                                   ptr(tz), float(eps), ptr(out.get("pot")),
                                   ptr(out.get("acc")), ptr(out.get("wsum")), ptr(ws),

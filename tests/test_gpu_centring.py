@@ -234,3 +234,28 @@ def test_soft_potential_source_cache_follows_changes(gpu):
     centring.clear_cache()
     assert torch.equal(c, centring.soft_potential(tests, other, m, off, t_off, eps))
     centring.clear_cache()
+
+
+def test_soft_packed_sources_match_separate_arrays(gpu):
+    """halma_pot_soft_batched_packed on (x, y, z, m) rows gives the same sums, gradients
+    and weights, bit for bit, as halma_pot_soft_batched on the separate arrays (with an
+    offset first source, as a sub-range of a larger catalogue)."""
+    import torch
+
+    from pyhalma_amd import engine
+
+    rng = np.random.default_rng(43)
+    n = 5000
+    P = rng.normal(scale=0.01, size=(n, 3)).astype(np.float32)
+    m = rng.uniform(1e4, 1e6, n).astype(np.float32)
+    src_off = np.array([100, 1800, 4700])
+    T = rng.normal(scale=0.004, size=(5, 3))
+    tst_off = np.array([0, 2, 5])
+    eps = 5e-4
+    sep = engine.pot_soft_batched(src_off, m, P[:, 0], P[:, 1], P[:, 2], tst_off, T[:, 0],
+                                  T[:, 1], T[:, 2], eps, pot=True, acc=True, wsum=True)
+    packed = torch.tensor(np.concatenate([P, m[:, None]], 1), device="cuda")
+    pk = engine.pot_soft_batched(src_off, None, None, None, None, tst_off, T[:, 0], T[:, 1],
+                                 T[:, 2], eps, pot=True, acc=True, wsum=True, packed=packed)
+    for k in ("pot", "acc", "wsum"):
+        assert torch.equal(sep[k], pk[k]), k
