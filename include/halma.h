@@ -425,10 +425,15 @@ int halma_energy_test_batched(int64_t n_halo, const int64_t *src_off, const floa
  *   wsum[i]   = sum_j m_j (r_ij^2 + eps^2)^(-3/2)
  * over the sources j of h (FP32 sm, sx, sy, sz) whose three coordinates all differ
  * from the test's (particle_subroutines.f90:499-501; drops the self pair).  Any of
- * pot, acc, wsum may be NULL (the gradient pass runs only when acc or wsum is asked
- * for).  FP32 compensated tiles merged in FP64: pot within halma_pot_error_bound()
- * relative; acc and wsum within the same bound relative to sum_j |term_j|.  The pair
- * terms are recomputed, never stored.  Stream-ordered.
+ * pot, acc, wsum may be NULL; when pot and acc or wsum are asked for together, one pass
+ * over the pairs computes them all.  FP32 compensated sums merged in FP64: pot within
+ * halma_pot_error_bound() relative; acc and wsum within the same bound relative to
+ * sum_j |term_j|.  A halo with at most 8 tests spreads each test's sources over the lanes
+ * of a wavefront (chunks of 4096 sources); larger haloes hold 64 tests per wavefront.
+ * Chunk sizes follow from each halo's own source and test counts and the chunk
+ * partials are added in chunk order, so a test's bits depend on its halo's inputs only
+ * (not on the other haloes of the call).  The pair terms are recomputed, never stored.
+ * Stream-ordered.
  * ---------------------------------------------------------------------------------- */
 size_t halma_soft_workspace_size(int64_t n_halo, const int64_t *src_off,
                                  const int64_t *tst_off);
@@ -448,6 +453,16 @@ int halma_pot_soft_batched_packed(int64_t n_halo, const int64_t *src_off,
                                   const double *tx, const double *ty, const double *tz,
                                   double eps, double *pot, double *acc, double *wsum,
                                   void *workspace, size_t workspace_bytes, void *stream);
+/* The halo potential-energy form: the tests are the sources themselves (halo h's
+ * members src_xyzm rows [offsets[h], offsets[h+1]), packed as above), so pot[k], acc and
+ * wsum are the sums of member k over the other members of its halo (the coordinate rule
+ * drops k itself).  Outputs indexed like the rows.  This is the forward (pot) and
+ * backward (acc: dW/dx_k = -G m_k acc_k for W_h = -G/2 sum m_k pot_k) of the halo
+ * potential energy. */
+size_t halma_soft_self_workspace_size(int64_t n_halo, const int64_t *offsets);
+int halma_pot_soft_self_batched(int64_t n_halo, const int64_t *offsets, const float *src_xyzm,
+                                double eps, double *pot, double *acc, double *wsum,
+                                void *workspace, size_t workspace_bytes, void *stream);
 
 /* Tuning knobs of the FP32 pair kernels (set by the autotuner; read back with get).
  *   HALMA_TUNE_TESTS_PER_LANE : test particles per lane (1, 2, 4)

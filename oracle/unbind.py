@@ -101,9 +101,10 @@ def unbind_iterative(offsets, members, x, y, z, vx, vy, vz, mass, factor, max_it
 
 def rps(gas_x, gas_y, gas_z, gas_vx, gas_vy, gas_vz, gas_mass, gas_temp,
         dm_x, dm_y, dm_z, dm_mass, st_x, st_y, st_z, st_mass, vx, vy, vz,
-        BRUTE_FORCE_LIM, mass_dm_part, num_dm_species, pot_fn=None):
+        BRUTE_FORCE_LIM, mass_dm_part, num_dm_species, pot_fn=None, return_energy=False):
     """halo_gas.py:285-492 restated; ``pot_fn`` defaults to the FP32 restatement of the
-    Fortran routine (what the reference calls)."""
+    Fortran routine (what the reference calls).  With ``return_energy`` the per-cell E
+    (the array the reference thresholds at :470-474) is returned after the four values."""
     if pot_fn is None:
         def pot_fn(m, x, y, z, tx, ty, tz):
             if len(tx) == 0:
@@ -160,13 +161,15 @@ def rps(gas_x, gas_y, gas_z, gas_vx, gas_vy, gas_vz, gas_mass, gas_temp,
     total = np.sum(gas_mass)
     cold_bound = np.sum(gas_mass[cold * bound])
     frac = cold_bound / total if total != 0.0 else 0.0
-    return total, frac, np.sum(gas_mass[unbound * cold]), np.sum(gas_mass[unbound * hot])
+    out = (total, frac, np.sum(gas_mass[unbound * cold]), np.sum(gas_mass[unbound * hot]))
+    return out + (E,) if return_energy else out
 
 
 def most_bound_particle(gas_x, gas_y, gas_z, gas_mass, dm_x, dm_y, dm_z, dm_mass,
                         st_x, st_y, st_z, st_mass, st_oripa, BRUTE_FORCE_LIM,
-                        mass_dm_part, pot_fn=None):
-    """halo_gas.py:498-634 restated (argmin of -S over the stars)."""
+                        mass_dm_part, pot_fn=None, return_energy=False):
+    """halo_gas.py:498-634 restated (argmin of -S over the stars).  With ``return_energy``
+    the per-star -S (the array of the argmin, :632) is returned after the four values."""
     if pot_fn is None:
         def pot_fn(m, x, y, z, tx, ty, tz):
             if len(tx) == 0:
@@ -205,4 +208,5 @@ def most_bound_particle(gas_x, gas_y, gas_z, gas_mass, dm_x, dm_y, dm_z, dm_mass
 
     be = -be
     k = np.argmin(be)
-    return st_x[k], st_y[k], st_z[k], st_oripa[k]
+    out = (st_x[k], st_y[k], st_z[k], st_oripa[k])
+    return out + (be,) if return_energy else out

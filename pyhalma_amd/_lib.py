@@ -43,6 +43,9 @@ SIGNATURES = {
                                           _dbl, _p, _p, _p, _p, _size, _p]),
     "halma_pot_soft_batched_packed": (_c.c_int, [_i64, _i64p, _p, _i64p, _p, _p, _p, _dbl, _p,
                                                  _p, _p, _p, _size, _p]),
+    "halma_soft_self_workspace_size": (_size, [_i64, _i64p]),
+    "halma_pot_soft_self_batched": (_c.c_int, [_i64, _i64p, _p, _dbl, _p, _p, _p, _p, _size,
+                                               _p]),
     "halma_pot_sum_f32": (_c.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p,
                                      _size, _p]),
     "halma_pot_error_bound": (_dbl, []),

@@ -13,7 +13,11 @@ kernels with tests and sources exchanged.
 
 ``potential_energy`` is the halo's softened potential energy
 W_h = -G/2 sum_{i in h} m_i S_i as a differentiable function of the particle positions
-(dW/dx_k = -G m_k a_k).  ``refine_centres`` moves a seed centre per halo downhill on
+and masses (dW/dx_k = -G m_k a_k, dW/dm_k = -G S_k, both by the symmetry of the pair
+terms), through ``HaloPotentialEnergy``: the forward pass is the library's softened sum
+of every member against its halo (tests = the members' own FP32 positions, so the rule
+drops each self pair), the backward pass its hand-written acceleration kernel over the
+same pairs, recomputed.  ``refine_centres`` moves a seed centre per halo downhill on
 the potential Phi(c) = -G S(c): every step takes the autograd gradient of -S at the
 current centres and scales it per halo by 1 / w(c), w = sum_j m_j (r^2+eps^2)^(-3/2),
 which makes the step the fixed-point update c <- sum_j m_j w_j x_j / sum_j m_j w_j and
@@ -84,11 +88,16 @@ class SoftPotential(torch.autograd.Function):
     @staticmethod
     def forward(ctx, tests, src_pos, src_m, src_off, tst_off, eps, aux=None):
         tx, ty, tz = _cols(tests.detach().double())
-        out = engine.pot_soft_batched(src_off, None, None, None, None, tst_off, tx, ty, tz,
-                                      eps, packed=_src_packed(src_pos, src_m))["pot"]
+        # with aux (the centring, which always takes the test gradient next) the gradient
+        # and w come from the same launch as S: one pass over the sources per step
+        fused = aux is not None and ctx.needs_input_grad[0]
+        r = engine.pot_soft_batched(src_off, None, None, None, None, tst_off, tx, ty, tz,
+                                    eps, pot=True, acc=fused, wsum=fused,
+                                    packed=_src_packed(src_pos, src_m))
         ctx.save_for_backward(tests, src_pos, src_m)
         ctx.src_off, ctx.tst_off, ctx.eps, ctx.aux = src_off, tst_off, float(eps), aux
-        return out
+        ctx.fused = (r["acc"], r["wsum"]) if fused else None
+        return r["pot"]
 
     @staticmethod
     def backward(ctx, g):
@@ -97,10 +106,13 @@ class SoftPotential(torch.autograd.Function):
         gt = gp = gm = None
         tx, ty, tz = _cols(tests.detach().double())
         if ctx.needs_input_grad[0]:
-            r = engine.pot_soft_batched(ctx.src_off, None, None, None, None, ctx.tst_off, tx,
-                                        ty, tz, ctx.eps, pot=False, acc=True,
-                                        wsum=ctx.aux is not None,
-                                        packed=_src_packed(src_pos, src_m))
+            if ctx.fused is not None:
+                r = {"acc": ctx.fused[0], "wsum": ctx.fused[1]}
+            else:
+                r = engine.pot_soft_batched(ctx.src_off, None, None, None, None, ctx.tst_off,
+                                            tx, ty, tz, ctx.eps, pot=False, acc=True,
+                                            wsum=ctx.aux is not None,
+                                            packed=_src_packed(src_pos, src_m))
             gt = (g[:, None] * r["acc"]).to(tests.dtype)
             if ctx.aux is not None:
                 ctx.aux["wsum"] = r["wsum"]
@@ -124,16 +136,62 @@ def soft_potential(tests, src_pos, src_m, src_off, tst_off, eps, aux=None):
                                np.asarray(tst_off, np.int64), float(eps), aux)
 
 
+def _halo_sums(v, offsets_dev, offsets):
+    """Per-halo FP64 sums of v over the CSR ranges (the library's double-double group
+    reduction: a fixed order, no atomics)."""
+    n = int(v.numel())
+    grp = engine.Groups(torch.arange(max(n, 1), dtype=torch.int32, device=v.device),
+                        offsets_dev, offsets)
+    z = torch.zeros(max(n, 1), dtype=torch.float32, device=v.device)
+    vv = v if n else torch.zeros(1, dtype=torch.float64, device=v.device)
+    return engine.group_reduce(grp, z, z, z, vv)[:, 0]
+
+
+class HaloPotentialEnergy(torch.autograd.Function):
+    """W_h = -G/2 sum_{i in h} m_i S_i (one per halo, float64) for members pos (N, 3) and
+    masses m (N,) in halo order (CSR host offsets), S_i the softened sum over the other
+    members of i's halo.  Forward: halma_pot_soft_self_batched (S); backward: the same
+    entry's acceleration kernel (a), pair terms recomputed:
+      dW/dx_k = -G m_k a_k,   dW/dm_k = -G S_k."""
+
+    @staticmethod
+    def forward(ctx, pos, m, offsets, eps, G):
+        packed = torch.cat([pos.detach().float(), m.detach().float()[:, None]], 1).contiguous()
+        S = engine.pot_soft_self(offsets, packed, eps, pot=True)["pot"]
+        off_dev = engine.to_dev(offsets, np.int64)
+        W = -0.5 * G * _halo_sums(m.detach().double() * S, off_dev, offsets)
+        ctx.save_for_backward(m, S)
+        ctx.packed, ctx.offsets, ctx.off_dev, ctx.eps, ctx.G = packed, offsets, off_dev, eps, G
+        ctx.pos_dtype = pos.dtype
+        return W
+
+    @staticmethod
+    def backward(ctx, gW):
+        m, S = ctx.saved_tensors
+        seg = torch.repeat_interleave(
+            torch.arange(len(ctx.offsets) - 1, device=S.device),
+            torch.as_tensor(np.diff(ctx.offsets), device=S.device))
+        g = gW.detach().double()[seg]
+        gp = gm = None
+        if ctx.needs_input_grad[0]:
+            a = engine.pot_soft_self(ctx.offsets, ctx.packed, ctx.eps, pot=False,
+                                     acc=True)["acc"]
+            gp = ((-ctx.G * g * m.detach().double())[:, None] * a).to(ctx.pos_dtype)
+        if ctx.needs_input_grad[1]:
+            gm = (-ctx.G * g * S).to(m.dtype)
+        return gp, gm, None, None, None
+
+
 def potential_energy(pos, m, offsets, eps, G=G_CONST):
     """W_h = -G/2 sum_{i in h} m_i S_i (float64, one per halo) for particle positions
-    pos (N, 3) in halo order with CSR host offsets; differentiable in pos (and m)."""
+    pos (N, 3) in halo order with CSR host offsets; differentiable in pos and m.  The
+    pair terms use the FP32 positions and masses (as every pair sum of the library), so
+    a member's own pair is dropped by the coordinate rule whatever the input precision."""
     offsets = np.asarray(offsets, np.int64)
-    S = soft_potential(pos.double(), pos.float(), m, offsets, offsets, eps)
-    e = m.double() * S
-    seg = torch.repeat_interleave(torch.arange(len(offsets) - 1, device=e.device),
-                                  torch.as_tensor(np.diff(offsets), device=e.device))
-    W = torch.zeros(len(offsets) - 1, dtype=torch.float64, device=e.device)
-    return -0.5 * G * W.index_add(0, seg, e)
+    if offsets.ndim != 1 or len(offsets) < 1 or offsets[0] != 0 or offsets[-1] != len(pos) \
+            or len(m) != len(pos) or np.any(np.diff(offsets) < 0):
+        raise ValueError("offsets must be CSR offsets from 0 to len(pos) over pos and m")
+    return HaloPotentialEnergy.apply(pos, m, offsets, float(eps), float(G))
 
 
 class CentringResult:

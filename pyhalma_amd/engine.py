@@ -165,6 +165,40 @@ def pot_soft_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, eps, pot=True, ac
     return out
 
 
+def pot_soft_self(offsets, packed, eps, pot=True, acc=False, wsum=False):
+    """Softened sums of every member of every halo against the other members of its halo
+    (halma_pot_soft_self_batched): the sources are the (N, 4) contiguous FP32 device tensor
+    ``packed`` of (x, y, z, m) rows, haloes the CSR host ``offsets``; the tests are the
+    same rows.  Returns a dict of float64 device tensors (acc: N x 3) for the requested
+    outputs."""
+    L = _lib_()
+    offsets = np.asarray(offsets, np.int64)
+    H = len(offsets) - 1
+    if not isinstance(packed, torch.Tensor) or packed.dim() != 2 or packed.shape[1] != 4:
+        raise ValueError("packed sources must be an (N, 4) tensor of (x, y, z, m) rows")
+    if H < 0 or int(packed.shape[0]) < int(offsets[-1]):
+        raise ValueError("packed sources shorter than the offsets")
+    packed = to_dev(packed, np.float32)
+    n = int(offsets[-1]) if H >= 0 else 0
+    dev = packed.device
+    out = {}
+    if pot:
+        out["pot"] = torch.zeros(n, dtype=torch.float64, device=dev)
+    if acc:
+        out["acc"] = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    if wsum:
+        out["wsum"] = torch.zeros(n, dtype=torch.float64, device=dev)
+    if H <= 0 or n == int(offsets[0]):
+        return out
+    oo = _lib.i64_array(offsets)
+    ws = workspace(L.halma_soft_self_workspace_size(H, oo))
+    rc = L.halma_pot_soft_self_batched(H, oo, ptr(packed), float(eps), ptr(out.get("pot")),
+                                       ptr(out.get("acc")), ptr(out.get("wsum")), ptr(ws),
+                                       ws.numel(), stream_ptr())
+    _lib.check(rc, "halma_pot_soft_self_batched")
+    return out
+
+
 def energy_test_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, vx, vy, vz, halo_v,
                         factor, with_pot=False):
     """One energy test per test particle, every halo at once (halma_energy_test_batched):

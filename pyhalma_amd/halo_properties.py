@@ -148,27 +148,54 @@ def escape_velocity_unbinding_fortran(rete, L, ncoarse, grid_data, gas_data, mas
                                       cx, cy, cz, vx, vy, vz, Rmax, part_list, st_x, st_y,
                                       st_z, st_vx, st_vy, st_vz, st_mass, factor_v, rho_B,
                                       gas_particles=None, dm_mass_to_sun=None):
-    """Reference signature (halo_properties.py:282-361).  The DM selection inside Rmax is
-    done here; the AMR gas cells must be passed already converted to particles
-    (``gas_particles`` = (x, y, z, mass in Msun, comoving) and the MASCLET mass unit as
-    ``dm_mass_to_sun``), because the AMR reader (halo_gas.AMRgrid_to_particles) needs
-    MASCLET grid data."""
-    if gas_particles is None or dm_mass_to_sun is None:
-        raise NotImplementedError(
-            "pass gas_particles=(x, y, z, mass) and dm_mass_to_sun: converting MASCLET AMR "
-            "cells to particles (halo_gas.py:56-141) is outside this package")
-    dm_x, dm_y, dm_z = (np.asarray(a) for a in masclet_dm_data[:3])
-    dm_m = np.asarray(masclet_dm_data[3]) * dm_mass_to_sun
+    """Reference signature and steps (halo_properties.py:282-361): the dark matter of
+    masclet_dm_data within Rmax (masses x masclet's mass unit, ``gathering.MASS_TO_SUN``
+    unless ``dm_mass_to_sun`` is given), the gas particles of the AMR cells within Rmax
+    (``halo_gas.AMRgrid_to_particles`` on the GPU, masses x rete**3) and the halo's stars
+    are the sources, in the reference's order gas, stars, dark matter; the stars are the
+    tests.  Returns the bool mask bound (E <= 0) over part_list.  A caller without the
+    MASCLET grid arrays may pass ``gas_particles`` = (x, y, z, mass in Msun, comoving)
+    instead; the Rmax cut and the rete**3 scaling are applied to them as to the AMR
+    particles."""
+    from . import halo_gas
+    from .gathering import MASS_TO_SUN
+
+    unit = MASS_TO_SUN if dm_mass_to_sun is None else dm_mass_to_sun
+    # dark matter (halo_properties.py:288-300)
+    dm_x = masclet_dm_data[0]
+    dm_y = masclet_dm_data[1]
+    dm_z = masclet_dm_data[2]
+    dm_mass = masclet_dm_data[3] * unit
     inside = np.sqrt((dm_x - cx) ** 2 + (dm_y - cy) ** 2 + (dm_z - cz) ** 2) < Rmax
-    gx, gy, gz, gm = (np.asarray(a) for a in gas_particles)
-    gin = np.sqrt((gx - cx) ** 2 + (gy - cy) ** 2 + (gz - cz) ** 2) < Rmax
-    gm = gm[gin] * rete ** 3
-    sources = (np.concatenate((gx[gin], dm_x[inside])), np.concatenate((gy[gin], dm_y[inside])),
-               np.concatenate((gz[gin], dm_z[inside])), np.concatenate((gm, dm_m[inside])))
-    # the reference orders the sources gas, stars, DM; the sum is order-insensitive up to
-    # rounding, and the GPU sum is compensated
-    return escape_velocity_unbinding(part_list, st_x, st_y, st_z, st_vx, st_vy, st_vz,
-                                     st_mass, vx, vy, vz, factor_v, extra_sources=sources)
+    dm_x, dm_y, dm_z, dm_mass = dm_x[inside], dm_y[inside], dm_z[inside], dm_mass[inside]
+    # the halo's stars (:303-309)
+    p = np.asarray(part_list, dtype=np.int64)
+    st_x_halo, st_y_halo, st_z_halo = (np.asarray(a)[p] for a in (st_x, st_y, st_z))
+    st_vx_halo, st_vy_halo, st_vz_halo = (np.asarray(a)[p] for a in (st_vx, st_vy, st_vz))
+    st_mass_halo = np.asarray(st_mass)[p]
+    # gas particles of the AMR cells (:313-328)
+    if gas_particles is None:
+        g = halo_gas.AMRgrid_to_particles(L, ncoarse, grid_data, gas_data, Rmax, cx, cy, cz,
+                                          rho_B)
+        gas_x, gas_y, gas_z, gas_mass = g[0], g[1], g[2], g[6]
+    else:
+        gas_x, gas_y, gas_z, gas_mass = (np.asarray(a) for a in gas_particles)
+    gin = np.sqrt((gas_x - cx) ** 2 + (gas_y - cy) ** 2 + (gas_z - cz) ** 2) < Rmax
+    gas_x, gas_y, gas_z = gas_x[gin], gas_y[gin], gas_z[gin]
+    gas_mass = gas_mass[gin]
+    gas_mass *= rete ** 3
+    # sources gas, stars, dark matter; tests the stars (:335-340)
+    total_x = np.concatenate((gas_x, st_x_halo, dm_x))
+    total_y = np.concatenate((gas_y, st_y_halo, dm_y))
+    total_z = np.concatenate((gas_z, st_z_halo, dm_z))
+    total_mass = np.concatenate((gas_mass, st_mass_halo, dm_mass))
+    be = halo_gas.brute_force_binding_energy_fortran(total_mass, total_x, total_y, total_z,
+                                                     st_x_halo, st_y_halo, st_z_halo)
+    be = -be
+    be *= G_CONST
+    be *= factor_v ** 2
+    kin = 0.5 * ((st_vx_halo - vx) ** 2 + (st_vy_halo - vy) ** 2 + (st_vz_halo - vz) ** 2)
+    return kin + be <= 0.0
 
 
 def _arange_grid(grid):

@@ -259,3 +259,108 @@ def test_soft_packed_sources_match_separate_arrays(gpu):
                                  T[:, 2], eps, pot=True, acc=True, wsum=True, packed=packed)
     for k in ("pot", "acc", "wsum"):
         assert torch.equal(sep[k], pk[k]), k
+
+
+def test_potential_energy_fp64_positions_drop_self_pair(gpu):
+    """Positions given in FP64 that FP32 cannot hold: the energy uses the FP32 values for
+    every pair (tests and sources alike), so each member's own pair is dropped and W
+    equals the FP64 oracle sum over j != i at the FP32 points, and the gradient is the
+    FP64 acceleration at the same points."""
+    from pyhalma_amd import centring
+
+    rng = np.random.default_rng(51)
+    n = 1500
+    p64 = rng.normal(scale=0.01, size=(n, 3)) + 0.123456789012345  # not FP32 values
+    assert np.any(p64.astype(np.float32).astype(np.float64) != p64)
+    m = rng.uniform(1e4, 1e6, n).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    pos = torch.from_numpy(p64).to(dev).requires_grad_(True)
+    mt = torch.from_numpy(m).to(dev)
+    W = centring.potential_energy(pos, mt, [0, n], EPS)
+    W.sum().backward()
+    p32 = p64.astype(np.float32)
+    S = oracle.pot_soft_f64(m, p32[:, 0], p32[:, 1], p32[:, 2], *(p32[:, k].astype(np.float64)
+                                                                 for k in range(3)), EPS)
+    G = centring.G_CONST
+    Wo = -0.5 * G * float(np.sum(m.astype(np.float64) * S))
+    B = engine_bound()
+    assert abs(W.item() - Wo) <= B * abs(Wo), (W.item(), Wo)
+    t = p32.astype(np.float64)
+    Ao = oracle.acc_soft_f64(m, p32[:, 0], p32[:, 1], p32[:, 2], t[:, 0], t[:, 1], t[:, 2], EPS)
+    mag, _ = _terms_abs(m, p32[:, 0], p32[:, 1], p32[:, 2], t, EPS)
+    want = -G * m[:, None].astype(np.float64) * Ao
+    g = pos.grad.cpu().numpy()
+    assert np.all(np.abs(g - want) <= B * G * m[:, None] * mag + 1e-300)
+    print(f"n={n} W={W.item():.6e} oracle={Wo:.6e}", flush=True)
+
+
+def engine_bound():
+    from pyhalma_amd import engine
+
+    return engine.pot_error_bound()
+
+
+def test_soft_sums_independent_of_other_haloes_and_merge_path(gpu):
+    """A halo's softened sums are the same bits whether it is alone in a call (fewer than
+    16384 tests: a wavefront merges each test) or shares it with a larger halo (a thread
+    per test), and the large call's sums stay within the bound of the FP64 oracle."""
+    from pyhalma_amd import engine
+
+    rng = np.random.default_rng(52)
+    na, nb = 3000, 17000
+    P = rng.normal(scale=0.01, size=(na + nb, 3)).astype(np.float32)
+    m = rng.uniform(1e4, 1e6, na + nb).astype(np.float32)
+    T = P.astype(np.float64) + rng.normal(scale=1e-4, size=P.shape)
+    one = engine.pot_soft_batched([0, na], m, P[:, 0], P[:, 1], P[:, 2], [0, na], T[:na, 0],
+                                  T[:na, 1], T[:na, 2], EPS, pot=True, acc=True, wsum=True)
+    both = engine.pot_soft_batched([0, na, na + nb], m, P[:, 0], P[:, 1], P[:, 2],
+                                   [0, na, na + nb], T[:, 0], T[:, 1], T[:, 2], EPS, pot=True,
+                                   acc=True, wsum=True)
+    for k in ("pot", "acc", "wsum"):
+        assert torch.equal(one[k], both[k][:na]), k
+    B = engine.pot_error_bound()
+    pot = both["pot"].cpu().numpy()[na:]
+    So = oracle.pot_soft_f64(m[na:], P[na:, 0], P[na:, 1], P[na:, 2], T[na:, 0], T[na:, 1],
+                             T[na:, 2], EPS)
+    assert np.all(np.abs(pot - So) <= B * So)
+    # the pot-only and gradient-only calls give the same bits as the combined one
+    po = engine.pot_soft_batched([0, na], m, P[:, 0], P[:, 1], P[:, 2], [0, na], T[:na, 0],
+                                 T[:na, 1], T[:na, 2], EPS, pot=True)
+    ao = engine.pot_soft_batched([0, na], m, P[:, 0], P[:, 1], P[:, 2], [0, na], T[:na, 0],
+                                 T[:na, 1], T[:na, 2], EPS, pot=False, acc=True, wsum=True)
+    assert torch.equal(po["pot"], one["pot"])
+    assert torch.equal(ao["acc"], one["acc"]) and torch.equal(ao["wsum"], one["wsum"])
+
+
+def test_soft_few_tests_lanes_over_sources(gpu):
+    """Haloes with one to eight tests (the wavefront spreads the sources over its lanes)
+    and one with nine (64 tests per wavefront), against the FP64 oracle, with a source
+    count that leaves a ragged last chunk."""
+    from pyhalma_amd import engine
+
+    rng = np.random.default_rng(53)
+    ns = [9000, 4097, 1, 0, 12345]
+    nt = [1, 8, 3, 2, 9]
+    so = np.concatenate([[0], np.cumsum(ns)])
+    to = np.concatenate([[0], np.cumsum(nt)])
+    P = rng.normal(scale=0.01, size=(so[-1], 3)).astype(np.float32)
+    m = rng.uniform(1e4, 1e6, so[-1]).astype(np.float32)
+    T = rng.normal(scale=0.01, size=(to[-1], 3))
+    T[0] = P[5]  # a test on a source: that pair is dropped
+    r = engine.pot_soft_batched(so, m, P[:, 0], P[:, 1], P[:, 2], to, T[:, 0], T[:, 1],
+                                T[:, 2], EPS, pot=True, acc=True, wsum=True)
+    pot, acc, ws = (r[k].cpu().numpy() for k in ("pot", "acc", "wsum"))
+    B = engine.pot_error_bound()
+    for h in range(len(ns)):
+        a, b = to[h], to[h + 1]
+        s, e = so[h], so[h + 1]
+        if e == s:
+            assert np.all(pot[a:b] == 0) and np.all(acc[a:b] == 0)
+            continue
+        args = (m[s:e], P[s:e, 0], P[s:e, 1], P[s:e, 2])
+        So = oracle.pot_soft_f64(*args, T[a:b, 0], T[a:b, 1], T[a:b, 2], EPS)
+        Ao = oracle.acc_soft_f64(*args, T[a:b, 0], T[a:b, 1], T[a:b, 2], EPS)
+        mag, wo = _terms_abs(*args, T[a:b], EPS)
+        assert np.all(np.abs(pot[a:b] - So) <= B * So + 1e-300), h
+        assert np.all(np.abs(acc[a:b] - Ao) <= B * mag + 1e-300), h
+        assert np.all(np.abs(ws[a:b] - wo) <= B * wo + 1e-300), h
