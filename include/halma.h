@@ -351,14 +351,17 @@ int halma_progenitors(int64_t n_new, int64_t n_new_members, const int64_t *new_o
  * lies inside the rigorous error band are re-tested with an FP64 sum in the oracle's
  * fixed order.  Outputs (device): bound[k] per member position (1 bound, 0 not); iters
  * (n_halo int32, may be NULL) passes per halo; vcm (3 per halo, may be NULL); pot (per
- * member, the last S used, may be NULL).  stats (HOST, 12 int64, may be NULL): [0] passes,
+ * member, the last S used, may be NULL).  stats (HOST, 13 int64, may be NULL): [0] passes,
  * [1] members re-tested in FP64, [2] symmetric-kernel queue items of the first pass,
  * [3] tests per lane, [4] sub-tile, [5] pair-kernel time in ns and [6] timed passes
  * (with HALMA_UNBIND_TIME_POT: hipEvents recorded on the stream around the pair
  * kernels), [7] pairs of the reference's loop (n_bound x n_sources summed over haloes
  * and passes), [8] symmetric-kernel time in ns (part of [5]), [9] its launches,
  * [10] pairs of the symmetric kernel's passes (part of [7]; it evaluates each unordered
- * pair once for two of them), [11] chunked-kernel launches.  Full passes (first pass, or
+ * pair once for two of them), [11] chunked-kernel launches, [12] members of [1] re-summed
+ * because their S had an error bound above 1e-6 of S (a kept member whose incremental
+ * updates took its rigorous bound past 1e-6 gets a fresh FP64 sum, so every S the call
+ * keeps is within 1e-6 of the FP64 direct sum).  Full passes (first pass, or
  * more than a quarter of the live set removed) go to the symmetric kernel for haloes of
  * at most HALMA_TUNE_SYM_MAX members, to the chunked kernel otherwise; incremental passes
  * go to the chunked kernel.  Passes are queued one ahead of the host: the counters of
@@ -401,8 +404,10 @@ int halma_unbind_plain_f64(int64_t n_halo, const int64_t *offsets, const int32_t
  * three coordinates all differ from test i's; tests [tst_off[h], tst_off[h+1]) (HOST
  * offsets).  halo_v: 3 doubles per halo (device).  bound and s_out (may be NULL) are
  * indexed like the tests.  Decisions equal those made with the FP64 sum in the oracle's
- * fixed order (band re-test, see halma_unbind_batched).  stats (HOST, may be NULL):
- * [0] tests re-tested in FP64 (the call then synchronises the stream).
+ * fixed order (band re-test, see halma_unbind_batched).  stats (HOST, 3 int64, may be
+ * NULL; the call then synchronises the stream): [0] tests re-tested in FP64, [1] time of
+ * the pair kernel in ns (hipEvents on the stream around its launch), [2] (test, source)
+ * pairs summed.
  * ---------------------------------------------------------------------------------- */
 size_t halma_energy_test_workspace_size(int64_t n_halo, const int64_t *src_off,
                                         const int64_t *tst_off);

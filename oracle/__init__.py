@@ -17,6 +17,7 @@ from .native import (  # noqa: F401
     pot_soft_f64,
     acc_soft_f64,
     fof_labels_c,
+    fof_labels_cells,
     ref_pot_sum_f32,
     have_ref,
 )

@@ -274,3 +274,129 @@ int orc_fof_labels(int64_t n, const double *x, const double *y, const double *z,
     free(cx);
     return 0;
 }
+
+/* The same labels as orc_fof_labels, computed for catalogues with dense clumps (a
+ * checker fast enough for millions of particles).  Particles are binned in sub-cells of
+ * side ll / sqrt(3) * (1 - 1e-6): any two particles of one sub-cell are closer than ll
+ * even after the FP64 rounding of the cell index and of d^2, so a sub-cell is one
+ * connected piece.  Two sub-cells within Chebyshev distance 2 (the farthest a link of
+ * length ll can reach) are joined as soon as one of their pairs passes the same FP64
+ * predicate ((dx*dx + dy*dy) + dz*dz) <= ll*ll; pairs of sub-cells already in one
+ * component are not tested.  labels[i] = min index of i's component.  Falls back to
+ * orc_fof_labels for ll <= 0 or for a grid of more than 2^21 sub-cells per axis.
+ * Returns 0, or -1 on allocation failure. */
+int orc_fof_labels_cells(int64_t n, const double *x, const double *y, const double *z,
+                         double ll, int64_t *labels) {
+    if (n <= 0) return 0;
+    if (!(ll > 0.0)) return orc_fof_labels(n, x, y, z, ll, labels);
+    const double ll2 = ll * ll;
+    double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+    for (int64_t i = 1; i < n; ++i) {
+        if (x[i] < lo[0]) lo[0] = x[i];
+        if (x[i] > hi[0]) hi[0] = x[i];
+        if (y[i] < lo[1]) lo[1] = y[i];
+        if (y[i] > hi[1]) hi[1] = y[i];
+        if (z[i] < lo[2]) lo[2] = z[i];
+        if (z[i] > hi[2]) hi[2] = z[i];
+    }
+    const double side = ll / sqrt(3.0) * (1.0 - 1e-6);
+    int64_t nc[3];
+    for (int d = 0; d < 3; ++d) {
+        const double cells = floor((hi[d] - lo[d]) / side) + 1.0;
+        if (!(cells < 2097152.0)) return orc_fof_labels(n, x, y, z, ll, labels);
+        nc[d] = (int64_t)cells;
+    }
+    key_idx *ki = (key_idx *)malloc(sizeof(key_idx) * (size_t)n);
+    if (!ki) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t c0 = (int64_t)floor((x[i] - lo[0]) / side);
+        const int64_t c1 = (int64_t)floor((y[i] - lo[1]) / side);
+        const int64_t c2 = (int64_t)floor((z[i] - lo[2]) / side);
+        ki[i].key = (c0 * nc[1] + c1) * nc[2] + c2;
+        ki[i].idx = i;
+    }
+    qsort(ki, (size_t)n, sizeof(key_idx), cmp_key_idx);
+    /* cells: runs of one key in the sorted order */
+    int64_t ncell = 0;
+    for (int64_t s = 0; s < n; ++s)
+        if (s == 0 || ki[s].key != ki[s - 1].key) ++ncell;
+    int64_t *start = (int64_t *)malloc(sizeof(int64_t) * (size_t)(ncell + 1));
+    int64_t *ckey = (int64_t *)malloc(sizeof(int64_t) * (size_t)ncell);
+    int64_t *parent = (int64_t *)malloc(sizeof(int64_t) * (size_t)ncell);
+    int64_t *cmin = (int64_t *)malloc(sizeof(int64_t) * (size_t)ncell);
+    if (!start || !ckey || !parent || !cmin) {
+        free(ki);
+        free(start);
+        free(ckey);
+        free(parent);
+        free(cmin);
+        return -1;
+    }
+    int64_t c = -1;
+    for (int64_t s = 0; s < n; ++s) {
+        if (s == 0 || ki[s].key != ki[s - 1].key) {
+            ++c;
+            start[c] = s;
+            ckey[c] = ki[s].key;
+            parent[c] = c;
+            cmin[c] = ki[s].idx;
+        } else if (ki[s].idx < cmin[c]) {
+            cmin[c] = ki[s].idx;
+        }
+    }
+    start[ncell] = n;
+    for (int64_t a = 0; a < ncell; ++a) {
+        const int64_t k = ckey[a];
+        const int64_t a2 = k % nc[2], a1 = (k / nc[2]) % nc[1], a0 = k / (nc[1] * nc[2]);
+        for (int64_t d0 = -2; d0 <= 2; ++d0) {
+            const int64_t b0 = a0 + d0;
+            if (b0 < 0 || b0 >= nc[0]) continue;
+            for (int64_t d1 = -2; d1 <= 2; ++d1) {
+                const int64_t b1 = a1 + d1;
+                if (b1 < 0 || b1 >= nc[1]) continue;
+                for (int64_t d2 = -2; d2 <= 2; ++d2) {
+                    const int64_t b2 = a2 + d2;
+                    if (b2 < 0 || b2 >= nc[2]) continue;
+                    const int64_t kb = (b0 * nc[1] + b1) * nc[2] + b2;
+                    if (kb <= k) continue; /* each unordered pair of cells once */
+                    int64_t l = a + 1, h = ncell;
+                    while (l < h) {
+                        const int64_t mid = (l + h) >> 1;
+                        if (ckey[mid] < kb) l = mid + 1;
+                        else h = mid;
+                    }
+                    if (l >= ncell || ckey[l] != kb) continue;
+                    if (uf_find(parent, a) == uf_find(parent, l)) continue;
+                    int linked = 0;
+                    for (int64_t s = start[a]; s < start[a + 1] && !linked; ++s) {
+                        const int64_t i = ki[s].idx;
+                        for (int64_t t = start[l]; t < start[l + 1]; ++t) {
+                            const int64_t j = ki[t].idx;
+                            const double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+                            if ((dx * dx + dy * dy) + dz * dz <= ll2) {
+                                linked = 1;
+                                break;
+                            }
+                        }
+                    }
+                    if (linked) uf_union(parent, a, l);
+                }
+            }
+        }
+    }
+    /* component minimum particle index at each root, then the labels */
+    for (int64_t a = 0; a < ncell; ++a) {
+        const int64_t r = uf_find(parent, a);
+        if (cmin[a] < cmin[r]) cmin[r] = cmin[a];
+    }
+    for (int64_t a = 0; a < ncell; ++a) {
+        const int64_t m = cmin[uf_find(parent, a)];
+        for (int64_t s = start[a]; s < start[a + 1]; ++s) labels[ki[s].idx] = m;
+    }
+    free(ki);
+    free(start);
+    free(ckey);
+    free(parent);
+    free(cmin);
+    return 0;
+}

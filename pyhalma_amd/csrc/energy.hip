@@ -156,7 +156,7 @@ int halma_energy_test_batched(int64_t n_halo, const int64_t* src_off, const floa
     if (vel_dtype != HALMA_F32 && vel_dtype != HALMA_F64)
         return set_error(HALMA_E_ARG, "bad velocity dtype");
     if (!isfinite(factor)) return set_error(HALMA_E_ARG, "factor must be finite");
-    if (stats) stats[0] = 0;
+    if (stats) stats[0] = stats[1] = stats[2] = 0;
     if (n_halo == 0) return 0;
     for (int64_t h = 0; h < n_halo; ++h)
         if (src_off[h + 1] < src_off[h] || tst_off[h + 1] < tst_off[h])
@@ -181,9 +181,20 @@ int halma_energy_test_batched(int64_t n_halo, const int64_t* src_off, const floa
     const int64_t t0 = tst_off[0];
     std::vector<int64_t> to_rel(n_halo + 1);  // tests from zero: S and the mask are dense
     for (int64_t h = 0; h <= n_halo; ++h) to_rel[h] = tst_off[h] - t0;
+    // with stats, events bracket the pair kernel (its time is reported)
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (stats)
+        for (auto& e : ev) HALMA_HIP(hipEventCreate(&e));
+    auto drop_events = [&]() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
     rc = pot_batched(n_halo, src_off, sm, sx, sy, sz, to_rel.data(), tx + t0, ty + t0, tz + t0,
-                     w.S, nullptr, w.pot_ws, w.pot_bytes, st);
-    if (rc) return rc;
+                     w.S, nullptr, w.pot_ws, w.pot_bytes, st, ev[0], ev[1]);
+    if (rc) {
+        drop_events();
+        return rc;
+    }
     k_et_tst_halo<<<static_cast<unsigned>(n_halo), kThreads, 0, st>>>(
         w.off + n_halo + 1, static_cast<int>(n_halo), w.tst_halo);
     HALMA_LAUNCH_CHECK();
@@ -222,10 +233,23 @@ int halma_energy_test_batched(int64_t n_halo, const int64_t* src_off, const floa
     HALMA_LAUNCH_CHECK();
     if (stats) {
         long long* host = pinned_words(2);
-        if (!host) return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
-        HALMA_HIP(hipMemcpyAsync(host, w.counters, sizeof(long long), hipMemcpyDeviceToHost, st));
-        HALMA_HIP(hipStreamSynchronize(st));
+        if (!host) {
+            drop_events();
+            return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
+        }
+        hipError_t e = hipMemcpyAsync(host, w.counters, sizeof(long long),
+                                      hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        drop_events();
+        if (e != hipSuccess) return hip_fail(e, "energy test statistics", __FILE__, __LINE__);
         stats[0] = host[0];
+        stats[1] = static_cast<int64_t>(static_cast<double>(ms) * 1e6);
+        int64_t pairs = 0;
+        for (int64_t h = 0; h < n_halo; ++h)
+            pairs += (src_off[h + 1] - src_off[h]) * (tst_off[h + 1] - tst_off[h]);
+        stats[2] = pairs;
     }
     return 0;
 }

@@ -55,10 +55,13 @@ struct DynSets {
 // Batched FP32 compensated pair sums over H haloes (HOST CSR offsets), FP64 (and/or
 // FP32) result per test; scratch from pot_batched_workspace_bytes.  Stream-ordered.
 size_t pot_batched_workspace_bytes(int64_t H, const int64_t* src_off, const int64_t* tst_off);
+// ev_begin / ev_end (may be null): events recorded on the stream right before and after
+// the pair kernel's launch.
 int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const float* sx,
                 const float* sy, const float* sz, const int64_t* tst_off, const float* tx,
                 const float* ty, const float* tz, double* out_f64, float* out_f32, void* ws,
-                size_t ws_bytes, hipStream_t stream);
+                size_t ws_bytes, hipStream_t stream, hipEvent_t ev_begin = nullptr,
+                hipEvent_t ev_end = nullptr);
 
 // Launches the persistent FP32 kernel for a plan whose tables are already on the
 // device.  head must be a device int holding 0 at launch: this launcher clears it

@@ -10,6 +10,7 @@
 // of split source ranges are merged in FP64 in chunk order (deterministic).
 #include <algorithm>
 #include <numeric>
+#include <vector>
 
 #include "halma_common.h"
 #include "potential.h"
@@ -396,7 +397,8 @@ size_t pot_batched_workspace_bytes(int64_t H, const int64_t* src_off, const int6
 int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const float* sx,
                        const float* sy, const float* sz, const int64_t* tst_off,
                        const float* tx, const float* ty, const float* tz, double* out_f64,
-                       float* out_f32, void* ws, size_t ws_bytes, hipStream_t stream) {
+                       float* out_f32, void* ws, size_t ws_bytes, hipStream_t stream,
+                       hipEvent_t ev_begin, hipEvent_t ev_end) {
     PotPlan plan;
     int rc = plan_pot(H, src_off, tst_off, plan);
     if (rc) return rc;
@@ -421,8 +423,13 @@ int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const float*
     k_tst_halo<<<static_cast<unsigned>(H), 256, 0, stream>>>(w.halos, static_cast<int>(H),
                                                              w.tst_halo);
     HALMA_LAUNCH_CHECK();
-    rc = launch_pot_f32(plan, w.src4, w.tst4, w.halos, DynSets(), w.items, w.part, w.head, stream);
+    k_fill_i32<<<1, 64, 0, stream>>>(w.head, 1, 0);
+    HALMA_LAUNCH_CHECK();
+    if (ev_begin) HALMA_HIP(hipEventRecord(ev_begin, stream));
+    rc = launch_pot_f32(plan, w.src4, w.tst4, w.halos, DynSets(), w.items, w.part, w.head, stream,
+                        false);
     if (rc) return rc;
+    if (ev_end) HALMA_HIP(hipEventRecord(ev_end, stream));
     const int64_t blocks = (plan.n_tst + 255) / 256;
     k_pot_merge<<<static_cast<unsigned>(blocks), 256, 0, stream>>>(
         w.halos, w.tst_halo, plan.n_tst, w.part, out_f64 ? out_f64 + t0 : nullptr,

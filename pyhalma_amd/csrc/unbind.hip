@@ -100,29 +100,26 @@ __global__ void k_member_halo(const int64_t* __restrict__ off, int H, int32_t* _
 }
 
 __global__ void k_init_halo(const int64_t* __restrict__ off, int H, int32_t* __restrict__ active,
-                            int32_t* __restrict__ iters, int32_t* __restrict__ removed,
-                            int32_t* __restrict__ need_full, int32_t* __restrict__ refresh) {
+                            int32_t* __restrict__ iters, int32_t* __restrict__ removed) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     active[h] = off[h + 1] > off[h] ? 1 : 0;
     iters[h] = 0;
     removed[h] = 0;
-    need_full[h] = 0;
-    refresh[h] = 0;
 }
 
 // A halo's pass is full (tests = sources = the live bound set, symmetric kernel) on
-// the first pass, whenever more than a quarter of the live set left in the previous
-// pass, and when a member kept by the previous pass had an accumulated error bound above
-// kPotRelMax of its S; otherwise incremental (the removed members' terms are subtracted).
-__device__ __forceinline__ bool pass_is_full(int pass_no, int run, int rrun, int need_full) {
-    return pass_no == 0 || need_full != 0 || 4 * static_cast<int64_t>(rrun) > run;
+// the first pass and whenever more than a quarter of the live set left in the previous
+// pass; otherwise incremental (the removed members' terms are subtracted).
+__device__ __forceinline__ bool pass_is_full(int pass_no, int run, int rrun) {
+    return pass_no == 0 || 4 * static_cast<int64_t>(rrun) > run;
 }
 
-// Largest relative error bound a member's S may keep: a pass whose update leaves a kept
-// member above it makes the halo's next pass a full one (and a halo that would stop there
-// takes one more full pass, which is not counted in its iterations).  Both full-pass
-// kernels are below it (halma_pot_error_bound at sub-tiles of 8, halma_pot_sym_error_bound).
+// Largest relative error bound a member's S may keep: a kept member whose accumulated
+// bound (the full pass's, plus that of every incremental subtraction since) goes above it
+// is re-summed in FP64 by the re-test kernel, like the members of the decision band.
+// Both full-pass kernels stay below it (halma_pot_error_bound at sub-tiles of 4 or 8,
+// halma_pot_sym_error_bound).
 constexpr double kPotRelMax = 1e-6;
 
 __global__ void k_zero_counters(unsigned long long* __restrict__ c, int n) {
@@ -250,7 +247,7 @@ __global__ __launch_bounds__(kThreads) void k_halo_sets(
     int32_t* __restrict__ mode, double* __restrict__ vcm, int32_t* __restrict__ tst_len,
     int32_t* __restrict__ src_len, int32_t* __restrict__ src_sel, int64_t* __restrict__ icnt,
     int64_t* __restrict__ icnt_sym, const SymDesc* __restrict__ sym,
-    int32_t* __restrict__ need_full, unsigned long long* __restrict__ counters) {
+    unsigned long long* __restrict__ counters) {
     const int lane = __lane_id();
     const int pos = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x +
                                       threadIdx.x) / kWave);
@@ -317,8 +314,7 @@ __global__ __launch_bounds__(kThreads) void k_halo_sets(
     const double M = acc[0][0] + acc[0][1];
     for (int d = 0; d < 3; ++d)
         vcm[3 * h + d] = M > 0.0 ? (acc[1 + d][0] + acc[1 + d][1]) / M : 0.0;
-    const int full = pass_is_full(*pass_no, run, rrun, need_full[h]) ? 1 : 0;
-    need_full[h] = 0;
+    const int full = pass_is_full(*pass_no, run, rrun) ? 1 : 0;
     n_bound[h] = run;
     n_rem[h] = rrun;
     mode[h] = full;
@@ -393,11 +389,10 @@ struct EnergyArgs {
     double* eacc;      // absolute error bound of sacc
     int32_t* removed;
     int32_t* band;
-    int32_t* need_full;  // per halo: a kept member's error bound exceeded kPotRelMax
-    int32_t* refresh;    // per halo: the pass only refreshes S (not an iteration)
     unsigned long long* counters;  // [0] removed total, [1] band count, [2] band total,
                                    // [3] pairs, [4] symmetric-kernel pairs, [5] haloes
-                                   // still active after the pass, [7] the re-test
+                                   // still active after the pass, [6] members re-summed
+                                   // for their error bound alone, [7] the re-test
                                    // kernel's block ticket
     double* s_out;     // optional: S per member position (FP64)
     int64_t nm;
@@ -475,13 +470,14 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     if (a.s_out) a.s_out[k] = S;
     const bool in_band = fabs(E) <= band;
     const bool gone = !in_band && E > 0.0;
-    // a kept member decided outside the band with a bound above the limit: the next pass
-    // recomputes the halo (band members get an FP64 S from the re-test)
-    if (!gone && !in_band && err > kPotRelMax * S) a.need_full[h] = 1;
-    if (in_band) {
+    // a kept member whose error bound went above kPotRelMax: its S is re-summed in FP64
+    // with the band (its decision, already exact, stays the same)
+    const bool stale = !gone && !in_band && err > kPotRelMax * S;
+    if (in_band || stale) {
         const unsigned long long at = atomicAdd(a.counters + 1, 1ull);
         a.band[at] = static_cast<int32_t>(g);
         atomicAdd(a.counters + 2, 1ull);
+        if (stale) atomicAdd(a.counters + 6, 1ull);
     } else if (gone) {
         a.state[k] = kJustGone;
     }
@@ -503,29 +499,24 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 // End of a pass for the halo h (every thread of the block calls it): per-halo iteration
-// counts, the halo stays active while it removed someone (or, once, for a full pass that
-// refreshes S when a kept member's error bound went above kPotRelMax: that pass removes
-// nobody, since every decision is exact, and is not counted as an iteration), and the
-// counters (wave-aggregated atomics): [0] removed total, [3] pairs of the reference's
-// loop this pass (n_bound x n_src), [4] those of the symmetric kernel's passes, [5]
-// haloes still active.
+// counts, the halo stays active while it removed someone, and the counters
+// (wave-aggregated atomics): [0] removed total, [3] pairs of the reference's loop this
+// pass (n_bound x n_src), [4] those of the symmetric kernel's passes, [5] haloes still
+// active.
 __device__ void finish_haloes(const EnergyArgs& a, int h) {
     unsigned long long rem = 0, pairs = 0, full_pairs = 0, live = 0;
     if (h < a.H && a.active[h]) {
         const int nb = a.n_bound[h];
         const int r = a.removed[h];
-        const bool refreshing = a.refresh[h] != 0;
         if (nb > 0) {
-            if (!(refreshing && r == 0)) a.iters[h] += 1;
+            a.iters[h] += 1;
             pairs = static_cast<unsigned long long>(nb) *
                     static_cast<unsigned long long>(a.n_src[h]);
             if (a.mode[h] && a.sym[h].nb > 0) full_pairs = pairs;  // symmetric kernel's
         }
-        bool stay = r > 0 && nb > 0;
-        const bool refresh = !stay && nb > 0 && !refreshing && a.need_full[h] != 0;
-        a.refresh[h] = refresh ? 1 : 0;
-        if (!stay && !refresh) a.active[h] = 0;
-        live = (stay || refresh) ? 1 : 0;
+        const bool stay = r > 0 && nb > 0;
+        if (!stay) a.active[h] = 0;
+        live = stay ? 1 : 0;
         rem = static_cast<unsigned long long>(r);
         a.removed[h] = 0;
     }
@@ -746,8 +737,6 @@ struct UnbWs {
     int32_t* active;
     int32_t* iters;
     int32_t* removed;
-    int32_t* need_full;
-    int32_t* refresh;
     int32_t* pass_no;
     double* vcm;
     int32_t* band;
@@ -839,8 +828,6 @@ UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, const Sy
     w.active = c.take<int32_t>(h1);
     w.iters = c.take<int32_t>(h1);
     w.removed = c.take<int32_t>(h1);
-    w.need_full = c.take<int32_t>(h1);
-    w.refresh = c.take<int32_t>(h1);
     w.pass_no = c.take<int32_t>(64);
     w.vcm = c.take<double>(3 * h1);
     w.band = c.take<int32_t>(n1);
@@ -906,7 +893,7 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
                   kThreads, 0, st>>>(
         static_cast<int>(L.H), w.order, w.halos, tile, w.cfirst, w.active, w.pass_no, w.ccnt,
         w.csum, w.n_bound, w.n_rem, w.mode, w.vcm, w.tst_len, w.src_len, w.src_sel, w.icnt,
-        w.sym_icnt, w.sym, w.need_full, w.counters);
+        w.sym_icnt, w.sym, w.counters);
     HALMA_LAUNCH_CHECK();
     if (L.H <= kFuseH) {
         const int64_t most = std::max<int64_t>(L.plan->n_items, w.sym_cap);
@@ -1085,7 +1072,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         HALMA_LAUNCH_CHECK();
     }
     k_init_halo<<<blocks_for(H), kThreads, 0, st>>>(w.off, static_cast<int>(H), w.active, w.iters,
-                                                    w.removed, w.need_full, w.refresh);
+                                                    w.removed);
     HALMA_LAUNCH_CHECK();
     k_zero_counters<<<1, 64, 0, st>>>(w.counters, 8);
     HALMA_LAUNCH_CHECK();
@@ -1099,7 +1086,9 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.nm = nm;
     const int cus = std::max(device_cu_count(), 1);
     L.grid_chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(n_chunks, cus * 8)));
-    L.grid_retest = 64;  // one ticket per block closes the pass: few blocks
+    // one block per band member at a time; the last block to take its ticket closes the
+    // pass, so the grid stays at one block per compute unit
+    L.grid_retest = cus;
     L.plan = &plan;
     L.splan = &splan;
     L.w = w;
@@ -1123,8 +1112,6 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.ea.eacc = w.eacc;
     L.ea.removed = w.removed;
     L.ea.band = w.band;
-    L.ea.need_full = w.need_full;
-    L.ea.refresh = w.refresh;
     L.ea.counters = w.counters;
     L.ea.s_out = s_out;
     L.ea.nm = nm;
@@ -1204,7 +1191,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         if (!r && timing) r = record(ev[q][2]);
         if (!r) r = run_part(3);
         if (r) return r;
-        const hipError_t e = hipMemcpyAsync(host + 8 * q, w.counters, 6 * sizeof(long long),
+        const hipError_t e = hipMemcpyAsync(host + 8 * q, w.counters, 7 * sizeof(long long),
                                             hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return hip_fail(e, "counter read-back", __FILE__, __LINE__);
         return record(done[q]);
@@ -1271,6 +1258,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         stats[9] = sym_launches;
         stats[10] = last[4];
         stats[11] = f32_launches;
+        stats[12] = last[6];
     }
     return 0;
 }

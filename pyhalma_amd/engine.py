@@ -200,11 +200,12 @@ def pot_soft_self(offsets, packed, eps, pot=True, acc=False, wsum=False):
 
 
 def energy_test_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, vx, vy, vz, halo_v,
-                        factor, with_pot=False):
+                        factor, with_pot=False, stats=None):
     """One energy test per test particle, every halo at once (halma_energy_test_batched):
     bound iff 1/2|v - v_h|^2 - factor*G*S <= 0, S over the halo's sources (CSR host
     offsets).  Returns (bound uint8 tensor, S float64 tensor or None, tests re-tested in
-    FP64)."""
+    FP64); ``stats`` (a dict) receives retested, pair_ns (the pair kernel's time) and
+    pairs."""
     import ctypes
     from .device import dtype_flag
     L = _lib_()
@@ -223,7 +224,7 @@ def energy_test_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, vx, vy, vz, ha
     nt = int(tx.numel())
     bound = torch.zeros(max(nt, 1), dtype=torch.uint8, device=tx.device)
     S = torch.zeros(max(nt, 1), dtype=torch.float64, device=tx.device) if with_pot else None
-    st = (ctypes.c_int64 * 1)()
+    st = (ctypes.c_int64 * 3)()
     if H > 0 and nt > 0:
         so = _lib.i64_array(src_off)
         to = _lib.i64_array(tst_off)
@@ -233,6 +234,8 @@ def energy_test_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, vx, vy, vz, ha
                                          dtype_flag(vx), ptr(hv), float(factor), ptr(bound),
                                          ptr(S), st, ptr(ws), ws.numel(), stream_ptr())
         _lib.check(rc, "halma_energy_test_batched")
+    if stats is not None:
+        stats.update(retested=int(st[0]), pair_ns=int(st[1]), pairs=int(st[2]))
     return bound[:nt], (S[:nt] if S is not None else None), int(st[0])
 
 
@@ -960,7 +963,7 @@ def unbind(groups, x, y, z, vx, vy, vz, m, factor=1.0, max_iter=100, use_graph=F
     pot = (torch.full((max(nm, 1),), float("nan"), dtype=torch.float64, device=dev)
            if with_pot else None)
     import ctypes
-    st = (ctypes.c_int64 * 12)()
+    st = (ctypes.c_int64 * 13)()
     if H > 0:
         off = _lib.i64_array(groups.offsets_host)
         ws = workspace(L.halma_unbind_workspace_size(H, off))
@@ -975,7 +978,8 @@ def unbind(groups, x, y, z, vx, vy, vz, m, factor=1.0, max_iter=100, use_graph=F
     s = dict(passes=int(st[0]), retested=int(st[1]), items=int(st[2]),
              tests_per_lane=int(st[3]), subtile=int(st[4]), pot_ns=int(st[5]),
              pot_launches=int(st[6]), pairs=int(st[7]), sym_ns=int(st[8]),
-             sym_launches=int(st[9]), full_pairs=int(st[10]), f32_launches=int(st[11]))
+             sym_launches=int(st[9]), full_pairs=int(st[10]), f32_launches=int(st[11]),
+             resummed=int(st[12]))
     return UnbindResult(bound[:nm], iters[:H], vcm[:H], None if pot is None else pot[:nm], s)
 
 

@@ -232,13 +232,14 @@ def _source_csr(groups, H):
     return off, out
 
 
-def RPS_batched(gas, gas_offsets, halo_v, dm=None, stars=None):
+def RPS_batched(gas, gas_offsets, halo_v, dm=None, stars=None, stats=None):
     """RPS (halo_gas.py:285-492) for every halo at once.  ``gas`` = (x, y, z, vx, vy, vz,
     mass, temp) in halo order with CSR host ``gas_offsets``; ``dm`` / ``stars`` =
     (x, y, z, mass, offsets) or None; ``halo_v`` (H, 3) the haloes' velocities.  Sources
     of halo h = its gas, then its DM, then its stars, all summed (no subsampling).
     Returns (H, 4) float64: total gas mass, cold bound fraction, cold unbound mass, hot
-    unbound mass (cold iff T < 5e4 K), and the per-cell bound mask (device uint8)."""
+    unbound mass (cold iff T < 5e4 K), and the per-cell bound mask (device uint8).
+    ``stats`` (a dict) receives the energy test's counters (engine.energy_test_batched)."""
     import torch
 
     from . import engine
@@ -249,7 +250,7 @@ def RPS_batched(gas, gas_offsets, halo_v, dm=None, stars=None):
     groups = [(gx, gy, gz, gm, gas_offsets)] + [g for g in (dm, stars) if g is not None]
     src_off, (sx, sy, sz, sm) = _source_csr(groups, H)
     bound, _, _ = engine.energy_test_batched(src_off, sm, sx, sy, sz, gas_offsets, gx, gy, gz,
-                                             gvx, gvy, gvz, halo_v, 2.0)
+                                             gvx, gvy, gvz, halo_v, 2.0, stats=stats)
     mass = to_dev(gm, np.float64)
     temp = to_dev(gt, np.float64)
     dev = mass.device

@@ -47,7 +47,8 @@ def test_unbind_matches_oracle(gpu, factor):
     # pass 1e-6 is recomputed in full
     assert rel.max() <= 1e-6
     print(f"passes={r.stats['passes']} retested={r.stats['retested']} "
-          f"unbound={int((~bound).sum())} max_rel_pot_err={rel.max():.2e}", flush=True)
+          f"resummed={r.stats['resummed']} unbound={int((~bound).sum())} "
+          f"max_rel_pot_err={rel.max():.2e}", flush=True)
 
 
 def test_unbind_equal_coordinates(gpu):

@@ -37,8 +37,10 @@ def test_oracles_agree(k):
     brute = oracle.fof_labels_brute(x, y, z, ll)
     sp = oracle.fof_labels_scipy(x, y, z, ll)
     cc = oracle.fof_labels_c(x, y, z, ll)
+    cells = oracle.fof_labels_cells(x, y, z, ll)
     assert np.array_equal(brute, sp)
     assert np.array_equal(brute, cc)
+    assert np.array_equal(brute, cells)
     assert oracle.fof_bfs(data, ll) == oracle.labels_to_groups(brute)
 
 
@@ -48,7 +50,8 @@ def test_golden_fixture(golden_dir):
         x = g[f"{name}_x"]
         ll = float(g[f"{name}_ll"])
         want = g[f"{name}_labels"]
-        for fn in (oracle.fof_labels_brute, oracle.fof_labels_c, oracle.fof_labels_scipy):
+        for fn in (oracle.fof_labels_brute, oracle.fof_labels_c, oracle.fof_labels_scipy,
+                   oracle.fof_labels_cells):
             assert np.array_equal(fn(x[:, 0], x[:, 1], x[:, 2], ll), want), (name, fn)
         assert oracle.fof_bfs(x.astype(np.float64), ll) == oracle.labels_to_groups(want)
 
@@ -61,6 +64,34 @@ def test_tie_rounding_both_ways():
     lab = oracle.fof_labels_brute(p[:, 0], p[:, 1], p[:, 2], ll)
     assert lab.tolist() == [0, 1, 2, 2, 4, 4]
     assert np.array_equal(oracle.fof_labels_c(p[:, 0], p[:, 1], p[:, 2], ll), lab)
+    assert np.array_equal(oracle.fof_labels_cells(p[:, 0], p[:, 1], p[:, 2], ll), lab)
+
+
+def test_cells_oracle_on_dense_clumps():
+    """The sub-cell oracle (cells of side ll/sqrt(3) linked whole) against the plain
+    cell-list oracle on dense clumps (thousands of particles per sub-cell), pairs of
+    clumps a little more and a little less than ll apart, lattices at spacings ll and
+    ll/sqrt(3) (points on sub-cell faces), and a wide box that needs the fallback."""
+    rng = np.random.default_rng(17)
+    ll = 0.01
+    dense = np.concatenate([rng.normal(c, 0.002, (3000, 3))
+                            for c in rng.uniform(-0.3, 0.3, (6, 3))])
+    pair = np.concatenate([rng.normal(0, 0.0004, (200, 3)),
+                           rng.normal(0, 0.0004, (200, 3)) + [0.0105, 0, 0],
+                           rng.normal(0, 0.0004, (200, 3)) + [0.5, 0, 0],
+                           rng.normal(0, 0.0004, (200, 3)) + [0.5 + 0.0125, 0, 0]])
+    g = np.arange(9) * ll
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    lat = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + 3.0
+    g2 = np.arange(9) * (ll / np.sqrt(3.0))
+    X, Y, Z = np.meshgrid(g2, g2, g2, indexing="ij")
+    lat2 = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) - 3.0
+    wide = np.concatenate([rng.uniform(0, 1, (300, 3)), [[1e5, 0, 0], [1e5 + ll, 0, 0]]])
+    for data in (dense, pair, lat, lat2, np.concatenate([dense, pair, lat, lat2]), wide):
+        data = data.astype(np.float32).astype(np.float64)
+        x, y, z = data[:, 0], data[:, 1], data[:, 2]
+        assert np.array_equal(oracle.fof_labels_cells(x, y, z, ll),
+                              oracle.fof_labels_c(x, y, z, ll))
 
 
 def test_groups_order_and_permutation():
