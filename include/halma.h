@@ -63,6 +63,22 @@ int halma_pot_sum_f32(const float *m, const float *x, const float *y, const floa
                       int64_t n_test, float *out, double *out_f64, void *workspace,
                       size_t workspace_bytes, void *stream);
 
+/* halma_pot_sum_f32 without a workspace argument and without any allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * scratch memory is a static device array of halma_pot_sum_noalloc_max_bytes() bytes
+ * (64 MiB) per GPU, held until the process ends; sizes whose plan needs more return
+ * HALMA_E_LIMIT before any GPU work.  Same plan and kernels as halma_pot_sum_f32, so the
+ * same bits for the same inputs and tuning.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other host threads) race, so
+ * callers must order them.  The array keeps the last call's contents; nothing in it is
+ * assumed to start at zero (the queue counter is cleared by a kernel each call).
+ * Stream-ordered. */
+size_t halma_pot_sum_noalloc_max_bytes(void);
+int halma_pot_sum_f32_noalloc(const float *m, const float *x, const float *y, const float *z,
+                              int64_t n_src, const float *tx, const float *ty, const float *tz,
+                              int64_t n_test, float *out, double *out_f64, void *stream);
+
 /* Relative error bound B (|S_f32 - S_exact| <= B * S_f32) of halma_pot_sum_f32 and the
  * batched kernels for the current tuning. */
 double halma_pot_error_bound(void);

@@ -49,6 +49,9 @@ SIGNATURES = {
     "halma_pot_sum_f32": (_c.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p,
                                      _size, _p]),
     "halma_pot_error_bound": (_dbl, []),
+    "halma_pot_sum_noalloc_max_bytes": (_size, []),
+    "halma_pot_sum_f32_noalloc": (_c.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p,
+                                             _p]),
     "halma_pot_sum_batched_workspace_size": (_size, [_i64, _i64p, _i64p]),
     "halma_pot_sum_batched_f32": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _i64p, _p, _p, _p,
                                              _p, _p, _size, _p]),

@@ -9,7 +9,9 @@
 // summed plainly, then Kahan-added into the running (sum, compensation) pair; partials
 // of split source ranges are merged in FP64 in chunk order (deterministic).
 #include <algorithm>
+#include <mutex>
 #include <numeric>
+#include <utility>
 #include <vector>
 
 #include "halma_common.h"
@@ -438,6 +440,52 @@ int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const float*
     return 0;
 }
 
+// ---- the entry point that allocates nothing -----------------------------------------
+// Its workspace is this file-scope device array (one copy per GPU, held until the
+// process ends).  Host code never uses the array's name as an address: the device
+// address comes from hipGetSymbolAddress, once per device, with that device current.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+
+}  // namespace halma
+
+static __device__ __align__(256) unsigned char scratch[halma::kNoAllocScratchBytes];
+
+namespace halma {
+
+namespace {
+
+// Device address of `scratch` on the current device (looked up on the first call there).
+int scratch_address(void** out) {
+    static std::mutex mu;
+    static std::vector<std::pair<int, void*>> addr;
+    int dev = 0;
+    HALMA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto& e : addr)
+        if (e.first == dev) {
+            *out = e.second;
+            return 0;
+        }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    if (!p) return set_error(HALMA_E_STATE, "no device address for the static scratch");
+    addr.emplace_back(dev, p);
+    *out = p;
+    return 0;
+}
+
+// The single halo's descriptor and its work items, written on the device (the order of
+// plan_pot: chunks outer, test tiles inner), so that nothing is uploaded from the host.
+__global__ void k_single_plan(HaloDesc d, int tiles, int n_items, HaloDesc* __restrict__ halos,
+                              int4* __restrict__ items) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) halos[0] = d;
+    if (k < n_items) items[k] = make_int4(0, k % tiles, k / tiles, 0);
+}
+
+}  // namespace
+
 }  // namespace halma
 
 using namespace halma;
@@ -445,6 +493,54 @@ using namespace halma;
 extern "C" {
 
 double halma_pot_error_bound(void) { return pot_error_bound(g_tuning.subtile); }
+
+size_t halma_pot_sum_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+int halma_pot_sum_f32_noalloc(const float* m, const float* x, const float* y, const float* z,
+                              int64_t n_src, const float* tx, const float* ty, const float* tz,
+                              int64_t n_test, float* out, double* out_f64, void* stream) {
+    if (n_src < 0 || n_test < 0) return set_error(HALMA_E_ARG, "negative size");
+    if (n_test > 0 && !out && !out_f64)
+        return set_error(HALMA_E_ARG, "halma_pot_sum_f32_noalloc: no output");
+    if ((n_src > 0 && (!m || !x || !y || !z)) || (n_test > 0 && (!tx || !ty || !tz)))
+        return set_error(HALMA_E_ARG, "halma_pot_sum_f32_noalloc: null data pointer");
+    if (n_src > INT32_MAX || n_test > INT32_MAX)
+        return set_error(HALMA_E_LIMIT, "at most 2^31-1 sources and tests per halo");
+    const int64_t so[2] = {0, n_src}, to[2] = {0, n_test};
+    PotPlan plan;
+    int rc = plan_pot(1, so, to, plan, false);
+    if (rc) return rc;
+    const size_t need = carve_pot(nullptr, plan).bytes;
+    if (need > kNoAllocScratchBytes)
+        return set_error(HALMA_E_LIMIT,
+                         "halma_pot_sum_f32_noalloc: %lld sources x %lld tests need %zu bytes of "
+                         "scratch, the static scratch holds %zu",
+                         (long long)n_src, (long long)n_test, need, kNoAllocScratchBytes);
+    if (n_test == 0) return 0;
+    void* base = nullptr;
+    rc = scratch_address(&base);
+    if (rc) return rc;
+    PotWs w = carve_pot(base, plan);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rc = launch_pack4(m, x, y, z, plan.n_src, w.src4, st);
+    if (rc) return rc;
+    rc = launch_pack4(nullptr, tx, ty, tz, plan.n_tst, w.tst4, st);
+    if (rc) return rc;
+    const int tile = plan.tests_per_lane * kWave;
+    const int tiles = static_cast<int>((n_test + tile - 1) / tile);
+    const int n_items = static_cast<int>(plan.n_items);
+    k_single_plan<<<static_cast<unsigned>(std::max(1, (n_items + 255) / 256)), 256, 0, st>>>(
+        plan.halos[0], std::max(tiles, 1), n_items, w.halos, w.items);
+    HALMA_LAUNCH_CHECK();
+    k_tst_halo<<<1, 256, 0, st>>>(w.halos, 1, w.tst_halo);
+    HALMA_LAUNCH_CHECK();
+    rc = launch_pot_f32(plan, w.src4, w.tst4, w.halos, DynSets(), w.items, w.part, w.head, st);
+    if (rc) return rc;
+    k_pot_merge<<<static_cast<unsigned>((plan.n_tst + 255) / 256), 256, 0, st>>>(
+        w.halos, w.tst_halo, plan.n_tst, w.part, out_f64, out);
+    HALMA_LAUNCH_CHECK();
+    return 0;
+}
 
 size_t halma_pot_sum_batched_workspace_size(int64_t H, const int64_t* src_off,
                                             const int64_t* tst_off) {
