@@ -2,25 +2,27 @@
 // wherever ((dx*dx + dy*dy) + dz*dz) <= ll*ll in FP64 (fof.py:18-79 through scipy's
 // KDTree; inclusive, non-periodic), canonical label = minimum particle index.
 //
-// 1. bounding box (two-pass reduction);
+// 1. bounding box (two-pass reduction), read back once: it fixes the bits of each cell
+//    coordinate;
 // 2. sub-cells of side s = ll/sqrt(3) * (1 - 1e-6): two particles in one sub-cell are
 //    within ll under the FP64 test (|dx|,|dy|,|dz| <= s(1+1e-9) after rounding of the
 //    cell index), so a sub-cell is one vertex of the cell graph and needs no pair test;
 //    every link spans at most 2 sub-cells per axis (ll/s < 1.733);
-// 3. occupied sub-cells go into an open-addressing hash table (64-bit keys, vector
-//    atomics); each particle records its cell's slot, and a new cell sets its bit in a
-//    hashed occupancy bitmap;
-// 4. a stable radix sort of (slot, particle) lists the particles cell by cell (cells
-//    numbered in slot order, particles ascending within a cell); coordinates are
-//    gathered into that order;
-// 5. links, in two rounds (the 13 neighbours at Chebyshev distance 1, then the 49 at
-//    distance 2): one lane per (cell, half-stencil offset) skips empty neighbours (a
-//    clear occupancy bit, else the table) and pairs already joined, tests small pairs
-//    of cells itself, and lists larger pairs for one wavefront each (64 lanes over the
-//    larger cell, early exit); linked cells are joined in a lock-free union-find over
-//    cells (the larger root index is hooked under the smaller, so parents only decrease
-//    and no cycle can form; finds halve paths);
-// 6. roots are read back with atomic reads (see k_roots), each component's minimum
+// 3. a stable LSD radix sort of the particles by cell key (x, y, z bits packed, z lowest)
+//    lists them cell by cell: cells of one column (x, y) are consecutive and ordered by
+//    z, columns are in (x, y) order, particles ascending within a cell; coordinates are
+//    gathered into that order, and each column enters a hash table (key -> column), sized
+//    from the column count;
+// 4. links, in two rounds (Chebyshev distance 1, then 2), one lane per (column,
+//    neighbour column): the half-stencil's 12 columns (dx > 0, or dx = 0 and dy > 0) and
+//    the column itself (cells above); one table probe finds the neighbour column, then the
+//    two z-ordered cell lists are merged for the cell pairs within the round's dz.  A
+//    pair of cells already joined is skipped; small pairs are tested by the lane, larger
+//    ones listed for one wavefront each (64 lanes over the larger cell, early exit);
+//    linked cells are joined in a lock-free union-find over cells (the larger root index
+//    is hooked under the smaller, so parents only decrease and no cycle can form; finds
+//    halve paths);
+// 5. roots are read back with atomic reads (see k_roots), each component's minimum
 //    particle index is taken over its cells, and every particle gets it as label.
 #include <math.h>
 
@@ -36,23 +38,14 @@ namespace halma {
 namespace {
 
 constexpr uint64_t kEmpty = ~0ull;
-constexpr int kCellBits = 21;
 constexpr double kCellLimit = 2097152.0;  // 2^21 cells per axis
 constexpr int kBoxBlocks = 1024;
 
-// Table slot of a cell key: the (x, y) column is hashed and z is added, so the cells of
-// one column sit in consecutive slots and the 5 z-neighbours of a stencil column are one
-// or two memory lines apart.
-__device__ __forceinline__ uint64_t mix64(uint64_t k);
-__device__ __forceinline__ uint64_t slot_of(uint64_t key, uint64_t mask) {
-    return (mix64(key >> kCellBits) + (key & ((1ull << kCellBits) - 1))) & mask;
-}
-
-// 16-byte table entry: cell key (kEmpty when free), particle count, cell number
-struct __attribute__((aligned(16))) Entry {
+// 16-byte column table entry: column key (kEmpty when free), column number
+struct __attribute__((aligned(16))) ColEntry {
     uint64_t key;
-    int32_t cnt;
-    int32_t cell;
+    int32_t col;
+    int32_t pad;
 };
 
 __device__ __forceinline__ uint64_t mix64(uint64_t k) {
@@ -163,15 +156,6 @@ __global__ __launch_bounds__(256) void k_bbox_final(const double* __restrict__ p
     }
 }
 
-__global__ void k_clear_table(Entry* t, int64_t n) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) {
-        t[i].key = kEmpty;
-        t[i].cnt = 0;
-        t[i].cell = INT32_MAX;
-    }
-}
-
 __global__ void k_fill_i32n(int* p, int64_t n, int v) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -193,130 +177,174 @@ struct Packed<double> {
     }
 };
 
-// Binning: every particle finds (or inserts) its sub-cell in the hash table and records
-// the slot; a stable radix sort of (slot, particle index) then lists the particles cell
-// by cell, ascending within a cell (so a cell's first particle is its minimum index).
-// A newly inserted cell also sets its bit in a hashed occupancy bitmap, which lets the
-// link phase skip the hash lookups of empty neighbour cells.
-// Occupancy bit of a cell: the low bits of its Morton code, so that the neighbourhood of
-// a cell falls on a few bitmap lines (cells far apart may share a bit: a false positive
-// only costs a table lookup).
-__device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
-    v &= 0x1fffffull;
-    v = (v | (v << 32)) & 0x1f00000000ffffull;
-    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
-    v = (v | (v << 8)) & 0x100f00f00f00f00full;
-    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
-    v = (v | (v << 2)) & 0x1249249249249249ull;
-    return v;
-}
 
-__device__ __forceinline__ uint64_t occ_bit(uint64_t key, uint64_t bmask) {
-    const uint64_t m = (1ull << kCellBits) - 1;
-    return (spread3(key >> (2 * kCellBits)) << 2 | spread3((key >> kCellBits) & m) << 1 |
-            spread3(key & m)) & bmask;
-}
+// Cell grid of one call: origin, inverse side, and the bits of each cell coordinate in
+// the packed key (x | y | z, z lowest).
+struct Grid {
+    double lo[3];
+    double inv_s;
+    int bx, by, bz;
+};
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_fof_keys(
-    const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, int64_t n,
-    const double* __restrict__ box, double inv_s, Entry* __restrict__ tab, uint64_t mask,
-    uint32_t* __restrict__ skey, uint32_t* __restrict__ bits, uint64_t bmask,
-    int* __restrict__ err, typename Packed<T>::type* __restrict__ pk) {
+__device__ __forceinline__ uint64_t cell_key(T xi, T yi, T zi, const Grid& g, bool& bad) {
+    const double q[3] = {(static_cast<double>(xi) - g.lo[0]) * g.inv_s,
+                         (static_cast<double>(yi) - g.lo[1]) * g.inv_s,
+                         (static_cast<double>(zi) - g.lo[2]) * g.inv_s};
+    const int bits[3] = {g.bx, g.by, g.bz};
+    uint64_t key = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const bool ok = q[d] >= 0.0 && q[d] < kCellLimit;  // also catches NaN
+        uint64_t c = ok ? static_cast<uint64_t>(floor(q[d])) : 0;
+        if (c >> bits[d]) {  // beyond the grid the box fixed: cannot happen for finite data
+            bad = true;
+            c = 0;
+        }
+        bad |= !ok;
+        key = (key << bits[d]) | c;
+    }
+    return key;
+}
+
+// per particle: the low and high 32 bits of its cell key, and a packed copy of its
+// coordinates for the cell-order gather (one line per particle instead of three)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fof_keys(const T* __restrict__ x,
+                                                  const T* __restrict__ y,
+                                                  const T* __restrict__ z, int64_t n, Grid g,
+                                                  uint32_t* __restrict__ klo,
+                                                  uint32_t* __restrict__ khi,
+                                                  int* __restrict__ err,
+                                                  typename Packed<T>::type* __restrict__ pk) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const T xi = x[i], yi = y[i], zi = z[i];
-    // packed copy for the cell-order gather (one line per particle instead of three)
     pk[i] = Packed<T>::make(xi, yi, zi);
-    const double q[3] = {(static_cast<double>(xi) - box[0]) * inv_s,
-                         (static_cast<double>(yi) - box[1]) * inv_s,
-                         (static_cast<double>(zi) - box[2]) * inv_s};
-    uint64_t key = 0;
     bool bad = false;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        bad |= !(q[d] >= 0.0 && q[d] < kCellLimit);  // also catches NaN
-        const uint64_t c = bad ? 0 : static_cast<uint64_t>(floor(q[d]));
-        key = (key << kCellBits) | c;
-    }
-    if (bad) {
-        atomicOr(err, 1);
-        key = 0;
-    }
-    uint64_t h = slot_of(key, mask);
-    for (uint64_t probe = 0;; ++probe) {
-        // a plain load: a slot's key never changes once set, so a stale value can only
-        // be a stale "empty", which the compare-and-swap below corrects
-        const uint64_t k = tab[h].key;
-        if (k == key) break;
-        if (k == kEmpty) {
-            uint64_t expected = kEmpty;
-            if (__hip_atomic_compare_exchange_strong(&tab[h].key, &expected, key,
-                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT)) {
-                const uint64_t b = occ_bit(key, bmask);  // this thread created the cell
-                atomicOr(bits + (b >> 5), 1u << (b & 31));
-                break;
-            }
-            if (expected == key) break;
-        }
-        if (probe > mask) {  // table full (cannot happen with capacity >= 2n)
-            atomicOr(err, 2);
-            h = 0;
-            break;
-        }
-        h = (h + 1) & mask;
-    }
-    skey[i] = static_cast<uint32_t>(h);
+    const uint64_t key = cell_key(xi, yi, zi, g, bad);
+    if (bad) atomicOr(err, 1);
+    klo[i] = static_cast<uint32_t>(key);
+    khi[i] = static_cast<uint32_t>(key >> 32);
 }
 
-__global__ void k_run_heads(const uint32_t* __restrict__ s, int64_t n, int* __restrict__ head) {
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const int32_t* __restrict__ perm,
+                             int64_t n, uint32_t* __restrict__ dst) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j < n) head[j] = (j == 0 || s[j] != s[j - 1]) ? 1 : 0;
+    if (j < n) dst[j] = src[perm[j]];
 }
 
-// per sorted position j: its cell c (cells numbered in slot order), the particle's
-// coordinates in cell order, and, at the first particle of a cell, the cell's records
+// per sorted position j: the particle's coordinates in cell order, its key (recomputed
+// from the same coordinates with the same arithmetic) and the cell / column heads
+// (bit 0 / bit 32: the key / the column part of the key differs from position j - 1)
 template <typename T>
-__global__ void k_cells(const uint32_t* __restrict__ skey, const int32_t* __restrict__ perm,
-                        const int64_t* __restrict__ before, const int* __restrict__ head,
-                        int64_t n, Entry* __restrict__ tab,
-                        const typename Packed<T>::type* __restrict__ pk, T* __restrict__ sx,
-                        T* __restrict__ sy, T* __restrict__ sz, int* __restrict__ pcell,
-                        int* __restrict__ cell_start, uint64_t* __restrict__ cell_key,
-                        int* __restrict__ cell_min, int* __restrict__ parent,
-                        int* __restrict__ comp_min) {
+__global__ __launch_bounds__(256) void k_fof_sorted(const int32_t* __restrict__ perm, int64_t n,
+                                                    Grid g,
+                                                    const typename Packed<T>::type* __restrict__ pk,
+                                                    T* __restrict__ sx, T* __restrict__ sy,
+                                                    T* __restrict__ sz,
+                                                    uint64_t* __restrict__ skey) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    const int hd = head[j];
-    const int c = static_cast<int>(before[j] + hd - 1);
-    const int32_t i = perm[j];
-    const typename Packed<T>::type p = pk[i];
+    const typename Packed<T>::type p = pk[perm[j]];
     sx[j] = p.x;
     sy[j] = p.y;
     sz[j] = p.z;
-    pcell[i] = c;
-    if (hd) {
-        const uint32_t slot = skey[j];
-        tab[slot].cell = c;
+    bool bad = false;
+    skey[j] = cell_key(p.x, p.y, p.z, g, bad);
+}
+
+__global__ void k_fof_heads(const uint64_t* __restrict__ skey, int64_t n, int bz,
+                            int64_t* __restrict__ heads) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t k = skey[j];
+    const uint64_t kp = j > 0 ? skey[j - 1] : ~k;
+    const int64_t cell = k != kp ? 1 : 0;
+    const int64_t col = (k >> bz) != (kp >> bz) ? 1 : 0;
+    heads[j] = cell | (col << 32);
+}
+
+// the column table's size from the column count (power of two >= 2 columns, >= 1024,
+// <= cap), kept as its mask in totals[1]; then cleared
+__global__ void k_col_table_size(int64_t* __restrict__ totals, int64_t cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int64_t ncol = totals[0] >> 32;
+    int64_t c = 1024;
+    while (c < 2 * ncol && c < cap) c <<= 1;
+    totals[1] = c - 1;
+}
+
+__global__ void k_clear_cols(const int64_t* __restrict__ totals, ColEntry* __restrict__ tab) {
+    const int64_t m = totals[1] + 1;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        tab[i].key = kEmpty;
+        tab[i].col = -1;
+    }
+}
+
+// Cells and columns from the sorted keys.  Cell c: its first sorted position, its z,
+// its smallest particle index (the sort is stable, so the first one), the union-find
+// root; column q: its first cell, its key, and its entry in the table.
+__global__ __launch_bounds__(256) void k_fof_cells(
+    const uint64_t* __restrict__ skey, const int32_t* __restrict__ perm,
+    const int64_t* __restrict__ heads, const int64_t* __restrict__ before, int64_t n, int bz,
+    const int64_t* __restrict__ totals, int* __restrict__ pcell, int* __restrict__ cell_start,
+    int* __restrict__ cell_z, int* __restrict__ cell_min, int* __restrict__ parent,
+    int* __restrict__ comp_min, int* __restrict__ col_cell, uint64_t* __restrict__ col_key,
+    ColEntry* __restrict__ tab, int* __restrict__ err) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t h = heads[j], b = before[j];
+    const int c = static_cast<int>((b & 0xffffffff) + (h & 0xffffffff)) - 1;
+    const int q = static_cast<int>((b >> 32) + (h >> 32)) - 1;
+    const uint64_t k = skey[j];
+    pcell[perm[j]] = c;
+    if (h & 0xffffffff) {
         cell_start[c] = static_cast<int>(j);
-        cell_key[c] = tab[slot].key;
-        cell_min[c] = i;  // stable sort: the cell's smallest particle index comes first
+        cell_z[c] = static_cast<int>(k & ((1ull << bz) - 1));
+        cell_min[c] = perm[j];
         parent[c] = c;
         comp_min[c] = INT32_MAX;
     }
-    if (j == n - 1) cell_start[c + 1] = static_cast<int>(n);
+    if (h >> 32) {
+        const uint64_t ck = k >> bz;
+        col_cell[q] = c;
+        col_key[q] = ck;
+        const uint64_t mask = static_cast<uint64_t>(totals[1]);
+        uint64_t s = mix64(ck) & mask;
+        for (uint64_t probe = 0;; ++probe) {
+            uint64_t expected = kEmpty;
+            if (__hip_atomic_compare_exchange_strong(&tab[s].key, &expected, ck,
+                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) {
+                tab[s].col = q;
+                break;
+            }
+            if (probe > mask) {  // table full (cannot happen: capacity >= 2 columns)
+                atomicOr(err, 2);
+                break;
+            }
+            s = (s + 1) & mask;
+        }
+    }
+    if (j == n - 1) {
+        cell_start[c + 1] = static_cast<int>(n);
+        col_cell[q + 1] = c + 1;
+    }
 }
 
-// cell number of a key, or -1
-__device__ __forceinline__ int table_find(const Entry* __restrict__ tab, uint64_t mask,
-                                          uint64_t key) {
-    uint64_t h = slot_of(key, mask);
+// column number of a column key, or -1.  The entry's key is written by a compare-and-swap
+// and its column right after, by the same thread, in an earlier kernel.
+__device__ __forceinline__ int col_find(const ColEntry* __restrict__ tab, uint64_t mask,
+                                        uint64_t key) {
+    uint64_t s = mix64(key) & mask;
     for (uint64_t probe = 0; probe <= mask; ++probe) {
-        const uint64_t k = tab[h].key;
-        if (k == key) return tab[h].cell;
+        const uint64_t k = tab[s].key;
+        if (k == key) return tab[s].col;
         if (k == kEmpty) return -1;
-        h = (h + 1) & mask;
+        s = (s + 1) & mask;
     }
     return -1;
 }
@@ -330,21 +358,6 @@ __device__ __forceinline__ int find_ro(const int* P, int x) {
     for (int p = P[x]; p != x; p = P[x]) x = p;
     return x;
 }
-
-// The 62 half-stencil offsets (dx, dy, dz) of a cell: the 13 at Chebyshev distance 1
-// first, then the 49 at distance 2, each group in (dz, dy, dx) order.
-__constant__ signed char kNbr[62][3] = {
-    {1, 0, 0}, {-1, 1, 0}, {0, 1, 0}, {1, 1, 0}, {-1, -1, 1}, {0, -1, 1}, {1, -1, 1},
-    {-1, 0, 1}, {0, 0, 1}, {1, 0, 1}, {-1, 1, 1}, {0, 1, 1}, {1, 1, 1},
-    {2, 0, 0}, {-2, 1, 0}, {2, 1, 0}, {-2, 2, 0}, {-1, 2, 0}, {0, 2, 0}, {1, 2, 0},
-    {2, 2, 0}, {-2, -2, 1}, {-1, -2, 1}, {0, -2, 1}, {1, -2, 1}, {2, -2, 1}, {-2, -1, 1},
-    {2, -1, 1}, {-2, 0, 1}, {2, 0, 1}, {-2, 1, 1}, {2, 1, 1}, {-2, 2, 1}, {-1, 2, 1},
-    {0, 2, 1}, {1, 2, 1}, {2, 2, 1}, {-2, -2, 2}, {-1, -2, 2}, {0, -2, 2}, {1, -2, 2},
-    {2, -2, 2}, {-2, -1, 2}, {-1, -1, 2}, {0, -1, 2}, {1, -1, 2}, {2, -1, 2}, {-2, 0, 2},
-    {-1, 0, 2}, {0, 0, 2}, {1, 0, 2}, {2, 0, 2}, {-2, 1, 2}, {-1, 1, 2}, {0, 1, 2},
-    {1, 1, 2}, {2, 1, 2}, {-2, 2, 2}, {-1, 2, 2}, {0, 2, 2}, {1, 2, 2}, {2, 2, 2}};
-constexpr int kNear = 13;
-constexpr int kLaneTest = 64;  // pair tests a lane does alone
 
 template <typename T>
 __device__ __forceinline__ bool within(const T* sx, const T* sy, const T* sz, int a, int b,
@@ -369,86 +382,104 @@ __global__ void k_flatten(const int64_t* __restrict__ totals, int* __restrict__ 
     }
 }
 
-// Link candidates, one lane per (cell, neighbour offset) of one distance class (NEAR:
-// the 13 at Chebyshev distance 1; otherwise the 49 at distance 2, launched after every
-// distance-1 link is in the forest, so that dense regions are recognised as joined and
-// need no search).  A lane skips empty neighbours (occupancy bit, then the table) and
-// pairs already joined, tests small pairs of cells itself and hooks them when linked,
-// and hands larger pairs to k_fof_big through a list (one atomic per wave); a full list
-// makes the lane test the pair itself.
-template <typename T, bool NEAR>
-__global__ __launch_bounds__(256) void k_fof_cand(
-    const Entry* __restrict__ tab, uint64_t mask, const uint32_t* __restrict__ bits,
-    uint64_t bmask, const int* __restrict__ cell_start,
-    const uint64_t* __restrict__ cell_key, const T* __restrict__ sx, const T* __restrict__ sy,
-    const T* __restrict__ sz, const int64_t* __restrict__ totals, double ll2,
+
+// The neighbour columns of the half stencil, (dx, dy) with dx > 0 or dx = 0 and dy > 0:
+// the four within Chebyshev distance 1 first, then the eight at distance 2.
+__constant__ signed char kCol[12][2] = {{0, 1}, {1, -1}, {1, 0}, {1, 1},  // near
+                                        {0, 2}, {1, -2}, {1, 2}, {2, -2},
+                                        {2, -1}, {2, 0}, {2, 1}, {2, 2}};
+constexpr int kLaneTest = 64;  // pair tests a lane does alone
+
+template <typename T>
+__device__ __forceinline__ bool cells_linked(const int* __restrict__ cell_start, const T* sx,
+                                             const T* sy, const T* sz, int c, int d,
+                                             double ll2) {
+    const int sc = cell_start[c], nc = cell_start[c + 1] - sc;
+    const int sd = cell_start[d], nd = cell_start[d + 1] - sd;
+    for (int a = 0; a < nc; ++a)
+        for (int bb = 0; bb < nd; ++bb)
+            if (within(sx, sy, sz, sc + a, sd + bb, ll2)) return true;
+    return false;
+}
+
+// One candidate pair of cells: skipped when already joined; tested here when small,
+// else listed for k_fof_big (tested here too when the list is full).
+template <typename T>
+__device__ __forceinline__ void candidate(int c, int d, const int* __restrict__ cell_start,
+                                          const T* sx, const T* sy, const T* sz, double ll2,
+                                          int* __restrict__ parent, int2* __restrict__ big,
+                                          int64_t big_cap,
+                                          unsigned long long* __restrict__ n_big) {
+    if (find_ro(parent, c) == find_ro(parent, d)) return;
+    const int64_t nc = cell_start[c + 1] - cell_start[c];
+    const int64_t nd = cell_start[d + 1] - cell_start[d];
+    if (nc * nd > kLaneTest) {
+        const unsigned long long at = atomicAdd(n_big, 1ull);
+        if (static_cast<int64_t>(at) < big_cap) {
+            big[at] = make_int2(c, d);
+            return;
+        }
+    }
+    if (cells_linked(cell_start, sx, sy, sz, c, d, ll2)) uf_union(parent, c, d);
+}
+
+// Link candidates, one lane per (column, column offset) of one round.  ROUND 1: the
+// column's own cells one z apart, and the four near columns with |dz| <= 1 (the 13 cell
+// offsets at Chebyshev distance 1).  ROUND 2, launched after every distance-1 link is in
+// the forest (so that dense regions are recognised as joined and need no search): the
+// own cells two apart, the near columns with |dz| = 2, the far columns with |dz| <= 2
+// (the other 49 offsets).
+template <typename T, int ROUND>
+__global__ __launch_bounds__(256) void k_fof_cols(
+    const ColEntry* __restrict__ tab, const int* __restrict__ col_cell,
+    const uint64_t* __restrict__ col_key, const int* __restrict__ cell_start,
+    const int* __restrict__ cell_z, const T* __restrict__ sx, const T* __restrict__ sy,
+    const T* __restrict__ sz, const int64_t* __restrict__ totals, int bx, int by, double ll2,
     int* __restrict__ parent, int2* __restrict__ big, int64_t big_cap,
     unsigned long long* __restrict__ n_big) {
-    constexpr int K = NEAR ? kNear : 62 - kNear;
-    constexpr int K0 = NEAR ? 0 : kNear;
-    const int lane = __lane_id();
-    const int64_t total = (totals[0] & 0xffffffff) * K;
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    const uint64_t cmask = (1ull << kCellBits) - 1;
-    // the wave iterates uniformly (lanes past the end idle) so that ballots are whole
-    for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~(kWave - 1));
-         i0 < total; i0 += stride) {
-        const int64_t idx = i0 + lane;
-        bool push = false;
-        int c = 0, d = -1;
-        if (idx < total) {
-            c = static_cast<int>(idx / K);
-            const int k = K0 + static_cast<int>(idx % K);
-            const uint64_t key = cell_key[c];
-            const int64_t nx = static_cast<int64_t>((key >> (2 * kCellBits)) & cmask) + kNbr[k][0];
-            const int64_t ny = static_cast<int64_t>((key >> kCellBits) & cmask) + kNbr[k][1];
-            const int64_t nz = static_cast<int64_t>(key & cmask) + kNbr[k][2];
-            if (nx >= 0 && ny >= 0 && nz >= 0 && nx <= static_cast<int64_t>(cmask) &&
-                ny <= static_cast<int64_t>(cmask) && nz <= static_cast<int64_t>(cmask)) {
-                const uint64_t nkey = (static_cast<uint64_t>(nx) << (2 * kCellBits)) |
-                                      (static_cast<uint64_t>(ny) << kCellBits) |
-                                      static_cast<uint64_t>(nz);
-                const uint64_t b = occ_bit(nkey, bmask);
-                if ((bits[b >> 5] >> (b & 31)) & 1u) d = table_find(tab, mask, nkey);
+    constexpr int K = ROUND == 1 ? 5 : 13;  // own column + neighbour columns
+    const int64_t ncol = totals[0] >> 32;
+    const uint64_t mask = static_cast<uint64_t>(totals[1]);
+    const int64_t total = ncol * K;
+    const int64_t ymask = (int64_t{1} << by) - 1;
+    const int64_t nxc = int64_t{1} << bx, nyc = int64_t{1} << by;
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         idx < total; idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int q = static_cast<int>(idx / K);
+        const int o = static_cast<int>(idx % K);
+        const int a0 = col_cell[q], a1 = col_cell[q + 1];
+        if (o == 0) {  // the column's own cells, ROUND apart in z
+            for (int c = a0; c + 1 < a1; ++c) {
+                const int zc = cell_z[c];
+                for (int d = c + 1; d < a1 && cell_z[d] - zc <= ROUND; ++d)
+                    if (cell_z[d] - zc == ROUND)
+                        candidate(c, d, cell_start, sx, sy, sz, ll2, parent, big, big_cap,
+                                  n_big);
             }
-            if (d >= 0 && find_ro(parent, c) != find_ro(parent, d)) {
-                const int sc = cell_start[c], nc = cell_start[c + 1] - sc;
-                const int sd = cell_start[d], nd = cell_start[d + 1] - sd;
-                if (static_cast<int64_t>(nc) * nd <= kLaneTest) {
-                    bool linked = false;
-                    for (int a = 0; a < nc && !linked; ++a)
-                        for (int bb = 0; bb < nd; ++bb)
-                            if (within(sx, sy, sz, sc + a, sd + bb, ll2)) {
-                                linked = true;
-                                break;
-                            }
-                    if (linked) uf_union(parent, c, d);
-                } else {
-                    push = true;
-                }
-            }
+            continue;
         }
-        const uint64_t pm = __ballot(push);
-        if (pm) {
-            unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(n_big, static_cast<unsigned long long>(__popcll(pm)));
-            base = __shfl(base, 0, kWave);
-            if (push) {
-                const int64_t at = static_cast<int64_t>(base + __popcll(pm & ((1ull << lane) - 1ull)));
-                if (at < big_cap) {
-                    big[at] = make_int2(c, d);
-                } else {  // list full: test here (correct, slower)
-                    const int sc = cell_start[c], nc = cell_start[c + 1] - sc;
-                    const int sd = cell_start[d], nd = cell_start[d + 1] - sd;
-                    bool linked = false;
-                    for (int a = 0; a < nc && !linked; ++a)
-                        for (int bb = 0; bb < nd; ++bb)
-                            if (within(sx, sy, sz, sc + a, sd + bb, ll2)) {
-                                linked = true;
-                                break;
-                            }
-                    if (linked) uf_union(parent, c, d);
-                }
+        const int k = o - 1;  // round 1: the near columns; round 2: all twelve
+        const uint64_t key = col_key[q];
+        const int64_t nx = static_cast<int64_t>(key >> by) + kCol[k][0];
+        const int64_t ny = static_cast<int64_t>(key & ymask) + kCol[k][1];
+        if (nx < 0 || ny < 0 || nx >= nxc || ny >= nyc) continue;
+        const int qb = col_find(tab, mask, (static_cast<uint64_t>(nx) << by) |
+                                               static_cast<uint64_t>(ny));
+        if (qb < 0) continue;
+        const int b0 = col_cell[qb], b1 = col_cell[qb + 1];
+        // |dz| range of this (round, column): near columns take |dz| <= 1 in round 1 and
+        // |dz| = 2 in round 2, far columns |dz| <= 2
+        const bool near = k < 4;
+        const int lo = (ROUND == 2 && near) ? 2 : 0;
+        const int hi = ROUND == 1 ? 1 : 2;
+        int p = b0;
+        for (int c = a0; c < a1; ++c) {
+            const int zc = cell_z[c];
+            while (p < b1 && cell_z[p] < zc - hi) ++p;
+            for (int d = p; d < b1 && cell_z[d] <= zc + hi; ++d) {
+                const int dz = cell_z[d] > zc ? cell_z[d] - zc : zc - cell_z[d];
+                if (dz >= lo)
+                    candidate(c, d, cell_start, sx, sy, sz, ll2, parent, big, big_cap, n_big);
             }
         }
     }
@@ -555,19 +586,25 @@ __global__ void k_labels(int64_t n, const int* __restrict__ pcell, const int* __
 struct FofWs {
     double* box_part;
     double* box;
-    Entry* tab;
-    uint32_t* bits;   // hashed occupancy bitmap of the cells
-    int64_t* totals;
+    ColEntry* tab;
+    int64_t* totals;  // [0] cells | columns << 32, [1] column table mask
     void* scan_ws;
     void* sort_ws;
-    uint32_t* skey;   // slot of each particle, then sorted
-    uint32_t* skey_sorted;
+    uint32_t* klo;    // low / high 32 bits of each particle's cell key
+    uint32_t* khi;
+    uint32_t* klo_s;  // sorted keys of the passes (unused beyond the sort)
+    uint32_t* khi_g;
+    uint32_t* khi_s;
+    int32_t* perm1;   // order after the low-bits pass
     int32_t* perm;    // particle of each sorted position
-    int* head;
-    int64_t* before;  // exclusive scan of head
+    uint64_t* skey;   // cell key of each sorted position
+    int64_t* heads;   // cell / column heads, then (in place) their exclusive scan
+    int64_t* before;
     int* pcell;
     int* cell_start;  // n_cells + 1 entries
-    uint64_t* cell_key;
+    int* cell_z;
+    int* col_cell;    // n_cols + 1 entries
+    uint64_t* col_key;
     int* parent;
     int* root;
     int* comp_min;
@@ -581,9 +618,7 @@ struct FofWs {
     int64_t big_cap;
     int* err;
     size_t bytes;
-    int64_t cap;
-    int64_t nbits;
-    int slot_bits;
+    int64_t cap;      // column table capacity for n particles (the largest it gets)
 };
 
 int64_t table_capacity(int64_t n) {
@@ -592,37 +627,32 @@ int64_t table_capacity(int64_t n) {
     return cap;
 }
 
-// occupancy bits: the power of two >= 16 n (>= 2^15, <= 2^31), so that at most one
-// empty neighbour in 16 passes the bit test
-int64_t bitmap_bits(int64_t n) {
-    int64_t b = int64_t{1} << 15;
-    while (b < 16 * n && b < (int64_t{1} << 31)) b <<= 1;
-    return b;
-}
-
 FofWs carve_fof(void* base, int64_t n, size_t elt) {
     Carver c(base);
     FofWs w;
     const int64_t nn = n > 0 ? n : 1;
     w.cap = table_capacity(n);
-    w.slot_bits = 0;
-    while ((int64_t{1} << w.slot_bits) < w.cap) ++w.slot_bits;
-    w.nbits = bitmap_bits(n);
     w.box_part = c.take<double>(kBoxBlocks * 6);
     w.box = c.take<double>(8);
-    w.tab = c.take<Entry>(w.cap);
-    w.bits = c.take<uint32_t>(w.nbits / 32);
+    w.tab = c.take<ColEntry>(w.cap);
     w.totals = c.take<int64_t>(4);
     w.scan_ws = c.take<char>(scan_workspace_bytes(nn));
     w.sort_ws = c.take<char>(radix_workspace_bytes(nn));
-    w.skey = c.take<uint32_t>(nn);
-    w.skey_sorted = c.take<uint32_t>(nn);
+    w.klo = c.take<uint32_t>(nn);
+    w.khi = c.take<uint32_t>(nn);
+    w.klo_s = c.take<uint32_t>(nn);
+    w.khi_g = c.take<uint32_t>(nn);
+    w.khi_s = c.take<uint32_t>(nn);
+    w.perm1 = c.take<int32_t>(nn);
     w.perm = c.take<int32_t>(nn);
-    w.head = c.take<int>(nn);
+    w.skey = c.take<uint64_t>(nn);
+    w.heads = c.take<int64_t>(nn);
     w.before = c.take<int64_t>(nn);
     w.pcell = c.take<int>(nn);
     w.cell_start = c.take<int>(nn + 1);
-    w.cell_key = c.take<uint64_t>(nn);
+    w.cell_z = c.take<int>(nn);
+    w.col_cell = c.take<int>(nn + 1);
+    w.col_key = c.take<uint64_t>(nn);
     w.parent = c.take<int>(nn);
     w.root = c.take<int>(nn);
     w.comp_min = c.take<int>(nn);
@@ -649,62 +679,99 @@ inline unsigned blocks_for(int64_t n, int t = 256) {
     return static_cast<unsigned>((n + t - 1) / t);
 }
 
+int bits_for(int64_t cells) {  // bits of the values 0 .. cells - 1
+    int b = 0;
+    while ((int64_t{1} << b) < cells) ++b;
+    return b;
+}
+
 template <typename T>
 int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* labels,
             const FofWs& w, hipStream_t st) {
-    const int grid_cells = device_cu_count() > 0 ? device_cu_count() * 8 : 2048;
+    const int cus = std::max(1, device_cu_count());
+    const int grid_cells = cus * 8;
     k_fill_i32n<<<1, 64, 0, st>>>(w.err, 1, 0);
     HALMA_LAUNCH_CHECK();
     k_bbox_partial<T><<<kBoxBlocks, 256, 0, st>>>(x, y, z, n, w.box_part);
     HALMA_LAUNCH_CHECK();
     k_bbox_final<<<1, 256, 0, st>>>(w.box_part, kBoxBlocks, w.box);
     HALMA_LAUNCH_CHECK();
-    k_clear_table<<<blocks_for(w.cap), 256, 0, st>>>(w.tab, w.cap);
-    HALMA_LAUNCH_CHECK();
-    k_clear_words<<<1024, 256, 0, st>>>(w.bits, w.nbits / 32);
-    HALMA_LAUNCH_CHECK();
+    // the box fixes the grid (one small read-back)
+    long long* hw = pinned_words(8);
+    if (!hw) return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
+    double* hb = reinterpret_cast<double*>(hw);
+    HALMA_HIP(hipMemcpyAsync(hb, w.box, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HALMA_HIP(hipStreamSynchronize(st));
     // sub-cell side: ll/sqrt(3) with a 1e-6 relative margin (see the file header)
     const double side = ll / sqrt(3.0) * (1.0 - 1e-6);
-    const double inv_s = 1.0 / side;
-    const uint64_t bmask = static_cast<uint64_t>(w.nbits - 1);
+    Grid g;
+    g.inv_s = 1.0 / side;
+    int bits[3];
+    for (int d = 0; d < 3; ++d) {
+        g.lo[d] = hb[d];
+        const double span = (hb[3 + d] - hb[d]) * g.inv_s;
+        if (!(span >= 0.0 && span < kCellLimit))  // also catches NaN and infinities
+            return set_error(HALMA_E_LIMIT,
+                             "friends-of-friends: coordinates span more than 2^21 sub-cells "
+                             "of side ll/sqrt(3) on an axis, or are not finite");
+        bits[d] = bits_for(static_cast<int64_t>(floor(span)) + 1);
+    }
+    g.bx = bits[0];
+    g.by = bits[1];
+    g.bz = bits[2];
+    const int kbits = g.bx + g.by + g.bz;
     auto* pk = static_cast<typename Packed<T>::type*>(w.pk);
-    k_fof_keys<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, w.box, inv_s, w.tab,
-                                                 static_cast<uint64_t>(w.cap - 1), w.skey,
-                                                 w.bits, bmask, w.err, pk);
+    k_fof_keys<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, g, w.klo, w.khi, w.err, pk);
     HALMA_LAUNCH_CHECK();
-    int rc = radix_sort_pairs(w.skey, nullptr, w.skey_sorted, w.perm, n, w.slot_bits,
+    // stable LSD sort by the key: the low 32 bits, then the high ones
+    int rc = radix_sort_pairs(w.klo, nullptr, w.klo_s, w.perm1, n, std::max(1, std::min(kbits, 32)),
                               w.sort_ws, st);
     if (rc) return rc;
-    k_run_heads<<<blocks_for(n), 256, 0, st>>>(w.skey_sorted, n, w.head);
+    const int32_t* perm = w.perm1;
+    if (kbits > 32) {
+        k_gather_u32<<<blocks_for(n), 256, 0, st>>>(w.khi, w.perm1, n, w.khi_g);
+        HALMA_LAUNCH_CHECK();
+        rc = radix_sort_pairs(w.khi_g, w.perm1, w.khi_s, w.perm, n, kbits - 32, w.sort_ws, st);
+        if (rc) return rc;
+        perm = w.perm;
+    }
+    T* sx = static_cast<T*>(w.sx);
+    T* sy = static_cast<T*>(w.sy);
+    T* sz = static_cast<T*>(w.sz);
+    k_fof_sorted<T><<<blocks_for(n), 256, 0, st>>>(perm, n, g, pk, sx, sy, sz, w.skey);
     HALMA_LAUNCH_CHECK();
-    rc = exclusive_scan_i32_to_i64(w.head, w.before, n, w.totals, w.scan_ws, st);
+    k_fof_heads<<<blocks_for(n), 256, 0, st>>>(w.skey, n, g.bz, w.heads);
+    HALMA_LAUNCH_CHECK();
+    rc = exclusive_scan_i64(w.heads, w.before, n, w.totals, w.scan_ws, st);
     if (rc) return rc;
-    k_cells<T><<<blocks_for(n), 256, 0, st>>>(
-        w.skey_sorted, w.perm, w.before, w.head, n, w.tab, pk, static_cast<T*>(w.sx),
-        static_cast<T*>(w.sy), static_cast<T*>(w.sz), w.pcell, w.cell_start, w.cell_key,
-        w.cell_min, w.parent, w.comp_min);
+    k_col_table_size<<<1, 64, 0, st>>>(w.totals, w.cap);
+    HALMA_LAUNCH_CHECK();
+    k_clear_cols<<<grid_cells, 256, 0, st>>>(w.totals, w.tab);
+    HALMA_LAUNCH_CHECK();
+    k_fof_cells<<<blocks_for(n), 256, 0, st>>>(w.skey, perm, w.heads, w.before, n, g.bz,
+                                               w.totals, w.pcell, w.cell_start, w.cell_z,
+                                               w.cell_min, w.parent, w.comp_min, w.col_cell,
+                                               w.col_key, w.tab, w.err);
     HALMA_LAUNCH_CHECK();
     k_clear_words<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(w.n_big), 4);
     HALMA_LAUNCH_CHECK();
-    const T* sx = static_cast<const T*>(w.sx);
-    const T* sy = static_cast<const T*>(w.sy);
-    const T* sz = static_cast<const T*>(w.sz);
-    const unsigned cand_grid = static_cast<unsigned>(std::max(1, device_cu_count()) * 32);
-    k_fof_cand<T, true><<<cand_grid, 256, 0, st>>>(
-        w.tab, static_cast<uint64_t>(w.cap - 1), w.bits, bmask, w.cell_start, w.cell_key, sx,
-        sy, sz, w.totals, ll * ll, w.parent, w.big, w.big_cap, w.n_big);
+    const unsigned cand_grid = static_cast<unsigned>(cus * 32);
+    const double ll2 = ll * ll;
+    k_fof_cols<T, 1><<<cand_grid, 256, 0, st>>>(w.tab, w.col_cell, w.col_key, w.cell_start,
+                                                w.cell_z, sx, sy, sz, w.totals, g.bx, g.by,
+                                                ll2, w.parent, w.big, w.big_cap, w.n_big);
     HALMA_LAUNCH_CHECK();
     k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big, w.big_cap, w.cell_start, sx, sy,
-                                             sz, ll * ll, w.parent);
+                                             sz, ll2, w.parent);
     HALMA_LAUNCH_CHECK();
     k_flatten<<<grid_cells, 256, 0, st>>>(w.totals, w.parent);
     HALMA_LAUNCH_CHECK();
-    k_fof_cand<T, false><<<cand_grid, 256, 0, st>>>(
-        w.tab, static_cast<uint64_t>(w.cap - 1), w.bits, bmask, w.cell_start, w.cell_key, sx,
-        sy, sz, w.totals, ll * ll, w.parent, w.big, w.big_cap, w.n_big + 1);
+    k_fof_cols<T, 2><<<cand_grid, 256, 0, st>>>(w.tab, w.col_cell, w.col_key, w.cell_start,
+                                                w.cell_z, sx, sy, sz, w.totals, g.bx, g.by,
+                                                ll2, w.parent, w.big, w.big_cap, w.n_big + 1);
     HALMA_LAUNCH_CHECK();
     k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big + 1, w.big_cap, w.cell_start, sx,
-                                             sy, sz, ll * ll, w.parent);
+                                             sy, sz, ll2, w.parent);
     HALMA_LAUNCH_CHECK();
     k_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.parent, w.root);
     HALMA_LAUNCH_CHECK();
@@ -718,8 +785,9 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
 
 }  // namespace
 
-// Shared with the group builder: runs FoF into `labels` and leaves the cell structures in
-// the workspace.  Asynchronous; *err (device) is non-zero on a range or table failure.
+// Runs FoF into `labels` (the cell structures stay in the workspace).  Asynchronous
+// after one read-back of the bounding box; *err (device) is non-zero on a range or table
+// failure.
 int fof_labels_async(const void* x, const void* y, const void* z, int dtype, int64_t n,
                      double ll, int* labels, void* ws, size_t ws_bytes, int** err_out,
                      int64_t** ncells_out, hipStream_t st) {
