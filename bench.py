@@ -1009,14 +1009,17 @@ def bench_sources(args):
 
 def bench_centring(args):
     """Differentiable centring: 200 Hernquist haloes of 5e4-5e5 particles (eps = 0.5 kpc).
-    One step = the halo potential energy W_h = -G/2 sum m_i S_i of every halo (forward:
-    the softened sum of every member against its halo, sum of n_h^2 pairs) + its autograd
-    backward (the hand-written acceleration kernel over the same pairs) + the
-    potential-minimum centre of every halo from seeds 0.1-0.5 a off (autograd-driven
-    descent).  Roofline: the forward kernel (FP32 VALU, 12 FLOP per pair), its time from HIP
-    events on the stream around the forward call.  Checks (CPU leg, the oracle's FP64 sums
-    on the three smallest haloes): W, the gradient, and each centre against the FP64
-    brute-force minimum over the halo's particles."""
+    One step = the halo potential energy W_h = -G/2 sum m_i S_i of every halo with its
+    autograd backward dW/dx (the softened sum and the hand-written acceleration kernel
+    over the sum of n_h^2 pairs: with positions that require a gradient the forward launch
+    evaluates both in one pass, and the backward launches nothing) + the potential-minimum
+    centre of every halo from seeds 0.1-0.5 a off (autograd-driven descent).  Roofline: the
+    fused pass (FP32 VALU; algorithmic work 12 FLOP per pair for S plus 20 for the
+    acceleration), its time from HIP events on the stream around the forward call; the two
+    kernels unfused (S alone, the acceleration alone) are timed once after the steps.
+    Checks (CPU leg, the oracle's FP64 sums on the three smallest haloes): W, the
+    gradient, and each centre against the FP64 brute-force minimum over the halo's
+    particles."""
     import numpy as np
     import torch
 
@@ -1060,22 +1063,39 @@ def bench_centring(args):
     c = res.centres.cpu().numpy()
     err = np.linalg.norm(c - w["centres"], axis=1) / w["a"]
     it = res.iters.cpu().numpy()
-    fwd_tf = FLOP_PER_SOFT_PAIR * pairs / fwd_s / 1e12
-    bwd_tf = FLOP_PER_ACC_PAIR * pairs / bwd_s / 1e12
-    log(f"centring: {step_s * 1e3:.1f} ms/step (forward {fwd_s * 1e3:.1f} ms {fwd_tf:.1f} "
-        f"TFLOP/s, backward {bwd_s * 1e3:.1f} ms {bwd_tf:.1f} TFLOP/s, centres "
-        f"{cen_s * 1e3:.2f} ms); iterations max {it.max()} median {np.median(it)}; "
-        f"|c - c_true|/a median {np.median(err):.2e}")
+    fused_flop = FLOP_PER_SOFT_PAIR + FLOP_PER_ACC_PAIR
+    fwd_tf = fused_flop * pairs / fwd_s / 1e12
+    # the two kernels unfused, once each (S alone; the acceleration alone)
+    from pyhalma_amd import engine
+    packed = torch.cat([pos, m[:, None]], 1).contiguous()
+    s = torch.cuda.current_stream()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    e[0].record(s)
+    engine.pot_soft_self(off, packed, eps, pot=True)
+    e[1].record(s)
+    engine.pot_soft_self(off, packed, eps, pot=False, acc=True)
+    e[2].record(s)
+    torch.cuda.synchronize()
+    pot_ms, acc_ms = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
+    pot_tf = FLOP_PER_SOFT_PAIR * pairs / pot_ms / 1e9
+    acc_tf = FLOP_PER_ACC_PAIR * pairs / acc_ms / 1e9
+    log(f"centring: {step_s * 1e3:.1f} ms/step (forward with the acceleration "
+        f"{fwd_s * 1e3:.1f} ms {fwd_tf:.1f} TFLOP/s, backward {bwd_s * 1e3:.2f} ms, centres "
+        f"{cen_s * 1e3:.2f} ms; unfused: S {pot_ms:.1f} ms {pot_tf:.1f} TFLOP/s, acceleration "
+        f"{acc_ms:.1f} ms {acc_tf:.1f} TFLOP/s); iterations max {it.max()} median "
+        f"{np.median(it)}; |c - c_true|/a median {np.median(err):.2e}")
     roofline = {"bound": "valu", "achieved": round(fwd_tf, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(fwd_tf / PEAK_FP32_TFLOPS, 4),
                 "traffic": None, "roof": VALU_ROOF,
-                "kernel": "k_soft_tile (softened FP32 pair sums, tests = the halo's members; "
-                          "the potential-energy forward)",
-                "flop_per_pair": FLOP_PER_SOFT_PAIR, "pairs_per_launch": pairs,
+                "kernel": "k_soft_tile<S and acceleration, tests = the halo's members> (the "
+                          "potential-energy forward with the gradient fused)",
+                "flop_per_pair": fused_flop, "pairs_per_launch": pairs,
                 "launch_ms": round(fwd_s * 1e3, 3), "launches_per_step": 1,
-                "backward": {"kernel": "k_soft_tile (acceleration)", "ms": bwd_s * 1e3,
-                             "flop_per_pair": FLOP_PER_ACC_PAIR, "tflops": bwd_tf,
-                             "frac": bwd_tf / PEAK_FP32_TFLOPS}}
+                "unfused": {"S": {"ms": pot_ms, "flop_per_pair": FLOP_PER_SOFT_PAIR,
+                                  "tflops": pot_tf, "frac": pot_tf / PEAK_FP32_TFLOPS},
+                            "acceleration": {"ms": acc_ms, "flop_per_pair": FLOP_PER_ACC_PAIR,
+                                             "tflops": acc_tf,
+                                             "frac": acc_tf / PEAK_FP32_TFLOPS}}}
     out = _line("particles/s halo potential energy forward + backward + potential-minimum "
                 "centres (200 Hernquist haloes)", n / step_s, "particles/s", step_s, args,
                 {"workload": "200 Hernquist haloes of 5e4-5e5 (log-uniform), FP32, eps 0.5 "

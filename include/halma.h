@@ -88,7 +88,11 @@ double halma_pot_error_bound(void);
  * three-coordinate rule (which also drops j = k).  Members are the CSR ranges
  * [offsets[h], offsets[h+1]) (HOST offsets) of the FP32 arrays m, x, y, z; out is FP64
  * and indexed like them.  Every pair distance is computed once for both of its terms;
- * the relative error is below halma_pot_sym_error_bound().  Stream-ordered. */
+ * the relative error is below halma_pot_sym_error_bound().  Haloes of more than 64
+ * blocks of 512 members run their block pairs in tiles of 64 x 64 blocks, one launch per
+ * tile, each tile's partial sums added (FP64, fixed order) before the next, so the
+ * workspace stays linear in the member count (at most 65 partial slots per member).
+ * Stream-ordered. */
 size_t halma_pot_sum_self_workspace_size(int64_t n_halo, const int64_t *offsets);
 int halma_pot_sum_self_batched(int64_t n_halo, const int64_t *offsets, const float *m,
                                const float *x, const float *y, const float *z, double *out,
@@ -434,6 +438,26 @@ int halma_energy_test_batched(int64_t n_halo, const int64_t *src_off, const floa
                               int vel_dtype, const double *halo_v, double factor,
                               uint8_t *bound, double *s_out, int64_t *stats, void *workspace,
                               size_t workspace_bytes, void *stream);
+
+/* The same test when each halo's tests are also its first sources (the gas cells of
+ * halo_gas.py:285-492, which are their own sources, followed by dark matter and stars):
+ * halo h's tests are [tst_off[h], tst_off[h+1]) of (tm, tx, ty, tz) and its sources are
+ * those tests then its extra sources [ext_off[h], ext_off[h+1]) of (em, ex, ey, ez) (HOST
+ * offsets; a halo may have no extra sources).  The tests' own pair sum runs on the
+ * symmetric kernel (each pair distance once, halma_pot_sum_self_batched), the extra
+ * sources on the chunked kernel; decisions, band re-test (FP64 in the oracle's order over
+ * the tests then the extra sources) and outputs as halma_energy_test_batched with
+ * src = tests ++ extras.  stats [2] counts (test, source) pairs of the reference's loop. */
+size_t halma_energy_test_self_workspace_size(int64_t n_halo, const int64_t *tst_off,
+                                             const int64_t *ext_off);
+int halma_energy_test_self_batched(int64_t n_halo, const int64_t *tst_off, const float *tm,
+                                   const float *tx, const float *ty, const float *tz,
+                                   const int64_t *ext_off, const float *em, const float *ex,
+                                   const float *ey, const float *ez, const void *vx,
+                                   const void *vy, const void *vz, int vel_dtype,
+                                   const double *halo_v, double factor, uint8_t *bound,
+                                   double *s_out, int64_t *stats, void *workspace,
+                                   size_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------------------------------------------
  * Softened potential and its gradient, for the potential-minimum centring of haloes

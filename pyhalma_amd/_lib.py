@@ -39,6 +39,10 @@ SIGNATURES = {
     "halma_energy_test_batched": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _i64p, _p, _p, _p,
                                              _p, _p, _p, _c.c_int, _p, _dbl, _p, _p, _i64p,
                                              _p, _size, _p]),
+    "halma_energy_test_self_workspace_size": (_size, [_i64, _i64p, _i64p]),
+    "halma_energy_test_self_batched": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _i64p, _p, _p,
+                                                  _p, _p, _p, _p, _p, _c.c_int, _p, _dbl, _p,
+                                                  _p, _i64p, _p, _size, _p]),
     "halma_pot_soft_batched": (_c.c_int, [_i64, _i64p, _p, _p, _p, _p, _i64p, _p, _p, _p,
                                           _dbl, _p, _p, _p, _p, _size, _p]),
     "halma_pot_soft_batched_packed": (_c.c_int, [_i64, _i64p, _p, _i64p, _p, _p, _p, _dbl, _p,

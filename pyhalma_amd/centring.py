@@ -152,16 +152,23 @@ class HaloPotentialEnergy(torch.autograd.Function):
     masses m (N,) in halo order (CSR host offsets), S_i the softened sum over the other
     members of i's halo.  Forward: halma_pot_soft_self_batched (S); backward: the same
     entry's acceleration kernel (a), pair terms recomputed:
-      dW/dx_k = -G m_k a_k,   dW/dm_k = -G S_k."""
+      dW/dx_k = -G m_k a_k,   dW/dm_k = -G S_k.
+    With ``fuse`` and positions that require a gradient, the forward launch evaluates the
+    acceleration with S in the same pass over the pairs (the differences, distances and
+    the coordinate rule are shared) and keeps it (N x 3) for the backward, which then
+    launches nothing; without it the backward recomputes every pair term."""
 
     @staticmethod
-    def forward(ctx, pos, m, offsets, eps, G):
+    def forward(ctx, pos, m, offsets, eps, G, fuse):
         packed = torch.cat([pos.detach().float(), m.detach().float()[:, None]], 1).contiguous()
-        S = engine.pot_soft_self(offsets, packed, eps, pot=True)["pot"]
+        fused = bool(fuse) and ctx.needs_input_grad[0]
+        r = engine.pot_soft_self(offsets, packed, eps, pot=True, acc=fused)
+        S = r["pot"]
         off_dev = engine.to_dev(offsets, np.int64)
         W = -0.5 * G * _halo_sums(m.detach().double() * S, off_dev, offsets)
         ctx.save_for_backward(m, S)
         ctx.packed, ctx.offsets, ctx.off_dev, ctx.eps, ctx.G = packed, offsets, off_dev, eps, G
+        ctx.acc = r["acc"] if fused else None
         ctx.pos_dtype = pos.dtype
         return W
 
@@ -174,24 +181,28 @@ class HaloPotentialEnergy(torch.autograd.Function):
         g = gW.detach().double()[seg]
         gp = gm = None
         if ctx.needs_input_grad[0]:
-            a = engine.pot_soft_self(ctx.offsets, ctx.packed, ctx.eps, pot=False,
-                                     acc=True)["acc"]
+            a = ctx.acc
+            if a is None:
+                a = engine.pot_soft_self(ctx.offsets, ctx.packed, ctx.eps, pot=False,
+                                         acc=True)["acc"]
             gp = ((-ctx.G * g * m.detach().double())[:, None] * a).to(ctx.pos_dtype)
         if ctx.needs_input_grad[1]:
             gm = (-ctx.G * g * S).to(m.dtype)
-        return gp, gm, None, None, None
+        return gp, gm, None, None, None, None
 
 
-def potential_energy(pos, m, offsets, eps, G=G_CONST):
+def potential_energy(pos, m, offsets, eps, G=G_CONST, fuse=True):
     """W_h = -G/2 sum_{i in h} m_i S_i (float64, one per halo) for particle positions
     pos (N, 3) in halo order with CSR host offsets; differentiable in pos and m.  The
     pair terms use the FP32 positions and masses (as every pair sum of the library), so
-    a member's own pair is dropped by the coordinate rule whatever the input precision."""
+    a member's own pair is dropped by the coordinate rule whatever the input precision.
+    ``fuse``: when pos requires a gradient, the forward pass also evaluates the
+    acceleration (one pass over the pairs for both); else the backward recomputes it."""
     offsets = np.asarray(offsets, np.int64)
     if offsets.ndim != 1 or len(offsets) < 1 or offsets[0] != 0 or offsets[-1] != len(pos) \
             or len(m) != len(pos) or np.any(np.diff(offsets) < 0):
         raise ValueError("offsets must be CSR offsets from 0 to len(pos) over pos and m")
-    return HaloPotentialEnergy.apply(pos, m, offsets, float(eps), float(G))
+    return HaloPotentialEnergy.apply(pos, m, offsets, float(eps), float(G), bool(fuse))
 
 
 class CentringResult:

@@ -540,6 +540,10 @@ __global__ __launch_bounds__(kBig) void k_density_peak(PeakArgs a) {
         grids_of(a, g, gr);
         const int64_t b = a.off[g], e = a.off[g + 1];
         for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+            // a group with too many distinct nodes goes to the dense grid: stop at once
+            // (probing a full table would cost up to kTable probes per member)
+            if (__hip_atomic_load(&overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                break;
             const int64_t i = a.mem[k];
             const int node = cell_of(a, gr, i);
             const double m = ldf(a.s.m, a.s.mass_t, i);
@@ -547,7 +551,9 @@ __global__ __launch_bounds__(kBig) void k_density_peak(PeakArgs a) {
             for (int probe = 0; probe < kTable; ++probe) {
                 const int old = atomicCAS(&skey[h], -1, node);
                 if (old == -1 || old == node) {
-                    if (old == -1 && atomicAdd(&used, 1) >= kTableFill) overflow = 1;
+                    if (old == -1 && atomicAdd(&used, 1) >= kTableFill)
+                        __hip_atomic_store(&overflow, 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                     atomicAdd(&sval[h], m);
                     break;
                 }

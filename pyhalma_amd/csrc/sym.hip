@@ -195,13 +195,15 @@ __global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
                     }
                 }
             }
-            // test side of tile a from this item: slot B
+            // test side of tile a from this item: slot B (mod the band width)
+            const int tslot = B % d.wb;
 #pragma unroll
             for (int r = 0; r < 2; ++r)
-                if (ii[r] < n) pp[static_cast<int64_t>(B) * stride + ii[r]] = make_float2(as[r], ac[r]);
+                if (ii[r] < n) pp[static_cast<int64_t>(tslot) * stride + ii[r]] = make_float2(as[r], ac[r]);
         }
-        // source side of B's tiles from this item: slot A, or slot nb on the diagonal
-        const int slot = A < B ? A : nb;
+        // source side of B's tiles from this item: slot A (mod the band width), or the
+        // diagonal slot min(wb, nb)
+        const int slot = A < B ? A % d.wb : min(d.wb, nb);
         for (int tb = 0; tb < kSymK; ++tb) {
             const int b = B * kSymK + tb;
             if (b >= nt) break;
@@ -218,34 +220,48 @@ __global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
     }
 }
 
-__global__ void k_sym_merge(const SymDesc* __restrict__ halos, const int* __restrict__ mhalo,
-                            int64_t n_tot, const int64_t* __restrict__ member_base,
-                            const float2* __restrict__ part, double* __restrict__ out) {
-    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g >= n_tot) return;
-    const int h = mhalo[g];
-    const SymDesc d = halos[h];
-    const int64_t i = g - member_base[h];
-    const int nb = sym_blocks(d.n);
-    // the nb + 1 slots in order, loads eight at a time in flight
-    const float2* pp = part + d.part_off + i;
-    double s = 0.0;
-    for (int c = 0; c <= nb; c += 8) {
-        float2 q[8];
+// The slots of one tile launch added into each affected member's FP64 sum (out, cleared
+// before the first launch): segment s = {halo, first member, count, role << 24 | local
+// slots}; the member adds its local slots 0 .. nloc - 1 in order, then, on a diagonal
+// tile, the diagonal slot.  With one band a member's slots 0..nb are added in order.
+__global__ __launch_bounds__(256) void k_sym_merge_tile(const SymDesc* __restrict__ halos,
+                                                        const int4* __restrict__ segs, int nseg,
+                                                        const int64_t* __restrict__ member_base,
+                                                        const float2* __restrict__ part,
+                                                        double* __restrict__ out) {
+    for (int s = blockIdx.y; s < nseg; s += gridDim.y) {
+        const int4 sg = segs[s];
+        const SymDesc d = halos[sg.x];
+        const int nloc = sg.w & 0xffffff;
+        const bool diag = (sg.w >> 24) == 2;
+        for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < sg.z; k += gridDim.x * blockDim.x) {
+            const int64_t i = sg.y + k;
+            const float2* pp = part + d.part_off + i;
+            const int64_t g = member_base[sg.x] + i;
+            double acc = out[g];
+            for (int c = 0; c < nloc; c += 8) {
+                float2 q[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            q[u] = c + u <= nb ? pp[static_cast<int64_t>(c + u) * d.n] : make_float2(0.0f, 0.0f);
+                for (int u = 0; u < 8; ++u)
+                    q[u] = c + u < nloc ? pp[static_cast<int64_t>(c + u) * d.n]
+                                        : make_float2(0.0f, 0.0f);
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (c + u <= nb) s += static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
+                for (int u = 0; u < 8; ++u)
+                    if (c + u < nloc) acc += static_cast<double>(q[u].x) + static_cast<double>(q[u].y);
+            }
+            if (diag) {
+                const float2 q = pp[static_cast<int64_t>(min(d.wb, d.nb)) * d.n];
+                acc += static_cast<double>(q.x) + static_cast<double>(q.y);
+            }
+            out[g] = acc;
+        }
     }
-    out[g] = s;
 }
 
-__global__ void k_sym_mhalo(const int64_t* __restrict__ base, int H, int* __restrict__ mh) {
-    const int h = blockIdx.x;
-    if (h >= H) return;
-    for (int64_t k = base[h] + threadIdx.x; k < base[h + 1]; k += blockDim.x) mh[k] = h;
+__global__ void k_sym_zero(double* __restrict__ p, int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[i] = 0.0;
 }
 
 __global__ void k_sym_pack(const float* __restrict__ m, const float* __restrict__ x,
@@ -264,20 +280,21 @@ struct SymWs {
     SymDesc* halos;
     int4* items;
     float2* part;
-    int* mhalo;
+    int4* segs;
     int64_t* base;
     int* head;
     size_t bytes;
 };
 
-SymWs carve_sym(void* b, const SymPlan& p, int64_t H) {
+// n_segs: the merge segments of a banded plan (at most two per halo and tile)
+SymWs carve_sym(void* b, const SymPlan& p, int64_t H, int64_t n_segs) {
     Carver c(b);
     SymWs w;
     w.src4 = c.take<float4>(std::max<int64_t>(p.n_members, 1));
     w.halos = c.take<SymDesc>(std::max<int64_t>(H, 1));
     w.items = c.take<int4>(std::max<int64_t>(p.n_items, 1));
     w.part = c.take<float2>(std::max<int64_t>(p.n_part, 1));
-    w.mhalo = c.take<int>(std::max<int64_t>(p.n_members, 1));
+    w.segs = c.take<int4>(std::max<int64_t>(n_segs, 1));
     w.base = c.take<int64_t>(H + 1);
     w.head = c.take<int>(64);
     w.bytes = c.used;
@@ -286,7 +303,8 @@ SymWs carve_sym(void* b, const SymPlan& p, int64_t H) {
 
 }  // namespace
 
-int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items, int max_members) {
+int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items, int max_members,
+             int band) {
     plan = SymPlan();
     plan.halos.resize(H);
     int64_t part = 0;
@@ -300,7 +318,9 @@ int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items, int 
         d.part_off = part;
         d.n = static_cast<int32_t>(n);
         d.nb = (max_members != 0 && d.n > max_members) ? 0 : sym_blocks(d.n);
-        part += static_cast<int64_t>(d.nb + 1) * n;
+        d.wb = (band > 0 && d.nb > band) ? band : d.nb + 1;
+        d.pad = 0;
+        part += static_cast<int64_t>(std::min(d.wb, d.nb) + 1) * n;
         work[h] = n;
     }
     plan.order = order_desc(work);
@@ -313,10 +333,44 @@ int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items, int 
     plan.n_items = n_items;
     if (with_items) {
         plan.items.reserve(static_cast<size_t>(n_items));
-        for (int32_t h : plan.order) {
-            const int nb = plan.halos[h].nb;
-            for (int B = 0; B < nb; ++B)
-                for (int A = 0; A <= B; ++A) plan.items.push_back(make_int4(h, A, B, 0));
+        int64_t tiles = 1;
+        for (int64_t h = 0; h < H; ++h) {
+            const SymDesc& d = plan.halos[h];
+            const int64_t K = d.nb > 0 ? (d.nb + d.wb - 1) / d.wb : 0;
+            tiles = std::max<int64_t>(tiles, K * (K + 1) / 2);
+        }
+        plan.launch_off.push_back(0);
+        plan.seg_off.push_back(0);
+        for (int64_t t = 0; t < tiles; ++t) {
+            for (int32_t h : plan.order) {
+                const SymDesc& d = plan.halos[h];
+                const int nb = d.nb, wb = d.wb;
+                const int64_t K = nb > 0 ? (nb + wb - 1) / wb : 0;
+                if (t >= K * (K + 1) / 2) continue;
+                int Q = 0;
+                while (static_cast<int64_t>(Q + 1) * (Q + 2) / 2 <= t) ++Q;
+                const int P = static_cast<int>(t - static_cast<int64_t>(Q) * (Q + 1) / 2);
+                const int b0 = Q * wb, b1 = std::min(nb, (Q + 1) * wb);
+                const int a0 = P * wb, a1 = std::min(nb, (P + 1) * wb);
+                for (int B = b0; B < b1; ++B)
+                    for (int A = a0; A < a1 && A <= B; ++A)
+                        plan.items.push_back(make_int4(h, A, B, 0));
+                // the members of bands P and Q take this tile's slots into their sums
+                auto seg = [&](int band, int role, int nloc) {
+                    const int64_t m0 = static_cast<int64_t>(band) * wb * 4 * 128;
+                    const int64_t m1 = std::min<int64_t>(d.n, m0 + static_cast<int64_t>(wb) * 512);
+                    plan.segs.push_back(make_int4(h, static_cast<int>(m0),
+                                                  static_cast<int>(m1 - m0), (role << 24) | nloc));
+                };
+                if (P == Q) {
+                    seg(P, 2, b1 - b0);
+                } else {
+                    seg(P, 0, b1 - b0);
+                    seg(Q, 1, a1 - a0);
+                }
+            }
+            plan.launch_off.push_back(static_cast<int64_t>(plan.items.size()));
+            plan.seg_off.push_back(static_cast<int64_t>(plan.segs.size()));
         }
     }
     plan.n_members = off[H] - off[0];
@@ -352,9 +406,11 @@ double halma_pot_sym_error_bound(void) { return pot_sym_error_bound(); }
 
 size_t halma_pot_sum_self_workspace_size(int64_t n_halo, const int64_t* offsets) {
     if (n_halo < 0 || !offsets) return 0;
+    for (int64_t h = 0; h < n_halo; ++h)
+        if (offsets[h + 1] < offsets[h]) return 0;
     SymPlan plan;
-    if (plan_sym(n_halo, offsets, plan, false)) return 0;
-    return carve_sym(nullptr, plan, n_halo).bytes;
+    if (plan_sym(n_halo, offsets, plan, true, 0, kSymBand)) return 0;
+    return carve_sym(nullptr, plan, n_halo, static_cast<int64_t>(plan.segs.size())).bytes;
 }
 
 int halma_pot_sum_self_batched(int64_t n_halo, const int64_t* offsets, const float* m,
@@ -368,10 +424,11 @@ int halma_pot_sum_self_batched(int64_t n_halo, const int64_t* offsets, const flo
     if (n == 0) return 0;
     if (!m || !x || !y || !z || !out) return set_error(HALMA_E_ARG, "null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
+    // haloes of more than kSymBand blocks run tile by tile (slots O(n kSymBand) per member)
     SymPlan plan;
-    int rc = plan_sym(n_halo, offsets, plan, true);
+    int rc = plan_sym(n_halo, offsets, plan, true, 0, kSymBand);
     if (rc) return rc;
-    SymWs w = carve_sym(ws, plan, n_halo);
+    SymWs w = carve_sym(ws, plan, n_halo, static_cast<int64_t>(plan.segs.size()));
     if (ws_bytes < w.bytes)
         return set_error(HALMA_E_WORKSPACE, "workspace %zu < required %zu", ws_bytes, w.bytes);
     const int64_t o0 = offsets[0];
@@ -384,16 +441,33 @@ int halma_pot_sum_self_batched(int64_t n_halo, const int64_t* offsets, const flo
     if (!rc) rc = upload_async(w.base, base.data(), base.size() * sizeof(int64_t), st);
     if (!rc && plan.n_items > 0)
         rc = upload_async(w.items, plan.items.data(), plan.items.size() * sizeof(int4), st);
+    if (!rc && !plan.segs.empty())
+        rc = upload_async(w.segs, plan.segs.data(), plan.segs.size() * sizeof(int4), st);
     if (rc) return rc;
-    k_sym_mhalo<<<static_cast<unsigned>(n_halo), 256, 0, st>>>(w.base, static_cast<int>(n_halo),
-                                                               w.mhalo);
+    double* o = out + o0;
+    k_sym_zero<<<256, 256, 0, st>>>(o, n);
     HALMA_LAUNCH_CHECK();
-    rc = launch_pot_sym(plan, w.src4, w.halos, nullptr, w.items, nullptr, w.part, w.head, st,
-                        true);
-    if (rc) return rc;
-    k_sym_merge<<<static_cast<unsigned>((n + 255) / 256), 256, 0, st>>>(w.halos, w.mhalo, n,
-                                                                       w.base, w.part, out + o0);
-    HALMA_LAUNCH_CHECK();
+    int cus = device_cu_count();
+    if (cus <= 0) cus = 256;
+    const int64_t T = static_cast<int64_t>(plan.launch_off.size()) - 1;
+    for (int64_t t = 0; t < T; ++t) {
+        const int64_t i0 = plan.launch_off[t], ni = plan.launch_off[t + 1] - i0;
+        const int64_t s0 = plan.seg_off[t], ns = plan.seg_off[t + 1] - s0;
+        if (ni > 0) {
+            k_sym_clear<<<1, 64, 0, st>>>(w.head);
+            HALMA_LAUNCH_CHECK();
+            const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(cus * 8, (ni + 3) / 4)));
+            k_pot_sym<<<grid, 256, 0, st>>>(w.src4, w.halos, nullptr, w.items + i0,
+                                            static_cast<int>(ni), nullptr, w.part, w.head);
+            HALMA_LAUNCH_CHECK();
+        }
+        if (ns > 0) {
+            const dim3 mg(32, static_cast<unsigned>(std::min<int64_t>(ns, 4096)));
+            k_sym_merge_tile<<<mg, 256, 0, st>>>(w.halos, w.segs + s0, static_cast<int>(ns),
+                                                 w.base, w.part, o);
+            HALMA_LAUNCH_CHECK();
+        }
+    }
     return 0;
 }
 

@@ -100,27 +100,44 @@ __global__ void k_member_halo(const int64_t* __restrict__ off, int H, int32_t* _
 }
 
 __global__ void k_init_halo(const int64_t* __restrict__ off, int H, int32_t* __restrict__ active,
-                            int32_t* __restrict__ iters, int32_t* __restrict__ removed) {
+                            int32_t* __restrict__ iters, int32_t* __restrict__ removed,
+                            int32_t* __restrict__ stale, int32_t* __restrict__ need_full,
+                            int32_t* __restrict__ refresh) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     active[h] = off[h + 1] > off[h] ? 1 : 0;
     iters[h] = 0;
     removed[h] = 0;
+    stale[h] = 0;
+    need_full[h] = 0;
+    refresh[h] = 0;
 }
 
 // A halo's pass is full (tests = sources = the live bound set, symmetric kernel) on
-// the first pass and whenever more than a quarter of the live set left in the previous
-// pass; otherwise incremental (the removed members' terms are subtracted).
-__device__ __forceinline__ bool pass_is_full(int pass_no, int run, int rrun) {
-    return pass_no == 0 || 4 * static_cast<int64_t>(rrun) > run;
+// the first pass, whenever more than a quarter of the live set left in the previous
+// pass, and after a pass that left too many stale members (below); otherwise
+// incremental (the removed members' terms are subtracted).
+__device__ __forceinline__ bool pass_is_full(int pass_no, int run, int rrun, int need_full) {
+    return pass_no == 0 || need_full != 0 || 4 * static_cast<int64_t>(rrun) > run;
 }
 
-// Largest relative error bound a member's S may keep: a kept member whose accumulated
+// Largest relative error bound a member's S may keep.  A kept member whose accumulated
 // bound (the full pass's, plus that of every incremental subtraction since) goes above it
-// is re-summed in FP64 by the re-test kernel, like the members of the decision band.
-// Both full-pass kernels stay below it (halma_pot_error_bound at sub-tiles of 4 or 8,
-// halma_pot_sym_error_bound).
+// is stale.  When a halo has few stale members (at most an eighth of its live set) the
+// re-test kernel re-sums them in FP64, like the members of the decision band; otherwise
+// the halo's next pass is a full one (the symmetric kernel recomputes every S for less
+// than the FP64 sums would cost), and a halo that removed nobody takes that full pass
+// only to refresh its S (not counted as an iteration: its decisions are exact and do not
+// change).  Both full-pass kernels stay below the limit (halma_pot_error_bound at
+// sub-tiles of 4 or 8, halma_pot_sym_error_bound).
 constexpr double kPotRelMax = 1e-6;
+
+__device__ __forceinline__ bool too_stale(int stale, int live) {
+    return 8 * static_cast<int64_t>(stale) > live;
+}
+
+// band-list entries of stale members carry this bit (members of the decision band do not)
+constexpr uint32_t kStaleBit = 0x80000000u;
 
 __global__ void k_zero_counters(unsigned long long* __restrict__ c, int n) {
     if (static_cast<int>(threadIdx.x) < n) c[threadIdx.x] = 0;
@@ -247,6 +264,7 @@ __global__ __launch_bounds__(kThreads) void k_halo_sets(
     int32_t* __restrict__ mode, double* __restrict__ vcm, int32_t* __restrict__ tst_len,
     int32_t* __restrict__ src_len, int32_t* __restrict__ src_sel, int64_t* __restrict__ icnt,
     int64_t* __restrict__ icnt_sym, const SymDesc* __restrict__ sym,
+    int32_t* __restrict__ stale, int32_t* __restrict__ need_full,
     unsigned long long* __restrict__ counters) {
     const int lane = __lane_id();
     const int pos = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x +
@@ -314,7 +332,9 @@ __global__ __launch_bounds__(kThreads) void k_halo_sets(
     const double M = acc[0][0] + acc[0][1];
     for (int d = 0; d < 3; ++d)
         vcm[3 * h + d] = M > 0.0 ? (acc[1 + d][0] + acc[1 + d][1]) / M : 0.0;
-    const int full = pass_is_full(*pass_no, run, rrun) ? 1 : 0;
+    const int full = pass_is_full(*pass_no, run, rrun, need_full[h]) ? 1 : 0;
+    need_full[h] = 0;
+    stale[h] = 0;
     n_bound[h] = run;
     n_rem[h] = rrun;
     mode[h] = full;
@@ -389,6 +409,9 @@ struct EnergyArgs {
     double* eacc;      // absolute error bound of sacc
     int32_t* removed;
     int32_t* band;
+    int32_t* stale;      // per halo: stale members this pass
+    int32_t* need_full;  // per halo: the next pass is a full one
+    int32_t* refresh;    // per halo: this pass only refreshes S (not an iteration)
     unsigned long long* counters;  // [0] removed total, [1] band count, [2] band total,
                                    // [3] pairs, [4] symmetric-kernel pairs, [5] haloes
                                    // still active after the pass, [6] members re-summed
@@ -475,20 +498,26 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     const bool stale = !gone && !in_band && err > kPotRelMax * S;
     if (in_band || stale) {
         const unsigned long long at = atomicAdd(a.counters + 1, 1ull);
-        a.band[at] = static_cast<int32_t>(g);
-        atomicAdd(a.counters + 2, 1ull);
-        if (stale) atomicAdd(a.counters + 6, 1ull);
+        a.band[at] = static_cast<int32_t>(static_cast<uint32_t>(g) | (stale ? kStaleBit : 0u));
     } else if (gone) {
         a.state[k] = kJustGone;
     }
-    // removals per halo, one atomic per halo present in the wave (members of a halo are
-    // contiguous, so a wave spans few haloes)
+    // removals and stale members per halo, one atomic per halo present in the wave
+    // (members of a halo are contiguous, so a wave spans few haloes)
     unsigned long long pending = __ballot(gone);
     while (pending) {
         const int leader = __ffsll(static_cast<long long>(pending)) - 1;
         const int hl = __shfl(h, leader, kWave);
         const unsigned long long same = __ballot(gone && h == hl);
         if (__lane_id() == leader) atomicAdd(a.removed + hl, __popcll(same));
+        pending &= ~same;
+    }
+    pending = __ballot(stale);
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const int hl = __shfl(h, leader, kWave);
+        const unsigned long long same = __ballot(stale && h == hl);
+        if (__lane_id() == leader) atomicAdd(a.stale + hl, __popcll(same));
         pending &= ~same;
     }
 }
@@ -499,24 +528,29 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 // End of a pass for the halo h (every thread of the block calls it): per-halo iteration
-// counts, the halo stays active while it removed someone, and the counters
-// (wave-aggregated atomics): [0] removed total, [3] pairs of the reference's loop this
-// pass (n_bound x n_src), [4] those of the symmetric kernel's passes, [5] haloes still
-// active.
+// counts, the halo stays active while it removed someone (or, once, to refresh the S of
+// too many stale members), and the counters (wave-aggregated atomics): [0] removed
+// total, [3] pairs of the reference's loop this pass (n_bound x n_src), [4] those of the
+// symmetric kernel's passes, [5] haloes still active.
 __device__ void finish_haloes(const EnergyArgs& a, int h) {
     unsigned long long rem = 0, pairs = 0, full_pairs = 0, live = 0;
     if (h < a.H && a.active[h]) {
         const int nb = a.n_bound[h];
         const int r = a.removed[h];
+        const bool refreshing = a.refresh[h] != 0;
         if (nb > 0) {
-            a.iters[h] += 1;
+            if (!(refreshing && r == 0)) a.iters[h] += 1;
             pairs = static_cast<unsigned long long>(nb) *
                     static_cast<unsigned long long>(a.n_src[h]);
             if (a.mode[h] && a.sym[h].nb > 0) full_pairs = pairs;  // symmetric kernel's
         }
         const bool stay = r > 0 && nb > 0;
-        if (!stay) a.active[h] = 0;
-        live = stay ? 1 : 0;
+        const bool redo = nb > 0 && too_stale(a.stale[h], nb);
+        a.need_full[h] = redo ? 1 : 0;
+        const bool refresh = !stay && redo && !refreshing;
+        a.refresh[h] = refresh ? 1 : 0;
+        if (!stay && !refresh) a.active[h] = 0;
+        live = (stay || refresh) ? 1 : 0;
         rem = static_cast<unsigned long long>(r);
         a.removed[h] = 0;
     }
@@ -542,8 +576,11 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
     __shared__ int last;
     const int64_t nb = static_cast<int64_t>(a.counters[1]);
     for (int64_t w = blockIdx.x; w < nb; w += gridDim.x) {
-        const int64_t g = a.band[w];
+        const uint32_t entry = static_cast<uint32_t>(a.band[w]);
+        const int64_t g = entry & ~kStaleBit;
         const int h = a.mhalo[g];
+        // stale members of a halo that takes a full pass next are left to that pass
+        if ((entry & kStaleBit) && too_stale(a.stale[h], a.n_bound[h])) continue;
         const HaloDesc d = a.halos[h];
         const int n = a.n_bound[h];
         const float4 t = src4[g];
@@ -555,6 +592,8 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
             a.sacc[k] = S;
             a.eacc[k] = 1e-12 * S;
             if (a.s_out) a.s_out[k] = S;
+            atomicAdd(a.counters + 2, 1ull);
+            if (entry & kStaleBit) atomicAdd(a.counters + 6, 1ull);
             if (E > 0.0) {
                 a.state[k] = kJustGone;
                 atomicAdd(a.removed + h, 1);
@@ -737,6 +776,9 @@ struct UnbWs {
     int32_t* active;
     int32_t* iters;
     int32_t* removed;
+    int32_t* stale;
+    int32_t* need_full;
+    int32_t* refresh;
     int32_t* pass_no;
     double* vcm;
     int32_t* band;
@@ -828,6 +870,9 @@ UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, const Sy
     w.active = c.take<int32_t>(h1);
     w.iters = c.take<int32_t>(h1);
     w.removed = c.take<int32_t>(h1);
+    w.stale = c.take<int32_t>(h1);
+    w.need_full = c.take<int32_t>(h1);
+    w.refresh = c.take<int32_t>(h1);
     w.pass_no = c.take<int32_t>(64);
     w.vcm = c.take<double>(3 * h1);
     w.band = c.take<int32_t>(n1);
@@ -893,7 +938,7 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
                   kThreads, 0, st>>>(
         static_cast<int>(L.H), w.order, w.halos, tile, w.cfirst, w.active, w.pass_no, w.ccnt,
         w.csum, w.n_bound, w.n_rem, w.mode, w.vcm, w.tst_len, w.src_len, w.src_sel, w.icnt,
-        w.sym_icnt, w.sym, w.counters);
+        w.sym_icnt, w.sym, w.stale, w.need_full, w.counters);
     HALMA_LAUNCH_CHECK();
     if (L.H <= kFuseH) {
         const int64_t most = std::max<int64_t>(L.plan->n_items, w.sym_cap);
@@ -1072,7 +1117,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
         HALMA_LAUNCH_CHECK();
     }
     k_init_halo<<<blocks_for(H), kThreads, 0, st>>>(w.off, static_cast<int>(H), w.active, w.iters,
-                                                    w.removed);
+                                                    w.removed, w.stale, w.need_full, w.refresh);
     HALMA_LAUNCH_CHECK();
     k_zero_counters<<<1, 64, 0, st>>>(w.counters, 8);
     HALMA_LAUNCH_CHECK();
@@ -1112,6 +1157,9 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.ea.eacc = w.eacc;
     L.ea.removed = w.removed;
     L.ea.band = w.band;
+    L.ea.stale = w.stale;
+    L.ea.need_full = w.need_full;
+    L.ea.refresh = w.refresh;
     L.ea.counters = w.counters;
     L.ea.s_out = s_out;
     L.ea.nm = nm;

@@ -239,6 +239,50 @@ def energy_test_batched(src_off, m, x, y, z, tst_off, tx, ty, tz, vx, vy, vz, ha
     return bound[:nt], (S[:nt] if S is not None else None), int(st[0])
 
 
+def energy_test_self_batched(tst_off, m, x, y, z, vx, vy, vz, halo_v, factor, ext_off=None,
+                             em=None, ex=None, ey=None, ez=None, with_pot=False, stats=None):
+    """The energy test with each halo's tests as its first sources
+    (halma_energy_test_self_batched): halo h's tests (m, x, y, z, v over CSR host tst_off)
+    are summed against each other on the symmetric kernel and against its extra sources
+    (em, ex, ey, ez over CSR host ext_off; none when ext_off is None) on the chunked
+    kernel.  Returns as energy_test_batched."""
+    import ctypes
+    from .device import dtype_flag
+    L = _lib_()
+    tst_off = np.asarray(tst_off, np.int64)
+    H = len(tst_off) - 1
+    if ext_off is None:
+        ext_off = np.zeros(H + 1, np.int64)
+        em = ex = ey = ez = np.zeros(1, np.float32)
+    ext_off = np.asarray(ext_off, np.int64)
+    if len(ext_off) != H + 1:
+        raise ValueError("offset arrays must have the same length")
+    m, x, y, z = (to_dev(a, np.float32) for a in (m, x, y, z))
+    em, ex, ey, ez = (to_dev(a, np.float32) for a in (em, ex, ey, ez))
+    vx, vy, vz = _coords(vx, vy, vz)
+    hv = to_dev(np.asarray(halo_v, np.float64).reshape(-1, 3) if not isinstance(halo_v, torch.Tensor)
+                else halo_v.double().reshape(-1, 3), np.float64).contiguous()
+    if hv.shape[0] != H:
+        raise ValueError("halo_v must hold 3 values per halo")
+    nt = int(x.numel())
+    bound = torch.zeros(max(nt, 1), dtype=torch.uint8, device=x.device)
+    S = torch.zeros(max(nt, 1), dtype=torch.float64, device=x.device) if with_pot else None
+    st = (ctypes.c_int64 * 3)()
+    if H > 0 and nt > 0:
+        to = _lib.i64_array(tst_off)
+        eo = _lib.i64_array(ext_off)
+        ws = workspace(L.halma_energy_test_self_workspace_size(H, to, eo))
+        rc = L.halma_energy_test_self_batched(H, to, ptr(m), ptr(x), ptr(y), ptr(z), eo,
+                                              ptr(em), ptr(ex), ptr(ey), ptr(ez), ptr(vx),
+                                              ptr(vy), ptr(vz), dtype_flag(vx), ptr(hv),
+                                              float(factor), ptr(bound), ptr(S), st, ptr(ws),
+                                              ws.numel(), stream_ptr())
+        _lib.check(rc, "halma_energy_test_self_batched")
+    if stats is not None:
+        stats.update(retested=int(st[0]), pair_ns=int(st[1]), pairs=int(st[2]))
+    return bound[:nt], (S[:nt] if S is not None else None), int(st[0])
+
+
 def _coords(x, y, z):
     """Device coordinate tensors sharing one float dtype (float32 kept, else float64)."""
     from .device import float_tensor

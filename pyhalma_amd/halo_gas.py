@@ -239,6 +239,8 @@ def RPS_batched(gas, gas_offsets, halo_v, dm=None, stars=None, stats=None):
     of halo h = its gas, then its DM, then its stars, all summed (no subsampling).
     Returns (H, 4) float64: total gas mass, cold bound fraction, cold unbound mass, hot
     unbound mass (cold iff T < 5e4 K), and the per-cell bound mask (device uint8).
+    The gas self-term runs on the symmetric kernel (each cell pair once), the dark matter
+    and stars against the cells on the chunked kernel (halma_energy_test_self_batched).
     ``stats`` (a dict) receives the energy test's counters (engine.energy_test_batched)."""
     import torch
 
@@ -247,10 +249,15 @@ def RPS_batched(gas, gas_offsets, halo_v, dm=None, stars=None, stats=None):
     gas_offsets = np.asarray(gas_offsets, np.int64)
     H = len(gas_offsets) - 1
     gx, gy, gz, gvx, gvy, gvz, gm, gt = gas
-    groups = [(gx, gy, gz, gm, gas_offsets)] + [g for g in (dm, stars) if g is not None]
-    src_off, (sx, sy, sz, sm) = _source_csr(groups, H)
-    bound, _, _ = engine.energy_test_batched(src_off, sm, sx, sy, sz, gas_offsets, gx, gy, gz,
-                                             gvx, gvy, gvz, halo_v, 2.0, stats=stats)
+    # the gas cells are their own first sources (symmetric kernel); dark matter and stars
+    # follow as extra sources
+    extra = [g for g in (dm, stars) if g is not None]
+    ext = dict(ext_off=None)
+    if extra:
+        ext_off, (ex, ey, ez, em) = _source_csr(extra, H)
+        ext = dict(ext_off=ext_off, em=em, ex=ex, ey=ey, ez=ez)
+    bound, _, _ = engine.energy_test_self_batched(gas_offsets, gm, gx, gy, gz, gvx, gvy, gvz,
+                                                  halo_v, 2.0, stats=stats, **ext)
     mass = to_dev(gm, np.float64)
     temp = to_dev(gt, np.float64)
     dev = mass.device

@@ -160,6 +160,11 @@ def test_potential_energy_gradient_vs_fp64_acceleration(gpu):
     W = centring.potential_energy(pos, m, off, EPS)
     W.sum().backward()
     g = pos.grad.cpu().numpy()
+    # the backward that recomputes the acceleration gives the same bits as the fused one
+    pos2 = pos.detach().clone().requires_grad_(True)
+    W2 = centring.potential_energy(pos2, m, off, EPS, fuse=False)
+    W2.sum().backward()
+    assert torch.equal(W, W2) and torch.equal(pos.grad, pos2.grad)
     G = centring.G_CONST
     for h in range(len(off) - 1):
         s, e = off[h], off[h + 1]

@@ -129,3 +129,42 @@ def test_energy_test_knife_edge_and_edges(gpu):
     assert 0 < want.sum() < n
     assert np.array_equal(got[:n], want)
     assert got[n:].tolist() == [True, False, True]
+
+
+def test_energy_test_self_form_vs_oracle(gpu):
+    """The self form (the gas cells are their own first sources: symmetric kernel for the
+    gas pairs, chunked kernel for dark matter and stars) decides every cell as the FP64
+    oracle does, its S is within the bound of the FP64 sum, and haloes without extra
+    sources work."""
+    from pyhalma_amd import engine
+    from pyhalma_amd.halo_gas import _source_csr
+
+    w = _workload(seed=6, n=8, nmax=40000)
+    off = w["offsets"]
+    H = len(off) - 1
+    ext_off, (ex, ey, ez, em) = _source_csr([w["dm"], w["stars"]], H)
+    bound, S, retested = engine.energy_test_self_batched(
+        off, w["mass"], w["x"], w["y"], w["z"], w["vx"], w["vy"], w["vz"], w["bulk"], 2.0,
+        ext_off=ext_off, em=em, ex=ex, ey=ey, ez=ez, with_pot=True)
+    want, S64 = _oracle_masks(w, w["bulk"])
+    got = bound.cpu().numpy().astype(bool)
+    print(f"haloes={H} cells={off[-1]} largest={np.diff(off).max()} retested={retested}",
+          flush=True)
+    assert np.array_equal(got, want), int((got != want).sum())
+    rel = np.abs(S.cpu().numpy() - S64) / S64
+    assert rel.max() <= max(engine.pot_error_bound(), engine.pot_sym_error_bound())
+    # no extra sources: the gas alone, against the plain form
+    b2, S2, _ = engine.energy_test_self_batched(off, w["mass"], w["x"], w["y"], w["z"], w["vx"],
+                                                w["vy"], w["vz"], w["bulk"], 2.0, with_pot=True)
+    b3, S3, _ = engine.energy_test_batched(off, w["mass"], w["x"], w["y"], w["z"], off, w["x"],
+                                           w["y"], w["z"], w["vx"], w["vy"], w["vz"], w["bulk"],
+                                           2.0, with_pot=True)
+    assert torch_equal(b2, b3)
+    rel = np.abs(S2.cpu().numpy() - S3.cpu().numpy()) / S3.cpu().numpy()
+    assert rel.max() <= engine.pot_error_bound() + engine.pot_sym_error_bound()
+
+
+def torch_equal(a, b):
+    import torch
+
+    return bool(torch.equal(a, b))

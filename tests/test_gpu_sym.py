@@ -151,3 +151,35 @@ def test_pair_kernels_nan_coordinates(gpu):
         assert np.array_equal(np.isnan(got), np.isnan(want))
         a, b = off[0], off[1]
         assert np.all(np.abs(got[a:b] - want[a:b]) <= bound * want[a:b])
+
+
+def test_pot_sym_banded_large_haloes(gpu):
+    """Haloes of more than 64 blocks of 512 members run their block pairs tile by tile
+    (bands of 64 blocks, partial sums merged between tiles): sampled members of haloes of
+    40,000 (two bands), 70,000 (three bands) and 32,768 members (exactly one band) are
+    within the bound of the FP64 oracle, and the small haloes of the call have the same
+    bits as in a call of their own."""
+    from pyhalma_amd import engine, synth
+
+    rng = np.random.default_rng(23)
+    sizes = [40000, 500, 70000, 3000, 32768]
+    parts = [synth.plummer_positions(rng, n, 0.01).astype(np.float32) for n in sizes]
+    P = np.concatenate(parts)
+    m = rng.uniform(1e4, 1e6, len(P)).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    got = engine.pot_sum_self(off, m, P[:, 0], P[:, 1], P[:, 2]).cpu().numpy()
+    B = engine.pot_sym_error_bound()
+    for h, n in enumerate(sizes):
+        a, b = off[h], off[h + 1]
+        pick = np.sort(rng.choice(n, min(n, 1500), replace=False)) + a
+        want = oracle.pot_sum_f64(m[a:b], P[a:b, 0], P[a:b, 1], P[a:b, 2], P[pick, 0],
+                                  P[pick, 1], P[pick, 2])
+        err = np.abs(got[pick] - want) / want
+        print(f"n={n} max rel err {err.max():.2e}", flush=True)
+        assert err.max() <= B, n
+    # halo 1 (500) and halo 3 (3000) alone: the same bits
+    for h in (1, 3):
+        a, b = off[h], off[h + 1]
+        alone = engine.pot_sum_self([0, b - a], m[a:b], P[a:b, 0], P[a:b, 1],
+                                    P[a:b, 2]).cpu().numpy()
+        assert np.array_equal(alone, got[a:b])
