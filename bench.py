@@ -46,8 +46,8 @@ FOF_BYTES_PER_PARTICLE = 16  # 12 B FP32 xyz read + 4 B label written
 # HBM bytes per k_pot_sym launch on the default workload, averaged over the launches of a
 # step (the first pass carries almost all of it): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
 # two rocprofv3 --pmc passes, each scaled by what the counter reads for a 256 MiB device
-# copy (profiles/bench_step_2M_hbm_traffic.md)
-SYM_TRAFFIC_BYTES = 80.9e6
+# copy (profiles/clumps_2M_hbm_traffic.md)
+SYM_TRAFFIC_BYTES = 80.1e6
 MINP = 50
 VALU_ROOF = ("FP32 vector ALU (157.3 TFLOP/s, MI355X's dense FP32 figure, which counts "
              "packed FMA); the kernel issues FP32 vector instructions, no matrix "
@@ -584,12 +584,20 @@ def bench_gas(args):
     achieved = FLOP_PER_PAIR * timed[-1]["pairs"] / pair_s / 1e12
     log(f"gas: {step_s * 1e3:.1f} ms/step; pair kernel {pair_s * 1e3:.1f} ms "
         f"({achieved:.1f} TFLOP/s)")
+    # the self-term runs on the symmetric kernel in bands of 64 blocks of 512 members: one
+    # launch per tile (P <= Q) of the halo with the most bands
+    blocks = -(-(-(-sizes // 128)) // 4)
+    bands = -(-blocks // 64)
+    launches = int((bands * (bands + 1) // 2).max())
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                 "traffic": None, "roof": VALU_ROOF,
-                "kernel": "k_pot_f32 (chunked FP32 pair kernel, one launch per step)",
-                "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": int(timed[-1]["pairs"]),
-                "launch_ms": round(pair_s * 1e3, 3), "launches_per_step": 1}
+                "kernel": "k_pot_sym (symmetric FP32 pair kernel, one launch per tile of 64 x 64 "
+                          "blocks of 512 cells; the HIP-event window also holds the tile merges)",
+                "flop_per_pair": FLOP_PER_PAIR,
+                "pairs_per_launch": int(timed[-1]["pairs"]) // launches,
+                "launch_ms": round(pair_s * 1e3 / launches, 3), "launches_per_step": launches,
+                "window_ms": round(pair_s * 1e3, 3)}
     # full-size check on a sample: FP64 fixed-order sums of sampled cells
     rng = np.random.default_rng(1)
     sample = np.sort(rng.choice(n, 256, replace=False))

@@ -55,7 +55,11 @@ int halma_device_cu_count(void);
  * Kahan-compensated tiles; the FP64 value of the compensated sum is written to
  * out_f64 when that pointer is non-NULL, and out (float32, the reference's output
  * type) may be NULL when only out_f64 is wanted.  The relative error against the
- * FP64 direct sum is below halma_pot_error_bound() (a rigorous bound).
+ * FP64 direct sum is below halma_pot_error_bound() (a rigorous bound).  The rule is
+ * evaluated as "the smallest |coordinate difference| is not zero", which is the
+ * reference's for finite and NaN coordinates; two equal infinite coordinates give
+ * inf - inf = NaN, so their pair is counted (the reference drops it): infinite
+ * coordinates are outside parity.
  * ---------------------------------------------------------------------------------- */
 size_t halma_pot_sum_workspace_size(int64_t n_src, int64_t n_test);
 int halma_pot_sum_f32(const float *m, const float *x, const float *y, const float *z,

@@ -142,4 +142,34 @@ __device__ double block_pot_f64_ordered(Src src, int64_t ns, float tx, float ty,
     return out;
 }
 
+// The same sum and the same bits from one wavefront: lane l keeps the partials l, l + 64,
+// l + 128 and l + 192 (each over its sources in ascending j), folds the tree's first two
+// levels in its registers (L[l] + L[l + 128], L[l + 64] + L[l + 192], then their sum, as
+// the levels s = 128 and s = 64 add them) and the remaining six by shuffles.  Every lane
+// returns the same value.
+template <class Src>
+__device__ double wave_pot_f64_ordered(Src src, int64_t ns, float tx, float ty, float tz) {
+    static_assert(kOrdWidth == 4 * kWave, "four partials per lane");
+    const int l = __lane_id();
+    const double xd = tx, yd = ty, zd = tz;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t j = l;
+    for (; j + 3 * kWave < ns; j += kOrdWidth) {
+        const float4 q0 = src(j), q1 = src(j + kWave), q2 = src(j + 2 * kWave),
+                     q3 = src(j + 3 * kWave);
+        a0 = ord_term(a0, q0, tx, ty, tz, xd, yd, zd);
+        a1 = ord_term(a1, q1, tx, ty, tz, xd, yd, zd);
+        a2 = ord_term(a2, q2, tx, ty, tz, xd, yd, zd);
+        a3 = ord_term(a3, q3, tx, ty, tz, xd, yd, zd);
+    }
+    if (j < ns) a0 = ord_term(a0, src(j), tx, ty, tz, xd, yd, zd);
+    if (j + kWave < ns) a1 = ord_term(a1, src(j + kWave), tx, ty, tz, xd, yd, zd);
+    if (j + 2 * kWave < ns) a2 = ord_term(a2, src(j + 2 * kWave), tx, ty, tz, xd, yd, zd);
+    a0 = a0 + a2;
+    a1 = a1 + a3;
+    double r = a0 + a1;
+    for (int s = kWave / 2; s >= 1; s >>= 1) r = r + __shfl_down(r, s, kWave);
+    return __shfl(r, 0, kWave);
+}
+
 }  // namespace halma

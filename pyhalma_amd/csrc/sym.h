@@ -38,17 +38,18 @@ struct __attribute__((aligned(8))) SymDesc {
 
 struct SymPlan {
     std::vector<SymDesc> halos;
-    std::vector<int4> items;    // {halo, block A, block B, 0}, A <= B, largest haloes first
-                                // (with bands: launch by launch)
     std::vector<int32_t> order; // the haloes by size, largest first
     int64_t n_items = 0, n_members = 0, n_part = 0;
     // with items: launch t runs items [launch_off[t], launch_off[t + 1]) (tile t of every
     // halo that has one; tile t = (P, Q) with t = Q (Q + 1) / 2 + P) and then merges the
     // member segments [seg_off[t], seg_off[t + 1]) of segs: {halo, first member, member
     // count, role << 24 | local slots}; role 0: the test side of band P (off-diagonal
-    // tile), 1: the source side of band Q, 2: a diagonal tile (and its diagonal slot)
+    // tile), 1: the source side of band Q, 2: a diagonal tile (and its diagonal slot).
+    // The items themselves ({halo, block A, block B, 0}, A <= B, B-major within a tile,
+    // largest haloes first within a launch) are written on the device from tiles:
+    // {halo, P, Q, index of the tile's first item}, O(haloes x tiles) on the host.
     std::vector<int64_t> launch_off, seg_off;
-    std::vector<int4> segs;
+    std::vector<int4> segs, tiles;
 };
 
 // Plans from HOST member offsets (H+1 entries); the item list only with with_items.
@@ -57,6 +58,10 @@ struct SymPlan {
 // listed tile by tile, see halma_pot_sum_self_batched); 0 keeps every halo in one band.
 int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items,
              int max_members = 0, int band = 0);
+
+// Writes the items of plan.tiles (n_tiles, on the device) into items.
+int expand_sym_items(const int4* d_tiles, int64_t n_tiles, const SymDesc* d_halos, int4* items,
+                     hipStream_t st);
 
 // Persistent launch over the items (count plan.n_items, or *n_items_dev when given);
 // live_n (device, may be null) overrides each halo's member count.  head must hold 0

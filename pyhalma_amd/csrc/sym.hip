@@ -271,6 +271,25 @@ __global__ void k_sym_pack(const float* __restrict__ m, const float* __restrict_
     if (i < n) out[i] = make_float4(x[i], y[i], z[i], m[i]);
 }
 
+// One block per tile (P, Q) of a halo: its items (A, B), B-major, A <= B, from the
+// tile's first item index on.
+__global__ __launch_bounds__(256) void k_sym_expand(const int4* __restrict__ tiles,
+                                                    const SymDesc* __restrict__ halos,
+                                                    int4* __restrict__ items) {
+    const int4 t = tiles[blockIdx.x];
+    const SymDesc d = halos[t.x];
+    const int wb = d.wb, nb = d.nb;
+    const int a0 = t.y * wb, a1 = min(nb, (t.y + 1) * wb);
+    const int b0 = t.z * wb, b1 = min(nb, (t.z + 1) * wb);
+    int64_t at = t.w;
+    for (int B = b0; B < b1; ++B) {
+        const int ae = t.y == t.z ? B + 1 : a1;  // diagonal tile: A <= B
+        for (int A = a0 + static_cast<int>(threadIdx.x); A < ae; A += blockDim.x)
+            items[at + (A - a0)] = make_int4(t.x, A, B, 0);
+        at += ae - a0;
+    }
+}
+
 __global__ void k_sym_clear(int* __restrict__ p) {
     if (threadIdx.x == 0) *p = 0;
 }
@@ -281,6 +300,7 @@ struct SymWs {
     int4* items;
     float2* part;
     int4* segs;
+    int4* tiles;
     int64_t* base;
     int* head;
     size_t bytes;
@@ -295,6 +315,7 @@ SymWs carve_sym(void* b, const SymPlan& p, int64_t H, int64_t n_segs) {
     w.items = c.take<int4>(std::max<int64_t>(p.n_items, 1));
     w.part = c.take<float2>(std::max<int64_t>(p.n_part, 1));
     w.segs = c.take<int4>(std::max<int64_t>(n_segs, 1));
+    w.tiles = c.take<int4>(std::max<size_t>(p.tiles.size(), 1));
     w.base = c.take<int64_t>(H + 1);
     w.head = c.take<int>(64);
     w.bytes = c.used;
@@ -332,7 +353,6 @@ int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items, int 
     if (n_items > INT32_MAX) return set_error(HALMA_E_LIMIT, "too many work items");
     plan.n_items = n_items;
     if (with_items) {
-        plan.items.reserve(static_cast<size_t>(n_items));
         int64_t tiles = 1;
         for (int64_t h = 0; h < H; ++h) {
             const SymDesc& d = plan.halos[h];
@@ -341,6 +361,7 @@ int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items, int 
         }
         plan.launch_off.push_back(0);
         plan.seg_off.push_back(0);
+        int64_t at = 0;  // items listed so far
         for (int64_t t = 0; t < tiles; ++t) {
             for (int32_t h : plan.order) {
                 const SymDesc& d = plan.halos[h];
@@ -352,9 +373,9 @@ int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items, int 
                 const int P = static_cast<int>(t - static_cast<int64_t>(Q) * (Q + 1) / 2);
                 const int b0 = Q * wb, b1 = std::min(nb, (Q + 1) * wb);
                 const int a0 = P * wb, a1 = std::min(nb, (P + 1) * wb);
-                for (int B = b0; B < b1; ++B)
-                    for (int A = a0; A < a1 && A <= B; ++A)
-                        plan.items.push_back(make_int4(h, A, B, 0));
+                plan.tiles.push_back(make_int4(h, P, Q, static_cast<int>(at)));
+                at += P == Q ? static_cast<int64_t>(b1 - b0) * (b1 - b0 + 1) / 2
+                             : static_cast<int64_t>(b1 - b0) * (a1 - a0);
                 // the members of bands P and Q take this tile's slots into their sums
                 auto seg = [&](int band, int role, int nloc) {
                     const int64_t m0 = static_cast<int64_t>(band) * wb * 4 * 128;
@@ -369,12 +390,22 @@ int plan_sym(int64_t H, const int64_t* off, SymPlan& plan, bool with_items, int 
                     seg(Q, 1, a1 - a0);
                 }
             }
-            plan.launch_off.push_back(static_cast<int64_t>(plan.items.size()));
+            plan.launch_off.push_back(at);
             plan.seg_off.push_back(static_cast<int64_t>(plan.segs.size()));
         }
+        if (at != n_items) return set_error(HALMA_E_STATE, "symmetric plan: item count");
     }
     plan.n_members = off[H] - off[0];
     plan.n_part = part;
+    return 0;
+}
+
+int expand_sym_items(const int4* d_tiles, int64_t n_tiles, const SymDesc* d_halos, int4* items,
+                     hipStream_t st) {
+    if (n_tiles <= 0) return 0;
+    if (n_tiles > INT32_MAX) return set_error(HALMA_E_LIMIT, "too many symmetric tiles");
+    k_sym_expand<<<static_cast<unsigned>(n_tiles), 256, 0, st>>>(d_tiles, d_halos, items);
+    HALMA_LAUNCH_CHECK();
     return 0;
 }
 
@@ -439,10 +470,13 @@ int halma_pot_sum_self_batched(int64_t n_halo, const int64_t* offsets, const flo
     for (int64_t h = 0; h <= n_halo; ++h) base[h] = offsets[h] - o0;
     rc = upload_async(w.halos, plan.halos.data(), plan.halos.size() * sizeof(SymDesc), st);
     if (!rc) rc = upload_async(w.base, base.data(), base.size() * sizeof(int64_t), st);
-    if (!rc && plan.n_items > 0)
-        rc = upload_async(w.items, plan.items.data(), plan.items.size() * sizeof(int4), st);
+    if (!rc && !plan.tiles.empty())
+        rc = upload_async(w.tiles, plan.tiles.data(), plan.tiles.size() * sizeof(int4), st);
     if (!rc && !plan.segs.empty())
         rc = upload_async(w.segs, plan.segs.data(), plan.segs.size() * sizeof(int4), st);
+    if (!rc)
+        rc = expand_sym_items(w.tiles, static_cast<int64_t>(plan.tiles.size()), w.halos,
+                              w.items, st);
     if (rc) return rc;
     double* o = out + o0;
     k_sym_zero<<<256, 256, 0, st>>>(o, n);

@@ -27,9 +27,10 @@
 //                    bound (the bounds of every FP32 sum that went into S add up),
 //                    evaluates E in FP64 and decides every member whose |E| exceeds
 //                    factor*G*bound + 8 ulp(kin + factor*G*S);
-//   4. k_retest      one block per member inside the band: S in FP64 with the oracle's
-//                    fixed summation order over the bound set, so E and the decision are
-//                    the oracle's bit for bit (S and its bound are reset to that value);
+//   4. k_retest      one wavefront per member inside the band (or stale): S in FP64 with
+//                    the oracle's fixed summation order over the bound set, so E and the
+//                    decision are the oracle's bit for bit (S and its bound are reset to
+//                    that value);
 //   5. k_finish      per-halo pass counts, convergence flags, the removed total.
 // The parts are plain launches (or, on request, hipGraphs captured once and replayed);
 // the host reads one counter per pass to stop.
@@ -572,10 +573,11 @@ __device__ void finish_haloes(const EnergyArgs& a, int h) {
 __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
                                                       const float4* __restrict__ src4) {
     static_assert(kOrdWidth == kThreads, "one thread per halo in the closing loop");
-    __shared__ double sh[kOrdWidth];
     __shared__ int last;
     const int64_t nb = static_cast<int64_t>(a.counters[1]);
-    for (int64_t w = blockIdx.x; w < nb; w += gridDim.x) {
+    const int waves = kThreads / kWave;
+    for (int64_t w = static_cast<int64_t>(blockIdx.x) * waves + threadIdx.x / kWave; w < nb;
+         w += static_cast<int64_t>(gridDim.x) * waves) {
         const uint32_t entry = static_cast<uint32_t>(a.band[w]);
         const int64_t g = entry & ~kStaleBit;
         const int h = a.mhalo[g];
@@ -585,8 +587,8 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
         const int n = a.n_bound[h];
         const float4 t = src4[g];
         SrcAoS src{src4 + d.src_off};
-        const double S = block_pot_f64_ordered(src, n, t.x, t.y, t.z, sh);
-        if (threadIdx.x == 0) {
+        const double S = wave_pot_f64_ordered(src, n, t.x, t.y, t.z);
+        if (__lane_id() == 0) {
             const int64_t k = a.cidx[g];
             const double E = kinetic(a.vel, k, a.vcm + 3 * h) - a.cg * S;
             a.sacc[k] = S;

@@ -183,3 +183,27 @@ def test_pot_sym_banded_large_haloes(gpu):
         alone = engine.pot_sum_self([0, b - a], m[a:b], P[a:b, 0], P[a:b, 1],
                                     P[a:b, 2]).cpu().numpy()
         assert np.array_equal(alone, got[a:b])
+
+
+def test_pair_kernels_equal_infinite_coordinates(gpu):
+    """Two members whose x is the same infinity.  The reference's x_j /= x_i is false for
+    them, so it drops their pair; the kernels test the smallest |difference| against zero
+    and inf - inf is NaN, so they count it and those two members' sums are NaN (a
+    documented divergence: infinite coordinates are outside parity).  Every other member
+    of the halo is unaffected: its terms with the infinite members are m / inf = 0, as in
+    the reference, and its sum is within the bound of the oracle's."""
+    from pyhalma_amd import engine
+
+    rng = np.random.default_rng(37)
+    n = 600
+    P = rng.normal(scale=0.01, size=(n, 3)).astype(np.float32)
+    P[10, 0] = P[20, 0] = np.inf
+    m = rng.uniform(1e4, 1e6, n).astype(np.float32)
+    want = oracle.pot_sum_f64(m, *(P[:, k] for k in range(3)), *(P[:, k] for k in range(3)))
+    off = np.array([0, n])
+    sym = engine.pot_sum_self(off, m, P[:, 0], P[:, 1], P[:, 2]).cpu().numpy()
+    finite = np.ones(n, bool)
+    finite[[10, 20]] = False
+    assert np.all(np.isfinite(want))
+    assert np.all(np.isnan(sym[~finite]))
+    assert np.all(np.abs(sym[finite] - want[finite]) <= engine.pot_sym_error_bound() * want[finite])
