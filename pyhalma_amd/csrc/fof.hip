@@ -383,6 +383,87 @@ __global__ void k_flatten(const int64_t* __restrict__ totals, int* __restrict__ 
 }
 
 
+// Lane order of the link kernels: columns grouped by the log2 of their particle count,
+// largest first, so that the lanes of a wavefront walk columns of similar length (a
+// wavefront lasts as long as its longest lane) and the longest start first.  A counting sort over kColClasses classes; each block takes one
+// contiguous stretch of columns, counts it, reserves its places with one atomic per class,
+// and writes it (the order within a class is not fixed; the links do not depend on it).
+constexpr int kColClasses = 8;
+constexpr int kClassBlocks = 256;
+
+__device__ __forceinline__ int col_class(const int* __restrict__ col_cell,
+                                         const int* __restrict__ cell_start, int64_t q) {
+    const int np = cell_start[col_cell[q + 1]] - cell_start[col_cell[q]];
+    const int lg = 31 - __clz(np > 0 ? np : 1);
+    return lg < kColClasses - 1 ? lg : kColClasses - 1;
+}
+
+// counts[0..7]: columns per class; counts[8..15]: the class cursors (zero here)
+__global__ __launch_bounds__(256) void k_col_class_count(const int64_t* __restrict__ totals,
+                                                         const int* __restrict__ col_cell,
+                                                         const int* __restrict__ cell_start,
+                                                         int* __restrict__ counts) {
+    __shared__ int h[kColClasses];
+    if (threadIdx.x < kColClasses) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t ncol = totals[0] >> 32;
+    const int64_t per = (ncol + gridDim.x - 1) / gridDim.x;
+    const int64_t q0 = blockIdx.x * per, q1 = q0 + per < ncol ? q0 + per : ncol;
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x)
+        atomicAdd(&h[col_class(col_cell, cell_start, q)], 1);
+    __syncthreads();
+    if (threadIdx.x < kColClasses && h[threadIdx.x] > 0) atomicAdd(counts + threadIdx.x, h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_col_class_scatter(const int64_t* __restrict__ totals,
+                                                           const int* __restrict__ col_cell,
+                                                           const int* __restrict__ cell_start,
+                                                           int* __restrict__ counts,
+                                                           int* __restrict__ perm) {
+    __shared__ int h[kColClasses];
+    __shared__ int base[kColClasses];
+    if (threadIdx.x < kColClasses) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t ncol = totals[0] >> 32;
+    const int64_t per = (ncol + gridDim.x - 1) / gridDim.x;
+    const int64_t q0 = blockIdx.x * per, q1 = q0 + per < ncol ? q0 + per : ncol;
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x)
+        atomicAdd(&h[col_class(col_cell, cell_start, q)], 1);
+    __syncthreads();
+    if (threadIdx.x < kColClasses) {
+        int before = 0;  // columns of the higher classes: the longest columns come first
+        for (int k = threadIdx.x + 1; k < kColClasses; ++k) before += counts[k];
+        base[threadIdx.x] =
+            h[threadIdx.x] > 0 ? before + atomicAdd(counts + kColClasses + threadIdx.x, h[threadIdx.x]) : 0;
+        h[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const int k = col_class(col_cell, cell_start, q);
+        perm[base[k] + atomicAdd(&h[k], 1)] = static_cast<int>(q);
+    }
+}
+
+// Between the rounds, after k_flatten (parent[c] is then c's root): the root shared by
+// all cells of a column, or -1 when they differ.  Two columns with the same shared root
+// are joined throughout, so round 2 skips the pair without looking at its cells.
+__global__ void k_col_roots(const int64_t* __restrict__ totals, const int* __restrict__ col_cell,
+                            const int* __restrict__ parent, int* __restrict__ col_root) {
+    const int64_t ncol = totals[0] >> 32;
+    for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < ncol;
+         q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int a0 = col_cell[q], a1 = col_cell[q + 1];
+        const int r = parent[a0];
+        int u = r;
+        for (int c = a0 + 1; c < a1; ++c)
+            if (parent[c] != r) {
+                u = -1;
+                break;
+            }
+        col_root[q] = u;
+    }
+}
+
 // The neighbour columns of the half stencil, (dx, dy) with dx > 0 or dx = 0 and dy > 0:
 // the four within Chebyshev distance 1 first, then the eight at distance 2.
 __constant__ signed char kCol[12][2] = {{0, 1}, {1, -1}, {1, 0}, {1, 1},  // near
@@ -390,37 +471,33 @@ __constant__ signed char kCol[12][2] = {{0, 1}, {1, -1}, {1, 0}, {1, 1},  // nea
                                         {2, -1}, {2, 0}, {2, 1}, {2, 2}};
 constexpr int kLaneTest = 64;  // pair tests a lane does alone
 
-template <typename T>
-__device__ __forceinline__ bool cells_linked(const int* __restrict__ cell_start, const T* sx,
-                                             const T* sy, const T* sz, int c, int d,
-                                             double ll2) {
-    const int sc = cell_start[c], nc = cell_start[c + 1] - sc;
-    const int sd = cell_start[d], nd = cell_start[d + 1] - sd;
-    for (int a = 0; a < nc; ++a)
-        for (int bb = 0; bb < nd; ++bb)
-            if (within(sx, sy, sz, sc + a, sd + bb, ll2)) return true;
-    return false;
-}
-
 // One candidate pair of cells: skipped when already joined; tested here when small,
-// else listed for k_fof_big (tested here too when the list is full).
+// else listed for k_fof_big (tested here too when the list is full).  The caller holds
+// cell c's root rc (from a read-only find, possibly out of date: equal roots still prove
+// the cells joined) and its particles [sc, sc + nc); rc is refreshed after a link.
 template <typename T>
-__device__ __forceinline__ void candidate(int c, int d, const int* __restrict__ cell_start,
-                                          const T* sx, const T* sy, const T* sz, double ll2,
+__device__ __forceinline__ void candidate(int c, int& rc, int sc, int nc, int d,
+                                          const int* __restrict__ cell_start, const T* sx,
+                                          const T* sy, const T* sz, double ll2,
                                           int* __restrict__ parent, int2* __restrict__ big,
                                           int64_t big_cap,
                                           unsigned long long* __restrict__ n_big) {
-    if (find_ro(parent, c) == find_ro(parent, d)) return;
-    const int64_t nc = cell_start[c + 1] - cell_start[c];
-    const int64_t nd = cell_start[d + 1] - cell_start[d];
-    if (nc * nd > kLaneTest) {
+    if (rc == find_ro(parent, d)) return;
+    const int sd = cell_start[d], nd = cell_start[d + 1] - sd;
+    if (static_cast<int64_t>(nc) * nd > kLaneTest) {
         const unsigned long long at = atomicAdd(n_big, 1ull);
         if (static_cast<int64_t>(at) < big_cap) {
             big[at] = make_int2(c, d);
             return;
         }
     }
-    if (cells_linked(cell_start, sx, sy, sz, c, d, ll2)) uf_union(parent, c, d);
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nd; ++b)
+            if (within(sx, sy, sz, sc + a, sd + b, ll2)) {
+                uf_union(parent, c, d);
+                rc = find_ro(parent, c);
+                return;
+            }
 }
 
 // Link candidates, one lane per (column, column offset) of one round.  ROUND 1: the
@@ -436,7 +513,8 @@ __global__ __launch_bounds__(256) void k_fof_cols(
     const int* __restrict__ cell_z, const T* __restrict__ sx, const T* __restrict__ sy,
     const T* __restrict__ sz, const int64_t* __restrict__ totals, int bx, int by, double ll2,
     int* __restrict__ parent, int2* __restrict__ big, int64_t big_cap,
-    unsigned long long* __restrict__ n_big) {
+    unsigned long long* __restrict__ n_big, const int* __restrict__ col_root,
+    const int* __restrict__ col_perm) {
     constexpr int K = ROUND == 1 ? 5 : 13;  // own column + neighbour columns
     const int64_t ncol = totals[0] >> 32;
     const uint64_t mask = static_cast<uint64_t>(totals[1]);
@@ -445,16 +523,25 @@ __global__ __launch_bounds__(256) void k_fof_cols(
     const int64_t nxc = int64_t{1} << bx, nyc = int64_t{1} << by;
     for (int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
          idx < total; idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int q = static_cast<int>(idx / K);
+        const int q = col_perm[idx / K];
         const int o = static_cast<int>(idx % K);
         const int a0 = col_cell[q], a1 = col_cell[q + 1];
+        const int ra = ROUND == 2 ? col_root[q] : -1;
         if (o == 0) {  // the column's own cells, ROUND apart in z
+            if (ra >= 0) continue;  // one component already
             for (int c = a0; c + 1 < a1; ++c) {
                 const int zc = cell_z[c];
+                int rc = -1, sc = 0, nc = 0;
                 for (int d = c + 1; d < a1 && cell_z[d] - zc <= ROUND; ++d)
-                    if (cell_z[d] - zc == ROUND)
-                        candidate(c, d, cell_start, sx, sy, sz, ll2, parent, big, big_cap,
-                                  n_big);
+                    if (cell_z[d] - zc == ROUND) {
+                        if (rc < 0) {
+                            rc = find_ro(parent, c);
+                            sc = cell_start[c];
+                            nc = cell_start[c + 1] - sc;
+                        }
+                        candidate(c, rc, sc, nc, d, cell_start, sx, sy, sz, ll2, parent, big,
+                                  big_cap, n_big);
+                    }
             }
             continue;
         }
@@ -466,6 +553,7 @@ __global__ __launch_bounds__(256) void k_fof_cols(
         const int qb = col_find(tab, mask, (static_cast<uint64_t>(nx) << by) |
                                                static_cast<uint64_t>(ny));
         if (qb < 0) continue;
+        if (ra >= 0 && ra == col_root[qb]) continue;  // joined throughout
         const int b0 = col_cell[qb], b1 = col_cell[qb + 1];
         // |dz| range of this (round, column): near columns take |dz| <= 1 in round 1 and
         // |dz| = 2 in round 2, far columns |dz| <= 2
@@ -475,11 +563,18 @@ __global__ __launch_bounds__(256) void k_fof_cols(
         int p = b0;
         for (int c = a0; c < a1; ++c) {
             const int zc = cell_z[c];
+            int rc = -1, sc = 0, nc = 0;
             while (p < b1 && cell_z[p] < zc - hi) ++p;
             for (int d = p; d < b1 && cell_z[d] <= zc + hi; ++d) {
                 const int dz = cell_z[d] > zc ? cell_z[d] - zc : zc - cell_z[d];
-                if (dz >= lo)
-                    candidate(c, d, cell_start, sx, sy, sz, ll2, parent, big, big_cap, n_big);
+                if (dz < lo) continue;
+                if (rc < 0) {
+                    rc = find_ro(parent, c);
+                    sc = cell_start[c];
+                    nc = cell_start[c + 1] - sc;
+                }
+                candidate(c, rc, sc, nc, d, cell_start, sx, sy, sz, ll2, parent, big, big_cap,
+                          n_big);
             }
         }
     }
@@ -605,6 +700,8 @@ struct FofWs {
     int* cell_z;
     int* col_cell;    // n_cols + 1 entries
     uint64_t* col_key;
+    int* col_perm;    // the columns in link-lane order (by class)
+    int* class_counts;
     int* parent;
     int* root;
     int* comp_min;
@@ -653,6 +750,8 @@ FofWs carve_fof(void* base, int64_t n, size_t elt) {
     w.cell_z = c.take<int>(nn);
     w.col_cell = c.take<int>(nn + 1);
     w.col_key = c.take<uint64_t>(nn);
+    w.col_perm = c.take<int>(nn);
+    w.class_counts = c.take<int>(2 * kColClasses);
     w.parent = c.take<int>(nn);
     w.root = c.take<int>(nn);
     w.comp_min = c.take<int>(nn);
@@ -755,20 +854,33 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     HALMA_LAUNCH_CHECK();
     k_clear_words<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(w.n_big), 4);
     HALMA_LAUNCH_CHECK();
+    k_clear_words<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(w.class_counts), 2 * kColClasses);
+    HALMA_LAUNCH_CHECK();
+    k_col_class_count<<<kClassBlocks, 256, 0, st>>>(w.totals, w.col_cell, w.cell_start,
+                                                     w.class_counts);
+    HALMA_LAUNCH_CHECK();
+    k_col_class_scatter<<<kClassBlocks, 256, 0, st>>>(w.totals, w.col_cell, w.cell_start,
+                                                       w.class_counts, w.col_perm);
+    HALMA_LAUNCH_CHECK();
     const unsigned cand_grid = static_cast<unsigned>(cus * 32);
     const double ll2 = ll * ll;
     k_fof_cols<T, 1><<<cand_grid, 256, 0, st>>>(w.tab, w.col_cell, w.col_key, w.cell_start,
                                                 w.cell_z, sx, sy, sz, w.totals, g.bx, g.by,
-                                                ll2, w.parent, w.big, w.big_cap, w.n_big);
+                                                ll2, w.parent, w.big, w.big_cap, w.n_big,
+                                                nullptr, w.col_perm);
     HALMA_LAUNCH_CHECK();
     k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big, w.big_cap, w.cell_start, sx, sy,
                                              sz, ll2, w.parent);
     HALMA_LAUNCH_CHECK();
     k_flatten<<<grid_cells, 256, 0, st>>>(w.totals, w.parent);
     HALMA_LAUNCH_CHECK();
+    // the shared root of each column, in w.root (free until k_roots)
+    k_col_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.col_cell, w.parent, w.root);
+    HALMA_LAUNCH_CHECK();
     k_fof_cols<T, 2><<<cand_grid, 256, 0, st>>>(w.tab, w.col_cell, w.col_key, w.cell_start,
                                                 w.cell_z, sx, sy, sz, w.totals, g.bx, g.by,
-                                                ll2, w.parent, w.big, w.big_cap, w.n_big + 1);
+                                                ll2, w.parent, w.big, w.big_cap, w.n_big + 1,
+                                                w.root, w.col_perm);
     HALMA_LAUNCH_CHECK();
     k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big + 1, w.big_cap, w.cell_start, sx,
                                              sy, sz, ll2, w.parent);
