@@ -50,6 +50,7 @@ struct DynSets {
     const int* src_sel = nullptr;
     const float4* src_alt = nullptr;
     const int64_t* n_items = nullptr;
+    const int* chunk = nullptr;  // per-halo source chunk of this launch (else the plan's)
 };
 
 // Batched FP32 compensated pair sums over H haloes (HOST CSR offsets), FP64 (and/or

@@ -80,9 +80,11 @@ __global__ __launch_bounds__(256) void k_pot_f32(const float4* __restrict__ src,
             ns = __builtin_amdgcn_readfirstlane(uniform_load(dyn.src_len + it.x));
             if (__builtin_amdgcn_readfirstlane(uniform_load(dyn.src_sel + it.x))) sbase = dyn.src_alt;
         }
+        const int chunk =
+            dyn.chunk ? __builtin_amdgcn_readfirstlane(uniform_load(dyn.chunk + it.x)) : d.chunk;
         const int i0 = it.y * (R * kWave);
-        const int j0 = it.z * d.chunk;
-        const int j1 = min(ns, j0 + d.chunk);
+        const int j0 = it.z * chunk;
+        const int j1 = min(ns, j0 + chunk);
         if (i0 >= nt || j0 >= j1) continue;
 
         float xi[R], yi[R], zi[R], as[R], ac[R];

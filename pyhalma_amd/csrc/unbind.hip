@@ -52,6 +52,8 @@ namespace halma {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kIncChunk = 256;      // smallest source chunk of an incremental pass
+constexpr int kRetestBlock = 2048;  // live sets from this size are re-summed by a block
 
 // member states
 constexpr uint8_t kGone = 0;      // removed by an earlier pass
@@ -263,8 +265,8 @@ __global__ __launch_bounds__(kThreads) void k_halo_sets(
     const int32_t* __restrict__ pass_no, int32_t* __restrict__ ccnt,
     const double* __restrict__ csum, int32_t* __restrict__ n_bound, int32_t* __restrict__ n_rem,
     int32_t* __restrict__ mode, double* __restrict__ vcm, int32_t* __restrict__ tst_len,
-    int32_t* __restrict__ src_len, int32_t* __restrict__ src_sel, int64_t* __restrict__ icnt,
-    int64_t* __restrict__ icnt_sym, const SymDesc* __restrict__ sym,
+    int32_t* __restrict__ src_len, int32_t* __restrict__ src_sel, int32_t* __restrict__ pchunk,
+    int64_t* __restrict__ icnt, int64_t* __restrict__ icnt_sym, const SymDesc* __restrict__ sym,
     int32_t* __restrict__ stale, int32_t* __restrict__ need_full,
     unsigned long long* __restrict__ counters) {
     const int lane = __lane_id();
@@ -348,7 +350,20 @@ __global__ __launch_bounds__(kThreads) void k_halo_sets(
     // pairs A <= B of its live size; incremental ones, and full ones of haloes planned
     // without slots (above kSymMaxMembers), to the chunked kernel
     const bool use_sym = full && sym[h].nb > 0;
-    const int ch = halos[h].chunk;
+    // source chunk of this pass: the plan's for a full pass; for an incremental one (few
+    // sources: the members just removed) as many of the planned chunk slots as chunks of
+    // at least kIncChunk sources fill, so that a large halo's removals still spread over
+    // several items (ns never exceeds the planned source count, so the chunks fit)
+    const HaloDesc hd = halos[h];
+    int ch = hd.chunk;
+    if (!full && ns > 0) {
+        const int slots = (hd.src_len + hd.chunk - 1) / hd.chunk;
+        int c = (ns + slots - 1) / slots;
+        c = c < kIncChunk ? kIncChunk : c;
+        c = (c + 7) / 8 * 8;
+        ch = c < ch ? c : ch;
+    }
+    if (lane == 0) pchunk[h] = ch;
     icnt[pos] = (!use_sym && nt > 0 && ns > 0)
                     ? static_cast<int64_t>((nt + tile - 1) / tile) * ((ns + ch - 1) / ch)
                     : 0;
@@ -395,6 +410,7 @@ struct EnergyArgs {
     const int32_t* mhalo;
     const int32_t* n_bound;
     const int32_t* n_src;
+    const int32_t* pchunk;
     const int32_t* mode;
     const int32_t* cidx;
     const float2* part;
@@ -409,7 +425,8 @@ struct EnergyArgs {
     double* sacc;      // running S per member position
     double* eacc;      // absolute error bound of sacc
     int32_t* removed;
-    int32_t* band;
+    int32_t* band;       // members to re-test: small live sets from the front, large from
+    int64_t band_cap;    // the back (band_cap entries)
     int32_t* stale;      // per halo: stale members this pass
     int32_t* need_full;  // per halo: the next pass is a full one
     int32_t* refresh;    // per halo: this pass only refreshes S (not an iteration)
@@ -457,7 +474,8 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
         pp = a.sym_part + sd.part_off + p;
     } else {
         const int ns = a.n_src[h];
-        nch = (ns + d.chunk - 1) / d.chunk;
+        const int ch = a.pchunk[h];
+        nch = (ns + ch - 1) / ch;
         stride = d.stride;
         pp = a.part + d.part_off + p;
     }
@@ -497,12 +515,29 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     // a kept member whose error bound went above kPotRelMax: its S is re-summed in FP64
     // with the band (its decision, already exact, stays the same)
     const bool stale = !gone && !in_band && err > kPotRelMax * S;
-    if (in_band || stale) {
-        const unsigned long long at = atomicAdd(a.counters + 1, 1ull);
-        a.band[at] = static_cast<int32_t>(static_cast<uint32_t>(g) | (stale ? kStaleBit : 0u));
-    } else if (gone) {
-        a.state[k] = kJustGone;
+    // members to re-test: of small live sets from the front of the band list, of large
+    // ones (a block each in k_retest) from its back; one atomic per wavefront and list
+    const bool push = in_band || stale;
+    const bool big = n >= kRetestBlock;
+    const int32_t val = static_cast<int32_t>(static_cast<uint32_t>(g) | (stale ? kStaleBit : 0u));
+    const int lane = __lane_id();
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const unsigned long long m_small = __ballot(push && !big), m_big = __ballot(push && big);
+    if (m_small) {
+        const int leader = __ffsll(static_cast<long long>(m_small)) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(a.counters + 1, static_cast<unsigned long long>(__popcll(m_small)));
+        base = __shfl(base, leader, kWave);
+        if (push && !big) a.band[base + __popcll(m_small & below)] = val;
     }
+    if (m_big) {
+        const int leader = __ffsll(static_cast<long long>(m_big)) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(a.counters + 8, static_cast<unsigned long long>(__popcll(m_big)));
+        base = __shfl(base, leader, kWave);
+        if (push && big) a.band[a.band_cap - 1 - static_cast<int64_t>(base + __popcll(m_big & below))] = val;
+    }
+    if (!push && gone) a.state[k] = kJustGone;
     // removals and stale members per halo, one atomic per halo present in the wave
     // (members of a halo are contiguous, so a wave spans few haloes)
     unsigned long long pending = __ballot(gone);
@@ -573,31 +608,57 @@ __device__ void finish_haloes(const EnergyArgs& a, int h) {
 __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
                                                       const float4* __restrict__ src4) {
     static_assert(kOrdWidth == kThreads, "one thread per halo in the closing loop");
+    __shared__ double sh[kOrdWidth];
     __shared__ int last;
     const int64_t nb = static_cast<int64_t>(a.counters[1]);
+    // the sum in the oracle's order (same bits either way): a block per member of a large
+    // live set (a shorter chain per thread), a wavefront per member of a small one (more
+    // members in flight)
+    auto decide = [&](int64_t g, int h, double S) {
+        const int64_t k = a.cidx[g];
+        const double E = kinetic(a.vel, k, a.vcm + 3 * h) - a.cg * S;
+        a.sacc[k] = S;
+        a.eacc[k] = 1e-12 * S;
+        if (a.s_out) a.s_out[k] = S;
+        atomicAdd(a.counters + 2, 1ull);
+        return E;
+    };
+    const int64_t nbig = static_cast<int64_t>(a.counters[8]);
+    for (int64_t w = blockIdx.x; w < nbig; w += gridDim.x) {  // large live sets: a block each
+        const uint32_t entry = static_cast<uint32_t>(a.band[a.band_cap - 1 - w]);
+        const int64_t g = entry & ~kStaleBit;
+        const int h = a.mhalo[g];
+        const int n = a.n_bound[h];
+        // stale members of a halo that takes a full pass next are left to that pass
+        if ((entry & kStaleBit) && too_stale(a.stale[h], n)) continue;  // uniform
+        const float4 t = src4[g];
+        SrcAoS src{src4 + a.halos[h].src_off};
+        const double S = block_pot_f64_ordered(src, n, t.x, t.y, t.z, sh);
+        if (threadIdx.x == 0) {
+            const double E = decide(g, h, S);
+            if (entry & kStaleBit) atomicAdd(a.counters + 6, 1ull);
+            if (E > 0.0) {
+                a.state[a.cidx[g]] = kJustGone;
+                atomicAdd(a.removed + h, 1);
+            }
+        }
+    }
     const int waves = kThreads / kWave;
     for (int64_t w = static_cast<int64_t>(blockIdx.x) * waves + threadIdx.x / kWave; w < nb;
          w += static_cast<int64_t>(gridDim.x) * waves) {
         const uint32_t entry = static_cast<uint32_t>(a.band[w]);
         const int64_t g = entry & ~kStaleBit;
         const int h = a.mhalo[g];
-        // stale members of a halo that takes a full pass next are left to that pass
-        if ((entry & kStaleBit) && too_stale(a.stale[h], a.n_bound[h])) continue;
-        const HaloDesc d = a.halos[h];
         const int n = a.n_bound[h];
+        if ((entry & kStaleBit) && too_stale(a.stale[h], n)) continue;
         const float4 t = src4[g];
-        SrcAoS src{src4 + d.src_off};
+        SrcAoS src{src4 + a.halos[h].src_off};
         const double S = wave_pot_f64_ordered(src, n, t.x, t.y, t.z);
         if (__lane_id() == 0) {
-            const int64_t k = a.cidx[g];
-            const double E = kinetic(a.vel, k, a.vcm + 3 * h) - a.cg * S;
-            a.sacc[k] = S;
-            a.eacc[k] = 1e-12 * S;
-            if (a.s_out) a.s_out[k] = S;
-            atomicAdd(a.counters + 2, 1ull);
+            const double E = decide(g, h, S);
             if (entry & kStaleBit) atomicAdd(a.counters + 6, 1ull);
             if (E > 0.0) {
-                a.state[k] = kJustGone;
+                a.state[a.cidx[g]] = kJustGone;
                 atomicAdd(a.removed + h, 1);
             }
         }
@@ -612,7 +673,8 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
     __threadfence();  // every block's decisions visible to the closing block
     for (int h0 = 0; h0 < a.H; h0 += kOrdWidth) finish_haloes(a, h0 + threadIdx.x);
     if (threadIdx.x == 0) {
-        a.counters[1] = 0;  // the band list is rebuilt by the next pass
+        a.counters[1] = 0;  // the band lists are rebuilt by the next pass
+        a.counters[8] = 0;
         a.counters[7] = 0;
         *a.pass_no += 1;
     }
@@ -621,7 +683,8 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
 // The pass's item list: the items of the haloes with live work, in the plan's halo
 // order, test tiles fastest ({halo, tile, chunk, 0}, as in the plan).  ioff is the
 // exclusive scan of the per-position counts; the list never exceeds the planned one
-// (live sets only shrink, chunks are fixed), and is cut at cap all the same.
+// (live sets only shrink, no pass has more chunks than the plan), and is cut at cap all
+// the same.
 __global__ void k_item_fill(int H, const int32_t* __restrict__ order,
                             const int64_t* __restrict__ ioff, const int64_t* __restrict__ itotal,
                             const int32_t* __restrict__ tst_len, int tile, int64_t cap,
@@ -775,6 +838,7 @@ struct UnbWs {
     int32_t* tst_len;
     int32_t* src_len;
     int32_t* src_sel;
+    int32_t* pchunk;  // per-halo source chunk of the current pass
     int32_t* active;
     int32_t* iters;
     int32_t* removed;
@@ -869,6 +933,7 @@ UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, const Sy
     w.tst_len = c.take<int32_t>(h1);
     w.src_len = c.take<int32_t>(h1);
     w.src_sel = c.take<int32_t>(h1);
+    w.pchunk = c.take<int32_t>(h1);
     w.active = c.take<int32_t>(h1);
     w.iters = c.take<int32_t>(h1);
     w.removed = c.take<int32_t>(h1);
@@ -878,7 +943,7 @@ UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, const Sy
     w.pass_no = c.take<int32_t>(64);
     w.vcm = c.take<double>(3 * h1);
     w.band = c.take<int32_t>(n1);
-    w.counters = c.take<unsigned long long>(8);
+    w.counters = c.take<unsigned long long>(16);
     w.items = c.take<int4>(std::max<int64_t>(plan.n_items, 1));
     w.icnt = c.take<int64_t>(h1);
     w.ioff = c.take<int64_t>(h1);
@@ -939,8 +1004,8 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
     k_halo_sets<<<static_cast<unsigned>((L.H + kThreads / kWave - 1) / (kThreads / kWave)),
                   kThreads, 0, st>>>(
         static_cast<int>(L.H), w.order, w.halos, tile, w.cfirst, w.active, w.pass_no, w.ccnt,
-        w.csum, w.n_bound, w.n_rem, w.mode, w.vcm, w.tst_len, w.src_len, w.src_sel, w.icnt,
-        w.sym_icnt, w.sym, w.stale, w.need_full, w.counters);
+        w.csum, w.n_bound, w.n_rem, w.mode, w.vcm, w.tst_len, w.src_len, w.src_sel, w.pchunk,
+        w.icnt, w.sym_icnt, w.sym, w.stale, w.need_full, w.counters);
     HALMA_LAUNCH_CHECK();
     if (L.H <= kFuseH) {
         const int64_t most = std::max<int64_t>(L.plan->n_items, w.sym_cap);
@@ -999,6 +1064,7 @@ int launch_pot_f32_part(const PassLaunch& L, hipStream_t st) {
     dyn.src_sel = w.src_sel;
     dyn.src_alt = w.rem4;
     dyn.n_items = w.itotal;
+    dyn.chunk = w.pchunk;
     return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, dyn, w.items, w.part, w.head, st,
                           false);
 }
@@ -1121,7 +1187,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     k_init_halo<<<blocks_for(H), kThreads, 0, st>>>(w.off, static_cast<int>(H), w.active, w.iters,
                                                     w.removed, w.stale, w.need_full, w.refresh);
     HALMA_LAUNCH_CHECK();
-    k_zero_counters<<<1, 64, 0, st>>>(w.counters, 8);
+    k_zero_counters<<<1, 64, 0, st>>>(w.counters, 16);
     HALMA_LAUNCH_CHECK();
     k_zero_counters<<<1, 64, 0, st>>>(reinterpret_cast<unsigned long long*>(w.pass_no), 1);
     HALMA_LAUNCH_CHECK();
@@ -1144,6 +1210,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.ea.mhalo = w.mhalo;
     L.ea.n_bound = w.n_bound;
     L.ea.n_src = w.src_len;
+    L.ea.pchunk = w.pchunk;
     L.ea.mode = w.mode;
     L.ea.cidx = w.cidx;
     L.ea.part = w.part;
@@ -1159,6 +1226,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.ea.eacc = w.eacc;
     L.ea.removed = w.removed;
     L.ea.band = w.band;
+    L.ea.band_cap = nm > 0 ? nm : 1;  // the entries of w.band
     L.ea.stale = w.stale;
     L.ea.need_full = w.need_full;
     L.ea.refresh = w.refresh;
