@@ -65,11 +65,10 @@ int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const float*
                 hipEvent_t ev_end = nullptr);
 
 // Launches the persistent FP32 kernel for a plan whose tables are already on the
-// device.  head must be a device int holding 0 at launch: this launcher clears it
-// first unless reset_head is false (the caller then clears it earlier in stream order).
+// device (items taken round-robin by the waves).
 int launch_pot_f32(const PotPlan& plan, const float4* src4, const float4* tst4,
                    const HaloDesc* d_halos, DynSets dyn, const int4* d_items, float2* part,
-                   int* head, hipStream_t stream, bool reset_head = true);
+                   hipStream_t stream);
 
 // Packs SoA FP32 (m, x, y, z) into AoS float4 (x, y, z, m); m may be null (-> 0).
 int launch_pack4(const float* m, const float* x, const float* y, const float* z,

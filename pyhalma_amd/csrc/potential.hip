@@ -52,25 +52,19 @@ __global__ __launch_bounds__(256) void k_pot_f32(const float4* __restrict__ src,
                                                  const HaloDesc* __restrict__ halos,
                                                  DynSets dyn,
                                                  const int4* __restrict__ items, int n_items,
-                                                 float2* __restrict__ part,
-                                                 int* __restrict__ head) {
+                                                 float2* __restrict__ part) {
     const int lane = __lane_id();
     const int n_waves = static_cast<int>(gridDim.x * (blockDim.x / kWave));
     if (dyn.n_items) {
         const int64_t nd = uniform_load(dyn.n_items);
         n_items = static_cast<int>(nd < n_items ? nd : n_items);
     }
-    // the first item of every wave is its wave number; later ones come from the queue
-    // (numbered from n_waves), so a launch with few items costs few atomics on `head`
-    int k = __builtin_amdgcn_readfirstlane(
+    // items round-robin over the waves (the list is in descending item size, so every
+    // wave gets a similar share): no queue, no atomics -- a queue head serialises at about
+    // 12 ns per claim, which dominated passes of many small items
+    const int k0 = __builtin_amdgcn_readfirstlane(
         static_cast<int>(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
-    for (bool first = true;; first = false) {
-        if (!first) {
-            int t = 0;
-            if (lane == 0) t = atomicAdd(head, 1);
-            k = n_waves + __builtin_amdgcn_readfirstlane(t);
-        }
-        if (k >= n_items) break;
+    for (int k = k0; k < n_items; k += n_waves) {
         const int4 it = uniform_load(items + k);
         const HaloDesc d = uniform_load(halos + it.x);
         int ns = d.src_len, nt = d.tst_len;
@@ -334,27 +328,22 @@ int launch_pack4(const float* m, const float* x, const float* y, const float* z,
 template <int R, int SUB>
 static int launch_pot_t(const PotPlan& plan, const float4* src4, const float4* tst4,
                         const HaloDesc* d_halos, DynSets dyn, const int4* d_items,
-                        float2* part, int* head, bool reset_head, hipStream_t stream) {
-    if (reset_head) {
-        k_fill_i32<<<1, 64, 0, stream>>>(head, 1, 0);
-        HALMA_LAUNCH_CHECK();
-    }
+                        float2* part, hipStream_t stream) {
     const int n_items = static_cast<int>(plan.n_items);
     if (n_items == 0) return 0;
     const int grid = std::max(1, std::min(plan.grid, (n_items + 3) / 4));
     k_pot_f32<R, SUB><<<grid, 256, 0, stream>>>(src4, tst4, d_halos, dyn, d_items, n_items,
-                                                part, head);
+                                                part);
     HALMA_LAUNCH_CHECK();
     return 0;
 }
 
 int launch_pot_f32(const PotPlan& plan, const float4* src4, const float4* tst4,
                    const HaloDesc* d_halos, DynSets dyn, const int4* d_items, float2* part,
-                   int* head, hipStream_t stream, bool reset_head) {
+                   hipStream_t stream) {
 #define HALMA_POT_CASE(R_, S_)                                                            \
     if (plan.tests_per_lane == R_ && plan.subtile == S_)                                  \
-        return launch_pot_t<R_, S_>(plan, src4, tst4, d_halos, dyn, d_items, part, head,  \
-                                    reset_head, stream);
+        return launch_pot_t<R_, S_>(plan, src4, tst4, d_halos, dyn, d_items, part, stream);
     HALMA_POT_CASE(1, 4)
     HALMA_POT_CASE(1, 8)
     HALMA_POT_CASE(2, 4)
@@ -374,7 +363,6 @@ struct PotWs {
     int4* items;
     float2* part;
     int* tst_halo;
-    int* head;
     size_t bytes;
 };
 
@@ -387,7 +375,6 @@ static PotWs carve_pot(void* base, const PotPlan& p) {
     w.items = c.take<int4>(std::max<int64_t>(p.n_items, 1));
     w.part = c.take<float2>(std::max<int64_t>(p.n_part, 1));
     w.tst_halo = c.take<int>(std::max<int64_t>(p.n_tst, 1));
-    w.head = c.take<int>(64);
     w.bytes = c.used;
     return w;
 }
@@ -427,11 +414,8 @@ int pot_batched(int64_t H, const int64_t* src_off, const float* sm, const float*
     k_tst_halo<<<static_cast<unsigned>(H), 256, 0, stream>>>(w.halos, static_cast<int>(H),
                                                              w.tst_halo);
     HALMA_LAUNCH_CHECK();
-    k_fill_i32<<<1, 64, 0, stream>>>(w.head, 1, 0);
-    HALMA_LAUNCH_CHECK();
     if (ev_begin) HALMA_HIP(hipEventRecord(ev_begin, stream));
-    rc = launch_pot_f32(plan, w.src4, w.tst4, w.halos, DynSets(), w.items, w.part, w.head, stream,
-                        false);
+    rc = launch_pot_f32(plan, w.src4, w.tst4, w.halos, DynSets(), w.items, w.part, stream);
     if (rc) return rc;
     if (ev_end) HALMA_HIP(hipEventRecord(ev_end, stream));
     const int64_t blocks = (plan.n_tst + 255) / 256;
@@ -536,7 +520,7 @@ int halma_pot_sum_f32_noalloc(const float* m, const float* x, const float* y, co
     HALMA_LAUNCH_CHECK();
     k_tst_halo<<<1, 256, 0, st>>>(w.halos, 1, w.tst_halo);
     HALMA_LAUNCH_CHECK();
-    rc = launch_pot_f32(plan, w.src4, w.tst4, w.halos, DynSets(), w.items, w.part, w.head, st);
+    rc = launch_pot_f32(plan, w.src4, w.tst4, w.halos, DynSets(), w.items, w.part, st);
     if (rc) return rc;
     k_pot_merge<<<static_cast<unsigned>((plan.n_tst + 255) / 256), 256, 0, st>>>(
         w.halos, w.tst_halo, plan.n_tst, w.part, out_f64, out);

@@ -688,8 +688,7 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
 __global__ void k_item_fill(int H, const int32_t* __restrict__ order,
                             const int64_t* __restrict__ ioff, const int64_t* __restrict__ itotal,
                             const int32_t* __restrict__ tst_len, int tile, int64_t cap,
-                            int4* __restrict__ items, int* __restrict__ head) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *head = 0;  // the pair kernel's queue
+                            int4* __restrict__ items) {
     const int64_t total = itotal[0] < cap ? itotal[0] : cap;
     for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
          k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -759,8 +758,7 @@ __global__ __launch_bounds__(kThreads) void k_items_fused(
     int H, const int32_t* __restrict__ order, const int64_t* __restrict__ icnt,
     const int64_t* __restrict__ icnt_sym, const int32_t* __restrict__ tst_len, int tile,
     int64_t cap, int64_t sym_cap, int4* __restrict__ items, int4* __restrict__ sym_items,
-    int64_t* __restrict__ itotal, int64_t* __restrict__ sym_itotal, int* __restrict__ head,
-    int* __restrict__ sym_head) {
+    int64_t* __restrict__ itotal, int64_t* __restrict__ sym_itotal, int* __restrict__ sym_head) {
     __shared__ int off_a[kFuseH];
     __shared__ int off_b[kFuseH];
     __shared__ int shs[kThreads / kWave];
@@ -789,7 +787,6 @@ __global__ __launch_bounds__(kThreads) void k_items_fused(
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *itotal = ta;
         *sym_itotal = tb;
-        *head = 0;
         *sym_head = 0;
     }
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -857,7 +854,6 @@ struct UnbWs {
     int64_t* itotal;
     void* iscan_ws;
     float2* part;
-    int* head;
     // the symmetric kernel of the full passes
     SymDesc* sym;     // per halo (in the tables region)
     int4* sym_items;
@@ -950,7 +946,6 @@ UnbWs carve_unb(void* base, int64_t H, int64_t nm, const PotPlan& plan, const Sy
     w.itotal = c.take<int64_t>(4);
     w.iscan_ws = c.take<char>(scan_workspace_bytes(h1));
     w.part = c.take<float2>(std::max<int64_t>(plan.n_part, 1));
-    w.head = c.take<int>(64);
     w.sym_cap = std::max<int64_t>(splan.n_items, 1);
     w.sym_items = c.take<int4>(w.sym_cap);
     w.sym_icnt = c.take<int64_t>(h1);
@@ -1014,7 +1009,7 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
         k_items_fused<<<grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.icnt,
                                                   w.sym_icnt, w.tst_len, tile, L.plan->n_items,
                                                   w.sym_cap, w.items, w.sym_items, w.itotal,
-                                                  w.sym_itotal, w.head, w.sym_head);
+                                                  w.sym_itotal, w.sym_head);
         HALMA_LAUNCH_CHECK();
     } else {
         {
@@ -1024,8 +1019,7 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
             const unsigned fill_grid = static_cast<unsigned>(
                 std::max<int64_t>(1, std::min<int64_t>(1024, (cap + 255) / 256)));
             k_item_fill<<<fill_grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.ioff,
-                                                         w.itotal, w.tst_len, tile, cap, w.items,
-                                                         w.head);
+                                                         w.itotal, w.tst_len, tile, cap, w.items);
             HALMA_LAUNCH_CHECK();
         }
         {
@@ -1065,8 +1059,7 @@ int launch_pot_f32_part(const PassLaunch& L, hipStream_t st) {
     dyn.src_alt = w.rem4;
     dyn.n_items = w.itotal;
     dyn.chunk = w.pchunk;
-    return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, dyn, w.items, w.part, w.head, st,
-                          false);
+    return launch_pot_f32(*L.plan, w.src4, w.src4, w.halos, dyn, w.items, w.part, st);
 }
 
 int launch_post(const PassLaunch& L, hipStream_t st) {
