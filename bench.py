@@ -773,15 +773,14 @@ def bench_substructure(args):
         k = 40
         gl = groups.to_arrays()[:k]
         ns = int(sum(len(g) for g in gl))
-        tc = time.perf_counter()
-        _, cnt = oracle.split_substructure(gl, w["x"], w["y"], w["z"], w["mass"], ll, 25, 2.0,
-                                           min_ll, 0.75, MINP)
-        cs = time.perf_counter() - tc
-        extra["cpu_baseline"] = {"value": ns / cs, "unit": "members/s", "cores": 1,
-                                 "kind": "port", "seconds": cs,
-                                 "sample": f"the first {k} groups ({ns} members, {cnt} "
-                                           "substructures): the oracle's restatement, C "
-                                           "cell-list FoF per group and level"}
+        cs, ts, (_, cnt) = _median_runs(
+            lambda: oracle.split_substructure(gl, w["x"], w["y"], w["z"], w["mass"], ll, 25,
+                                              2.0, min_ll, 0.75, MINP), args.cpu_runs, warmup=0)
+        extra["cpu_baseline"] = _cpu_entry(
+            ns / cs, "members/s", 1, "port", cs, ts,
+            f"the first {k} groups ({ns} members, {cnt} substructures): the oracle's "
+            f"restatement, C cell-list FoF per group and level, one core; median of {len(ts)} "
+            "runs")
     return _line("particles/s FoF + substructure splitting, 8M-particle bridged pairs",
                  n / step_s, "particles/s", step_s, args,
                  {"workload": "4000 Plummer pairs at 1-3 a joined by 5% bridges, 8M FP32, "
@@ -829,14 +828,13 @@ def bench_cleaning(args):
         k = 40
         gl = groups.to_arrays()[:k]
         ns = int(sum(len(g) for g in gl))
-        tc = time.perf_counter()
-        oracle.phase_space_clean(gl, *(w[q] for q in keys), ll, 1.0, 3.0)
-        cs = time.perf_counter() - tc
-        extra["cpu_baseline"] = {"value": ns / cs, "unit": "members/s", "cores": 1,
-                                 "kind": "port", "seconds": cs,
-                                 "sample": f"the first {k} groups ({ns} members): the "
-                                           "oracle's restatement (sparse cells, one group "
-                                           "at a time)"}
+        cs, ts, _ = _median_runs(
+            lambda: oracle.phase_space_clean(gl, *(w[q] for q in keys), ll, 1.0, 3.0),
+            args.cpu_runs, warmup=0)
+        extra["cpu_baseline"] = _cpu_entry(
+            ns / cs, "members/s", 1, "port", cs, ts,
+            f"the first {k} groups ({ns} members): the oracle's restatement (sparse cells, "
+            f"one group at a time), one core; median of {len(ts)} runs")
     return _line("particles/s FoF + phase-space cleaning, 2.1M-particle clumps",
                  n / step_s, "particles/s", step_s, args,
                  {"workload": "1000 Plummer clumps in a 40 Mpc box, 2.1M FP32, FoF at ll, "
@@ -906,16 +904,16 @@ def bench_progenitors(args):
 
         k = 8
         groups = new.to_arrays()[:k]
-        tc = time.perf_counter()
-        want = oracle.progenitors(groups, oripa, new_mass[:k], old_lists, omm)
-        cs = time.perf_counter() - tc
+        cs, ts, want = _median_runs(
+            lambda: oracle.progenitors(groups, oripa, new_mass[:k], old_lists, omm),
+            args.cpu_runs, warmup=0)
         same = all((a[:k] == b).all() for a, b in zip((pro1, pro2, nmerg, mtype), want))
         ns = int(sum(len(g) for g in groups))
-        extra["cpu_baseline"] = {"value": ns / cs, "unit": "new members/s", "cores": 1,
-                                 "kind": "port", "seconds": cs,
-                                 "sample": f"the first {k} new haloes ({ns} members) against "
-                                           f"all {old_npart.size} old haloes: the oracle's "
-                                           "restatement, np.isin per halo pair"}
+        extra["cpu_baseline"] = _cpu_entry(
+            ns / cs, "new members/s", 1, "port", cs, ts,
+            f"the first {k} new haloes ({ns} members) against all {old_npart.size} old "
+            f"haloes: the oracle's restatement, np.isin per halo pair, one core; median of "
+            f"{len(ts)} runs")
         extra["sample_matches_oracle"] = bool(same)
     return _line("new members/s progenitor matching, 2.1M-particle clumps",
                  new.n_members / step_s, "new members/s", step_s, args,
@@ -984,27 +982,31 @@ def bench_sources(args):
         k = 10
         dm_tree = KDTree(np.array((dm["x"], dm["y"], dm["z"])).T.astype(np.float64))
         st_tree = KDTree(np.array((w["x"], w["y"], w["z"])).T.astype(np.float64))
-        tc = time.perf_counter()
+        def run():
+            res = []
+            for h in range(k):
+                c, R = centres[h], radii[h]
+                g = og.amr_to_particles(amr["L"], amr["ncoarse"], amr["levels"], amr["rx"],
+                                        amr["ry"], amr["rz"], amr["cells"], R, *c, rho_B)
+                keep = og.inside(g[0], g[1], g[2], R, *c)
+                res.append((g[0][keep], sorted(dm_tree.query_ball_point(c, R)),
+                            sorted(st_tree.query_ball_point(c, R))))
+            return res
+
+        cs, ts, res = _median_runs(run, args.cpu_runs, warmup=0)
         same = True
-        for h in range(k):
-            c, R = centres[h], radii[h]
-            g = og.amr_to_particles(amr["L"], amr["ncoarse"], amr["levels"], amr["rx"],
-                                    amr["ry"], amr["rz"], amr["cells"], R, *c, rho_B)
-            keep = og.inside(g[0], g[1], g[2], R, *c)
-            di = sorted(dm_tree.query_ball_point(c, R))
-            si = sorted(st_tree.query_ball_point(c, R))
+        for h, (gx, di, si) in enumerate(res):
             a, b = out["gas_offsets"][h], out["gas_offsets"][h + 1]
-            same &= bool(np.array_equal(out["gas"]["x"][a:b].cpu().numpy(), g[0][keep]))
+            same &= bool(np.array_equal(out["gas"]["x"][a:b].cpu().numpy(), gx))
             a, b = out["dm_offsets"][h], out["dm_offsets"][h + 1]
             same &= out["dm_index"][a:b].cpu().tolist() == di
             a, b = out["st_offsets"][h], out["st_offsets"][h + 1]
             same &= out["st_index"][a:b].cpu().tolist() == si
-        cs = time.perf_counter() - tc
-        extra["cpu_baseline"] = {"value": k / cs, "unit": "haloes/s", "cores": 1,
-                                 "kind": "port", "seconds": cs,
-                                 "sample": f"the first {k} haloes: AMR cells by the oracle's "
-                                           "pure-Python loops, dark matter and stars by "
-                                           "scipy KD-trees built once (untimed)"}
+        extra["cpu_baseline"] = _cpu_entry(
+            k / cs, "haloes/s", 1, "port", cs, ts,
+            f"the first {k} haloes: AMR cells by the oracle's pure-Python loops, dark matter "
+            f"and stars by scipy KD-trees built once (untimed), one core; median of "
+            f"{len(ts)} runs")
         extra["sample_matches_oracle"] = same
     return _line("haloes/s source gathering (gas from AMR cells, dark matter, stars)",
                  H / step_s, "haloes/s", step_s, args,
