@@ -89,6 +89,30 @@ struct SrcSoA {
     }
 };
 
+// One FP32 pair term m_j / r added to acc: the three coordinate differences, r^2 by
+// FMAs, v_rsq_f32, and the reference's rule (a pair counts only when all three
+// differences are non-zero: the smallest |difference| compared with zero).
+__device__ __forceinline__ float pair_acc(const float4 s, float xi, float yi, float zi,
+                                          float acc) {
+    const float dx = s.x - xi;
+    const float dy = s.y - yi;
+    const float dz = s.z - zi;
+    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float mn = fminf(fminf(fabsf(dx), fabsf(dy)), fabsf(dz));
+    const float inv = mn != 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+    return __builtin_fmaf(s.w, inv, acc);
+}
+
+// Kahan-compensated addition: c holds the low part lost so far (s + c is the sum; the
+// sequence of s is that of the textbook form, whose compensation is -c)
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    const float y = v + c;
+    const float t = s + y;
+    c = y - (t - s);
+    s = t;
+}
+
+
 // FP64 sum in the oracle's fixed order for one test position against ns sources:
 // kOrdWidth strided partials L[k] = sum over j = k, k + kOrdWidth, ... ascending (each
 // from 0.0), then the pairwise tree L[k] = L[k] + L[k + s] for s = kOrdWidth/2, ..., 1

@@ -28,24 +28,6 @@ namespace halma {
 // v_min3_f32 with |.| modifiers; a NaN difference, like the reference's /= on NaN,
 // counts: the minimum skips NaN operands and is NaN only when all three are NaN).  The
 // mass multiply and the add are one FMA.
-__device__ __forceinline__ float pair_acc(const float4 s, float xi, float yi, float zi,
-                                          float acc) {
-    const float dx = s.x - xi;
-    const float dy = s.y - yi;
-    const float dz = s.z - zi;
-    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-    const float mn = fminf(fminf(fabsf(dx), fabsf(dy)), fabsf(dz));
-    const float inv = mn != 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
-    return __builtin_fmaf(s.w, inv, acc);
-}
-
-__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
-    const float y = v - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
-
 template <int R, int SUB>
 __global__ __launch_bounds__(256) void k_pot_f32(const float4* __restrict__ src,
                                                  const float4* __restrict__ tst,

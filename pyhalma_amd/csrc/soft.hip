@@ -71,10 +71,12 @@ __device__ __forceinline__ TestPt split_test(double x, double y, double z) {
     return t;
 }
 
+// Kahan-compensated addition: c holds the low part lost so far (s + c is the sum; the
+// sequence of s is that of the textbook form, whose compensation is -c)
 __device__ __forceinline__ void kahan(float& s, float& c, float v) {
-    const float y = v - c;
+    const float y = v + c;
     const float t = s + y;
-    c = (t - s) - y;
+    c = y - (t - s);
     s = t;
 }
 

@@ -42,10 +42,12 @@ __device__ __forceinline__ float inv_r(float4 q, float x, float y, float z) {
     return mn != 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;  // the rule of pair_acc
 }
 
+// Kahan-compensated addition: c holds the low part lost so far (s + c is the sum; the
+// sequence of s is that of the textbook form, whose compensation is -c)
 __device__ __forceinline__ void kahan(float& s, float& c, float v) {
-    const float y = v - c;
+    const float y = v + c;
     const float t = s + y;
-    c = (t - s) - y;
+    c = y - (t - s);
     s = t;
 }
 

@@ -27,11 +27,12 @@
 //                    bound (the bounds of every FP32 sum that went into S add up),
 //                    evaluates E in FP64 and decides every member whose |E| exceeds
 //                    factor*G*bound + 8 ulp(kin + factor*G*S);
-//   4. k_retest      one wavefront per member inside the band (or stale): S in FP64 with
-//                    the oracle's fixed summation order over the bound set, so E and the
-//                    decision are the oracle's bit for bit (S and its bound are reset to
-//                    that value);
-//   5. k_finish      per-halo pass counts, convergence flags, the removed total.
+//   4. k_retest      members inside the band: S in FP64 with the oracle's fixed summation
+//                    order over the bound set (a block or a wavefront per member), so E
+//                    and the decision are the oracle's bit for bit (S and its bound are
+//                    reset to that value); stale members: a fresh FP32 S;
+//   5. k_finish      per-halo pass counts, convergence flags, the removed total (a thread
+//                    per halo).
 // The parts are plain launches (or, on request, hipGraphs captured once and replayed);
 // the host reads one counter per pass to stop.
 #include <math.h>
@@ -126,13 +127,13 @@ __device__ __forceinline__ bool pass_is_full(int pass_no, int run, int rrun, int
 
 // Largest relative error bound a member's S may keep.  A kept member whose accumulated
 // bound (the full pass's, plus that of every incremental subtraction since) goes above it
-// is stale.  When a halo has few stale members (at most an eighth of its live set) the
-// re-test kernel re-sums them in FP64, like the members of the decision band; otherwise
-// the halo's next pass is a full one (the symmetric kernel recomputes every S for less
-// than the FP64 sums would cost), and a halo that removed nobody takes that full pass
-// only to refresh its S (not counted as an iteration: its decisions are exact and do not
-// change).  Both full-pass kernels stay below the limit (halma_pot_error_bound at
-// sub-tiles of 4 or 8, halma_pot_sym_error_bound).
+// is stale.  Its decision is exact (it lies outside the decision band), so it only needs
+// a fresh S: when its halo has few stale members (kStaleDiv below) the re-test kernel
+// re-sums them in FP32 with every term Kahan-added (bound 8u = 4.8e-7); otherwise the
+// halo's next pass is a full one (the symmetric kernel recomputes every S), and a halo
+// that removed nobody takes that full pass only to refresh its S (not counted as an
+// iteration: its decisions are exact and do not change).  Both full-pass kernels stay
+// below the limit (halma_pot_error_bound at sub-tiles of 4 or 8, halma_pot_sym_error_bound).
 constexpr double kPotRelMax = 1e-6;
 
 __device__ __forceinline__ bool too_stale(int stale, int live) {
@@ -421,6 +422,7 @@ struct EnergyArgs {
     double cg;         // factor * G
     double bound_rel;  // FP32 relative error bound B of one pair sum (incremental passes)
     double bound_sym;  // the same for the symmetric kernel (full passes)
+    double bound_resum;  // the same for the FP32 re-sum of a stale member
     uint8_t* state;
     double* sacc;      // running S per member position
     double* eacc;      // absolute error bound of sacc
@@ -433,11 +435,11 @@ struct EnergyArgs {
     unsigned long long* counters;  // [0] removed total, [1] band count, [2] band total,
                                    // [3] pairs, [4] symmetric-kernel pairs, [5] haloes
                                    // still active after the pass, [6] members re-summed
-                                   // for their error bound alone, [7] the re-test
-                                   // kernel's block ticket
+                                   // for their error bound alone, [8] the band
+                                   // count of large live sets
     double* s_out;     // optional: S per member position (FP64)
     int64_t nm;
-    // the end of a pass (the re-test kernel's last block)
+    // the end of a pass (k_finish)
     int H;
     int32_t* active;
     int32_t* iters;
@@ -512,8 +514,8 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     if (a.s_out) a.s_out[k] = S;
     const bool in_band = fabs(E) <= band;
     const bool gone = !in_band && E > 0.0;
-    // a kept member whose error bound went above kPotRelMax: its S is re-summed in FP64
-    // with the band (its decision, already exact, stays the same)
+    // a kept member whose error bound went above kPotRelMax: its S is re-summed (its
+    // decision, already exact, stays the same)
     const bool stale = !gone && !in_band && err > kPotRelMax * S;
     // members to re-test: of small live sets from the front of the band list, of large
     // ones (a block each in k_retest) from its back; one atomic per wavefront and list
@@ -602,18 +604,62 @@ __device__ void finish_haloes(const EnergyArgs& a, int h) {
     }
 }
 
-// One block per band member (grid-stride): the FP64 sum in the oracle's order decides.
-// The last block to finish (a ticket taken after every block's re-tests) then ends the
-// pass for all haloes (finish_haloes) and clears the band list and the ticket.
+// A stale member (kept, outside the decision band: its decision is exact) only needs a
+// fresh S within kPotRelMax: one FP32 sum over its halo's live set with the chunked
+// kernel's term and every term Kahan-added (relative bound pot_error_bound(1), 8u), the
+// threads of a wavefront or block in stride over the sources (four loads in flight),
+// merged in FP64.  About five times cheaper per term than the FP64 ordered sum.
+__device__ __forceinline__ double resum_f32_part(const float4* __restrict__ src, int n,
+                                                 float4 t, int i0, int step) {
+    float s = 0.0f, c = 0.0f;
+    int j = i0;
+    for (; j + 3 * step < n; j += 4 * step) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = src[j + u * step];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) kahan_add(s, c, pair_acc(q[u], t.x, t.y, t.z, 0.0f));
+    }
+    for (; j < n; j += step) kahan_add(s, c, pair_acc(src[j], t.x, t.y, t.z, 0.0f));
+    return static_cast<double>(s) + static_cast<double>(c);
+}
+
+__device__ __forceinline__ double wave_resum_f32(const float4* __restrict__ src, int n, float4 t) {
+    double r = resum_f32_part(src, n, t, __lane_id(), kWave);
+    for (int m = kWave / 2; m >= 1; m >>= 1) r += __shfl_xor(r, m, kWave);
+    return r;
+}
+
+__device__ double block_resum_f32(const float4* __restrict__ src, int n, float4 t, double* sh) {
+    double r = resum_f32_part(src, n, t, threadIdx.x, blockDim.x);
+    for (int m = kWave / 2; m >= 1; m >>= 1) r += __shfl_xor(r, m, kWave);
+    if (__lane_id() == 0) sh[threadIdx.x / kWave] = r;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < static_cast<int>(blockDim.x) / kWave; ++w) tot += sh[w];
+    __syncthreads();
+    return tot;
+}
+
+// The FP64 sum in the oracle's order decides every listed member (a block each for live
+// sets of kRetestBlock or more, a wavefront each for smaller ones); stale members get an
+// FP32 re-sum.  k_finish then ends the pass.
 __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
                                                       const float4* __restrict__ src4) {
-    static_assert(kOrdWidth == kThreads, "one thread per halo in the closing loop");
+    static_assert(kOrdWidth == kThreads, "the FP64 block sums take kThreads threads");
     __shared__ double sh[kOrdWidth];
-    __shared__ int last;
     const int64_t nb = static_cast<int64_t>(a.counters[1]);
-    // the sum in the oracle's order (same bits either way): a block per member of a large
-    // live set (a shorter chain per thread), a wavefront per member of a small one (more
-    // members in flight)
+    // members of the decision band: the sum in the oracle's order (same bits either way),
+    // a block per member of a large live set (a shorter chain per thread), a wavefront per
+    // member of a small one (more members in flight); stale members: an FP32 re-sum
+    auto fresh = [&](int64_t g, double S) {
+        const int64_t k = a.cidx[g];
+        a.sacc[k] = S;
+        a.eacc[k] = a.bound_resum * S;
+        if (a.s_out) a.s_out[k] = S;
+        atomicAdd(a.counters + 2, 1ull);
+        atomicAdd(a.counters + 6, 1ull);
+    };
     auto decide = [&](int64_t g, int h, double S) {
         const int64_t k = a.cidx[g];
         const double E = kinetic(a.vel, k, a.vcm + 3 * h) - a.cg * S;
@@ -632,15 +678,16 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
         // stale members of a halo that takes a full pass next are left to that pass
         if ((entry & kStaleBit) && too_stale(a.stale[h], n)) continue;  // uniform
         const float4 t = src4[g];
+        if (entry & kStaleBit) {  // a fresh S only
+            const double S = block_resum_f32(src4 + a.halos[h].src_off, n, t, sh);
+            if (threadIdx.x == 0) fresh(g, S);
+            continue;
+        }
         SrcAoS src{src4 + a.halos[h].src_off};
         const double S = block_pot_f64_ordered(src, n, t.x, t.y, t.z, sh);
-        if (threadIdx.x == 0) {
-            const double E = decide(g, h, S);
-            if (entry & kStaleBit) atomicAdd(a.counters + 6, 1ull);
-            if (E > 0.0) {
-                a.state[a.cidx[g]] = kJustGone;
-                atomicAdd(a.removed + h, 1);
-            }
+        if (threadIdx.x == 0 && decide(g, h, S) > 0.0) {
+            a.state[a.cidx[g]] = kJustGone;
+            atomicAdd(a.removed + h, 1);
         }
     }
     const int waves = kThreads / kWave;
@@ -652,30 +699,28 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
         const int n = a.n_bound[h];
         if ((entry & kStaleBit) && too_stale(a.stale[h], n)) continue;
         const float4 t = src4[g];
+        if (entry & kStaleBit) {  // a fresh S only
+            const double S = wave_resum_f32(src4 + a.halos[h].src_off, n, t);
+            if (__lane_id() == 0) fresh(g, S);
+            continue;
+        }
         SrcAoS src{src4 + a.halos[h].src_off};
         const double S = wave_pot_f64_ordered(src, n, t.x, t.y, t.z);
-        if (__lane_id() == 0) {
-            const double E = decide(g, h, S);
-            if (entry & kStaleBit) atomicAdd(a.counters + 6, 1ull);
-            if (E > 0.0) {
-                a.state[a.cidx[g]] = kJustGone;
-                atomicAdd(a.removed + h, 1);
-            }
+        if (__lane_id() == 0 && decide(g, h, S) > 0.0) {
+            a.state[a.cidx[g]] = kJustGone;
+            atomicAdd(a.removed + h, 1);
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();  // this block's decisions before its ticket
-        last = atomicAdd(a.counters + 7, 1ull) == gridDim.x - 1 ? 1 : 0;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();  // every block's decisions visible to the closing block
-    for (int h0 = 0; h0 < a.H; h0 += kOrdWidth) finish_haloes(a, h0 + threadIdx.x);
-    if (threadIdx.x == 0) {
+}
+
+// The end of a pass, after every re-test: per-halo counts and modes (finish_haloes), one
+// thread per halo; the band lists cleared for the next pass.
+__global__ __launch_bounds__(kThreads) void k_finish(EnergyArgs a) {
+    const int h = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    finish_haloes(a, h);
+    if (h == 0) {
         a.counters[1] = 0;  // the band lists are rebuilt by the next pass
         a.counters[8] = 0;
-        a.counters[7] = 0;
         *a.pass_no += 1;
     }
 }
@@ -1068,8 +1113,10 @@ int launch_post(const PassLaunch& L, hipStream_t st) {
         k_energy<<<blocks_for(L.nm), kThreads, 0, st>>>(L.ea);
         HALMA_LAUNCH_CHECK();
     }
-    // re-tests, then the end of the pass in the kernel's last block
+    // re-tests, then the end of the pass
     k_retest<<<L.grid_retest, kOrdWidth, 0, st>>>(L.ea, w.src4);
+    HALMA_LAUNCH_CHECK();
+    k_finish<<<blocks_for(L.H), kThreads, 0, st>>>(L.ea);
     HALMA_LAUNCH_CHECK();
     return 0;
 }
@@ -1192,9 +1239,8 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.nm = nm;
     const int cus = std::max(device_cu_count(), 1);
     L.grid_chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(n_chunks, cus * 8)));
-    // one block per band member at a time; the last block to take its ticket closes the
-    // pass, so the grid stays at one block per compute unit
-    L.grid_retest = cus;
+    // re-tests: blocks (large live sets) and wavefronts (small ones) over the band lists
+    L.grid_retest = cus * 4;
     L.plan = &plan;
     L.splan = &splan;
     L.w = w;
@@ -1214,6 +1260,7 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.ea.cg = factor * G;
     L.ea.bound_rel = pot_error_bound(plan.subtile);
     L.ea.bound_sym = pot_sym_error_bound();
+    L.ea.bound_resum = pot_error_bound(1);
     L.ea.state = w.state;
     L.ea.sacc = w.sacc;
     L.ea.eacc = w.eacc;
