@@ -831,24 +831,37 @@ int halma_density_peak(int64_t n_groups, const int64_t* offsets, const int32_t* 
     const unsigned grid = static_cast<unsigned>(n_groups < 65536 ? n_groups : 65536);
     k_density_peak<<<grid, kBig, 0, st>>>(a);
     HALMA_LAUNCH_CHECK();
-    long long* h = pinned_words(2);
-    if (!h) return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
-    h[0] = 0;
-    HALMA_HIP(hipMemcpyAsync(h, w.over_count, sizeof(int), hipMemcpyDeviceToHost, st));
-    HALMA_HIP(hipStreamSynchronize(st));
-    const int nover = static_cast<int>(h[0] & 0xffffffff);
-    for (int first = 0; first < nover; first += kDenseSlots) {
+    // the first round of the dense fallback is queued whatever the overflow count (its
+    // blocks return at once when no group overflowed), so that one read-back gives both
+    // the count and the error flags; further rounds, when needed, cost a second one
+    auto dense_round = [&](int first) -> int {
         k_dense_zero<<<kDenseSlots, kThreads, 0, st>>>(a, first);
         HALMA_LAUNCH_CHECK();
         k_dense_accum<<<dim3(64, kDenseSlots), kThreads, 0, st>>>(a, first);
         HALMA_LAUNCH_CHECK();
         k_dense_pick<<<kDenseSlots, kThreads, 0, st>>>(a, first);
         HALMA_LAUNCH_CHECK();
-    }
-    h[0] = 0;
-    HALMA_HIP(hipMemcpyAsync(h, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
+        return 0;
+    };
+    int rc = dense_round(0);
+    if (rc) return rc;
+    long long* h = pinned_words(2);
+    if (!h) return set_error(HALMA_E_STATE, "pinned host buffer unavailable");
+    h[0] = h[1] = 0;
+    HALMA_HIP(hipMemcpyAsync(h, w.over_count, sizeof(int), hipMemcpyDeviceToHost, st));
+    HALMA_HIP(hipMemcpyAsync(h + 1, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
     HALMA_HIP(hipStreamSynchronize(st));
-    if (h[0] & 8)
+    const int nover = static_cast<int>(h[0] & 0xffffffff);
+    if (nover > kDenseSlots) {
+        for (int first = kDenseSlots; first < nover; first += kDenseSlots) {
+            rc = dense_round(first);
+            if (rc) return rc;
+        }
+        h[1] = 0;
+        HALMA_HIP(hipMemcpyAsync(h + 1, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HALMA_HIP(hipStreamSynchronize(st));
+    }
+    if (h[1] & 8)
         return set_error(HALMA_E_LIMIT, "density peak: more than %d near-tied nodes",
                          kMaxCandidates);
     return 0;

@@ -348,7 +348,8 @@ def groups_from_labels(labels, min_size=2):
     labels = to_dev(labels, np.int32)
     n = int(labels.numel())
     members = torch.empty(max(n, 1), dtype=torch.int32, device=labels.device)
-    offsets = torch.zeros(n + 1, dtype=torch.int64, device=labels.device)
+    # the call writes offsets[0..G] (G the group count it returns)
+    offsets = torch.empty(n + 1, dtype=torch.int64, device=labels.device)
     ng = ctypes_i64()
     nm = ctypes_i64()
     if n > 0:
@@ -567,11 +568,12 @@ def split_substructure(groups, x, y, z, m, ll, minp_sub=25, dec_factor=2.0, min_
                     new_tab.append((int(hh[q]), level, q, int(starts[q]), int(lens[q]),
                                     len(new_mem)))
                 new_mem.append(cur[taken])
-                kept_per = torch.zeros(H, dtype=torch.int64, device=dev)
-                kept_per.index_add_(0, halo_of_pos, keep.to(torch.int64))
+                # kept members per group: a prefix sum read at the group offsets (the
+                # groups are contiguous, so per-group atomics would queue on one address)
+                cs = torch.cat([torch.zeros(1, dtype=torch.int64, device=dev),
+                                torch.cumsum(keep.to(torch.int64), 0)])
+                cur_off = cs[torch.from_numpy(cur_off).to(dev)].cpu().numpy().astype(np.int64)
                 cur = cur[keep]
-                cur_off = np.concatenate([[0], np.cumsum(kept_per.cpu().numpy())]).astype(
-                    np.int64)
         sub_ll = sub_ll / dec_factor
         level += 1
     # assemble: original groups (emptied below 30 %), then the new ones
