@@ -585,10 +585,11 @@ __global__ __launch_bounds__(kBig) void k_density_peak(PeakArgs a) {
 // zeroing (plain stores), the accumulation (device atomics) and the read-back (plain
 // loads) are separated by kernel boundaries.
 __global__ __launch_bounds__(kThreads) void k_dense_zero(PeakArgs a, int first) {
-    const int o = first + blockIdx.x;
+    const int o = first + blockIdx.y;
     if (o >= *a.over_count) return;
-    double* grid = a.dense + static_cast<int64_t>(blockIdx.x) * kCells;
-    for (int k = threadIdx.x; k < kCells; k += kThreads) grid[k] = 0.0;
+    double* grid = a.dense + static_cast<int64_t>(blockIdx.y) * kCells;
+    for (int k = blockIdx.x * kThreads + threadIdx.x; k < kCells; k += gridDim.x * kThreads)
+        grid[k] = 0.0;
 }
 
 __global__ __launch_bounds__(kThreads) void k_dense_accum(PeakArgs a, int first) {
@@ -606,7 +607,7 @@ __global__ __launch_bounds__(kThreads) void k_dense_accum(PeakArgs a, int first)
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_dense_pick(PeakArgs a, int first) {
+__global__ __launch_bounds__(kBig) void k_dense_pick(PeakArgs a, int first) {
     __shared__ double shd[kMaxWaves];
     __shared__ int shi[kMaxWaves + 1];
     __shared__ int cand[kMaxCandidates];
@@ -835,11 +836,11 @@ int halma_density_peak(int64_t n_groups, const int64_t* offsets, const int32_t* 
     // blocks return at once when no group overflowed), so that one read-back gives both
     // the count and the error flags; further rounds, when needed, cost a second one
     auto dense_round = [&](int first) -> int {
-        k_dense_zero<<<kDenseSlots, kThreads, 0, st>>>(a, first);
+        k_dense_zero<<<dim3(64, kDenseSlots), kThreads, 0, st>>>(a, first);
         HALMA_LAUNCH_CHECK();
         k_dense_accum<<<dim3(64, kDenseSlots), kThreads, 0, st>>>(a, first);
         HALMA_LAUNCH_CHECK();
-        k_dense_pick<<<kDenseSlots, kThreads, 0, st>>>(a, first);
+        k_dense_pick<<<kDenseSlots, kBig, 0, st>>>(a, first);
         HALMA_LAUNCH_CHECK();
         return 0;
     };
