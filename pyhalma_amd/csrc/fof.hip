@@ -11,19 +11,23 @@
 // 3. a stable LSD radix sort of the particles by cell key (x, y, z bits packed, z lowest)
 //    lists them cell by cell: cells of one column (x, y) are consecutive and ordered by
 //    z, columns are in (x, y) order, particles ascending within a cell; coordinates are
-//    gathered into that order, and each column enters a hash table (key -> column), sized
-//    from the column count;
+//    gathered into that order as one vector per particle, each cell is one 8-byte record
+//    (first position, z), and each column enters a hash table whose 16-byte entry holds
+//    its key, cell count, first cell and (between the rounds) shared root, so that a
+//    neighbour column costs one line; the table is sized from the column count;
 // 4. links, in two rounds (Chebyshev distance 1, then 2), one lane per (column,
 //    neighbour column): the half-stencil's 12 columns (dx > 0, or dx = 0 and dy > 0) and
 //    the column itself (cells above); one table probe finds the neighbour column, then the
-//    two z-ordered cell lists are merged for the cell pairs within the round's dz.  A
-//    pair of cells already joined is skipped; small pairs are tested by the lane, larger
-//    ones listed for one wavefront each (64 lanes over the larger cell, early exit);
-//    linked cells are joined in a lock-free union-find over cells (the larger root index
-//    is hooked under the smaller, so parents only decrease and no cycle can form; finds
-//    halve paths);
+//    two z-ordered cell lists are merged for the cell pairs within the round's dz.  Pairs
+//    of cells with at most 8 particle pairs are tested at once (roots are looked up only
+//    for a link); larger ones already joined are skipped, the rest tested by the lane or,
+//    beyond 64 particle pairs, listed for one wavefront each (64 lanes over the larger
+//    cell, early exit); linked cells are joined in a lock-free union-find over cells (the
+//    larger root index is hooked under the smaller, so parents only decrease and no cycle
+//    can form; finds halve paths);
 // 5. roots are read back with atomic reads (see k_roots), each component's minimum
-//    particle index is taken over its cells, and every particle gets it as label.
+//    particle index is taken over its cells, and every particle gets it as label (in
+//    sorted order, where its cell is known).
 #include <math.h>
 
 #include <algorithm>
@@ -41,11 +45,25 @@ constexpr uint64_t kEmpty = ~0ull;
 constexpr double kCellLimit = 2097152.0;  // 2^21 cells per axis
 constexpr int kBoxBlocks = 1024;
 
-// 16-byte column table entry: column key (kEmpty when free), column number
+// A column's word: its key (x | y, at most 42 bits) in the low bits and its cell count
+// (at most 2^21) above them.
+constexpr int kKeyBits = 42;
+constexpr uint64_t kKeyMask = (uint64_t{1} << kKeyBits) - 1;
+
+// 16-byte column table entry, everything a lane needs of a neighbour column in one line:
+// the word (kEmpty when free), the first cell, and between the rounds the root shared by
+// all its cells (-1 when they differ)
 struct __attribute__((aligned(16))) ColEntry {
-    uint64_t key;
-    int32_t col;
-    int32_t pad;
+    uint64_t word;
+    int32_t c0;
+    int32_t root;
+};
+
+// A column in link-lane order: its word, its first cell and its table slot
+struct __attribute__((aligned(16))) ColRec {
+    uint64_t word;
+    int32_t c0;
+    int32_t slot;
 };
 
 __device__ __forceinline__ uint64_t mix64(uint64_t k) {
@@ -234,26 +252,25 @@ __global__ void k_gather_u32(const uint32_t* __restrict__ src, const int32_t* __
     if (j < n) dst[j] = src[perm[j]];
 }
 
-// per sorted position j: the particle's coordinates in cell order, its key (recomputed
-// from the same coordinates with the same arithmetic) and the cell / column heads
-// (bit 0 / bit 32: the key / the column part of the key differs from position j - 1)
+// per sorted position j: the particle's packed coordinates in cell order (the link
+// kernels load a particle as one vector) and its key (recomputed from the same
+// coordinates with the same arithmetic)
 template <typename T>
 __global__ __launch_bounds__(256) void k_fof_sorted(const int32_t* __restrict__ perm, int64_t n,
                                                     Grid g,
                                                     const typename Packed<T>::type* __restrict__ pk,
-                                                    T* __restrict__ sx, T* __restrict__ sy,
-                                                    T* __restrict__ sz,
+                                                    typename Packed<T>::type* __restrict__ spk,
                                                     uint64_t* __restrict__ skey) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const typename Packed<T>::type p = pk[perm[j]];
-    sx[j] = p.x;
-    sy[j] = p.y;
-    sz[j] = p.z;
+    spk[j] = p;
     bool bad = false;
     skey[j] = cell_key(p.x, p.y, p.z, g, bad);
 }
 
+// per sorted position j: the cell / column heads (bit 0 / bit 32: the key / the column
+// part of the key differs from position j - 1)
 __global__ void k_fof_heads(const uint64_t* __restrict__ skey, int64_t n, int bz,
                             int64_t* __restrict__ heads) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -279,74 +296,57 @@ __global__ void k_clear_cols(const int64_t* __restrict__ totals, ColEntry* __res
     const int64_t m = totals[1] + 1;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        tab[i].key = kEmpty;
-        tab[i].col = -1;
+        tab[i].word = kEmpty;
+        tab[i].c0 = -1;
+        tab[i].root = -1;
     }
 }
 
-// Cells and columns from the sorted keys.  Cell c: its first sorted position, its z,
-// its smallest particle index (the sort is stable, so the first one), the union-find
-// root; column q: its first cell, its key, and its entry in the table.
+// Cells and columns from the sorted keys.  Cell c: its record (first sorted position,
+// z), its smallest particle index (the sort is stable, so the first one), the union-find
+// root; column q: its first cell and its key.
 __global__ __launch_bounds__(256) void k_fof_cells(
     const uint64_t* __restrict__ skey, const int32_t* __restrict__ perm,
     const int64_t* __restrict__ heads, const int64_t* __restrict__ before, int64_t n, int bz,
-    const int64_t* __restrict__ totals, int* __restrict__ pcell, int* __restrict__ cell_start,
-    int* __restrict__ cell_z, int* __restrict__ cell_min, int* __restrict__ parent,
-    int* __restrict__ comp_min, int* __restrict__ col_cell, uint64_t* __restrict__ col_key,
-    ColEntry* __restrict__ tab, int* __restrict__ err) {
+    int2* __restrict__ crec, int* __restrict__ cell_min, int* __restrict__ parent,
+    int* __restrict__ comp_min, int* __restrict__ col_cell, uint64_t* __restrict__ col_key) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const int64_t h = heads[j], b = before[j];
     const int c = static_cast<int>((b & 0xffffffff) + (h & 0xffffffff)) - 1;
     const int q = static_cast<int>((b >> 32) + (h >> 32)) - 1;
     const uint64_t k = skey[j];
-    pcell[perm[j]] = c;
     if (h & 0xffffffff) {
-        cell_start[c] = static_cast<int>(j);
-        cell_z[c] = static_cast<int>(k & ((1ull << bz) - 1));
+        crec[c] = make_int2(static_cast<int>(j), static_cast<int>(k & ((1ull << bz) - 1)));
         cell_min[c] = perm[j];
         parent[c] = c;
         comp_min[c] = INT32_MAX;
     }
     if (h >> 32) {
-        const uint64_t ck = k >> bz;
         col_cell[q] = c;
-        col_key[q] = ck;
-        const uint64_t mask = static_cast<uint64_t>(totals[1]);
-        uint64_t s = mix64(ck) & mask;
-        for (uint64_t probe = 0;; ++probe) {
-            uint64_t expected = kEmpty;
-            if (__hip_atomic_compare_exchange_strong(&tab[s].key, &expected, ck,
-                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT)) {
-                tab[s].col = q;
-                break;
-            }
-            if (probe > mask) {  // table full (cannot happen: capacity >= 2 columns)
-                atomicOr(err, 2);
-                break;
-            }
-            s = (s + 1) & mask;
-        }
+        col_key[q] = k >> bz;
     }
     if (j == n - 1) {
-        cell_start[c + 1] = static_cast<int>(n);
+        crec[c + 1] = make_int2(static_cast<int>(n), INT32_MAX);
         col_cell[q + 1] = c + 1;
     }
 }
 
-// column number of a column key, or -1.  The entry's key is written by a compare-and-swap
-// and its column right after, by the same thread, in an earlier kernel.
-__device__ __forceinline__ int col_find(const ColEntry* __restrict__ tab, uint64_t mask,
-                                        uint64_t key) {
+// The table entry of a column key, or an entry whose word is kEmpty when the column has
+// no cells.  Every word is final before the link kernels run.
+__device__ __forceinline__ ColEntry col_find(const ColEntry* __restrict__ tab, uint64_t mask,
+                                             uint64_t key) {
     uint64_t s = mix64(key) & mask;
     for (uint64_t probe = 0; probe <= mask; ++probe) {
-        const uint64_t k = tab[s].key;
-        if (k == key) return tab[s].col;
-        if (k == kEmpty) return -1;
+        const ColEntry e = tab[s];
+        if (e.word == kEmpty || (e.word & kKeyMask) == key) return e;
         s = (s + 1) & mask;
     }
-    return -1;
+    ColEntry e;
+    e.word = kEmpty;
+    e.c0 = -1;
+    e.root = -1;
+    return e;
 }
 
 // Read-only find with plain loads, for the "already joined?" test before a pair search.
@@ -359,12 +359,11 @@ __device__ __forceinline__ int find_ro(const int* P, int x) {
     return x;
 }
 
-template <typename T>
-__device__ __forceinline__ bool within(const T* sx, const T* sy, const T* sz, int a, int b,
-                                       double ll2) {
-    const double dx = static_cast<double>(sx[a]) - static_cast<double>(sx[b]);
-    const double dy = static_cast<double>(sy[a]) - static_cast<double>(sy[b]);
-    const double dz = static_cast<double>(sz[a]) - static_cast<double>(sz[b]);
+template <typename V>
+__device__ __forceinline__ bool within(const V& a, const V& b, double ll2) {
+    const double dx = static_cast<double>(a.x) - static_cast<double>(b.x);
+    const double dy = static_cast<double>(a.y) - static_cast<double>(b.y);
+    const double dz = static_cast<double>(a.z) - static_cast<double>(b.z);
     return (dx * dx + dy * dy) + dz * dz <= ll2;
 }
 
@@ -385,41 +384,71 @@ __global__ void k_flatten(const int64_t* __restrict__ totals, int* __restrict__ 
 
 // Lane order of the link kernels: columns grouped by the log2 of their particle count,
 // largest first, so that the lanes of a wavefront walk columns of similar length (a
-// wavefront lasts as long as its longest lane) and the longest start first.  A counting sort over kColClasses classes; each block takes one
-// contiguous stretch of columns, counts it, reserves its places with one atomic per class,
-// and writes it (the order within a class is not fixed; the links do not depend on it).
+// wavefront lasts as long as its longest lane) and the longest start first.  A counting
+// sort over kColClasses classes; each block takes one contiguous stretch of columns,
+// counts it, reserves its places with one atomic per class, and writes it (the order
+// within a class is not fixed; the links do not depend on it).
 constexpr int kColClasses = 8;
 constexpr int kClassBlocks = 256;
 
 __device__ __forceinline__ int col_class(const int* __restrict__ col_cell,
-                                         const int* __restrict__ cell_start, int64_t q) {
-    const int np = cell_start[col_cell[q + 1]] - cell_start[col_cell[q]];
+                                         const int2* __restrict__ crec, int64_t q) {
+    const int np = crec[col_cell[q + 1]].x - crec[col_cell[q]].x;
     const int lg = 31 - __clz(np > 0 ? np : 1);
     return lg < kColClasses - 1 ? lg : kColClasses - 1;
 }
 
-// counts[0..7]: columns per class; counts[8..15]: the class cursors (zero here)
+// counts[0..7]: columns per class; counts[8..15]: the class cursors (zero here).  Also
+// enters each column in the table (word, first cell) and keeps its slot.
 __global__ __launch_bounds__(256) void k_col_class_count(const int64_t* __restrict__ totals,
                                                          const int* __restrict__ col_cell,
-                                                         const int* __restrict__ cell_start,
-                                                         int* __restrict__ counts) {
+                                                         const uint64_t* __restrict__ col_key,
+                                                         int* __restrict__ col_slot,
+                                                         const int2* __restrict__ crec,
+                                                         ColEntry* __restrict__ tab,
+                                                         int* __restrict__ counts,
+                                                         int* __restrict__ err) {
     __shared__ int h[kColClasses];
     if (threadIdx.x < kColClasses) h[threadIdx.x] = 0;
     __syncthreads();
     const int64_t ncol = totals[0] >> 32;
     const int64_t per = (ncol + gridDim.x - 1) / gridDim.x;
     const int64_t q0 = blockIdx.x * per, q1 = q0 + per < ncol ? q0 + per : ncol;
-    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x)
-        atomicAdd(&h[col_class(col_cell, cell_start, q)], 1);
+    const uint64_t mask = static_cast<uint64_t>(totals[1]);
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        atomicAdd(&h[col_class(col_cell, crec, q)], 1);
+        const int a0 = col_cell[q];
+        const uint64_t key = col_key[q];
+        const uint64_t word = key | (static_cast<uint64_t>(col_cell[q + 1] - a0) << kKeyBits);
+        uint64_t s = mix64(key) & mask;
+        for (uint64_t probe = 0;; ++probe) {
+            uint64_t expected = kEmpty;
+            if (__hip_atomic_compare_exchange_strong(&tab[s].word, &expected, word,
+                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) {
+                tab[s].c0 = a0;
+                col_slot[q] = static_cast<int>(s);
+                break;
+            }
+            if (probe > mask) {  // table full (cannot happen: capacity >= 2 columns)
+                atomicOr(err, 2);
+                col_slot[q] = 0;
+                break;
+            }
+            s = (s + 1) & mask;
+        }
+    }
     __syncthreads();
     if (threadIdx.x < kColClasses && h[threadIdx.x] > 0) atomicAdd(counts + threadIdx.x, h[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(256) void k_col_class_scatter(const int64_t* __restrict__ totals,
                                                            const int* __restrict__ col_cell,
-                                                           const int* __restrict__ cell_start,
+                                                           const uint64_t* __restrict__ col_key,
+                                                           const int* __restrict__ col_slot,
+                                                           const int2* __restrict__ crec,
                                                            int* __restrict__ counts,
-                                                           int* __restrict__ perm) {
+                                                           ColRec* __restrict__ lanes) {
     __shared__ int h[kColClasses];
     __shared__ int base[kColClasses];
     if (threadIdx.x < kColClasses) h[threadIdx.x] = 0;
@@ -428,7 +457,7 @@ __global__ __launch_bounds__(256) void k_col_class_scatter(const int64_t* __rest
     const int64_t per = (ncol + gridDim.x - 1) / gridDim.x;
     const int64_t q0 = blockIdx.x * per, q1 = q0 + per < ncol ? q0 + per : ncol;
     for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x)
-        atomicAdd(&h[col_class(col_cell, cell_start, q)], 1);
+        atomicAdd(&h[col_class(col_cell, crec, q)], 1);
     __syncthreads();
     if (threadIdx.x < kColClasses) {
         int before = 0;  // columns of the higher classes: the longest columns come first
@@ -439,20 +468,28 @@ __global__ __launch_bounds__(256) void k_col_class_scatter(const int64_t* __rest
     }
     __syncthreads();
     for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-        const int k = col_class(col_cell, cell_start, q);
-        perm[base[k] + atomicAdd(&h[k], 1)] = static_cast<int>(q);
+        const int k = col_class(col_cell, crec, q);
+        const int a0 = col_cell[q];
+        ColRec r;
+        r.word = col_key[q] | (static_cast<uint64_t>(col_cell[q + 1] - a0) << kKeyBits);
+        r.c0 = a0;
+        r.slot = col_slot[q];
+        lanes[base[k] + atomicAdd(&h[k], 1)] = r;
     }
 }
 
 // Between the rounds, after k_flatten (parent[c] is then c's root): the root shared by
-// all cells of a column, or -1 when they differ.  Two columns with the same shared root
-// are joined throughout, so round 2 skips the pair without looking at its cells.
-__global__ void k_col_roots(const int64_t* __restrict__ totals, const int* __restrict__ col_cell,
-                            const int* __restrict__ parent, int* __restrict__ col_root) {
-    const int64_t ncol = totals[0] >> 32;
-    for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < ncol;
-         q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int a0 = col_cell[q], a1 = col_cell[q + 1];
+// all cells of a column, or -1 when they differ, into the column's table entry (the
+// table is walked in slot order).  Two columns with the same shared root are joined
+// throughout, so round 2 skips the pair without looking at its cells.
+__global__ void k_col_roots(const int64_t* __restrict__ totals, const int* __restrict__ parent,
+                            ColEntry* __restrict__ tab) {
+    const int64_t m = totals[1] + 1;
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < m;
+         s += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t word = tab[s].word;
+        if (word == kEmpty) continue;
+        const int a0 = tab[s].c0, a1 = a0 + static_cast<int>(word >> kKeyBits);
         const int r = parent[a0];
         int u = r;
         for (int c = a0 + 1; c < a1; ++c)
@@ -460,7 +497,7 @@ __global__ void k_col_roots(const int64_t* __restrict__ totals, const int* __res
                 u = -1;
                 break;
             }
-        col_root[q] = u;
+        tab[s].root = u;
     }
 }
 
@@ -470,34 +507,47 @@ __constant__ signed char kCol[12][2] = {{0, 1}, {1, -1}, {1, 0}, {1, 1},  // nea
                                         {0, 2}, {1, -2}, {1, 2}, {2, -2},
                                         {2, -1}, {2, 0}, {2, 1}, {2, 2}};
 constexpr int kLaneTest = 64;  // pair tests a lane does alone
+// Round 1: pairs of cells this small are tested before any root lookup.  Round 2 looks
+// the roots up first (after round 1 most of its pairs are already joined).
+constexpr int kTestFirst = 8;
 
-// One candidate pair of cells: skipped when already joined; tested here when small,
-// else listed for k_fof_big (tested here too when the list is full).  The caller holds
-// cell c's root rc (from a read-only find, possibly out of date: equal roots still prove
-// the cells joined) and its particles [sc, sc + nc); rc is refreshed after a link.
-template <typename T>
-__device__ __forceinline__ void candidate(int c, int& rc, int sc, int nc, int d,
-                                          const int* __restrict__ cell_start, const T* sx,
-                                          const T* sy, const T* sz, double ll2,
+// One candidate pair of cells c (particles [sc, sc + nc)) and d (particles [sd, sd + nd)).
+// Pairs of at most TF particle pairs are tested first, and the roots looked at only when
+// a pair is within ll (a test costs less than the root lookups).  Larger pairs are skipped
+// when already joined, else tested here when below kLaneTest pairs, else listed for
+// k_fof_big (tested here too when the list is full).  rc caches c's root (-1: not looked
+// up yet; from a read-only find, possibly out of date: equal roots still prove the cells
+// joined); it is refreshed after a link.
+template <int TF, typename V>
+__device__ __forceinline__ void candidate(int c, int& rc, int sc, int nc, int d, int sd,
+                                          int nd, const V* __restrict__ spk, double ll2,
                                           int* __restrict__ parent, int2* __restrict__ big,
                                           int64_t big_cap,
                                           unsigned long long* __restrict__ n_big) {
-    if (rc == find_ro(parent, d)) return;
-    const int sd = cell_start[d], nd = cell_start[d + 1] - sd;
-    if (static_cast<int64_t>(nc) * nd > kLaneTest) {
-        const unsigned long long at = atomicAdd(n_big, 1ull);
-        if (static_cast<int64_t>(at) < big_cap) {
-            big[at] = make_int2(c, d);
-            return;
-        }
-    }
-    for (int a = 0; a < nc; ++a)
-        for (int b = 0; b < nd; ++b)
-            if (within(sx, sy, sz, sc + a, sd + b, ll2)) {
-                uf_union(parent, c, d);
-                rc = find_ro(parent, c);
+    const int64_t pairs = static_cast<int64_t>(nc) * nd;
+    if (pairs > TF) {
+        if (rc < 0) rc = find_ro(parent, c);
+        if (rc == find_ro(parent, d)) return;
+        if (pairs > kLaneTest) {
+            const unsigned long long at = atomicAdd(n_big, 1ull);
+            if (static_cast<int64_t>(at) < big_cap) {
+                big[at] = make_int2(c, d);
                 return;
             }
+        }
+    }
+    for (int b = 0; b < nd; ++b) {
+        const V pb = spk[sd + b];
+        for (int a = 0; a < nc; ++a)
+            if (within(spk[sc + a], pb, ll2)) {
+                if (rc < 0) rc = find_ro(parent, c);
+                if (rc != find_ro(parent, d)) {
+                    uf_union(parent, c, d);
+                    rc = find_ro(parent, c);
+                }
+                return;
+            }
+    }
 }
 
 // Link candidates, one lane per (column, column offset) of one round.  ROUND 1: the
@@ -505,16 +555,15 @@ __device__ __forceinline__ void candidate(int c, int& rc, int sc, int nc, int d,
 // offsets at Chebyshev distance 1).  ROUND 2, launched after every distance-1 link is in
 // the forest (so that dense regions are recognised as joined and need no search): the
 // own cells two apart, the near columns with |dz| = 2, the far columns with |dz| <= 2
-// (the other 49 offsets).
+// (the other 49 offsets).  The lanes of one column are consecutive and read its record
+// together; a neighbour column costs one table line (first cell, cell count, shared
+// root), its cell records and its particles.
 template <typename T, int ROUND>
 __global__ __launch_bounds__(256) void k_fof_cols(
-    const ColEntry* __restrict__ tab, const int* __restrict__ col_cell,
-    const uint64_t* __restrict__ col_key, const int* __restrict__ cell_start,
-    const int* __restrict__ cell_z, const T* __restrict__ sx, const T* __restrict__ sy,
-    const T* __restrict__ sz, const int64_t* __restrict__ totals, int bx, int by, double ll2,
-    int* __restrict__ parent, int2* __restrict__ big, int64_t big_cap,
-    unsigned long long* __restrict__ n_big, const int* __restrict__ col_root,
-    const int* __restrict__ col_perm) {
+    const ColEntry* __restrict__ tab, const ColRec* __restrict__ lanes,
+    const int2* __restrict__ crec, const typename Packed<T>::type* __restrict__ spk,
+    const int64_t* __restrict__ totals, int bx, int by, double ll2, int* __restrict__ parent,
+    int2* __restrict__ big, int64_t big_cap, unsigned long long* __restrict__ n_big) {
     constexpr int K = ROUND == 1 ? 5 : 13;  // own column + neighbour columns
     const int64_t ncol = totals[0] >> 32;
     const uint64_t mask = static_cast<uint64_t>(totals[1]);
@@ -523,59 +572,64 @@ __global__ __launch_bounds__(256) void k_fof_cols(
     const int64_t nxc = int64_t{1} << bx, nyc = int64_t{1} << by;
     for (int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
          idx < total; idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int q = col_perm[idx / K];
+        const ColRec col = lanes[idx / K];
         const int o = static_cast<int>(idx % K);
-        const int a0 = col_cell[q], a1 = col_cell[q + 1];
-        const int ra = ROUND == 2 ? col_root[q] : -1;
+        const int a0 = col.c0, a1 = a0 + static_cast<int>(col.word >> kKeyBits);
+        const int ra = ROUND == 2 ? tab[col.slot].root : -1;
         if (o == 0) {  // the column's own cells, ROUND apart in z
             if (ra >= 0) continue;  // one component already
+            int2 cc = crec[a0];
             for (int c = a0; c + 1 < a1; ++c) {
-                const int zc = cell_z[c];
-                int rc = -1, sc = 0, nc = 0;
-                for (int d = c + 1; d < a1 && cell_z[d] - zc <= ROUND; ++d)
-                    if (cell_z[d] - zc == ROUND) {
-                        if (rc < 0) {
-                            rc = find_ro(parent, c);
-                            sc = cell_start[c];
-                            nc = cell_start[c + 1] - sc;
-                        }
-                        candidate(c, rc, sc, nc, d, cell_start, sx, sy, sz, ll2, parent, big,
-                                  big_cap, n_big);
-                    }
+                const int2 cn = crec[c + 1];
+                int rc = -1;
+                const int sc = cc.x, nc = cn.x - cc.x;
+                int2 dd = cn;
+                for (int d = c + 1; d < a1 && dd.y - cc.y <= ROUND; ++d) {
+                    const int2 dn = crec[d + 1];
+                    if (dd.y - cc.y == ROUND)
+                        candidate<ROUND == 1 ? kTestFirst : 0>(c, rc, sc, nc, d, dd.x,
+                                                               dn.x - dd.x, spk, ll2, parent,
+                                                               big, big_cap, n_big);
+                    dd = dn;
+                }
+                cc = cn;
             }
             continue;
         }
         const int k = o - 1;  // round 1: the near columns; round 2: all twelve
-        const uint64_t key = col_key[q];
+        const uint64_t key = col.word & kKeyMask;
         const int64_t nx = static_cast<int64_t>(key >> by) + kCol[k][0];
         const int64_t ny = static_cast<int64_t>(key & ymask) + kCol[k][1];
         if (nx < 0 || ny < 0 || nx >= nxc || ny >= nyc) continue;
-        const int qb = col_find(tab, mask, (static_cast<uint64_t>(nx) << by) |
-                                               static_cast<uint64_t>(ny));
-        if (qb < 0) continue;
-        if (ra >= 0 && ra == col_root[qb]) continue;  // joined throughout
-        const int b0 = col_cell[qb], b1 = col_cell[qb + 1];
+        const ColEntry e = col_find(tab, mask, (static_cast<uint64_t>(nx) << by) |
+                                                   static_cast<uint64_t>(ny));
+        if (e.word == kEmpty) continue;
+        if (ra >= 0 && ra == e.root) continue;  // joined throughout
+        const int b0 = e.c0, b1 = b0 + static_cast<int>(e.word >> kKeyBits);
         // |dz| range of this (round, column): near columns take |dz| <= 1 in round 1 and
         // |dz| = 2 in round 2, far columns |dz| <= 2
         const bool near = k < 4;
         const int lo = (ROUND == 2 && near) ? 2 : 0;
         const int hi = ROUND == 1 ? 1 : 2;
         int p = b0;
+        int2 pp = crec[b0];
+        int2 cc = crec[a0];
         for (int c = a0; c < a1; ++c) {
-            const int zc = cell_z[c];
-            int rc = -1, sc = 0, nc = 0;
-            while (p < b1 && cell_z[p] < zc - hi) ++p;
-            for (int d = p; d < b1 && cell_z[d] <= zc + hi; ++d) {
-                const int dz = cell_z[d] > zc ? cell_z[d] - zc : zc - cell_z[d];
-                if (dz < lo) continue;
-                if (rc < 0) {
-                    rc = find_ro(parent, c);
-                    sc = cell_start[c];
-                    nc = cell_start[c + 1] - sc;
-                }
-                candidate(c, rc, sc, nc, d, cell_start, sx, sy, sz, ll2, parent, big, big_cap,
-                          n_big);
+            const int2 cn = crec[c + 1];
+            const int zc = cc.y;
+            int rc = -1;
+            while (p < b1 && pp.y < zc - hi) pp = crec[++p];
+            int2 dd = pp;
+            for (int d = p; d < b1 && dd.y <= zc + hi; ++d) {
+                const int2 dn = crec[d + 1];
+                const int dz = dd.y > zc ? dd.y - zc : zc - dd.y;
+                if (dz >= lo)
+                    candidate<ROUND == 1 ? kTestFirst : 0>(c, rc, cc.x, cn.x - cc.x, d, dd.x,
+                                                           dn.x - dd.x, spk, ll2, parent, big,
+                                                           big_cap, n_big);
+                dd = dn;
             }
+            cc = cn;
         }
     }
 }
@@ -586,8 +640,8 @@ __global__ __launch_bounds__(256) void k_fof_cols(
 template <typename T>
 __global__ __launch_bounds__(256) void k_fof_big(
     const int2* __restrict__ big, const unsigned long long* __restrict__ n_big,
-    int64_t big_cap, const int* __restrict__ cell_start, const T* __restrict__ sx,
-    const T* __restrict__ sy, const T* __restrict__ sz, double ll2, int* __restrict__ parent) {
+    int64_t big_cap, const int2* __restrict__ crec,
+    const typename Packed<T>::type* __restrict__ spk, double ll2, int* __restrict__ parent) {
     constexpr int U = 4;
     const int lane = __lane_id();
     const int64_t nb = static_cast<int64_t>(n_big[0]) < big_cap
@@ -598,8 +652,8 @@ __global__ __launch_bounds__(256) void k_fof_big(
         const int2 pr = big[e];
         const int c = pr.x, d = pr.y;
         if (find_ro(parent, c) == find_ro(parent, d)) continue;
-        const int sc = cell_start[c], nc = cell_start[c + 1] - sc;
-        const int sd = cell_start[d], nd = cell_start[d + 1] - sd;
+        const int sc = crec[c].x, nc = crec[c + 1].x - sc;
+        const int sd = crec[d].x, nd = crec[d + 1].x - sd;
         const bool c_small = nc <= nd;
         const int sA = c_small ? sc : sd, nA = c_small ? nc : nd;
         const int sB = c_small ? sd : sc, nB = c_small ? nd : nc;
@@ -611,15 +665,16 @@ __global__ __launch_bounds__(256) void k_fof_big(
             for (int u = 0; u < U; ++u) {
                 const int b = b0 + u * kWave + lane;
                 valid[u] = b < nB;
-                const int bb = valid[u] ? sB + b : sB;
-                xb[u] = static_cast<double>(sx[bb]);
-                yb[u] = static_cast<double>(sy[bb]);
-                zb[u] = static_cast<double>(sz[bb]);
+                const auto pb = spk[valid[u] ? sB + b : sB];
+                xb[u] = static_cast<double>(pb.x);
+                yb[u] = static_cast<double>(pb.y);
+                zb[u] = static_cast<double>(pb.z);
             }
             for (int a = 0; a < nA; ++a) {
-                const double xa = static_cast<double>(sx[sA + a]);
-                const double ya = static_cast<double>(sy[sA + a]);
-                const double za = static_cast<double>(sz[sA + a]);
+                const auto pa = spk[sA + a];
+                const double xa = static_cast<double>(pa.x);
+                const double ya = static_cast<double>(pa.y);
+                const double za = static_cast<double>(pa.z);
                 bool hit = false;
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -659,23 +714,48 @@ __global__ void k_roots(const int64_t* __restrict__ totals, int* __restrict__ pa
     }
 }
 
-// also folds every cell's minimum particle index into its component's minimum
-__global__ void k_check_roots(const int64_t* __restrict__ totals, int* __restrict__ parent,
-                              const int* __restrict__ root, const int* __restrict__ cell_min,
-                              int* __restrict__ comp_min, int* __restrict__ err) {
+// Also folds every cell's minimum particle index into its component's minimum.  The
+// lanes of a wavefront whose root is the first lane's (consecutive cells mostly share
+// one) are merged into one check and one atomic; the others act alone.
+__global__ __launch_bounds__(256) void k_check_roots(const int64_t* __restrict__ totals,
+                                                     int* __restrict__ parent,
+                                                     const int* __restrict__ root,
+                                                     const int* __restrict__ cell_min,
+                                                     int* __restrict__ comp_min,
+                                                     int* __restrict__ err) {
     const int64_t n_cells = totals[0] & 0xffffffff;
-    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n_cells;
-         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int r = root[c];
-        if (coherent_read(parent + r) != r) atomicOr(err, 4);
-        atomicMin(comp_min + r, cell_min[c]);
+    const int lane = __lane_id();
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~(kWave - 1));
+         base < n_cells; base += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t c = base + lane;
+        const bool valid = c < n_cells;  // lane 0 always is
+        const int r = valid ? root[c] : -1;
+        const int m = valid ? cell_min[c] : INT32_MAX;
+        const int r0 = __shfl(r, 0, kWave);
+        const bool same = r == r0;
+        int v = same ? m : INT32_MAX;
+        for (int s = 32; s >= 1; s >>= 1) v = min(v, __shfl_xor(v, s, kWave));
+        if (lane == 0) {
+            if (coherent_read(parent + r0) != r0) atomicOr(err, 4);
+            atomicMin(comp_min + r0, v);
+        }
+        if (valid && !same) {
+            if (coherent_read(parent + r) != r) atomicOr(err, 4);
+            atomicMin(comp_min + r, m);
+        }
     }
 }
 
-__global__ void k_labels(int64_t n, const int* __restrict__ pcell, const int* __restrict__ root,
-                         const int* __restrict__ comp_min, int* __restrict__ labels) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) labels[i] = comp_min[root[pcell[i]]];
+// labels in sorted order: position j lies in cell (cell heads before and at j) - 1, and
+// its particle perm[j] gets the minimum of that cell's component
+__global__ void k_labels(int64_t n, const int32_t* __restrict__ perm,
+                         const int64_t* __restrict__ heads, const int64_t* __restrict__ before,
+                         const int* __restrict__ root, const int* __restrict__ comp_min,
+                         int* __restrict__ labels) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int c = static_cast<int>((before[j] & 0xffffffff) + (heads[j] & 0xffffffff)) - 1;
+    labels[perm[j]] = comp_min[root[c]];
 }
 
 struct FofWs {
@@ -693,22 +773,19 @@ struct FofWs {
     int32_t* perm1;   // order after the low-bits pass
     int32_t* perm;    // particle of each sorted position
     uint64_t* skey;   // cell key of each sorted position
-    int64_t* heads;   // cell / column heads, then (in place) their exclusive scan
-    int64_t* before;
-    int* pcell;
-    int* cell_start;  // n_cells + 1 entries
-    int* cell_z;
-    int* col_cell;    // n_cols + 1 entries
+    int64_t* heads;   // cell / column heads of each sorted position
+    int64_t* before;  // their exclusive scan (cell and column numbers)
+    int2* crec;       // per cell: first sorted position, z (n_cells + 1 entries)
+    int* col_cell;    // first cell of each column (n_cols + 1 entries)
     uint64_t* col_key;
-    int* col_perm;    // the columns in link-lane order (by class)
+    int* col_slot;    // table slot of each column
+    ColRec* lanes;    // the columns in link-lane order (by class)
     int* class_counts;
     int* parent;
     int* root;
     int* comp_min;
     int* cell_min;
-    void* sx;
-    void* sy;
-    void* sz;
+    void* spk;        // (x, y, z, 0) per sorted position
     void* pk;         // (x, y, z, 0) per particle in input order
     int2* big;        // large pairs of cells left for k_fof_big
     unsigned long long* n_big;
@@ -745,20 +822,17 @@ FofWs carve_fof(void* base, int64_t n, size_t elt) {
     w.skey = c.take<uint64_t>(nn);
     w.heads = c.take<int64_t>(nn);
     w.before = c.take<int64_t>(nn);
-    w.pcell = c.take<int>(nn);
-    w.cell_start = c.take<int>(nn + 1);
-    w.cell_z = c.take<int>(nn);
+    w.crec = c.take<int2>(nn + 1);
     w.col_cell = c.take<int>(nn + 1);
     w.col_key = c.take<uint64_t>(nn);
-    w.col_perm = c.take<int>(nn);
+    w.col_slot = c.take<int>(nn);
+    w.lanes = c.take<ColRec>(nn);
     w.class_counts = c.take<int>(2 * kColClasses);
     w.parent = c.take<int>(nn);
     w.root = c.take<int>(nn);
     w.comp_min = c.take<int>(nn);
     w.cell_min = c.take<int>(nn);
-    w.sx = c.take<char>(nn * elt);
-    w.sy = c.take<char>(nn * elt);
-    w.sz = c.take<char>(nn * elt);
+    w.spk = c.take<char>(nn * 4 * elt);
     w.pk = c.take<char>(nn * 4 * elt);
     w.big_cap = 2 * nn;
     w.big = c.take<int2>(w.big_cap);
@@ -834,10 +908,9 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
         if (rc) return rc;
         perm = w.perm;
     }
-    T* sx = static_cast<T*>(w.sx);
-    T* sy = static_cast<T*>(w.sy);
-    T* sz = static_cast<T*>(w.sz);
-    k_fof_sorted<T><<<blocks_for(n), 256, 0, st>>>(perm, n, g, pk, sx, sy, sz, w.skey);
+    typedef typename Packed<T>::type V;
+    V* spk = static_cast<V*>(w.spk);
+    k_fof_sorted<T><<<blocks_for(n), 256, 0, st>>>(perm, n, g, pk, spk, w.skey);
     HALMA_LAUNCH_CHECK();
     k_fof_heads<<<blocks_for(n), 256, 0, st>>>(w.skey, n, g.bz, w.heads);
     HALMA_LAUNCH_CHECK();
@@ -848,49 +921,48 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     k_clear_cols<<<grid_cells, 256, 0, st>>>(w.totals, w.tab);
     HALMA_LAUNCH_CHECK();
     k_fof_cells<<<blocks_for(n), 256, 0, st>>>(w.skey, perm, w.heads, w.before, n, g.bz,
-                                               w.totals, w.pcell, w.cell_start, w.cell_z,
-                                               w.cell_min, w.parent, w.comp_min, w.col_cell,
-                                               w.col_key, w.tab, w.err);
+                                               w.crec, w.cell_min, w.parent, w.comp_min,
+                                               w.col_cell, w.col_key);
     HALMA_LAUNCH_CHECK();
     k_clear_words<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(w.n_big), 4);
     HALMA_LAUNCH_CHECK();
     k_clear_words<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(w.class_counts), 2 * kColClasses);
     HALMA_LAUNCH_CHECK();
-    k_col_class_count<<<kClassBlocks, 256, 0, st>>>(w.totals, w.col_cell, w.cell_start,
-                                                     w.class_counts);
+    k_col_class_count<<<kClassBlocks, 256, 0, st>>>(w.totals, w.col_cell, w.col_key,
+                                                     w.col_slot, w.crec, w.tab,
+                                                     w.class_counts, w.err);
     HALMA_LAUNCH_CHECK();
-    k_col_class_scatter<<<kClassBlocks, 256, 0, st>>>(w.totals, w.col_cell, w.cell_start,
-                                                       w.class_counts, w.col_perm);
+    k_col_class_scatter<<<kClassBlocks, 256, 0, st>>>(w.totals, w.col_cell, w.col_key,
+                                                       w.col_slot, w.crec, w.class_counts,
+                                                       w.lanes);
     HALMA_LAUNCH_CHECK();
     const unsigned cand_grid = static_cast<unsigned>(cus * 32);
     const double ll2 = ll * ll;
-    k_fof_cols<T, 1><<<cand_grid, 256, 0, st>>>(w.tab, w.col_cell, w.col_key, w.cell_start,
-                                                w.cell_z, sx, sy, sz, w.totals, g.bx, g.by,
-                                                ll2, w.parent, w.big, w.big_cap, w.n_big,
-                                                nullptr, w.col_perm);
+    k_fof_cols<T, 1><<<cand_grid, 256, 0, st>>>(w.tab, w.lanes, w.crec, spk, w.totals, g.bx,
+                                                g.by, ll2, w.parent, w.big, w.big_cap,
+                                                w.n_big);
     HALMA_LAUNCH_CHECK();
-    k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big, w.big_cap, w.cell_start, sx, sy,
-                                             sz, ll2, w.parent);
+    k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big, w.big_cap, w.crec, spk, ll2,
+                                             w.parent);
     HALMA_LAUNCH_CHECK();
     k_flatten<<<grid_cells, 256, 0, st>>>(w.totals, w.parent);
     HALMA_LAUNCH_CHECK();
-    // the shared root of each column, in w.root (free until k_roots)
-    k_col_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.col_cell, w.parent, w.root);
+    k_col_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.parent, w.tab);
     HALMA_LAUNCH_CHECK();
-    k_fof_cols<T, 2><<<cand_grid, 256, 0, st>>>(w.tab, w.col_cell, w.col_key, w.cell_start,
-                                                w.cell_z, sx, sy, sz, w.totals, g.bx, g.by,
-                                                ll2, w.parent, w.big, w.big_cap, w.n_big + 1,
-                                                w.root, w.col_perm);
+    k_fof_cols<T, 2><<<cand_grid, 256, 0, st>>>(w.tab, w.lanes, w.crec, spk, w.totals, g.bx,
+                                                g.by, ll2, w.parent, w.big, w.big_cap,
+                                                w.n_big + 1);
     HALMA_LAUNCH_CHECK();
-    k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big + 1, w.big_cap, w.cell_start, sx,
-                                             sy, sz, ll2, w.parent);
+    k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big + 1, w.big_cap, w.crec, spk, ll2,
+                                             w.parent);
     HALMA_LAUNCH_CHECK();
     k_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.parent, w.root);
     HALMA_LAUNCH_CHECK();
     k_check_roots<<<grid_cells, 256, 0, st>>>(w.totals, w.parent, w.root, w.cell_min,
                                               w.comp_min, w.err);
     HALMA_LAUNCH_CHECK();
-    k_labels<<<blocks_for(n), 256, 0, st>>>(n, w.pcell, w.root, w.comp_min, labels);
+    k_labels<<<blocks_for(n), 256, 0, st>>>(n, perm, w.heads, w.before, w.root, w.comp_min,
+                                            labels);
     HALMA_LAUNCH_CHECK();
     return 0;
 }
