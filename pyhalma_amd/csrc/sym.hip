@@ -4,16 +4,22 @@
 // its two terms, m_j / r to S_i and m_i / r to S_j, accumulated together.
 //
 // A halo of n members is cut into tiles of kTile = 128 members and the tiles into blocks
-// of kSymK tiles.  A work item is a pair of blocks (A, B), A <= B.  One wavefront holds
-// the 128 members of one tile of A as tests in registers (2 per lane), streams the
-// members of B's tiles as wave-uniform sources, and accumulates
-//   - test side: FP32 sums of 8 terms, Kahan-added, per test (the tile of A);
-//   - source side: per source, the wave-wide sum of its terms over the tile's tests
-//     (a fixed reduction tree across the lanes), Kahan-added in the wave's LDS.
-// Within a diagonal item (A == B) only pairs of tiles a <= b are taken, and a tile
-// against itself is summed on the test side alone.  Every member then has kSymK-block
-// partial sums in nb + 1 slots (nb = blocks of its halo), each written by exactly one
-// item: slot s < I (its block) from the source side of item (s, I), slot I from the test
+// of kSymK = 4 tiles.  A work item is a pair of blocks (A, B), A <= B.  Tests are held in
+// registers (a tile of A: 2 per lane), the members of B's tiles streamed as wave-uniform
+// sources; accumulated are
+//   - test side: FP32 sums of 8 terms, Kahan-added, per test;
+//   - source side: per source, the wave-wide sum of its terms over a tile's tests (a
+//     fixed reduction tree across the lanes), Kahan-added in LDS over the tiles of A in
+//     order.
+// The test side keeps one Kahan sum per source tile; the tiles' sums are added in FP64
+// in tile order.  A launch with many items gives each item to one wavefront (all tiles of
+// B in turn); a launch with few items (a lone large halo, the last passes of an
+// unbinding) gives each to a workgroup whose wavefront w takes tile w of B, so that an
+// item lasts four tile pairs instead of sixteen.  Both forms do the same arithmetic in
+// the same order: a halo's sums do not depend on the other haloes of the call.  Within a diagonal item
+// (A == B) only pairs of tiles a <= b are taken, and a tile against itself is summed on
+// the test side alone.  Every member then has kSymK-block partial sums in nb + 1 slots
+// (nb = blocks of its halo), each written by exactly one item: slot s < I (its block) from the source side of item (s, I), slot I from the test
 // side of item (I, I), slots s > I from the test side of item (I, s), and slot nb from
 // the source side of item (I, I).  The slots are merged in FP64 in slot order.
 #include <math.h>
@@ -110,25 +116,25 @@ __device__ __forceinline__ void pair_group(const float4* __restrict__ q0, int g,
     }
 }
 
-__global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
-                                                 const SymDesc* __restrict__ halos,
-                                                 const int* __restrict__ live_n,
-                                                 const int4* __restrict__ items, int n_items,
-                                                 const int64_t* __restrict__ n_items_dev,
-                                                 float2* __restrict__ part,
-                                                 int* __restrict__ head) {
-    // wave-private LDS accumulators of the source side: kSymK tiles x 128 (sum, comp)
-    __shared__ float lsum[4][kSymK * kTile];
-    __shared__ float lcmp[4][kSymK * kTile];
+// Below this many items in a launch an item is taken by a workgroup (four wavefronts,
+// one source tile each), else by one wavefront: a wavefront per item fills the device
+// once there are a few items per SIMD and keeps diagonal items balanced; with fewer
+// items the workgroup form cuts an item's latency fourfold.
+constexpr int kGroupItems = 2048;
+
+// One item per wavefront: tiles of A in turn against all tiles of B; test side a Kahan
+// sum per tile of B, the tiles' sums added in FP64 in tile order; source side in the
+// wavefront's LDS (ls, lc: kSymK tiles each).  Same arithmetic, in the same order, as
+// the workgroup form below, so both give the same bits.
+__device__ __forceinline__ void sym_items_wave(const float4* __restrict__ src,
+                                               const SymDesc* __restrict__ halos,
+                                               const int* __restrict__ live_n,
+                                               const int4* __restrict__ items, int n_items,
+                                               float2* __restrict__ part, int* __restrict__ head,
+                                               float* ls, float* lc) {
     const int lane = __lane_id();
     const int wv = threadIdx.x / kWave;
-    float* ls = lsum[wv];
-    float* lc = lcmp[wv];
     const int n_waves = static_cast<int>(gridDim.x * (blockDim.x / kWave));
-    if (n_items_dev) {
-        const int64_t nd = uniform_load(n_items_dev);
-        n_items = static_cast<int>(nd < n_items ? nd : n_items);
-    }
     int k = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * (blockDim.x / kWave) + wv));
     for (bool first = true;; first = false) {
         if (!first) {
@@ -155,7 +161,8 @@ __global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
         for (int ta = 0; ta < kSymK; ++ta) {
             const int a = A * kSymK + ta;
             if (a >= nt) break;
-            float xi[2], yi[2], zi[2], mi[2], as[2], ac[2];
+            float xi[2], yi[2], zi[2], mi[2];
+            double S[2] = {0.0, 0.0};
             int ii[2];
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
@@ -165,8 +172,6 @@ __global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
                 yi[r] = p.y;
                 zi[r] = p.z;
                 mi[r] = ii[r] < n ? p.w : 0.0f;  // padding lanes add nothing to sources
-                as[r] = 0.0f;
-                ac[r] = 0.0f;
             }
             for (int tb = 0; tb < kSymK; ++tb) {
                 const int b = B * kSymK + tb;
@@ -175,6 +180,7 @@ __global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
                 const int j0 = b * kTile;
                 const int cnt = min(kTile, n - j0);
                 const bool diag = b == a;
+                float as[2] = {0.0f, 0.0f}, ac[2] = {0.0f, 0.0f};
                 for (int g = 0; g < cnt; g += 8) {
                     float ps[2] = {0.0f, 0.0f};
                     float pj[8];
@@ -196,12 +202,20 @@ __global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
                         }
                     }
                 }
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+                    S[r] += static_cast<double>(as[r]) + static_cast<double>(ac[r]);
             }
-            // test side of tile a from this item: slot B (mod the band width)
+            // test side of tile a from this item: slot B (mod the band width), the source
+            // tiles' sums added in FP64 in tile order (the same bits as a workgroup's)
             const int tslot = B % d.wb;
 #pragma unroll
             for (int r = 0; r < 2; ++r)
-                if (ii[r] < n) pp[static_cast<int64_t>(tslot) * stride + ii[r]] = make_float2(as[r], ac[r]);
+                if (ii[r] < n) {
+                    const float hi = static_cast<float>(S[r]);
+                    pp[static_cast<int64_t>(tslot) * stride + ii[r]] =
+                        make_float2(hi, static_cast<float>(S[r] - static_cast<double>(hi)));
+                }
         }
         // source side of B's tiles from this item: slot A (mod the band width), or the
         // diagonal slot min(wb, nb)
@@ -220,6 +234,144 @@ __global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
             }
         }
     }
+}
+
+// One item per workgroup: wavefront w streams tile w of B against each tile of A in turn;
+// its source side stays in its LDS (ls, lc: one tile), its test-side sums per tile of A go
+// to tpart[ta][w] and are added in FP64 in tile order once the four are done.
+__device__ __forceinline__ void sym_items_group(const float4* __restrict__ src,
+                                                const SymDesc* __restrict__ halos,
+                                                const int* __restrict__ live_n,
+                                                const int4* __restrict__ items, int n_items,
+                                                float2* __restrict__ part,
+                                                int* __restrict__ head, float* ls, float* lc,
+                                                float2* tpart, int* next_item) {
+    const int lane = __lane_id();
+    const int wv = threadIdx.x / kWave;
+    int k = static_cast<int>(blockIdx.x);
+    for (bool first = true;; first = false) {
+        if (!first) {
+            __syncthreads();  // the previous item's LDS has been read
+            if (threadIdx.x == 0) *next_item = atomicAdd(head, 1);
+            __syncthreads();
+            k = static_cast<int>(gridDim.x) + *next_item;
+        }
+        if (k >= n_items) break;
+        const int4 it = uniform_load(items + k);
+        const SymDesc d = uniform_load(halos + it.x);
+        const int n = live_n ? __builtin_amdgcn_readfirstlane(uniform_load(live_n + it.x)) : d.n;
+        const int nt = (n + kTile - 1) / kTile;
+        const int nb = (nt + kSymK - 1) / kSymK;
+        const int A = it.y, B = it.z;
+        if (B >= nb || A > B) continue;
+        const float4* sp = src + d.off;
+        float2* pp = part + d.part_off;
+        const int64_t stride = d.n;  // slots are laid out for the planned size
+        const int b = B * kSymK + wv;  // this wavefront's source tile
+        const int j0 = b * kTile;
+        const int cnt = b < nt ? min(kTile, n - j0) : 0;
+#pragma unroll
+        for (int e = 0; e < kTile / kWave; ++e) {
+            ls[e * kWave + lane] = 0.0f;
+            lc[e * kWave + lane] = 0.0f;
+        }
+        for (int ta = 0; ta < kSymK; ++ta) {
+            const int a = A * kSymK + ta;
+            if (a >= nt) break;
+            float as[2] = {0.0f, 0.0f}, ac[2] = {0.0f, 0.0f};
+            if (cnt > 0 && !(A == B && b < a)) {
+                float xi[2], yi[2], zi[2], mi[2];
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int i = a * kTile + r * kWave + lane;
+                    const float4 p = sp[min(i, n - 1)];
+                    xi[r] = p.x;
+                    yi[r] = p.y;
+                    zi[r] = p.z;
+                    mi[r] = i < n ? p.w : 0.0f;  // padding lanes add nothing to sources
+                }
+                const bool diag = b == a;
+                for (int g = 0; g < cnt; g += 8) {
+                    float ps[2] = {0.0f, 0.0f};
+                    float pj[8];
+                    if (g + 8 <= cnt)
+                        pair_group<true>(sp + j0, g, cnt, xi, yi, zi, mi, ps, pj);
+                    else
+                        pair_group<false>(sp + j0, g, cnt, xi, yi, zi, mi, ps, pj);
+                    kahan(as[0], ac[0], ps[0]);
+                    kahan(as[1], ac[1], ps[1]);
+                    if (!diag) {
+                        const float tot = reduce8(pj, lane);
+                        const int s = (lane >> 3) & 7;
+                        if ((lane & 7) == 0 && g + s < cnt) {
+                            float sv = ls[g + s], cv = lc[g + s];
+                            kahan(sv, cv, tot);
+                            ls[g + s] = sv;
+                            lc[g + s] = cv;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+                tpart[(ta * kSymK + wv) * kTile + r * kWave + lane] = make_float2(as[r], ac[r]);
+        }
+        __syncthreads();
+        // test side of A's tiles from this item: slot B (mod the band width); the four
+        // source tiles' compensated sums added in FP64 in tile order, stored as a pair of
+        // floats (hi, lo) whose sum keeps the FP64 value to u^2
+        const int tslot = B % d.wb;
+        for (int t = threadIdx.x; t < kSymK * kTile; t += blockDim.x) {
+            const int ta = t / kTile, e = t % kTile;
+            const int i = (A * kSymK + ta) * kTile + e;
+            if (A * kSymK + ta >= nt || i >= n) continue;
+            double S = 0.0;
+#pragma unroll
+            for (int w = 0; w < kSymK; ++w) {
+                const float2 v = tpart[(ta * kSymK + w) * kTile + e];
+                S += static_cast<double>(v.x) + static_cast<double>(v.y);
+            }
+            const float hi = static_cast<float>(S);
+            pp[static_cast<int64_t>(tslot) * stride + i] =
+                make_float2(hi, static_cast<float>(S - static_cast<double>(hi)));
+        }
+        // source side of this wavefront's tile of B: slot A (mod the band width), or the
+        // diagonal slot min(wb, nb)
+        const int slot = A < B ? A % d.wb : min(d.wb, nb);
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int e = r * kWave + lane;
+            if (e < cnt)
+                pp[static_cast<int64_t>(slot) * stride + j0 + e] = make_float2(ls[e], lc[e]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
+                                                 const SymDesc* __restrict__ halos,
+                                                 const int* __restrict__ live_n,
+                                                 const int4* __restrict__ items, int n_items,
+                                                 const int64_t* __restrict__ n_items_dev,
+                                                 float2* __restrict__ part,
+                                                 int* __restrict__ head) {
+    static_assert(kSymK * kWave == 256, "one wavefront per tile of a block");
+    // LDS of either form: a wavefront per item keeps (sum, comp) of kSymK source tiles
+    // per wavefront (16 KiB); a workgroup per item one source tile per wavefront (4 KiB)
+    // and the test-side sums of each (tile of A, wavefront) (16 KiB)
+    __shared__ float2 lds[2560];
+    __shared__ int next_item;
+    float* f = reinterpret_cast<float*>(lds);
+    const int wv = threadIdx.x / kWave;
+    if (n_items_dev) {
+        const int64_t nd = uniform_load(n_items_dev);
+        n_items = static_cast<int>(nd < n_items ? nd : n_items);
+    }
+    if (n_items >= kGroupItems)
+        sym_items_wave(src, halos, live_n, items, n_items, part, head,
+                       f + wv * kSymK * kTile, f + (kSymK + wv) * kSymK * kTile);
+    else
+        sym_items_group(src, halos, live_n, items, n_items, part, head, f + wv * kTile,
+                        f + (kSymK + wv) * kTile, lds + kSymK * kTile, &next_item);
 }
 
 // The slots of one tile launch added into each affected member's FP64 sum (out, cleared
@@ -422,7 +574,7 @@ int launch_pot_sym(const SymPlan& plan, const float4* src4, const SymDesc* d_hal
     if (n_items == 0) return 0;
     int cus = device_cu_count();
     if (cus <= 0) cus = 256;
-    const int grid = std::max(1, std::min(cus * 8, (n_items + 3) / 4));
+    const int grid = std::max(1, std::min(cus * 8, n_items));  // a workgroup or more per item
     k_pot_sym<<<grid, 256, 0, st>>>(src4, d_halos, live_n, d_items, n_items, n_items_dev, part,
                                     head);
     HALMA_LAUNCH_CHECK();
@@ -492,7 +644,7 @@ int halma_pot_sum_self_batched(int64_t n_halo, const int64_t* offsets, const flo
         if (ni > 0) {
             k_sym_clear<<<1, 64, 0, st>>>(w.head);
             HALMA_LAUNCH_CHECK();
-            const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(cus * 8, (ni + 3) / 4)));
+            const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(cus * 8, ni)));
             k_pot_sym<<<grid, 256, 0, st>>>(w.src4, w.halos, nullptr, w.items + i0,
                                             static_cast<int>(ni), nullptr, w.part, w.head);
             HALMA_LAUNCH_CHECK();

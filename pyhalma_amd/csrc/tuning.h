@@ -25,7 +25,9 @@ inline double pot_error_bound(int sub) {
     return (sub + 7.0) * u + 1e-11;
 }
 
-// The symmetric self-potential kernel: test side as above with sub-tiles of 8 terms;
+// The symmetric self-potential kernel: test side as above with sub-tiles of 8 terms (a
+// Kahan sum per source tile, the tiles' sums added in FP64 and kept as a float pair:
+// relative error below 1e-15, inside the 0.5u);
 // source side per term 4.5u (differences, squares, v_rsq_f32), the lane's two-term
 // product sum 2u, the 6-level reduction tree across the wave 6u, Kahan 2u, + 0.5u:
 // 15u either way.
