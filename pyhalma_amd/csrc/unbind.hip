@@ -651,14 +651,17 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
     const int64_t nb = static_cast<int64_t>(a.counters[1]);
     // members of the decision band: the sum in the oracle's order (same bits either way),
     // a block per member of a large live set (a shorter chain per thread), a wavefront per
-    // member of a small one (more members in flight); stale members: an FP32 re-sum
+    // member of a small one (more members in flight); stale members: an FP32 re-sum.  The
+    // re-test and re-sum counts are kept by the thread that writes and added to the
+    // global counters once per thread at the end (one address, many members per pass).
+    unsigned long long n_retest = 0, n_resum = 0;
     auto fresh = [&](int64_t g, double S) {
         const int64_t k = a.cidx[g];
         a.sacc[k] = S;
         a.eacc[k] = a.bound_resum * S;
         if (a.s_out) a.s_out[k] = S;
-        atomicAdd(a.counters + 2, 1ull);
-        atomicAdd(a.counters + 6, 1ull);
+        ++n_retest;
+        ++n_resum;
     };
     auto decide = [&](int64_t g, int h, double S) {
         const int64_t k = a.cidx[g];
@@ -666,7 +669,7 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
         a.sacc[k] = S;
         a.eacc[k] = 1e-12 * S;
         if (a.s_out) a.s_out[k] = S;
-        atomicAdd(a.counters + 2, 1ull);
+        ++n_retest;
         return E;
     };
     const int64_t nbig = static_cast<int64_t>(a.counters[8]);
@@ -711,6 +714,8 @@ __global__ __launch_bounds__(kOrdWidth) void k_retest(EnergyArgs a,
             atomicAdd(a.removed + h, 1);
         }
     }
+    if (n_retest) atomicAdd(a.counters + 2, n_retest);
+    if (n_resum) atomicAdd(a.counters + 6, n_resum);
 }
 
 // The end of a pass, after every re-test: per-halo counts and modes (finish_haloes), one
