@@ -128,16 +128,23 @@ __device__ __forceinline__ bool pass_is_full(int pass_no, int run, int rrun, int
 // Largest relative error bound a member's S may keep.  A kept member whose accumulated
 // bound (the full pass's, plus that of every incremental subtraction since) goes above it
 // is stale.  Its decision is exact (it lies outside the decision band), so it only needs
-// a fresh S: when its halo has few stale members (kStaleDiv below) the re-test kernel
-// re-sums them in FP32 with every term Kahan-added (bound 8u = 4.8e-7); otherwise the
+// a fresh S: when its halo has few stale members (at most 1/kStaleDiv of the live set)
+// the re-test kernel re-sums them in FP32 with every term Kahan-added (bound 8u =
+// 4.8e-7); otherwise the
 // halo's next pass is a full one (the symmetric kernel recomputes every S), and a halo
 // that removed nobody takes that full pass only to refresh its S (not counted as an
 // iteration: its decisions are exact and do not change).  Both full-pass kernels stay
 // below the limit (halma_pot_error_bound at sub-tiles of 4 or 8, halma_pot_sym_error_bound).
 constexpr double kPotRelMax = 1e-6;
 
+// A wavefront re-sums one stale member, a latency-bound chain of about live/64 terms per
+// lane, while a full pass adds a halo's block pairs to a launch that runs anyway: beyond
+// 1/32 of the live set the full pass costs less (measured on clump and bridged-pair
+// catalogues).
+constexpr int kStaleDiv = 32;
+
 __device__ __forceinline__ bool too_stale(int stale, int live) {
-    return 8 * static_cast<int64_t>(stale) > live;
+    return kStaleDiv * static_cast<int64_t>(stale) > live;
 }
 
 // band-list entries of stale members carry this bit (members of the decision band do not)
