@@ -15,10 +15,11 @@
 //    (first position, z), and each column enters a hash table whose 16-byte entry holds
 //    its key, cell count, first cell and (between the rounds) shared root, so that a
 //    neighbour column costs one line; the table is sized from the column count;
-// 4. links, in two rounds (Chebyshev distance 1, then 2), one lane per (column,
-//    neighbour column): the half-stencil's 12 columns (dx > 0, or dx = 0 and dy > 0) and
-//    the column itself (cells above); one table probe finds the neighbour column, then the
-//    two z-ordered cell lists are merged for the cell pairs within the round's dz.  Pairs
+// 4. links, in two rounds (Chebyshev distance 1, then 2), one lane per (segment of up to
+//    4 cells of a column, neighbour column): the half-stencil's 12 columns (dx > 0, or
+//    dx = 0 and dy > 0) and the column itself (cells above); one table probe finds the
+//    neighbour column, then the two z-ordered cell lists are merged for the cell pairs
+//    within the round's dz.  Pairs
 //    of cells with at most 8 particle pairs are tested at once (roots are looked up only
 //    for a link); larger ones already joined are skipped, the rest tested by the lane or,
 //    beyond 64 particle pairs, listed for one wavefront each (64 lanes over the larger
@@ -59,12 +60,17 @@ struct __attribute__((aligned(16))) ColEntry {
     int32_t root;
 };
 
-// A column in link-lane order: its word, its first cell and its table slot
+// A column segment in link-lane order: its word (the column's key in the low bits, the
+// segment's cell count above them, then how many cells of the column follow the segment,
+// at most 2: all that an own-column pair can reach), its first cell and the column's
+// table slot
 struct __attribute__((aligned(16))) ColRec {
     uint64_t word;
     int32_t c0;
     int32_t slot;
 };
+constexpr int kSegLenShift = kKeyBits;       // 10 bits
+constexpr int kSegMoreShift = kKeyBits + 10;  // 2 bits
 
 __device__ __forceinline__ uint64_t mix64(uint64_t k) {
     k ^= k >> 30;
@@ -382,23 +388,32 @@ __global__ void k_flatten(const int64_t* __restrict__ totals, int* __restrict__ 
 }
 
 
-// Lane order of the link kernels: columns grouped by the log2 of their particle count,
-// largest first, so that the lanes of a wavefront walk columns of similar length (a
-// wavefront lasts as long as its longest lane) and the longest start first.  A counting
-// sort over kColClasses classes; each block takes one contiguous stretch of columns,
-// counts it, reserves its places with one atomic per class, and writes it (the order
-// within a class is not fixed; the links do not depend on it).
+// Link lanes: a column of more than kSegMin cells is cut into segments of kSegCells
+// cells, and each segment takes its own lanes (a long column's cells are then linked by
+// several lanes at once instead of one long chain of finds and unions).  Lane order:
+// segments grouped by the log2 of their particle count, largest first, so that the lanes
+// of a wavefront walk segments of similar length (a wavefront lasts as long as its
+// longest lane) and the longest start first.  A counting sort over kColClasses classes;
+// each block takes one contiguous stretch of columns, counts their segments, reserves
+// their places with one atomic per class, and writes them (the order within a class is
+// not fixed; the links do not depend on it).
 constexpr int kColClasses = 8;
 constexpr int kClassBlocks = 256;
+constexpr int kSegMin = 4;
+constexpr int kSegCells = 4;
+static_assert(kSegCells < 1024 && kSegMin < 1024, "a segment's cell count has 10 bits");
 
-__device__ __forceinline__ int col_class(const int* __restrict__ col_cell,
-                                         const int2* __restrict__ crec, int64_t q) {
-    const int np = crec[col_cell[q + 1]].x - crec[col_cell[q]].x;
+// cells per segment of a column of n cells
+__device__ __forceinline__ int seg_step(int n) { return n > kSegMin ? kSegCells : (n > 0 ? n : 1); }
+
+// a segment's class: log2 of its particle count
+__device__ __forceinline__ int seg_class(const int2* __restrict__ crec, int c0, int c1) {
+    const int np = crec[c1].x - crec[c0].x;
     const int lg = 31 - __clz(np > 0 ? np : 1);
     return lg < kColClasses - 1 ? lg : kColClasses - 1;
 }
 
-// counts[0..7]: columns per class; counts[8..15]: the class cursors (zero here).  Also
+// counts[0..7]: segments per class; counts[8..15]: the class cursors (zero here).  Also
 // enters each column in the table (word, first cell) and keeps its slot.
 __global__ __launch_bounds__(256) void k_col_class_count(const int64_t* __restrict__ totals,
                                                          const int* __restrict__ col_cell,
@@ -416,10 +431,11 @@ __global__ __launch_bounds__(256) void k_col_class_count(const int64_t* __restri
     const int64_t q0 = blockIdx.x * per, q1 = q0 + per < ncol ? q0 + per : ncol;
     const uint64_t mask = static_cast<uint64_t>(totals[1]);
     for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-        atomicAdd(&h[col_class(col_cell, crec, q)], 1);
-        const int a0 = col_cell[q];
+        const int a0 = col_cell[q], a1 = col_cell[q + 1];
+        const int step = seg_step(a1 - a0);
+        for (int c = a0; c < a1; c += step) atomicAdd(&h[seg_class(crec, c, min(c + step, a1))], 1);
         const uint64_t key = col_key[q];
-        const uint64_t word = key | (static_cast<uint64_t>(col_cell[q + 1] - a0) << kKeyBits);
+        const uint64_t word = key | (static_cast<uint64_t>(a1 - a0) << kKeyBits);
         uint64_t s = mix64(key) & mask;
         for (uint64_t probe = 0;; ++probe) {
             uint64_t expected = kEmpty;
@@ -456,11 +472,14 @@ __global__ __launch_bounds__(256) void k_col_class_scatter(const int64_t* __rest
     const int64_t ncol = totals[0] >> 32;
     const int64_t per = (ncol + gridDim.x - 1) / gridDim.x;
     const int64_t q0 = blockIdx.x * per, q1 = q0 + per < ncol ? q0 + per : ncol;
-    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x)
-        atomicAdd(&h[col_class(col_cell, crec, q)], 1);
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const int a0 = col_cell[q], a1 = col_cell[q + 1];
+        const int step = seg_step(a1 - a0);
+        for (int c = a0; c < a1; c += step) atomicAdd(&h[seg_class(crec, c, min(c + step, a1))], 1);
+    }
     __syncthreads();
     if (threadIdx.x < kColClasses) {
-        int before = 0;  // columns of the higher classes: the longest columns come first
+        int before = 0;  // segments of the higher classes: the longest come first
         for (int k = threadIdx.x + 1; k < kColClasses; ++k) before += counts[k];
         base[threadIdx.x] =
             h[threadIdx.x] > 0 ? before + atomicAdd(counts + kColClasses + threadIdx.x, h[threadIdx.x]) : 0;
@@ -468,13 +487,18 @@ __global__ __launch_bounds__(256) void k_col_class_scatter(const int64_t* __rest
     }
     __syncthreads();
     for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-        const int k = col_class(col_cell, crec, q);
-        const int a0 = col_cell[q];
-        ColRec r;
-        r.word = col_key[q] | (static_cast<uint64_t>(col_cell[q + 1] - a0) << kKeyBits);
-        r.c0 = a0;
-        r.slot = col_slot[q];
-        lanes[base[k] + atomicAdd(&h[k], 1)] = r;
+        const int a0 = col_cell[q], a1 = col_cell[q + 1];
+        const int step = seg_step(a1 - a0);
+        for (int c = a0; c < a1; c += step) {
+            const int c1 = min(c + step, a1);
+            const int k = seg_class(crec, c, c1);
+            ColRec r;
+            r.word = col_key[q] | (static_cast<uint64_t>(c1 - c) << kSegLenShift) |
+                     (static_cast<uint64_t>(min(a1 - c1, 2)) << kSegMoreShift);
+            r.c0 = c;
+            r.slot = col_slot[q];
+            lanes[base[k] + atomicAdd(&h[k], 1)] = r;
+        }
     }
 }
 
@@ -550,41 +574,48 @@ __device__ __forceinline__ void candidate(int c, int& rc, int sc, int nc, int d,
     }
 }
 
-// Link candidates, one lane per (column, column offset) of one round.  ROUND 1: the
-// column's own cells one z apart, and the four near columns with |dz| <= 1 (the 13 cell
-// offsets at Chebyshev distance 1).  ROUND 2, launched after every distance-1 link is in
-// the forest (so that dense regions are recognised as joined and need no search): the
-// own cells two apart, the near columns with |dz| = 2, the far columns with |dz| <= 2
-// (the other 49 offsets).  The lanes of one column are consecutive and read its record
-// together; a neighbour column costs one table line (first cell, cell count, shared
-// root), its cell records and its particles.
+// Link candidates, one lane per (column segment, column offset) of one round: the
+// segment's cells against the cells ROUND apart above them in their own column, or
+// against a neighbour column.  ROUND 1: the own cells one z apart, and the four near
+// columns with |dz| <= 1 (the 13 cell offsets at Chebyshev distance 1).  ROUND 2,
+// launched after every distance-1 link is in the forest (so that dense regions are
+// recognised as joined and need no search): the own cells two apart, the near columns
+// with |dz| = 2, the far columns with |dz| <= 2 (the other 49 offsets).  The lanes of
+// one segment are consecutive and read its record together; a neighbour column costs
+// one table line (first cell, cell count, shared root), its cell records and its
+// particles.
 template <typename T, int ROUND>
 __global__ __launch_bounds__(256) void k_fof_cols(
     const ColEntry* __restrict__ tab, const ColRec* __restrict__ lanes,
     const int2* __restrict__ crec, const typename Packed<T>::type* __restrict__ spk,
     const int64_t* __restrict__ totals, int bx, int by, double ll2, int* __restrict__ parent,
-    int2* __restrict__ big, int64_t big_cap, unsigned long long* __restrict__ n_big) {
+    int2* __restrict__ big, int64_t big_cap, unsigned long long* __restrict__ n_big,
+    const int* __restrict__ seg_counts) {
     constexpr int K = ROUND == 1 ? 5 : 13;  // own column + neighbour columns
-    const int64_t ncol = totals[0] >> 32;
+    int64_t nseg = 0;
+#pragma unroll
+    for (int k = 0; k < kColClasses; ++k) nseg += seg_counts[k];
     const uint64_t mask = static_cast<uint64_t>(totals[1]);
-    const int64_t total = ncol * K;
+    const int64_t total = nseg * K;
     const int64_t ymask = (int64_t{1} << by) - 1;
     const int64_t nxc = int64_t{1} << bx, nyc = int64_t{1} << by;
     for (int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
          idx < total; idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const ColRec col = lanes[idx / K];
         const int o = static_cast<int>(idx % K);
-        const int a0 = col.c0, a1 = a0 + static_cast<int>(col.word >> kKeyBits);
-        const int ra = ROUND == 2 ? tab[col.slot].root : -1;
-        if (o == 0) {  // the column's own cells, ROUND apart in z
+        const int a0 = col.c0;  // the segment's cells [a0, a1)
+        const int a1 = a0 + static_cast<int>((col.word >> kSegLenShift) & 1023);
+        const int ra = ROUND == 2 ? tab[col.slot].root : -1;  // the column's shared root
+        if (o == 0) {  // the segment's cells against those ROUND above them in the column
             if (ra >= 0) continue;  // one component already
+            const int e1 = a1 + static_cast<int>((col.word >> kSegMoreShift) & 3);
             int2 cc = crec[a0];
-            for (int c = a0; c + 1 < a1; ++c) {
+            for (int c = a0; c < a1 && c + 1 < e1; ++c) {
                 const int2 cn = crec[c + 1];
                 int rc = -1;
                 const int sc = cc.x, nc = cn.x - cc.x;
                 int2 dd = cn;
-                for (int d = c + 1; d < a1 && dd.y - cc.y <= ROUND; ++d) {
+                for (int d = c + 1; d < e1 && dd.y - cc.y <= ROUND; ++d) {
                     const int2 dn = crec[d + 1];
                     if (dd.y - cc.y == ROUND)
                         candidate<ROUND == 1 ? kTestFirst : 0>(c, rc, sc, nc, d, dd.x,
@@ -779,7 +810,7 @@ struct FofWs {
     int* col_cell;    // first cell of each column (n_cols + 1 entries)
     uint64_t* col_key;
     int* col_slot;    // table slot of each column
-    ColRec* lanes;    // the columns in link-lane order (by class)
+    ColRec* lanes;    // the column segments in link-lane order (by class)
     int* class_counts;
     int* parent;
     int* root;
@@ -826,7 +857,7 @@ FofWs carve_fof(void* base, int64_t n, size_t elt) {
     w.col_cell = c.take<int>(nn + 1);
     w.col_key = c.take<uint64_t>(nn);
     w.col_slot = c.take<int>(nn);
-    w.lanes = c.take<ColRec>(nn);
+    w.lanes = c.take<ColRec>(nn + nn / kSegCells + 1);  // segments <= columns + cells / 4
     w.class_counts = c.take<int>(2 * kColClasses);
     w.parent = c.take<int>(nn);
     w.root = c.take<int>(nn);
@@ -940,7 +971,7 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     const double ll2 = ll * ll;
     k_fof_cols<T, 1><<<cand_grid, 256, 0, st>>>(w.tab, w.lanes, w.crec, spk, w.totals, g.bx,
                                                 g.by, ll2, w.parent, w.big, w.big_cap,
-                                                w.n_big);
+                                                w.n_big, w.class_counts);
     HALMA_LAUNCH_CHECK();
     k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big, w.big_cap, w.crec, spk, ll2,
                                              w.parent);
@@ -951,7 +982,7 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     HALMA_LAUNCH_CHECK();
     k_fof_cols<T, 2><<<cand_grid, 256, 0, st>>>(w.tab, w.lanes, w.crec, spk, w.totals, g.bx,
                                                 g.by, ll2, w.parent, w.big, w.big_cap,
-                                                w.n_big + 1);
+                                                w.n_big + 1, w.class_counts);
     HALMA_LAUNCH_CHECK();
     k_fof_big<T><<<grid_cells, 256, 0, st>>>(w.big, w.n_big + 1, w.big_cap, w.crec, spk, ll2,
                                              w.parent);
