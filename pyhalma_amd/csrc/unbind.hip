@@ -180,29 +180,8 @@ __device__ int block_excl_scan(int v, int& excl, int* sh) {
 // chunk's kept and just-removed members and sums the kept members' mass and momentum in
 // double-double; k_halo_sets turns the counts into per-chunk output offsets and merges
 // the sums in chunk order into v_cm; k_chunk_write writes the compacted lists.  Large
-// haloes thus spread over many blocks instead of one.
+// haloes thus spread over many wavefronts and blocks instead of one.
 constexpr int kChunk = 1024;
-
-// double-double block sum of (hi, lo) pairs; the result is valid in thread 0
-__device__ void block_dd_sum(double& hi, double& lo, double* shd) {
-    for (int s = 32; s >= 1; s >>= 1) {
-        const double h2 = __shfl_down(hi, s, kWave);
-        const double l2 = __shfl_down(lo, s, kWave);
-        dd_add_dd(hi, lo, h2, l2);
-    }
-    const int w = threadIdx.x / kWave;
-    if (__lane_id() == 0) {
-        shd[2 * w] = hi;
-        shd[2 * w + 1] = lo;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        hi = shd[0];
-        lo = shd[1];
-        for (int k = 1; k < kThreads / kWave; ++k) dd_add_dd(hi, lo, shd[2 * k], shd[2 * k + 1]);
-    }
-    __syncthreads();
-}
 
 __device__ __forceinline__ void chunk_range(const int64_t* off, int2 ch, int64_t& b,
                                             int64_t& e) {
@@ -210,21 +189,25 @@ __device__ __forceinline__ void chunk_range(const int64_t* off, int2 ch, int64_t
     e = off[ch.x + 1] < b + kChunk ? off[ch.x + 1] : b + kChunk;
 }
 
+// One wavefront per chunk: lanes stride over the chunk's members, then the counts and the
+// double-double sums are added across the lanes by a fixed butterfly (no barriers: a
+// pass's chunks, many of them in haloes already converged, cost a wavefront each).
 __global__ __launch_bounds__(kThreads) void k_chunk_sums(
     const int2* __restrict__ chunks, int n_chunks, const int64_t* __restrict__ off,
     const int32_t* __restrict__ active, const uint8_t* __restrict__ state,
     const double* __restrict__ vel, const double* __restrict__ m64,
     int32_t* __restrict__ ccnt, double* __restrict__ csum) {
-    __shared__ int shs[kThreads / kWave];
-    __shared__ double shd[2 * (kThreads / kWave)];
-    for (int ci = blockIdx.x; ci < n_chunks; ci += gridDim.x) {
+    const int lane = __lane_id();
+    const int64_t nw = static_cast<int64_t>(gridDim.x) * (kThreads / kWave);
+    for (int64_t ci = (static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x) / kWave;
+         ci < n_chunks; ci += nw) {
         const int2 ch = chunks[ci];
-        if (!active[ch.x]) continue;  // block-uniform
+        if (!active[ch.x]) continue;  // wave-uniform
         int64_t b, e;
         chunk_range(off, ch, b, e);
-        int packed = 0;  // kept count | just-removed count << 16
+        int packed = 0;  // kept count | just-removed count << 16 (at most kChunk each)
         double acc[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-        for (int64_t k = b + threadIdx.x; k < e; k += kThreads) {
+        for (int64_t k = b + lane; k < e; k += kWave) {
             const uint8_t st = state[k];
             if (st == kBound) {
                 packed += 1;
@@ -237,18 +220,23 @@ __global__ __launch_bounds__(kThreads) void k_chunk_sums(
                 packed += 1 << 16;
             }
         }
-        int excl;
-        const int tot = block_excl_scan(packed, excl, shs);
-        for (int q = 0; q < 4; ++q) {
-            block_dd_sum(acc[q][0], acc[q][1], shd);
-            if (threadIdx.x == 0) {
-                csum[8 * static_cast<int64_t>(ci) + 2 * q] = acc[q][0];
-                csum[8 * static_cast<int64_t>(ci) + 2 * q + 1] = acc[q][1];
+        for (int m = kWave / 2; m >= 1; m >>= 1) {
+            packed += __shfl_xor(packed, m, kWave);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double h2 = __shfl_xor(acc[q][0], m, kWave);
+                const double l2 = __shfl_xor(acc[q][1], m, kWave);
+                dd_add_dd(acc[q][0], acc[q][1], h2, l2);
             }
         }
-        if (threadIdx.x == 0) {
-            ccnt[2 * ci] = tot & 0xffff;
-            ccnt[2 * ci + 1] = tot >> 16;
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                csum[8 * ci + 2 * q] = acc[q][0];
+                csum[8 * ci + 2 * q + 1] = acc[q][1];
+            }
+            ccnt[2 * ci] = packed & 0xffff;
+            ccnt[2 * ci + 1] = packed >> 16;
         }
     }
 }
@@ -1032,7 +1020,7 @@ hipStream_t capture_stream() {
 
 struct PassLaunch {
     int64_t H, nm;
-    int grid_chunks, grid_retest;
+    int grid_chunks, grid_chunk_waves, grid_retest;
     const PotPlan* plan;
     const SymPlan* splan;
     UnbWs w;
@@ -1049,8 +1037,9 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
     const int nc = static_cast<int>(w.n_chunks);
     const int tile = L.plan->tests_per_lane * kWave;
     if (nc > 0) {
-        k_chunk_sums<<<L.grid_chunks, kThreads, 0, st>>>(w.chunks, nc, w.off, w.active, w.state,
-                                                         w.vel, w.m64, w.ccnt, w.csum);
+        k_chunk_sums<<<L.grid_chunk_waves, kThreads, 0, st>>>(w.chunks, nc, w.off, w.active,
+                                                              w.state, w.vel, w.m64, w.ccnt,
+                                                              w.csum);
         HALMA_LAUNCH_CHECK();
     }
     k_halo_sets<<<static_cast<unsigned>((L.H + kThreads / kWave - 1) / (kThreads / kWave)),
@@ -1251,6 +1240,9 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     L.nm = nm;
     const int cus = std::max(device_cu_count(), 1);
     L.grid_chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(n_chunks, cus * 8)));
+    // k_chunk_sums: a wavefront per chunk, four per block
+    L.grid_chunk_waves =
+        static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n_chunks + 3) / 4, cus * 8)));
     // re-tests: blocks (large live sets) and wavefronts (small ones) over the band lists
     L.grid_retest = cus * 4;
     L.plan = &plan;
