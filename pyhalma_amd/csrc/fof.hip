@@ -8,10 +8,11 @@
 //    within ll under the FP64 test (|dx|,|dy|,|dz| <= s(1+1e-9) after rounding of the
 //    cell index), so a sub-cell is one vertex of the cell graph and needs no pair test;
 //    every link spans at most 2 sub-cells per axis (ll/s < 1.733);
-// 3. a stable LSD radix sort of the particles by cell key (x, y, z bits packed, z lowest)
-//    lists them cell by cell: cells of one column (x, y) are consecutive and ordered by
-//    z, columns are in (x, y) order, particles ascending within a cell; coordinates are
-//    gathered into that order as one vector per particle, each cell is one 8-byte record
+// 3. a stable LSD radix sort of the particles by cell key (the mixed-radix index
+//    (x * ny + y) * nz + z) lists them cell by cell: cells of one column (x, y) are
+//    consecutive and ordered by z, columns are in (x, y) order, particles ascending within
+//    a cell; coordinates are gathered into that order as one vector per particle, each
+//    cell is one 8-byte record
 //    (first position, z), and each column enters a hash table whose 16-byte entry holds
 //    its key, cell count, first cell and (between the rounds) shared root, so that a
 //    neighbour column costs one line; the table is sized from the column count;
@@ -202,12 +203,16 @@ struct Packed<double> {
 };
 
 
-// Cell grid of one call: origin, inverse side, and the bits of each cell coordinate in
-// the packed key (x | y | z, z lowest).
+// Cell grid of one call: origin, inverse side, the cell count of each axis, and the bits
+// of the x and y cell coordinates in a column's packed table key (x << by | y).  The sort
+// key of a cell is its mixed-radix index (x * ny + y) * nz + z: the same (x, y, z) order
+// as packed bit fields, in as few bits as the grid needs (31 instead of 33 for a box of
+// 1114^3 cells, so one 32-bit sort instead of two).
 struct Grid {
     double lo[3];
     double inv_s;
-    int bx, by, bz;
+    uint64_t n[3];
+    int bx, by;
 };
 
 template <typename T>
@@ -215,20 +220,18 @@ __device__ __forceinline__ uint64_t cell_key(T xi, T yi, T zi, const Grid& g, bo
     const double q[3] = {(static_cast<double>(xi) - g.lo[0]) * g.inv_s,
                          (static_cast<double>(yi) - g.lo[1]) * g.inv_s,
                          (static_cast<double>(zi) - g.lo[2]) * g.inv_s};
-    const int bits[3] = {g.bx, g.by, g.bz};
-    uint64_t key = 0;
+    uint64_t c[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const bool ok = q[d] >= 0.0 && q[d] < kCellLimit;  // also catches NaN
-        uint64_t c = ok ? static_cast<uint64_t>(floor(q[d])) : 0;
-        if (c >> bits[d]) {  // beyond the grid the box fixed: cannot happen for finite data
+        c[d] = ok ? static_cast<uint64_t>(floor(q[d])) : 0;
+        if (c[d] >= g.n[d]) {  // beyond the grid the box fixed: cannot happen for finite data
             bad = true;
-            c = 0;
+            c[d] = 0;
         }
         bad |= !ok;
-        key = (key << bits[d]) | c;
     }
-    return key;
+    return (c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
 }
 
 // per particle: the low and high 32 bits of its cell key, and a packed copy of its
@@ -276,15 +279,15 @@ __global__ __launch_bounds__(256) void k_fof_sorted(const int32_t* __restrict__ 
 }
 
 // per sorted position j: the cell / column heads (bit 0 / bit 32: the key / the column
-// part of the key differs from position j - 1)
-__global__ void k_fof_heads(const uint64_t* __restrict__ skey, int64_t n, int bz,
+// part of the key, key / nz, differs from position j - 1)
+__global__ void k_fof_heads(const uint64_t* __restrict__ skey, int64_t n, uint64_t nz,
                             int64_t* __restrict__ heads) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const uint64_t k = skey[j];
     const uint64_t kp = j > 0 ? skey[j - 1] : ~k;
     const int64_t cell = k != kp ? 1 : 0;
-    const int64_t col = (k >> bz) != (kp >> bz) ? 1 : 0;
+    const int64_t col = j == 0 || (k / nz) != (kp / nz) ? 1 : 0;
     heads[j] = cell | (col << 32);
 }
 
@@ -309,11 +312,12 @@ __global__ void k_clear_cols(const int64_t* __restrict__ totals, ColEntry* __res
 }
 
 // Cells and columns from the sorted keys.  Cell c: its record (first sorted position,
-// z), its smallest particle index (the sort is stable, so the first one), the union-find
-// root; column q: its first cell and its key.
+// z = key mod nz), its smallest particle index (the sort is stable, so the first one),
+// the union-find root; column q: its first cell and its packed key (x << by | y, from
+// key / nz = x * ny + y).
 __global__ __launch_bounds__(256) void k_fof_cells(
     const uint64_t* __restrict__ skey, const int32_t* __restrict__ perm,
-    const int64_t* __restrict__ heads, const int64_t* __restrict__ before, int64_t n, int bz,
+    const int64_t* __restrict__ heads, const int64_t* __restrict__ before, int64_t n, Grid g,
     int2* __restrict__ crec, int* __restrict__ cell_min, int* __restrict__ parent,
     int* __restrict__ comp_min, int* __restrict__ col_cell, uint64_t* __restrict__ col_key) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -322,15 +326,17 @@ __global__ __launch_bounds__(256) void k_fof_cells(
     const int c = static_cast<int>((b & 0xffffffff) + (h & 0xffffffff)) - 1;
     const int q = static_cast<int>((b >> 32) + (h >> 32)) - 1;
     const uint64_t k = skey[j];
+    const uint64_t ck = k / g.n[2];  // x * ny + y
     if (h & 0xffffffff) {
-        crec[c] = make_int2(static_cast<int>(j), static_cast<int>(k & ((1ull << bz) - 1)));
+        crec[c] = make_int2(static_cast<int>(j), static_cast<int>(k - ck * g.n[2]));
         cell_min[c] = perm[j];
         parent[c] = c;
         comp_min[c] = INT32_MAX;
     }
     if (h >> 32) {
         col_cell[q] = c;
-        col_key[q] = k >> bz;
+        const uint64_t cx = ck / g.n[1];
+        col_key[q] = (cx << g.by) | (ck - cx * g.n[1]);
     }
     if (j == n - 1) {
         crec[c + 1] = make_int2(static_cast<int>(n), INT32_MAX);
@@ -910,7 +916,6 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     const double side = ll / sqrt(3.0) * (1.0 - 1e-6);
     Grid g;
     g.inv_s = 1.0 / side;
-    int bits[3];
     for (int d = 0; d < 3; ++d) {
         g.lo[d] = hb[d];
         const double span = (hb[3 + d] - hb[d]) * g.inv_s;
@@ -918,12 +923,14 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
             return set_error(HALMA_E_LIMIT,
                              "friends-of-friends: coordinates span more than 2^21 sub-cells "
                              "of side ll/sqrt(3) on an axis, or are not finite");
-        bits[d] = bits_for(static_cast<int64_t>(floor(span)) + 1);
+        g.n[d] = static_cast<uint64_t>(floor(span)) + 1;  // at most 2^21
     }
-    g.bx = bits[0];
-    g.by = bits[1];
-    g.bz = bits[2];
-    const int kbits = g.bx + g.by + g.bz;
+    g.bx = bits_for(static_cast<int64_t>(g.n[0]));
+    g.by = bits_for(static_cast<int64_t>(g.n[1]));
+    // bits of the mixed-radix keys 0 .. nx * ny * nz - 1 (the product is at most 2^63)
+    const uint64_t ncells = g.n[0] * g.n[1] * g.n[2];
+    int kbits = 0;
+    while (kbits < 64 && (ncells - 1) >> kbits) ++kbits;
     auto* pk = static_cast<typename Packed<T>::type*>(w.pk);
     k_fof_keys<T><<<blocks_for(n), 256, 0, st>>>(x, y, z, n, g, w.klo, w.khi, w.err, pk);
     HALMA_LAUNCH_CHECK();
@@ -943,7 +950,7 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     V* spk = static_cast<V*>(w.spk);
     k_fof_sorted<T><<<blocks_for(n), 256, 0, st>>>(perm, n, g, pk, spk, w.skey);
     HALMA_LAUNCH_CHECK();
-    k_fof_heads<<<blocks_for(n), 256, 0, st>>>(w.skey, n, g.bz, w.heads);
+    k_fof_heads<<<blocks_for(n), 256, 0, st>>>(w.skey, n, g.n[2], w.heads);
     HALMA_LAUNCH_CHECK();
     rc = exclusive_scan_i64(w.heads, w.before, n, w.totals, w.scan_ws, st);
     if (rc) return rc;
@@ -951,7 +958,7 @@ int fof_run(const T* x, const T* y, const T* z, int64_t n, double ll, int* label
     HALMA_LAUNCH_CHECK();
     k_clear_cols<<<grid_cells, 256, 0, st>>>(w.totals, w.tab);
     HALMA_LAUNCH_CHECK();
-    k_fof_cells<<<blocks_for(n), 256, 0, st>>>(w.skey, perm, w.heads, w.before, n, g.bz,
+    k_fof_cells<<<blocks_for(n), 256, 0, st>>>(w.skey, perm, w.heads, w.before, n, g,
                                                w.crec, w.cell_min, w.parent, w.comp_min,
                                                w.col_cell, w.col_key);
     HALMA_LAUNCH_CHECK();
