@@ -234,24 +234,27 @@ struct Src {
 // reductions with two barriers instead of forty-five.
 constexpr int kGrpQ = 8 * 2 + 6 + 1;
 
+// Only the first nw wavefronts hold members (the others skip the trees and are left out
+// of the fold), so a small group pays for the trees of the wavefronts it uses.
 __device__ void block_group_reduce(double (&acc)[8][2], double (&lo)[3], double (&hi)[3],
-                                   int& gexp, double* sh /* kGrpQ * kMaxWaves */) {
-    for (int k = 0; k < 8; ++k)
-        for (int s = 32; s >= 1; s >>= 1) {
-            const double h2 = __shfl_down(acc[k][0], s, kWave);
-            const double l2 = __shfl_down(acc[k][1], s, kWave);
-            dd_add_dd(acc[k][0], acc[k][1], h2, l2);
-        }
-    for (int d = 0; d < 3; ++d)
-        for (int s = 32; s >= 1; s >>= 1) {
-            lo[d] = fmin(lo[d], __shfl_down(lo[d], s, kWave));
-            hi[d] = fmax(hi[d], __shfl_down(hi[d], s, kWave));
-        }
-    for (int s = 32; s >= 1; s >>= 1) gexp = min(gexp, __shfl_down(gexp, s, kWave));
+                                   int& gexp, double* sh /* kGrpQ * kMaxWaves */, int nw) {
     const int w = threadIdx.x / kWave;
-    const int nw = static_cast<int>(blockDim.x) / kWave;
+    if (w < nw) {
+        for (int k = 0; k < 8; ++k)
+            for (int s = 32; s >= 1; s >>= 1) {
+                const double h2 = __shfl_down(acc[k][0], s, kWave);
+                const double l2 = __shfl_down(acc[k][1], s, kWave);
+                dd_add_dd(acc[k][0], acc[k][1], h2, l2);
+            }
+        for (int d = 0; d < 3; ++d)
+            for (int s = 32; s >= 1; s >>= 1) {
+                lo[d] = fmin(lo[d], __shfl_down(lo[d], s, kWave));
+                hi[d] = fmax(hi[d], __shfl_down(hi[d], s, kWave));
+            }
+        for (int s = 32; s >= 1; s >>= 1) gexp = min(gexp, __shfl_down(gexp, s, kWave));
+    }
     __syncthreads();
-    if (__lane_id() == 0) {
+    if (w < nw && __lane_id() == 0) {
         double* o = sh + w * kGrpQ;
         for (int k = 0; k < 8; ++k) {
             o[2 * k] = acc[k][0];
@@ -310,7 +313,12 @@ __global__ __launch_bounds__(kBig) void k_group_reduce(Src s, const int64_t* __r
         for (int k = 0; k < 8; ++k) acc[k][0] = acc[k][1] = 0.0;
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         int gexp = 4096;
-        for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+        // wavefronts that take members: enough for four or more members per lane
+        const int64_t ng = e - b;
+        const int nw = static_cast<int>(std::min<int64_t>(
+            std::max<int64_t>((ng + 4 * kWave - 1) / (4 * kWave), 1), blockDim.x / kWave));
+        const int nthr = nw * kWave;
+        for (int64_t k = b + threadIdx.x; threadIdx.x < nthr && k < e; k += nthr) {
             const int64_t i = mem[k];
             const double m = ldf(s.m, s.mass_t, i);
             const double p[3] = {ldf(s.x, s.pos_t, i), ldf(s.y, s.pos_t, i), ldf(s.z, s.pos_t, i)};
@@ -329,7 +337,7 @@ __global__ __launch_bounds__(kBig) void k_group_reduce(Src s, const int64_t* __r
             dd_add(acc[7][0], acc[7][1], fabs(m));
             gexp = min(gexp, mass_grain_exp(m));
         }
-        block_group_reduce(acc, lo, hi, gexp, sh);
+        block_group_reduce(acc, lo, hi, gexp, sh, nw);
         const double* bl = lo;
         const double* bh = hi;
         if (threadIdx.x == 0) {
