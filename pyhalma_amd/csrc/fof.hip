@@ -559,6 +559,18 @@ __device__ __forceinline__ void candidate(int c, int& rc, int sc, int nc, int d,
         if (rc < 0) rc = find_ro(parent, c);
         if (rc == find_ro(parent, d)) return;
         if (pairs > kLaneTest) {
+            // neighbouring dense cells nearly always hold a close pair among their first
+            // members: a 2 x 2 sample first, the whole pair to a wavefront only without one
+            const int sa = nc < 2 ? nc : 2, sb = nd < 2 ? nd : 2;
+            for (int b = 0; b < sb; ++b) {
+                const V pb = spk[sd + b];
+                for (int a = 0; a < sa; ++a)
+                    if (within(spk[sc + a], pb, ll2)) {
+                        uf_union(parent, c, d);
+                        rc = find_ro(parent, c);
+                        return;
+                    }
+            }
             const unsigned long long at = atomicAdd(n_big, 1ull);
             if (static_cast<int64_t>(at) < big_cap) {
                 big[at] = make_int2(c, d);
