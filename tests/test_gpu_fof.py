@@ -65,6 +65,45 @@ def test_fof_vs_oracle(gpu, dtype, kind):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fof_long_z_columns(gpu, dtype):
+    """Lines of particles along z, so that one column of sub-cells holds hundreds of cells
+    and the link lanes cut it into segments: links must cross every segment boundary.
+    Spacings below, at and just above ll (inclusive test; the margin of 1e-3 ll outlives
+    FP32 rounding of the z values), dense runs (many particles per cell)
+    split by one gap wider than ll, and pairs of lines half a linking length apart in x
+    (links between neighbouring columns).  The group counts are fixed by construction;
+    the labels must equal the oracle's."""
+    ll = 0.5
+    lines, expect = [], 0
+    for j, (step, n, linked) in enumerate([(0.95 * ll, 300, True), (ll, 300, True),
+                                          (ll * 1.001, 300, False),
+                                          (0.3 * ll, 400, True)]):
+        z = np.arange(n) * step
+        lines.append(np.stack([np.full(n, 100.0 * j), np.zeros(n), z], 1))
+        expect += 1 if linked else n
+    # dense runs: 40 particles in each of 30 consecutive sub-cells, then a gap of 1.2 ll
+    rng = np.random.default_rng(7)
+    side = ll / np.sqrt(3.0)
+    zr = np.concatenate([c * side + rng.uniform(0.05, 0.95, 40) * side for c in range(30)])
+    run = np.stack([np.full(zr.size, 500.0), np.full(zr.size, 0.01), zr], 1)
+    lines += [run, run + [0.0, 0.0, zr.max() + 1.2 * ll - zr.min()]]
+    expect += 2
+    # two lines 0.5 ll apart in x, each with gaps of 1.5 ll that only the other bridges
+    za = np.arange(0, 200) * 1.5 * ll
+    lines += [np.stack([np.full(za.size, 900.0), np.zeros(za.size), za], 1),
+              np.stack([np.full(za.size, 900.0 + 0.5 * ll), np.zeros(za.size),
+                        za + 0.75 * ll], 1)]
+    expect += 1
+    d = np.concatenate(lines).astype(dtype)
+    d = d[rng.permutation(len(d))]
+    want = oracle.fof_labels_c(d[:, 0], d[:, 1], d[:, 2], ll)
+    print(f"dtype={np.dtype(dtype).name} n={len(d)} groups={len(np.unique(want))}", flush=True)
+    assert len(np.unique(want)) == expect
+    got = _gpu_labels(d[:, 0], d[:, 1], d[:, 2], ll)
+    assert np.array_equal(got, want)
+
+
 def test_fof_clumps_workload_and_repeat(gpu):
     from pyhalma_amd import synth
 
