@@ -32,10 +32,6 @@ inline unsigned blocks_for(int64_t n) {
     return static_cast<unsigned>((n + kThreads - 1) / kThreads);
 }
 
-__device__ __forceinline__ double ldf(const void* p, int dtype, int64_t i) {
-    return dtype == HALMA_F64 ? static_cast<const double*>(p)[i]
-                              : static_cast<double>(static_cast<const float*>(p)[i]);
-}
 
 // v * m as the reference forms it: in FP32 when both arrays are FP32 (numba keeps the
 // product of two float32 values in float32), in FP64 otherwise

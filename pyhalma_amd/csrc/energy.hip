@@ -26,11 +26,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ __forceinline__ double ld_vel(const void* p, int dtype, int64_t i) {
-    return dtype == HALMA_F64 ? static_cast<const double*>(p)[i]
-                              : static_cast<double>(static_cast<const float*>(p)[i]);
-}
-
 struct TestArgs {
     const int* tst_halo;
     const int64_t* src_off;  // device copy, absolute (into sm, sx, sy, sz)
@@ -54,9 +49,9 @@ struct TestArgs {
 };
 
 __device__ __forceinline__ double kin_of(const TestArgs& a, int64_t i, int h) {
-    const double dvx = ld_vel(a.vx, a.vel_t, i) - a.halo_v[3 * h + 0];
-    const double dvy = ld_vel(a.vy, a.vel_t, i) - a.halo_v[3 * h + 1];
-    const double dvz = ld_vel(a.vz, a.vel_t, i) - a.halo_v[3 * h + 2];
+    const double dvx = ldf(a.vx, a.vel_t, i) - a.halo_v[3 * h + 0];
+    const double dvy = ldf(a.vy, a.vel_t, i) - a.halo_v[3 * h + 1];
+    const double dvz = ldf(a.vz, a.vel_t, i) - a.halo_v[3 * h + 2];
     return 0.5 * ((dvx * dvx + dvy * dvy) + dvz * dvz);
 }
 

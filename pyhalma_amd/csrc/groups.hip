@@ -45,9 +45,9 @@ inline unsigned blocks_for(int64_t n, int t = kThreads) {
     return static_cast<unsigned>((n + t - 1) / t);
 }
 
-__device__ __forceinline__ double ldf(const void* p, int dtype, int64_t i) {
-    return dtype == HALMA_F64 ? static_cast<const double*>(p)[i]
-                              : static_cast<double>(static_cast<const float*>(p)[i]);
+
+inline int common_dtype(int a, int b, int c) {
+    return a == b && b == c ? (a == HALMA_F32 ? 1 : 2) : 0;
 }
 
 // ------------------------------------------------------------------ CSR build ----
@@ -300,6 +300,7 @@ __device__ void block_group_reduce(double (&acc)[8][2], double (&lo)[3], double 
     __syncthreads();
 }
 
+template <int DT>
 __global__ __launch_bounds__(kBig) void k_group_reduce(Src s, const int64_t* __restrict__ off,
                                                            const int32_t* __restrict__ mem,
                                                            int64_t G,
@@ -318,24 +319,53 @@ __global__ __launch_bounds__(kBig) void k_group_reduce(Src s, const int64_t* __r
         const int nw = static_cast<int>(std::min<int64_t>(
             std::max<int64_t>((ng + 4 * kWave - 1) / (4 * kWave), 1), blockDim.x / kWave));
         const int nthr = nw * kWave;
-        for (int64_t k = b + threadIdx.x; threadIdx.x < nthr && k < e; k += nthr) {
-            const int64_t i = mem[k];
-            const double m = ldf(s.m, s.mass_t, i);
-            const double p[3] = {ldf(s.x, s.pos_t, i), ldf(s.y, s.pos_t, i), ldf(s.z, s.pos_t, i)};
-            dd_add(acc[0][0], acc[0][1], m);
+        // four members per thread in flight: their indices, then mass and position, then
+        // velocity; every accumulator takes the members in the same order as one member
+        // at a time
+        for (int64_t k0 = b + threadIdx.x; threadIdx.x < nthr && k0 < e; k0 += 4 * nthr) {
+            int32_t iv[4];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                dd_add(acc[1 + d][0], acc[1 + d][1], m * p[d]);
-                lo[d] = fmin(lo[d], p[d]);
-                hi[d] = fmax(hi[d], p[d]);
+            for (int u = 0; u < 4; ++u) {
+                const int64_t k = k0 + u * nthr;
+                iv[u] = k < e ? mem[k] : -1;
             }
-            if (s.vx) {
-                dd_add(acc[4][0], acc[4][1], m * ldf(s.vx, s.vel_t, i));
-                dd_add(acc[5][0], acc[5][1], m * ldf(s.vy, s.vel_t, i));
-                dd_add(acc[6][0], acc[6][1], m * ldf(s.vz, s.vel_t, i));
+            double mu[4], f[4][3];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (iv[u] < 0) continue;
+                mu[u] = ldt<DT>(s.m, s.mass_t, iv[u]);
+                f[u][0] = ldt<DT>(s.x, s.pos_t, iv[u]);
+                f[u][1] = ldt<DT>(s.y, s.pos_t, iv[u]);
+                f[u][2] = ldt<DT>(s.z, s.pos_t, iv[u]);
             }
-            dd_add(acc[7][0], acc[7][1], fabs(m));
-            gexp = min(gexp, mass_grain_exp(m));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (iv[u] < 0) continue;
+                const double m = mu[u];
+                dd_add(acc[0][0], acc[0][1], m);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    dd_add(acc[1 + d][0], acc[1 + d][1], m * f[u][d]);
+                    lo[d] = fmin(lo[d], f[u][d]);
+                    hi[d] = fmax(hi[d], f[u][d]);
+                }
+                dd_add(acc[7][0], acc[7][1], fabs(m));
+                gexp = min(gexp, mass_grain_exp(m));
+            }
+            if (!s.vx) continue;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (iv[u] < 0) continue;
+                f[u][0] = ldt<DT>(s.vx, s.vel_t, iv[u]);
+                f[u][1] = ldt<DT>(s.vy, s.vel_t, iv[u]);
+                f[u][2] = ldt<DT>(s.vz, s.vel_t, iv[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (iv[u] < 0) continue;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) dd_add(acc[4 + d][0], acc[4 + d][1], mu[u] * f[u][d]);
+            }
         }
         block_group_reduce(acc, lo, hi, gexp, sh, nw);
         const double* bl = lo;
@@ -362,12 +392,21 @@ __global__ __launch_bounds__(kBig) void k_group_reduce(Src s, const int64_t* __r
         }
         __syncthreads();
         double r2 = -1.0;
-        for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
-            const int64_t i = mem[k];
-            const double dx = cen[0] - ldf(s.x, s.pos_t, i);
-            const double dy = cen[1] - ldf(s.y, s.pos_t, i);
-            const double dz = cen[2] - ldf(s.z, s.pos_t, i);
-            r2 = fmax(r2, (dx * dx + dy * dy) + dz * dz);
+        for (int64_t k0 = b + threadIdx.x; k0 < e; k0 += 4 * static_cast<int64_t>(blockDim.x)) {
+            int32_t iv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t k = k0 + u * static_cast<int64_t>(blockDim.x);
+                iv[u] = k < e ? mem[k] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (iv[u] < 0) continue;
+                const double dx = cen[0] - ldt<DT>(s.x, s.pos_t, iv[u]);
+                const double dy = cen[1] - ldt<DT>(s.y, s.pos_t, iv[u]);
+                const double dz = cen[2] - ldt<DT>(s.z, s.pos_t, iv[u]);
+                r2 = fmax(r2, (dx * dx + dy * dy) + dz * dz);
+            }
         }
         r2 = block_max(r2, sh);
         if (threadIdx.x == 0) out[g * kRedFields + 7] = r2 > 0.0 ? sqrt(r2) : 0.0;
@@ -441,10 +480,11 @@ __device__ __forceinline__ void grids_of(const PeakArgs& a, int64_t g, Grid1* gr
     }
 }
 
+template <int DT = 0>
 __device__ __forceinline__ int cell_of(const PeakArgs& a, const Grid1* gr, int64_t i) {
-    const int ix = nearest_node(gr[0], ldf(a.s.x, a.s.pos_t, i));
-    const int iy = nearest_node(gr[1], ldf(a.s.y, a.s.pos_t, i));
-    const int iz = nearest_node(gr[2], ldf(a.s.z, a.s.pos_t, i));
+    const int ix = nearest_node(gr[0], ldt<DT>(a.s.x, a.s.pos_t, i));
+    const int iy = nearest_node(gr[1], ldt<DT>(a.s.y, a.s.pos_t, i));
+    const int iz = nearest_node(gr[2], ldt<DT>(a.s.z, a.s.pos_t, i));
     return (ix * kNodes + iy) * kNodes + iz;
 }
 
@@ -525,6 +565,7 @@ __device__ int pick_peak(const PeakArgs& a, const Grid1* gr, int64_t g, int64_t 
     return winner;
 }
 
+template <int DT>
 __global__ __launch_bounds__(kBig) void k_density_peak(PeakArgs a) {
     __shared__ int skey[kTable];
     __shared__ double sval[kTable];
@@ -547,25 +588,44 @@ __global__ __launch_bounds__(kBig) void k_density_peak(PeakArgs a) {
         Grid1 gr[3];
         grids_of(a, g, gr);
         const int64_t b = a.off[g], e = a.off[g + 1];
-        for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+        for (int64_t k0 = b + threadIdx.x; k0 < e; k0 += 4 * static_cast<int64_t>(blockDim.x)) {
             // a group with too many distinct nodes goes to the dense grid: stop at once
             // (probing a full table would cost up to kTable probes per member)
             if (__hip_atomic_load(&overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
                 break;
-            const int64_t i = a.mem[k];
-            const int node = cell_of(a, gr, i);
-            const double m = ldf(a.s.m, a.s.mass_t, i);
-            uint32_t h = (static_cast<uint32_t>(node) * 2654435761u) & (kTable - 1);
-            for (int probe = 0; probe < kTable; ++probe) {
-                const int old = atomicCAS(&skey[h], -1, node);
-                if (old == -1 || old == node) {
-                    if (old == -1 && atomicAdd(&used, 1) >= kTableFill)
-                        __hip_atomic_store(&overflow, 1, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                    atomicAdd(&sval[h], m);
-                    break;
+            // four members per thread in flight: their indices, then their fields
+            int32_t iv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t k = k0 + u * static_cast<int64_t>(blockDim.x);
+                iv[u] = k < e ? a.mem[k] : -1;
+            }
+            int nodes[4];
+            double ms[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (iv[u] < 0) continue;
+                nodes[u] = cell_of<DT>(a, gr, iv[u]);
+                ms[u] = ldt<DT>(a.s.m, a.s.mass_t, iv[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (iv[u] < 0 || __hip_atomic_load(&overflow, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP))
+                    break;  // past the group's end, or the group goes to the dense grid
+                const int node = nodes[u];
+                uint32_t h = (static_cast<uint32_t>(node) * 2654435761u) & (kTable - 1);
+                for (int probe = 0; probe < kTable; ++probe) {
+                    const int old = atomicCAS(&skey[h], -1, node);
+                    if (old == -1 || old == node) {
+                        if (old == -1 && atomicAdd(&used, 1) >= kTableFill)
+                            __hip_atomic_store(&overflow, 1, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        atomicAdd(&sval[h], ms[u]);
+                        break;
+                    }
+                    h = (h + 1) & (kTable - 1);
                 }
-                h = (h + 1) & (kTable - 1);
             }
         }
         __syncthreads();
@@ -643,20 +703,20 @@ __global__ __launch_bounds__(kBig) void k_dense_pick(PeakArgs a, int first) {
 // one memory line per value.  Packing each particle into one 32-byte (FP32) or 64-byte
 // (FP64) record first makes the gather one line per member; the member-ordered arrays
 // it writes are then read contiguously by every per-group kernel.
-template <typename T>
+template <typename T, int DT>
 __global__ void k_pack_records(const void* x, const void* y, const void* z, int pos_t,
                                const void* vx, const void* vy, const void* vz, int vel_t,
                                const void* m, int mass_t, int64_t n, T* __restrict__ rec) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     T* r = rec + 8 * i;
-    r[0] = static_cast<T>(ldf(x, pos_t, i));
-    r[1] = static_cast<T>(ldf(y, pos_t, i));
-    r[2] = static_cast<T>(ldf(z, pos_t, i));
-    r[3] = static_cast<T>(ldf(m, mass_t, i));
-    r[4] = vx ? static_cast<T>(ldf(vx, vel_t, i)) : T(0);
-    r[5] = vx ? static_cast<T>(ldf(vy, vel_t, i)) : T(0);
-    r[6] = vx ? static_cast<T>(ldf(vz, vel_t, i)) : T(0);
+    r[0] = static_cast<T>(ldt<DT>(x, pos_t, i));
+    r[1] = static_cast<T>(ldt<DT>(y, pos_t, i));
+    r[2] = static_cast<T>(ldt<DT>(z, pos_t, i));
+    r[3] = static_cast<T>(ldt<DT>(m, mass_t, i));
+    r[4] = vx ? static_cast<T>(ldt<DT>(vx, vel_t, i)) : T(0);
+    r[5] = vx ? static_cast<T>(ldt<DT>(vy, vel_t, i)) : T(0);
+    r[6] = vx ? static_cast<T>(ldt<DT>(vz, vel_t, i)) : T(0);
     r[7] = T(0);
 }
 
@@ -677,6 +737,22 @@ __global__ void k_gather_records(const T* __restrict__ rec, const int32_t* __res
         ovy[k] = r[5];
         ovz[k] = r[6];
     }
+}
+
+template <typename T>
+void pack_records(const void* x, const void* y, const void* z, int pos_t, const void* vx,
+                  const void* vy, const void* vz, int vel_t, const void* m, int mass_t,
+                  int64_t n, T* rec, hipStream_t st) {
+    const int dt = common_dtype(pos_t, mass_t, vx ? vel_t : mass_t);
+    if (dt == 1)
+        k_pack_records<T, 1><<<blocks_for(n), kThreads, 0, st>>>(x, y, z, pos_t, vx, vy, vz,
+                                                                  vel_t, m, mass_t, n, rec);
+    else if (dt == 2)
+        k_pack_records<T, 2><<<blocks_for(n), kThreads, 0, st>>>(x, y, z, pos_t, vx, vy, vz,
+                                                                  vel_t, m, mass_t, n, rec);
+    else
+        k_pack_records<T, 0><<<blocks_for(n), kThreads, 0, st>>>(x, y, z, pos_t, vx, vy, vz,
+                                                                  vel_t, m, mass_t, n, rec);
 }
 
 }  // namespace
@@ -798,8 +874,14 @@ int halma_group_reduce(int64_t n_groups, const int64_t* offsets, const int32_t* 
     if (vx && (!vy || !vz)) return set_error(HALMA_E_ARG, "velocities need vx, vy and vz");
     Src s = make_src(x, y, z, pos_dtype, vx, vy, vz, vel_dtype, m, mass_dtype);
     const unsigned grid = static_cast<unsigned>(n_groups < 65536 ? n_groups : 65536);
-    k_group_reduce<<<grid, kBig, 0, static_cast<hipStream_t>(stream)>>>(
-        s, offsets, members, n_groups, centres, out);
+    const int dt = common_dtype(pos_dtype, mass_dtype, vx ? vel_dtype : mass_dtype);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (dt == 1)
+        k_group_reduce<1><<<grid, kBig, 0, st>>>(s, offsets, members, n_groups, centres, out);
+    else if (dt == 2)
+        k_group_reduce<2><<<grid, kBig, 0, st>>>(s, offsets, members, n_groups, centres, out);
+    else
+        k_group_reduce<0><<<grid, kBig, 0, st>>>(s, offsets, members, n_groups, centres, out);
     HALMA_LAUNCH_CHECK();
     return 0;
 }
@@ -838,7 +920,13 @@ int halma_density_peak(int64_t n_groups, const int64_t* offsets, const int32_t* 
     k_fill_ints<<<1, 64, 0, st>>>(w.err, 1);
     HALMA_LAUNCH_CHECK();
     const unsigned grid = static_cast<unsigned>(n_groups < 65536 ? n_groups : 65536);
-    k_density_peak<<<grid, kBig, 0, st>>>(a);
+    const int dt = common_dtype(pos_dtype, mass_dtype, mass_dtype);
+    if (dt == 1)
+        k_density_peak<1><<<grid, kBig, 0, st>>>(a);
+    else if (dt == 2)
+        k_density_peak<2><<<grid, kBig, 0, st>>>(a);
+    else
+        k_density_peak<0><<<grid, kBig, 0, st>>>(a);
     HALMA_LAUNCH_CHECK();
     // the first round of the dense fallback is queued whatever the overflow count (its
     // blocks return at once when no group overflowed), so that one read-back gives both
@@ -902,8 +990,8 @@ int halma_gather_members(const int32_t* members, int64_t n_members, int64_t n_pa
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (out_dtype == HALMA_F64) {
         double* rec = static_cast<double*>(ws);
-        k_pack_records<double><<<blocks_for(n_particles), kThreads, 0, st>>>(
-            x, y, z, pos_dtype, vx, vy, vz, vel_dtype, m, mass_dtype, n_particles, rec);
+        pack_records(x, y, z, pos_dtype, vx, vy, vz, vel_dtype, m, mass_dtype, n_particles, rec,
+                     st);
         HALMA_LAUNCH_CHECK();
         k_gather_records<double><<<blocks_for(n_members), kThreads, 0, st>>>(
             rec, members, n_members, static_cast<double*>(ox), static_cast<double*>(oy),
@@ -911,8 +999,8 @@ int halma_gather_members(const int32_t* members, int64_t n_members, int64_t n_pa
             static_cast<double*>(ovy), static_cast<double*>(ovz));
     } else {
         float* rec = static_cast<float*>(ws);
-        k_pack_records<float><<<blocks_for(n_particles), kThreads, 0, st>>>(
-            x, y, z, pos_dtype, vx, vy, vz, vel_dtype, m, mass_dtype, n_particles, rec);
+        pack_records(x, y, z, pos_dtype, vx, vy, vz, vel_dtype, m, mass_dtype, n_particles, rec,
+                     st);
         HALMA_LAUNCH_CHECK();
         k_gather_records<float><<<blocks_for(n_members), kThreads, 0, st>>>(
             rec, members, n_members, static_cast<float*>(ox), static_cast<float*>(oy),

@@ -85,6 +85,29 @@ __device__ __forceinline__ int wave_sum_i32(int v) {
 
 // Error-free transformation a + b = s + e (Knuth TwoSum), FP64.  Requires no
 // contraction or reassociation (the library is compiled with -ffp-contract=off).
+// Element i of an FP32 or FP64 array (dtype HALMA_F32 or HALMA_F64) as a double.  The raw
+// bits are loaded inside the branch on the type and converted after it: a conversion
+// inside the branch would wait for its load at once, and the loads of several calls would
+// no longer overlap.
+__device__ __forceinline__ double ldf(const void* p, int dtype, int64_t i) {
+    uint64_t bits;
+    if (dtype == HALMA_F64)
+        bits = static_cast<const uint64_t*>(p)[i];
+    else
+        bits = static_cast<const uint32_t*>(p)[i];
+    return dtype == HALMA_F64 ? __longlong_as_double(static_cast<long long>(bits))
+                              : static_cast<double>(__uint_as_float(static_cast<uint32_t>(bits)));
+}
+
+// The same with the type fixed at compile time where every array of a call has one type
+// (DT 1: all FP32, 2: all FP64; 0: each array's type read at run time).
+template <int DT>
+__device__ __forceinline__ double ldt(const void* p, int dtype, int64_t i) {
+    if constexpr (DT == 1) return static_cast<double>(static_cast<const float*>(p)[i]);
+    else if constexpr (DT == 2) return static_cast<const double*>(p)[i];
+    else return ldf(p, dtype, i);
+}
+
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
     s = a + b;
     double bb = s - a;

@@ -28,10 +28,6 @@ constexpr int kRedStride = 16;  // fields per group of the reduction output
 
 inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
 
-__device__ __forceinline__ double ldf(const void* p, int dtype, int64_t i) {
-    return dtype == HALMA_F64 ? static_cast<const double*>(p)[i]
-                              : static_cast<double>(static_cast<const float*>(p)[i]);
-}
 
 struct Pos {
     const void* x;

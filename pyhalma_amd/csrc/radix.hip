@@ -1,11 +1,12 @@
-// Stable LSD radix sort (8-bit digits).  Per pass: a per-block digit histogram, one
-// exclusive scan over the digit-major [digit][block] table, then a stable scatter in
+// Stable LSD radix sort (8-bit digits).  Per pass: a per-block digit histogram, an
+// exclusive scan of each digit's row of the digit-major [digit][block] table (a block per
+// digit, which also writes the digit's total), then a stable scatter (each block adds the
+// exclusive scan of the 256 digit totals, taken in LDS, to its row offsets) in
 // which each block ranks its tile's keys by digit (256 keys per round; ranks among equal
 // digits inside a wavefront from eight ballots, wavefronts of a round offset by their
 // predecessors' counts through LDS), sorts the tile by digit in LDS and writes each
 // digit's run to its place, consecutive threads to consecutive addresses.
 #include "radix.h"
-#include "scan.h"
 
 namespace halma {
 
@@ -47,13 +48,44 @@ __device__ int block_excl_scan(int v, int* sh /* kWaves */) {
     return before + x - v;
 }
 
+// Exclusive scan of one digit's row hist[d][0 .. nb) into offs (a block per digit, 1024
+// entries per round, four consecutive per thread); the row's total goes to tot[d].
+__global__ __launch_bounds__(kThreads) void k_radix_rows(const int32_t* __restrict__ hist,
+                                                         int64_t nb, int32_t* __restrict__ offs,
+                                                         int32_t* __restrict__ tot) {
+    __shared__ int scan_sh[kWaves];
+    __shared__ int last;
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * nb;
+    int run = 0;
+    for (int64_t c = 0; c < nb; c += 4 * kThreads) {
+        const int64_t j0 = c + 4 * threadIdx.x;
+        int v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = j0 + u < nb ? hist[row + j0 + u] : 0;
+        const int local = (v[0] + v[1]) + (v[2] + v[3]);
+        const int excl = block_excl_scan(local, scan_sh);
+        int o = run + excl;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (j0 + u < nb) offs[row + j0 + u] = o;
+            o += v[u];
+        }
+        // the round's total: the last thread's exclusive prefix plus its own entries
+        if (threadIdx.x == kThreads - 1) last = excl + local;
+        __syncthreads();
+        run += last;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = run;
+}
+
 // Stable scatter of one tile: every key's rank among the tile's keys of its digit (rounds
 // in order, wavefronts in order, lanes by ballot), then the tile sorted by digit in LDS,
 // then written out digit run by digit run, consecutive threads to consecutive addresses.
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const uint32_t* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int shift,
-    int64_t nb, const int64_t* __restrict__ offs, uint32_t* __restrict__ keys_out,
-    int32_t* __restrict__ vals_out) {
+    int64_t nb, const int32_t* __restrict__ offs, const int32_t* __restrict__ tot,
+    uint32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out) {
     __shared__ int wcnt[kWaves][256];
     __shared__ int tcnt[256];
     __shared__ int loff[256];
@@ -64,7 +96,10 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const int tid = threadIdx.x;
     const int lane = __lane_id();
     const int w = tid / kWave;
-    run[tid] = offs[static_cast<int64_t>(tid) * nb + blockIdx.x];
+    {  // start of the digit's run: the totals of the smaller digits plus this block's row offset
+        const int before = block_excl_scan(tot[tid], scan_sh);
+        run[tid] = static_cast<int64_t>(before) + offs[static_cast<int64_t>(tid) * nb + blockIdx.x];
+    }
     tcnt[tid] = 0;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile;
@@ -142,8 +177,8 @@ size_t radix_workspace_bytes(int64_t n) {
     const int64_t nb = n > 0 ? (n + kTile - 1) / kTile : 1;
     Carver c(nullptr);
     c.take<int32_t>(256 * nb);
-    c.take<int64_t>(256 * nb);
-    c.take<char>(scan_workspace_bytes(256 * nb));
+    c.take<int32_t>(256 * nb);
+    c.take<int32_t>(256);
     c.take<uint32_t>(n > 0 ? n : 1);
     c.take<int32_t>(n > 0 ? n : 1);
     return c.used;
@@ -156,8 +191,8 @@ int radix_sort_pairs(const uint32_t* keys_in, const int32_t* vals_in, uint32_t* 
     const int64_t nb = (n + kTile - 1) / kTile;
     Carver c(ws);
     int32_t* hist = c.take<int32_t>(256 * nb);
-    int64_t* offs = c.take<int64_t>(256 * nb);
-    void* sws = c.take<char>(scan_workspace_bytes(256 * nb));
+    int32_t* offs = c.take<int32_t>(256 * nb);
+    int32_t* tot = c.take<int32_t>(256);
     uint32_t* tk = c.take<uint32_t>(n);
     int32_t* tv = c.take<int32_t>(n);
     const int passes = bits <= 0 ? 1 : (bits + 7) / 8;
@@ -170,10 +205,10 @@ int radix_sort_pairs(const uint32_t* keys_in, const int32_t* vals_in, uint32_t* 
         int32_t* vout = to_out ? vals_out : tv;
         k_radix_hist<<<static_cast<unsigned>(nb), kThreads, 0, st>>>(kin, n, 8 * p, nb, hist);
         HALMA_LAUNCH_CHECK();
-        int rc = exclusive_scan_i32_to_i64(hist, offs, 256 * nb, nullptr, sws, st);
-        if (rc) return rc;
+        k_radix_rows<<<256, kThreads, 0, st>>>(hist, nb, offs, tot);
+        HALMA_LAUNCH_CHECK();
         k_radix_scatter<<<static_cast<unsigned>(nb), kThreads, 0, st>>>(kin, vin, n, 8 * p, nb,
-                                                                         offs, kout, vout);
+                                                                         offs, tot, kout, vout);
         HALMA_LAUNCH_CHECK();
         kin = kout;
         vin = vout;

@@ -65,10 +65,6 @@ inline unsigned blocks_for(int64_t n, int t = kThreads) {
     return static_cast<unsigned>((n + t - 1) / t);
 }
 
-__device__ __forceinline__ double ldf(const void* p, int dtype, int64_t i) {
-    return dtype == HALMA_F64 ? static_cast<const double*>(p)[i]
-                              : static_cast<double>(static_cast<const float*>(p)[i]);
-}
 
 struct Inputs {
     const void *x, *y, *z;
@@ -80,20 +76,21 @@ struct Inputs {
 };
 
 // member k -> FP32 position/mass (the potential's inputs), FP64 velocity and mass
+template <int DT>
 __global__ void k_gather(Inputs in, const int32_t* __restrict__ mem, int64_t nm,
                          float4* __restrict__ p4, double* __restrict__ vel,
                          double* __restrict__ m64, uint8_t* __restrict__ state) {
     const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= nm) return;
     const int64_t i = mem[k];
-    p4[k] = make_float4(static_cast<float>(ldf(in.x, in.pos_t, i)),
-                        static_cast<float>(ldf(in.y, in.pos_t, i)),
-                        static_cast<float>(ldf(in.z, in.pos_t, i)),
-                        static_cast<float>(ldf(in.m, in.mass_t, i)));
-    vel[3 * k + 0] = ldf(in.vx, in.vel_t, i);
-    vel[3 * k + 1] = ldf(in.vy, in.vel_t, i);
-    vel[3 * k + 2] = ldf(in.vz, in.vel_t, i);
-    m64[k] = ldf(in.m, in.mass_t, i);
+    const double m = ldt<DT>(in.m, in.mass_t, i);
+    p4[k] = make_float4(static_cast<float>(ldt<DT>(in.x, in.pos_t, i)),
+                        static_cast<float>(ldt<DT>(in.y, in.pos_t, i)),
+                        static_cast<float>(ldt<DT>(in.z, in.pos_t, i)), static_cast<float>(m));
+    vel[3 * k + 0] = ldt<DT>(in.vx, in.vel_t, i);
+    vel[3 * k + 1] = ldt<DT>(in.vy, in.vel_t, i);
+    vel[3 * k + 2] = ldt<DT>(in.vz, in.vel_t, i);
+    m64[k] = m;
     state[k] = kBound;
 }
 
@@ -205,19 +202,44 @@ __global__ __launch_bounds__(kThreads) void k_chunk_sums(
         if (!active[ch.x]) continue;  // wave-uniform
         int64_t b, e;
         chunk_range(off, ch, b, e);
+        // every load of the chunk is issued before its first use (the states, then mass
+        // and velocity of the bound members eight positions at a time): three memory
+        // round trips per wavefront instead of two per position; the lane's terms are
+        // added in the same order as one position at a time
+        constexpr int kPer = kChunk / kWave;
+        uint8_t sv[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int64_t k = b + lane + static_cast<int64_t>(j) * kWave;
+            sv[j] = k < e ? state[k] : kGone;
+        }
         int packed = 0;  // kept count | just-removed count << 16 (at most kChunk each)
         double acc[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-        for (int64_t k = b + lane; k < e; k += kWave) {
-            const uint8_t st = state[k];
-            if (st == kBound) {
-                packed += 1;
-                const double m = m64[k];
-                dd_add(acc[0][0], acc[0][1], m);
-                dd_add(acc[1][0], acc[1][1], m * vel[3 * k + 0]);
-                dd_add(acc[2][0], acc[2][1], m * vel[3 * k + 1]);
-                dd_add(acc[3][0], acc[3][1], m * vel[3 * k + 2]);
-            } else if (st == kJustGone) {
-                packed += 1 << 16;
+#pragma unroll
+        for (int j0 = 0; j0 < kPer; j0 += 8) {
+            if (b + static_cast<int64_t>(j0) * kWave >= e) break;  // wave-uniform
+            double mv[8][4];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int64_t k = b + lane + static_cast<int64_t>(j0 + u) * kWave;
+                const bool ld = sv[j0 + u] == kBound;  // positions past e hold kGone
+                mv[u][0] = ld ? m64[k] : 0.0;
+                mv[u][1] = ld ? vel[3 * k + 0] : 0.0;
+                mv[u][2] = ld ? vel[3 * k + 1] : 0.0;
+                mv[u][3] = ld ? vel[3 * k + 2] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (sv[j0 + u] == kBound) {
+                    packed += 1;
+                    const double m = mv[u][0];
+                    dd_add(acc[0][0], acc[0][1], m);
+                    dd_add(acc[1][0], acc[1][1], m * mv[u][1]);
+                    dd_add(acc[2][0], acc[2][1], m * mv[u][2]);
+                    dd_add(acc[3][0], acc[3][1], m * mv[u][3]);
+                } else if (sv[j0 + u] == kJustGone) {
+                    packed += 1 << 16;
+                }
             }
         }
         for (int m = kWave / 2; m >= 1; m >>= 1) {
@@ -380,19 +402,36 @@ __global__ __launch_bounds__(kThreads) void k_chunk_write(
         chunk_range(off, ch, b, e);
         const int64_t kbase = off[ch.x] + ccnt[2 * ci];
         const int64_t gbase = off[ch.x] + ccnt[2 * ci + 1];
+        // the chunk's states and the positions of its live and just-removed members are
+        // loaded before the first scan (one memory round trip instead of two per round)
+        constexpr int kPer = kChunk / kThreads;
+        uint8_t sv[kPer];
+        float4 pv[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int64_t k = b + static_cast<int64_t>(j) * kThreads + threadIdx.x;
+            sv[j] = k < e ? state[k] : kGone;
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int64_t k = b + static_cast<int64_t>(j) * kThreads + threadIdx.x;
+            pv[j] = sv[j] != kGone ? p4[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         int run = 0, rrun = 0;
-        for (int64_t k0 = b; k0 < e; k0 += kThreads) {
-            const int64_t k = k0 + threadIdx.x;
-            const uint8_t st = k < e ? state[k] : kGone;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            if (b + static_cast<int64_t>(j) * kThreads >= e) break;  // block-uniform
+            const int64_t k = b + static_cast<int64_t>(j) * kThreads + threadIdx.x;
+            const uint8_t st = sv[j];
             const int v = st == kBound ? 1 : (st == kJustGone ? (1 << 16) : 0);
             int excl;
             const int tot = block_excl_scan(v, excl, shs);
             if (st == kBound) {
                 const int64_t pos = kbase + run + (excl & 0xffff);
-                src4[pos] = p4[k];
+                src4[pos] = pv[j];
                 cidx[pos] = static_cast<int32_t>(k);
             } else if (st == kJustGone) {
-                rem4[gbase + rrun + (excl >> 16)] = p4[k];
+                rem4[gbase + rrun + (excl >> 16)] = pv[j];
                 state[k] = kGone;
             }
             run += tot & 0xffff;
@@ -1219,7 +1258,9 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     if (rc) return rc;
     const int32_t* mem = members + h_off[0];
     if (nm > 0) {
-        k_gather<<<blocks_for(nm), kThreads, 0, st>>>(in, mem, nm, w.p4, w.vel, w.m64, w.state);
+        const bool one_type = in.pos_t == in.vel_t && in.pos_t == in.mass_t;
+        auto gather = one_type ? (in.pos_t == HALMA_F32 ? k_gather<1> : k_gather<2>) : k_gather<0>;
+        gather<<<blocks_for(nm), kThreads, 0, st>>>(in, mem, nm, w.p4, w.vel, w.m64, w.state);
         HALMA_LAUNCH_CHECK();
         k_member_halo<<<static_cast<unsigned>(H), kThreads, 0, st>>>(w.off, static_cast<int>(H),
                                                                      w.mhalo);
