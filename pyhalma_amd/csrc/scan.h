@@ -1,4 +1,4 @@
-// Device-wide exclusive prefix sums (three-kernel reduce-then-scan; deterministic).
+// Device-wide exclusive prefix sums (reduce-then-scan, two or three kernels; deterministic).
 #pragma once
 #include "halma_common.h"
 
