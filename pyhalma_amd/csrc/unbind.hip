@@ -1059,7 +1059,7 @@ hipStream_t capture_stream() {
 
 struct PassLaunch {
     int64_t H, nm;
-    int grid_chunks, grid_chunk_waves, grid_retest;
+    int grid_chunks, grid_chunk_waves, grid_retest, grid_items;
     const PotPlan* plan;
     const SymPlan* splan;
     UnbWs w;
@@ -1088,10 +1088,7 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
         w.icnt, w.sym_icnt, w.sym, w.stale, w.need_full, w.counters);
     HALMA_LAUNCH_CHECK();
     if (L.H <= kFuseH) {
-        const int64_t most = std::max<int64_t>(L.plan->n_items, w.sym_cap);
-        const unsigned grid = static_cast<unsigned>(
-            std::max<int64_t>(1, std::min<int64_t>(1024, (most + 255) / 256)));
-        k_items_fused<<<grid, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.icnt,
+        k_items_fused<<<L.grid_items, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.icnt,
                                                   w.sym_icnt, w.tst_len, tile, L.plan->n_items,
                                                   w.sym_cap, w.items, w.sym_items, w.itotal,
                                                   w.sym_itotal, w.sym_head);
@@ -1284,6 +1281,11 @@ int unbind_run(int64_t H, const int64_t* h_off, const int32_t* members, Inputs i
     // k_chunk_sums: a wavefront per chunk, four per block
     L.grid_chunk_waves =
         static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n_chunks + 3) / 4, cus * 8)));
+    // item lists: every block scans the per-halo counts itself, so one block per compute
+    // unit (more would repeat that scan in a second round)
+    const int64_t most_items = std::max<int64_t>(plan.n_items, w.sym_cap);
+    L.grid_items = static_cast<int>(
+        std::max<int64_t>(1, std::min<int64_t>(cus, (most_items + 255) / 256)));
     // re-tests: blocks (large live sets) and wavefronts (small ones) over the band lists
     L.grid_retest = cus * 4;
     L.plan = &plan;

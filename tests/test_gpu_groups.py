@@ -45,6 +45,26 @@ def test_groups_edge_cases(gpu):
     assert g.n_groups == 1 and g.to_lists()[0] == list(range(300000))
 
 
+def test_groups_from_labels_large(gpu):
+    """9M labels in 200,000 groups (FoF-style: a group's label is its minimum index): the
+    sort and the prefix sums then take more than 4096 tiles of 2048 elements, the sizes
+    where the prefix sum scans its block sums in a kernel of its own."""
+    from pyhalma_amd import engine
+
+    n, n_grp, min_size = 9_000_000, 200_000, 50
+    rng = np.random.default_rng(11)
+    assign = rng.integers(0, n_grp, n)
+    _, first, inv = np.unique(assign, return_index=True, return_inverse=True)
+    lab = first[inv].astype(np.int32)
+    print(f"case=large n={n} groups={len(first)}", flush=True)
+    g = engine.groups_from_labels(lab, min_size)
+    want = oracle.labels_to_groups(lab, min_size)
+    assert g.n_groups == len(want)
+    np.testing.assert_array_equal(g.sizes(), [len(w) for w in want])
+    got = g.members[: g.n_members].cpu().numpy()
+    np.testing.assert_array_equal(got, np.concatenate(want))
+
+
 @pytest.mark.parametrize("pos_dt", [np.float32, np.float64])
 def test_group_reduce(gpu, pos_dt):
     from pyhalma_amd import engine
