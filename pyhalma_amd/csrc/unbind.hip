@@ -551,12 +551,28 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
     // a kept member whose error bound went above kPotRelMax: its S is re-summed (its
     // decision, already exact, stays the same)
     const bool stale = !gone && !in_band && err > kPotRelMax * S;
+    const int lane = __lane_id();
+    // stale members per halo, one atomic per halo present in the wave (members of a halo
+    // are contiguous, so a wave spans few haloes); the count returned tells whether the
+    // halo already has too many for a re-sum (it then takes a full pass next and
+    // k_retest would skip them): such members are not listed
+    bool list_stale = false;
+    unsigned long long pending = __ballot(stale);
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const int hl = __shfl(h, leader, kWave);
+        const unsigned long long same = __ballot(stale && h == hl);
+        int before = 0;
+        if (lane == leader) before = atomicAdd(a.stale + hl, __popcll(same));
+        before = __shfl(before, leader, kWave);
+        if (stale && h == hl) list_stale = !too_stale(before + __popcll(same), n);
+        pending &= ~same;
+    }
     // members to re-test: of small live sets from the front of the band list, of large
     // ones (a block each in k_retest) from its back; one atomic per wavefront and list
-    const bool push = in_band || stale;
+    const bool push = in_band || list_stale;
     const bool big = n >= kRetestBlock;
     const int32_t val = static_cast<int32_t>(static_cast<uint32_t>(g) | (stale ? kStaleBit : 0u));
-    const int lane = __lane_id();
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const unsigned long long m_small = __ballot(push && !big), m_big = __ballot(push && big);
     if (m_small) {
@@ -573,23 +589,14 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs a) {
         base = __shfl(base, leader, kWave);
         if (push && big) a.band[a.band_cap - 1 - static_cast<int64_t>(base + __popcll(m_big & below))] = val;
     }
-    if (!push && gone) a.state[k] = kJustGone;
-    // removals and stale members per halo, one atomic per halo present in the wave
-    // (members of a halo are contiguous, so a wave spans few haloes)
-    unsigned long long pending = __ballot(gone);
+    if (gone) a.state[k] = kJustGone;
+    // removals per halo, one atomic per halo present in the wave
+    pending = __ballot(gone);
     while (pending) {
         const int leader = __ffsll(static_cast<long long>(pending)) - 1;
         const int hl = __shfl(h, leader, kWave);
         const unsigned long long same = __ballot(gone && h == hl);
-        if (__lane_id() == leader) atomicAdd(a.removed + hl, __popcll(same));
-        pending &= ~same;
-    }
-    pending = __ballot(stale);
-    while (pending) {
-        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
-        const int hl = __shfl(h, leader, kWave);
-        const unsigned long long same = __ballot(stale && h == hl);
-        if (__lane_id() == leader) atomicAdd(a.stale + hl, __popcll(same));
+        if (lane == leader) atomicAdd(a.removed + hl, __popcll(same));
         pending &= ~same;
     }
 }
