@@ -32,7 +32,6 @@ inline unsigned blocks_for(int64_t n) {
     return static_cast<unsigned>((n + kThreads - 1) / kThreads);
 }
 
-
 // v * m as the reference forms it: in FP32 when both arrays are FP32 (numba keeps the
 // product of two float32 values in float32), in FP64 otherwise
 __device__ __forceinline__ double vm_product(const void* v, int vt, const void* m, int mt,

@@ -34,7 +34,6 @@ inline unsigned blocks_for(int64_t n) {
     return static_cast<unsigned>((n + kThreads - 1) / kThreads);
 }
 
-
 struct Pts {
     const void* x;
     const void* y;

@@ -45,7 +45,7 @@ inline unsigned blocks_for(int64_t n, int t = kThreads) {
     return static_cast<unsigned>((n + t - 1) / t);
 }
 
-
+// The ldt<DT> form for a call's arrays: 1 all FP32, 2 all FP64, 0 mixed.
 inline int common_dtype(int a, int b, int c) {
     return a == b && b == c ? (a == HALMA_F32 ? 1 : 2) : 0;
 }

@@ -83,8 +83,6 @@ __device__ __forceinline__ int wave_sum_i32(int v) {
     return v;
 }
 
-// Error-free transformation a + b = s + e (Knuth TwoSum), FP64.  Requires no
-// contraction or reassociation (the library is compiled with -ffp-contract=off).
 // Element i of an FP32 or FP64 array (dtype HALMA_F32 or HALMA_F64) as a double.  The raw
 // bits are loaded inside the branch on the type and converted after it: a conversion
 // inside the branch would wait for its load at once, and the loads of several calls would
@@ -108,6 +106,8 @@ __device__ __forceinline__ double ldt(const void* p, int dtype, int64_t i) {
     else return ldf(p, dtype, i);
 }
 
+// Error-free transformation a + b = s + e (Knuth TwoSum), FP64.  Requires no
+// contraction or reassociation (the library is compiled with -ffp-contract=off).
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
     s = a + b;
     double bb = s - a;

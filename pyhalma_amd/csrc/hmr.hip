@@ -28,7 +28,6 @@ constexpr int kRedStride = 16;  // fields per group of the reduction output
 
 inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
 
-
 struct Pos {
     const void* x;
     const void* y;

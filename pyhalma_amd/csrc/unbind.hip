@@ -65,7 +65,6 @@ inline unsigned blocks_for(int64_t n, int t = kThreads) {
     return static_cast<unsigned>((n + t - 1) / t);
 }
 
-
 struct Inputs {
     const void *x, *y, *z;
     int pos_t;
