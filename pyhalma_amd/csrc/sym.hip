@@ -122,6 +122,11 @@ __device__ __forceinline__ void pair_group(const float4* __restrict__ q0, int g,
 // items the workgroup form cuts an item's latency fourfold.
 constexpr int kGroupItems = 2048;
 
+// Below this many items an item is taken by a 1024-thread workgroup whose sixteen
+// wavefronts take its sixteen tile pairs (k_pot_sym_fine): an item then lasts one tile
+// pair, for the last passes of an unbinding where a few small haloes take full passes.
+constexpr int kFineItems = 1024;
+
 // One item per wavefront: tiles of A in turn against all tiles of B; test side a Kahan
 // sum per tile of B, the tiles' sums added in FP64 in tile order; source side in the
 // wavefront's LDS (ls, lc: kSymK tiles each).  Same arithmetic, in the same order, as
@@ -366,12 +371,136 @@ __global__ __launch_bounds__(256) void k_pot_sym(const float4* __restrict__ src,
         const int64_t nd = uniform_load(n_items_dev);
         n_items = static_cast<int>(nd < n_items ? nd : n_items);
     }
+    if (n_items < kFineItems) return;  // k_pot_sym_fine takes the launch
     if (n_items >= kGroupItems)
         sym_items_wave(src, halos, live_n, items, n_items, part, head,
                        f + wv * kSymK * kTile, f + (kSymK + wv) * kSymK * kTile);
     else
         sym_items_group(src, halos, live_n, items, n_items, part, head, f + wv * kTile,
                         f + (kSymK + wv) * kTile, lds + kSymK * kTile, &next_item);
+}
+
+// One item per 1024-thread workgroup (launches of fewer than kFineItems items): wavefront
+// 4 ta + w takes tile ta of A against tile w of B.  The pair's test-side compensated sum
+// goes to tpart[ta][w], its source-side terms (one per source of tile w, the wave-wide sum
+// over tile ta's tests) to spart[ta][w]; then the test side of each tile of A is added in
+// FP64 over the tiles of B in tile order and the source side of each tile of B is
+// Kahan-added over the tiles of A in order, skipping the pairs the other forms skip: the
+// same operations in the same order as the other forms, so the same bits.
+__global__ __launch_bounds__(1024) void k_pot_sym_fine(const float4* __restrict__ src,
+                                                       const SymDesc* __restrict__ halos,
+                                                       const int* __restrict__ live_n,
+                                                       const int4* __restrict__ items,
+                                                       int n_items,
+                                                       const int64_t* __restrict__ n_items_dev,
+                                                       float2* __restrict__ part,
+                                                       int* __restrict__ head) {
+    static_assert(kSymK * kSymK * kWave == 1024, "one wavefront per tile pair of an item");
+    __shared__ float2 tpart[kSymK * kSymK * kTile];  // 16 KiB
+    __shared__ float spart[kSymK * kSymK * kTile];   // 8 KiB
+    __shared__ int next_item;
+    if (n_items_dev) {
+        const int64_t nd = uniform_load(n_items_dev);
+        n_items = static_cast<int>(nd < n_items ? nd : n_items);
+    }
+    if (n_items >= kFineItems) return;  // k_pot_sym takes the launch
+    const int lane = __lane_id();
+    const int wv = threadIdx.x / kWave;
+    const int ta = wv / kSymK, w = wv % kSymK;
+    float2* tp = tpart + wv * kTile;
+    float* spw = spart + wv * kTile;
+    int k = static_cast<int>(blockIdx.x);
+    for (bool first = true;; first = false) {
+        if (!first) {
+            __syncthreads();  // the previous item's LDS has been read
+            if (threadIdx.x == 0) next_item = atomicAdd(head, 1);
+            __syncthreads();
+            k = static_cast<int>(gridDim.x) + next_item;
+        }
+        if (k >= n_items) break;
+        const int4 it = uniform_load(items + k);
+        const SymDesc d = uniform_load(halos + it.x);
+        const int n = live_n ? __builtin_amdgcn_readfirstlane(uniform_load(live_n + it.x)) : d.n;
+        const int nt = (n + kTile - 1) / kTile;
+        const int nb = (nt + kSymK - 1) / kSymK;
+        const int A = it.y, B = it.z;
+        if (B >= nb || A > B) continue;  // the same for the whole workgroup
+        const float4* sp = src + d.off;
+        float2* pp = part + d.part_off;
+        const int64_t stride = d.n;  // slots are laid out for the planned size
+        const int a = A * kSymK + ta, b = B * kSymK + w;
+        const int j0 = b * kTile;
+        const int cnt = b < nt ? min(kTile, n - j0) : 0;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) spw[r * kWave + lane] = 0.0f;
+        float as[2] = {0.0f, 0.0f}, ac[2] = {0.0f, 0.0f};
+        if (a < nt && cnt > 0 && !(A == B && b < a)) {
+            float xi[2], yi[2], zi[2], mi[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int i = a * kTile + r * kWave + lane;
+                const float4 p = sp[min(i, n - 1)];
+                xi[r] = p.x;
+                yi[r] = p.y;
+                zi[r] = p.z;
+                mi[r] = i < n ? p.w : 0.0f;  // padding lanes add nothing to sources
+            }
+            const bool diag = b == a;
+            for (int g = 0; g < cnt; g += 8) {
+                float ps[2] = {0.0f, 0.0f};
+                float pj[8];
+                if (g + 8 <= cnt)
+                    pair_group<true>(sp + j0, g, cnt, xi, yi, zi, mi, ps, pj);
+                else
+                    pair_group<false>(sp + j0, g, cnt, xi, yi, zi, mi, ps, pj);
+                kahan(as[0], ac[0], ps[0]);
+                kahan(as[1], ac[1], ps[1]);
+                if (!diag) {
+                    const float tot = reduce8(pj, lane);
+                    const int s = (lane >> 3) & 7;
+                    if ((lane & 7) == 0 && g + s < cnt) spw[g + s] = tot;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) tp[r * kWave + lane] = make_float2(as[r], ac[r]);
+        __syncthreads();
+        // test side of A's tiles: slot B (mod the band width), the four source tiles'
+        // compensated sums added in FP64 in tile order, kept as a float pair (hi, lo)
+        const int tslot = B % d.wb;
+        for (int t = threadIdx.x; t < kSymK * kTile; t += blockDim.x) {
+            const int t_a = t / kTile, e = t % kTile;
+            const int i = (A * kSymK + t_a) * kTile + e;
+            if (A * kSymK + t_a >= nt || i >= n) continue;
+            double S = 0.0;
+#pragma unroll
+            for (int t_b = 0; t_b < kSymK; ++t_b) {
+                const float2 v = tpart[(t_a * kSymK + t_b) * kTile + e];
+                S += static_cast<double>(v.x) + static_cast<double>(v.y);
+            }
+            const float hi = static_cast<float>(S);
+            pp[static_cast<int64_t>(tslot) * stride + i] =
+                make_float2(hi, static_cast<float>(S - static_cast<double>(hi)));
+        }
+        // source side of B's tiles: slot A (mod the band width), or the diagonal slot
+        // min(wb, nb); the terms from the tiles of A Kahan-added in tile order
+        const int slot = A < B ? A % d.wb : min(d.wb, nb);
+        for (int t = threadIdx.x; t < kSymK * kTile; t += blockDim.x) {
+            const int t_b = t / kTile, e = t % kTile;
+            const int bb = B * kSymK + t_b;
+            if (bb >= nt) continue;
+            const int jb = bb * kTile;
+            if (e >= min(kTile, n - jb)) continue;
+            float sv = 0.0f, cv = 0.0f;
+            for (int t_a = 0; t_a < kSymK; ++t_a) {
+                const int aa = A * kSymK + t_a;
+                if (aa >= nt) break;
+                if ((A == B && bb < aa) || bb == aa) continue;
+                kahan(sv, cv, spart[(t_a * kSymK + t_b) * kTile + e]);
+            }
+            pp[static_cast<int64_t>(slot) * stride + jb + e] = make_float2(sv, cv);
+        }
+    }
 }
 
 // The slots of one tile launch added into each affected member's FP64 sum (out, cleared
@@ -574,10 +703,20 @@ int launch_pot_sym(const SymPlan& plan, const float4* src4, const SymDesc* d_hal
     if (n_items == 0) return 0;
     int cus = device_cu_count();
     if (cus <= 0) cus = 256;
-    const int grid = std::max(1, std::min(cus * 8, n_items));  // a workgroup or more per item
-    k_pot_sym<<<grid, 256, 0, st>>>(src4, d_halos, live_n, d_items, n_items, n_items_dev, part,
-                                    head);
-    HALMA_LAUNCH_CHECK();
+    // both forms are queued when the item count is known only on the device (the one not
+    // chosen returns at once)
+    if (n_items >= kFineItems || n_items_dev) {
+        const int grid = std::max(1, std::min(cus * 8, n_items));  // a workgroup or more per item
+        k_pot_sym<<<grid, 256, 0, st>>>(src4, d_halos, live_n, d_items, n_items, n_items_dev,
+                                        part, head);
+        HALMA_LAUNCH_CHECK();
+    }
+    if (n_items < kFineItems || n_items_dev) {
+        const int grid = std::max(1, std::min(cus, n_items));
+        k_pot_sym_fine<<<grid, 1024, 0, st>>>(src4, d_halos, live_n, d_items, n_items,
+                                              n_items_dev, part, head);
+        HALMA_LAUNCH_CHECK();
+    }
     return 0;
 }
 
@@ -644,9 +783,16 @@ int halma_pot_sum_self_batched(int64_t n_halo, const int64_t* offsets, const flo
         if (ni > 0) {
             k_sym_clear<<<1, 64, 0, st>>>(w.head);
             HALMA_LAUNCH_CHECK();
-            const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(cus * 8, ni)));
-            k_pot_sym<<<grid, 256, 0, st>>>(w.src4, w.halos, nullptr, w.items + i0,
-                                            static_cast<int>(ni), nullptr, w.part, w.head);
+            if (ni >= kFineItems) {
+                const int grid = static_cast<int>(std::min<int64_t>(cus * 8, ni));
+                k_pot_sym<<<grid, 256, 0, st>>>(w.src4, w.halos, nullptr, w.items + i0,
+                                                static_cast<int>(ni), nullptr, w.part, w.head);
+            } else {
+                const int grid = static_cast<int>(std::min<int64_t>(cus, ni));
+                k_pot_sym_fine<<<grid, 1024, 0, st>>>(w.src4, w.halos, nullptr, w.items + i0,
+                                                      static_cast<int>(ni), nullptr, w.part,
+                                                      w.head);
+            }
             HALMA_LAUNCH_CHECK();
         }
         if (ns > 0) {

@@ -185,6 +185,33 @@ def test_pot_sym_banded_large_haloes(gpu):
         assert np.array_equal(alone, got[a:b])
 
 
+def test_pot_sym_forms_same_bits(gpu):
+    """A halo's sums do not depend on how its launch is split: a 3000-member halo alone
+    (21 block pairs, each taken by a 1024-thread workgroup, one tile pair per wavefront)
+    and beside a 23,040-member halo (1056 block pairs, each taken by a 256-thread
+    workgroup) give the same bits; the large halo is within the bound of the FP64 oracle
+    on a sample of its members."""
+    from pyhalma_amd import engine, synth
+
+    rng = np.random.default_rng(29)
+    sizes = [23040, 3000]
+    parts = [synth.plummer_positions(rng, n, 0.01).astype(np.float32) for n in sizes]
+    P = np.concatenate(parts)
+    m = rng.uniform(1e4, 1e6, len(P)).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    got = engine.pot_sum_self(off, m, P[:, 0], P[:, 1], P[:, 2]).cpu().numpy()
+    for h in (0, 1):
+        a, b = off[h], off[h + 1]
+        alone = engine.pot_sum_self([0, b - a], m[a:b], P[a:b, 0], P[a:b, 1],
+                                    P[a:b, 2]).cpu().numpy()
+        print(f"halo {h} ({b - a} members): alone == joint", flush=True)
+        assert np.array_equal(alone, got[a:b])
+    pick = np.sort(rng.choice(sizes[0], 1000, replace=False))
+    want = oracle.pot_sum_f64(m[:sizes[0]], P[:sizes[0], 0], P[:sizes[0], 1], P[:sizes[0], 2],
+                              P[pick, 0], P[pick, 1], P[pick, 2])
+    assert np.max(np.abs(got[pick] - want) / want) <= engine.pot_sym_error_bound()
+
+
 def test_pair_kernels_equal_infinite_coordinates(gpu):
     """Two members whose x is the same infinity.  The reference's x_j /= x_i is false for
     them, so it drops their pair; the kernels test the smallest |difference| against zero
