@@ -388,13 +388,15 @@ __global__ __launch_bounds__(kThreads) void k_halo_sets(
     icnt_sym[pos] = nbs * (nbs + 1) / 2;
 }
 
-__global__ __launch_bounds__(kThreads) void k_chunk_write(
+// The compaction by blocks bid, bid + nblk, ... of a launch (its own kernel, or the
+// second part of k_items_write)
+__device__ __forceinline__ void chunk_write_blocks(
     const int2* __restrict__ chunks, int n_chunks, const int64_t* __restrict__ off,
     const int32_t* __restrict__ active, const int32_t* __restrict__ ccnt,
     uint8_t* __restrict__ state, const float4* __restrict__ p4, float4* __restrict__ src4,
-    int32_t* __restrict__ cidx, float4* __restrict__ rem4) {
+    int32_t* __restrict__ cidx, float4* __restrict__ rem4, int bid, int nblk) {
     __shared__ int shs[kThreads / kWave];
-    for (int ci = blockIdx.x; ci < n_chunks; ci += gridDim.x) {
+    for (int ci = bid; ci < n_chunks; ci += nblk) {
         const int2 ch = chunks[ci];
         if (!active[ch.x]) continue;  // block-uniform
         int64_t b, e;
@@ -437,6 +439,15 @@ __global__ __launch_bounds__(kThreads) void k_chunk_write(
             rrun += tot >> 16;
         }
     }
+}
+
+__global__ __launch_bounds__(kThreads) void k_chunk_write(
+    const int2* __restrict__ chunks, int n_chunks, const int64_t* __restrict__ off,
+    const int32_t* __restrict__ active, const int32_t* __restrict__ ccnt,
+    uint8_t* __restrict__ state, const float4* __restrict__ p4, float4* __restrict__ src4,
+    int32_t* __restrict__ cidx, float4* __restrict__ rem4) {
+    chunk_write_blocks(chunks, n_chunks, off, active, ccnt, state, p4, src4, cidx, rem4,
+                       blockIdx.x, gridDim.x);
 }
 
 struct EnergyArgs {
@@ -844,11 +855,12 @@ __device__ __forceinline__ int last_le(const int* off, int H, int64_t k) {
     return lo;
 }
 
-__global__ __launch_bounds__(kThreads) void k_items_fused(
+__device__ __forceinline__ void items_fused_blocks(
     int H, const int32_t* __restrict__ order, const int64_t* __restrict__ icnt,
     const int64_t* __restrict__ icnt_sym, const int32_t* __restrict__ tst_len, int tile,
     int64_t cap, int64_t sym_cap, int4* __restrict__ items, int4* __restrict__ sym_items,
-    int64_t* __restrict__ itotal, int64_t* __restrict__ sym_itotal, int* __restrict__ sym_head) {
+    int64_t* __restrict__ itotal, int64_t* __restrict__ sym_itotal, int* __restrict__ sym_head,
+    int bid, int nblk) {
     __shared__ int off_a[kFuseH];
     __shared__ int off_b[kFuseH];
     __shared__ int shs[kThreads / kWave];
@@ -874,13 +886,13 @@ __global__ __launch_bounds__(kThreads) void k_items_fused(
         eb += vb[k];
     }
     __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
         *itotal = ta;
         *sym_itotal = tb;
         *sym_head = 0;
     }
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    const int64_t t0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t stride = static_cast<int64_t>(nblk) * blockDim.x;
+    const int64_t t0 = static_cast<int64_t>(bid) * blockDim.x + threadIdx.x;
     const int64_t na = ta < cap ? ta : cap;
     for (int64_t k = t0; k < na; k += stride) {
         const int pos = last_le(off_a, H, k);
@@ -900,6 +912,25 @@ __global__ __launch_bounds__(kThreads) void k_items_fused(
         const int64_t A = local - B * (B + 1) / 2;
         sym_items[k] = make_int4(order[pos], static_cast<int>(A), static_cast<int>(B), 0);
     }
+}
+
+// The pass's item lists (blocks 0 .. items_blocks - 1) and the bound-set compaction (the
+// other blocks) in one launch: the two only need k_halo_sets' offsets.
+__global__ __launch_bounds__(kThreads) void k_items_write(
+    int H, const int32_t* __restrict__ order, const int64_t* __restrict__ icnt,
+    const int64_t* __restrict__ icnt_sym, const int32_t* __restrict__ tst_len, int tile,
+    int64_t cap, int64_t sym_cap, int4* __restrict__ items, int4* __restrict__ sym_items,
+    int64_t* __restrict__ itotal, int64_t* __restrict__ sym_itotal, int* __restrict__ sym_head,
+    int items_blocks, const int2* __restrict__ chunks, int n_chunks,
+    const int64_t* __restrict__ off, const int32_t* __restrict__ active,
+    const int32_t* __restrict__ ccnt, uint8_t* __restrict__ state, const float4* __restrict__ p4,
+    float4* __restrict__ src4, int32_t* __restrict__ cidx, float4* __restrict__ rem4) {
+    if (static_cast<int>(blockIdx.x) < items_blocks)
+        items_fused_blocks(H, order, icnt, icnt_sym, tst_len, tile, cap, sym_cap, items,
+                           sym_items, itotal, sym_itotal, sym_head, blockIdx.x, items_blocks);
+    else
+        chunk_write_blocks(chunks, n_chunks, off, active, ccnt, state, p4, src4, cidx, rem4,
+                           blockIdx.x - items_blocks, gridDim.x - items_blocks);
 }
 
 __global__ void k_final_mask(int64_t nm, const uint8_t* __restrict__ state,
@@ -1094,11 +1125,13 @@ int launch_pre(const PassLaunch& L, hipStream_t st) {
         w.icnt, w.sym_icnt, w.sym, w.stale, w.need_full, w.counters);
     HALMA_LAUNCH_CHECK();
     if (L.H <= kFuseH) {
-        k_items_fused<<<L.grid_items, kThreads, 0, st>>>(static_cast<int>(L.H), w.order, w.icnt,
-                                                  w.sym_icnt, w.tst_len, tile, L.plan->n_items,
-                                                  w.sym_cap, w.items, w.sym_items, w.itotal,
-                                                  w.sym_itotal, w.sym_head);
+        const int wblocks = nc > 0 ? L.grid_chunks : 0;
+        k_items_write<<<L.grid_items + wblocks, kThreads, 0, st>>>(
+            static_cast<int>(L.H), w.order, w.icnt, w.sym_icnt, w.tst_len, tile, L.plan->n_items,
+            w.sym_cap, w.items, w.sym_items, w.itotal, w.sym_itotal, w.sym_head, L.grid_items,
+            w.chunks, nc, w.off, w.active, w.ccnt, w.state, w.p4, w.src4, w.cidx, w.rem4);
         HALMA_LAUNCH_CHECK();
+        return 0;
     } else {
         {
             const int rc = exclusive_scan_i64(w.icnt, w.ioff, L.H, w.itotal, w.iscan_ws, st);
