@@ -47,7 +47,7 @@ FOF_BYTES_PER_PARTICLE = 16  # 12 B FP32 xyz read + 4 B label written
 # step (the first pass carries almost all of it): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
 # two rocprofv3 --pmc passes, each scaled by what the counter reads for a 256 MiB device
 # copy (profiles/clumps_2M_hbm_traffic.md)
-SYM_TRAFFIC_BYTES = 80.2e6
+SYM_TRAFFIC_BYTES = 80.0e6
 MINP = 50
 VALU_ROOF = ("FP32 vector ALU (157.3 TFLOP/s, MI355X's dense FP32 figure, which counts "
              "packed FMA); the kernel issues FP32 vector instructions, no matrix "
