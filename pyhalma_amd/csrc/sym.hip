@@ -703,9 +703,10 @@ int launch_pot_sym(const SymPlan& plan, const float4* src4, const SymDesc* d_hal
     if (n_items == 0) return 0;
     int cus = device_cu_count();
     if (cus <= 0) cus = 256;
-    // both forms are queued when the item count is known only on the device (the one not
-    // chosen returns at once)
-    if (n_items >= kFineItems || n_items_dev) {
+    // both forms are queued when the item count is known only on the device and the plan
+    // allows either (the one not chosen returns at once); the device count never exceeds
+    // the planned n_items
+    if (n_items >= kFineItems) {
         const int grid = std::max(1, std::min(cus * 8, n_items));  // a workgroup or more per item
         k_pot_sym<<<grid, 256, 0, st>>>(src4, d_halos, live_n, d_items, n_items, n_items_dev,
                                         part, head);
