@@ -14,8 +14,8 @@
 //                    bound members into the front of its range (AoS float4), of the
 //                    members removed by the previous pass into a second list, the
 //                    double-double v_cm over the bound set, and the pass mode: FULL
-//                    (first pass, or the previous pass removed more than a quarter of
-//                    the bound set) sums over the bound set afresh; INCREMENTAL subtracts
+//                    (first pass, or the previous pass removed more than half as many
+//                    members as it kept) sums over the bound set afresh; INCREMENTAL subtracts
 //                    the removed members' contributions from each bound member's stored
 //                    sum, S_new = S_old - sum_{j removed} m_j / r_ij, which costs
 //                    n_bound x n_removed pairs instead of n_bound^2.  k_halo_sets also
@@ -114,11 +114,12 @@ __global__ void k_init_halo(const int64_t* __restrict__ off, int H, int32_t* __r
 }
 
 // A halo's pass is full (tests = sources = the live bound set, symmetric kernel) on
-// the first pass, whenever more than a quarter of the live set left in the previous
-// pass, and after a pass that left too many stale members (below); otherwise
-// incremental (the removed members' terms are subtracted).
+// the first pass, whenever the previous pass removed more than half as many members as
+// it kept (an incremental pass costs kept x removed pair terms, a full one kept^2 / 2
+// with the pair symmetry), and after a pass that left too many stale members (below);
+// otherwise incremental (the removed members' terms are subtracted).
 __device__ __forceinline__ bool pass_is_full(int pass_no, int run, int rrun, int need_full) {
-    return pass_no == 0 || need_full != 0 || 4 * static_cast<int64_t>(rrun) > run;
+    return pass_no == 0 || need_full != 0 || 2 * static_cast<int64_t>(rrun) > run;
 }
 
 // Largest relative error bound a member's S may keep.  A kept member whose accumulated
@@ -263,8 +264,8 @@ __global__ __launch_bounds__(kThreads) void k_chunk_sums(
 }
 
 // One thread per halo, in the plan's halo order (h = order[pos]): chunk output offsets,
-// bound and removed counts, v_cm, the pass mode (full recompute, or incremental when at
-// most a quarter of the set left last pass), the live sets of the pair kernel (tests =
+// bound and removed counts, v_cm, the pass mode (full recompute, or incremental when the
+// last pass removed at most half as many members as it kept), the live sets of the pair kernel (tests =
 // the bound set; sources = the bound set in a full pass, the members removed by the
 // previous pass in an incremental one; none when inactive) and the halo's number of
 // pair-kernel items (test tiles x source chunks).  Also clears the removed-members
