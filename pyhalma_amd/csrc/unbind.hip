@@ -9,21 +9,25 @@
 // E > 0.  (The reference evaluates the test once; iterating it is this path's design.)
 //
 // One pass on the GPU, for all haloes at once:
-//   1. compaction    k_chunk_sums / k_halo_sets / k_chunk_write over fixed chunks of
-//                    1024 member positions: stable compaction of each active halo's
+//   1. compaction    k_chunk_sums / k_halo_sets, then k_items_write, over fixed chunks
+//                    of 1024 member positions: stable compaction of each active halo's
 //                    bound members into the front of its range (AoS float4), of the
 //                    members removed by the previous pass into a second list, the
 //                    double-double v_cm over the bound set, and the pass mode: FULL
 //                    (first pass, or the previous pass removed more than half as many
-//                    members as it kept) sums over the bound set afresh; INCREMENTAL subtracts
-//                    the removed members' contributions from each bound member's stored
-//                    sum, S_new = S_old - sum_{j removed} m_j / r_ij, which costs
-//                    n_bound x n_removed pairs instead of n_bound^2.  k_halo_sets also
-//                    sets the live test/source counts and each halo's number of
-//                    pair-kernel items; a scan and k_item_fill write the pass's queue
-//                    (haloes in the plan's largest-first order);
-//   2. k_pot_f32     the persistent FP32 pair kernel over that queue;
-//   3. k_energy      merges the chunk partials (FP64), updates S and its absolute error
+//                    members as it kept) sums over the bound set afresh; INCREMENTAL
+//                    subtracts the removed members' contributions from each bound
+//                    member's stored sum, S_new = S_old - sum_{j removed} m_j / r_ij,
+//                    which costs n_bound x n_removed pairs instead of n_bound^2.
+//                    k_halo_sets also sets the live test/source counts and each halo's
+//                    number of pair-kernel items; k_items_write writes both pair
+//                    kernels' item lists (haloes in the plan's largest-first order) and,
+//                    in its other blocks, the compacted lists;
+//   2. pair kernels  full passes on the symmetric kernel (k_pot_sym, or k_pot_sym_fine
+//                    for fewer than 1024 block pairs), incremental passes (and full
+//                    passes of haloes planned without slots) on the chunked FP32 kernel
+//                    k_pot_f32;
+//   3. k_energy      merges the slots or chunk partials (FP64), updates S and its error
 //                    bound (the bounds of every FP32 sum that went into S add up),
 //                    evaluates E in FP64 and decides every member whose |E| exceeds
 //                    factor*G*bound + 8 ulp(kin + factor*G*S);
