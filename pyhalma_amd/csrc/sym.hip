@@ -13,9 +13,10 @@
 //     order.
 // The test side keeps one Kahan sum per source tile; the tiles' sums are added in FP64
 // in tile order.  A launch with many items gives each item to one wavefront (all tiles of
-// B in turn); a launch with few items (a lone large halo, the last passes of an
-// unbinding) gives each to a workgroup whose wavefront w takes tile w of B, so that an
-// item lasts four tile pairs instead of sixteen.  Both forms do the same arithmetic in
+// B in turn); a launch with fewer (a lone large halo, the last passes of an unbinding)
+// gives each to a workgroup whose wavefront w takes tile w of B, so that an item lasts
+// four tile pairs instead of sixteen, and one with fewer than 1024 items to a 1024-thread
+// workgroup with one tile pair per wavefront (k_pot_sym_fine).  Both forms do the same arithmetic in
 // the same order: a halo's sums do not depend on the other haloes of the call.  Within a diagonal item
 // (A == B) only pairs of tiles a <= b are taken, and a tile against itself is summed on
 // the test side alone.  Every member then has kSymK-block partial sums in nb + 1 slots
